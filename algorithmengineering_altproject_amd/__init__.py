@@ -1,0 +1,193 @@
+"""MI355X-native TOP3D_XL finite-element inner loop.
+
+Python host binding of the C ABI in include/t3d_capi.h (ctypes; no torch).
+The compute path is the HIP library libt3d.so built for gfx950 by
+csrc/Makefile; importing this module fails loudly when it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libt3d.so")
+REPO = os.path.dirname(HERE)
+
+_lib = None
+
+T3D_OK = 0
+
+
+class T3DError(RuntimeError):
+    pass
+
+
+def build(verbose=False):
+    """Compile libt3d.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    import subprocess
+    cmd = ["make", "-C", os.path.join(HERE, "csrc"), "-j8"]
+    r = subprocess.run(cmd, capture_output=not verbose, text=True)
+    if r.returncode != 0:
+        raise T3DError("libt3d build failed:\n" + (r.stdout or "") + (r.stderr or ""))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise T3DError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "t3d_last_error": (C.c_char_p, []),
+        "t3d_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+        "t3d_malloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+        "t3d_free": (C.c_int, [vp]),
+        "t3d_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "t3d_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+        "t3d_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "t3d_memset": (C.c_int, [vp, C.c_int, C.c_size_t]),
+        "t3d_synchronize": (C.c_int, []),
+        "t3d_problem_create_cuboid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "t3d_problem_destroy": (C.c_int, [vp]),
+        "t3d_problem_sizes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "t3d_problem_model": (C.c_int, [vp] + [vp] * 8),
+        "t3d_free_to_full": (C.c_int, [vp, vp, vp]),
+        "t3d_full_to_free": (C.c_int, [vp, vp, vp]),
+        "t3d_load_vector": (C.c_int, [vp, vp]),
+        "t3d_set_modulus": (C.c_int, [vp, vp]),
+        "t3d_kxu_f64": (C.c_int, [vp, vp, vp]),
+        "t3d_kxu_free_host": (C.c_int, [vp, vp, vp, vp]),
+        "t3d_dot_f64": (C.c_int, [vp, vp, vp, C.c_int64, C.POINTER(C.c_double)]),
+        "t3d_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != T3D_OK:
+        msg = lib().t3d_last_error().decode(errors="replace")
+        raise T3DError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A device allocation owned by Python (t3d_malloc/t3d_free)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().t3d_malloc(C.byref(p), max(1, self.nbytes)), "t3d_malloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().t3d_memcpy_h2d(self.ptr, _ptr(a), self.nbytes), "h2d")
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().t3d_memcpy_d2h(_ptr(out), self.ptr, self.nbytes), "d2h")
+        return out
+
+    def zero(self):
+        check(lib().t3d_memset(self.ptr, 0, self.nbytes), "memset")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().t3d_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Problem:
+    """Device problem: CreateVoxelFEAmodel_Cuboid + ApplyBoundaryConditions
+    (cpp/src/fea.cpp:181-447) with device mirrors."""
+
+    def __init__(self, nely, nelx, nelz):
+        p = C.c_void_p()
+        check(lib().t3d_problem_create_cuboid(nely, nelx, nelz, C.byref(p)), "t3d_problem_create_cuboid")
+        self.h = p
+        s = (C.c_int64 * 10)()
+        check(lib().t3d_problem_sizes(self.h, s), "t3d_problem_sizes")
+        (self.resX, self.resY, self.resZ, self.ne, self.nN, self.nDOF, self.nF,
+         self.nnx, self.nny, self.nnz) = [int(v) for v in s]
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_problem_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def model(self):
+        ne, nN, nF = self.ne, self.nN, self.nF
+        out = dict(eleMapBack=np.zeros(ne, np.int32), eNodMat=np.zeros(8 * ne, np.int32),
+                   color=np.zeros(ne, np.int32), Ke=np.zeros(576), F=np.zeros(3 * nN),
+                   isFree=np.zeros(3 * nN, np.uint8), freeDofIndex=np.zeros(nF, np.int32),
+                   eDofFree=np.zeros(24 * ne, np.int32))
+        check(lib().t3d_problem_model(self.h, *[_ptr(out[k]) for k in
+                                                ("eleMapBack", "eNodMat", "color", "Ke", "F", "isFree",
+                                                 "freeDofIndex", "eDofFree")]), "t3d_problem_model")
+        return out
+
+    def free_to_full(self, dfree, dfull):
+        check(lib().t3d_free_to_full(self.h, dfree.ptr, dfull.ptr), "t3d_free_to_full")
+
+    def full_to_free(self, dfull, dfree):
+        check(lib().t3d_full_to_free(self.h, dfull.ptr, dfree.ptr), "t3d_full_to_free")
+
+    def load_vector(self, dfull):
+        check(lib().t3d_load_vector(self.h, dfull.ptr), "t3d_load_vector")
+
+    def set_modulus(self, dE):
+        check(lib().t3d_set_modulus(self.h, dE.ptr), "t3d_set_modulus")
+
+    def kxu_f64(self, du, dy):
+        check(lib().t3d_kxu_f64(self.h, du.ptr, dy.ptr), "t3d_kxu_f64")
+
+    def kxu_free_host(self, E, uFree):
+        E = np.ascontiguousarray(E, np.float64)
+        u = np.ascontiguousarray(uFree, np.float64)
+        y = np.zeros(self.nF)
+        check(lib().t3d_kxu_free_host(self.h, _ptr(E), _ptr(u), _ptr(y)), "t3d_kxu_free_host")
+        return y
+
+    def dot(self, da, db, n=None):
+        out = C.c_double()
+        check(lib().t3d_dot_f64(self.h, da.ptr, db.ptr, n if n is not None else da.n, C.byref(out)), "t3d_dot_f64")
+        return out.value
+
+    def time_op(self, op, din, dout, reps):
+        ms = C.c_double()
+        check(lib().t3d_time_op(self.h, op, din.ptr, dout.ptr, reps, C.byref(ms)), "t3d_time_op")
+        return ms.value
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    cus = C.c_int()
+    mem = C.c_size_t()
+    check(lib().t3d_device_info(name, 64, C.byref(cus), C.byref(mem)), "t3d_device_info")
+    return name.value.decode(), cus.value, mem.value

@@ -1,0 +1,74 @@
+// t3d_internal.hpp -- internal declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/t3d_capi.h"
+#include "model.hpp"
+
+namespace t3d {
+
+// ---- error handling: every C-ABI entry returns an int status ----
+void set_error(const std::string& msg);
+#define T3D_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::t3d::set_error(std::string(#call) + ": " + hipGetErrorString(_e));              \
+            return T3D_ERR_HIP;                                                               \
+        }                                                                                     \
+    } while (0)
+#define T3D_CHECK(cond, code, msg)                \
+    do {                                          \
+        if (!(cond)) {                            \
+            ::t3d::set_error(msg);                \
+            return (code);                        \
+        }                                         \
+    } while (0)
+
+// canonical reduction geometry (identical to the oracle's definition):
+// tile = 256 lanes x 4 items; lane l of tile t sums items t*1024 + k*256 + l
+constexpr int CANON_LANES = 256;
+constexpr int CANON_PER_LANE = 4;
+constexpr int CANON_TILE = CANON_LANES * CANON_PER_LANE;
+
+struct Grid {
+    int resX, resY, resZ;   // cells
+    int nnx, nny, nnz;      // nodes per axis
+    int64_t nN;             // nodes
+    int64_t ncell;
+};
+
+}  // namespace t3d
+
+// device-side problem context
+struct t3d_problem {
+    t3d::HostModel host;
+    t3d::Grid g;
+    int ne = 0;
+    int64_t nF = 0;
+    hipStream_t stream = nullptr;
+    // device arrays
+    double* Ke = nullptr;               // [576]
+    uint8_t* nodeFixed = nullptr;       // [nN] bit c set = DOF c fixed
+    int32_t* compactToCell = nullptr;   // [ne]
+    int32_t* cellToCompact = nullptr;   // [ncell], -1 = void
+    int32_t* freeDofIndex = nullptr;    // [nF]
+    double* Ecell = nullptr;            // [ncell] modulus per structured cell (0 = void)
+    float* EcellF = nullptr;            // [ncell] fp32 copy for the single precision operator
+    double* bFull = nullptr;            // [3*nN] restrict_to_free(F) in the full layout
+    // reduction scratch
+    double* red = nullptr;              // partial sums
+    int64_t redCap = 0;
+    double* scal = nullptr;             // device scalars
+};
+
+namespace t3d {
+int ensure_red(t3d_problem* p, int64_t items);
+// canonical sum of n items produced by kernel `first` into p->red; final value to dst (device)
+int canon_finish(t3d_problem* p, int64_t ntiles, double* dst);
+int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst);
+}  // namespace t3d

@@ -1,0 +1,95 @@
+/*
+ * t3d_capi.h -- C ABI of the MI355X (gfx950) TOP3D_XL finite-element inner loop.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Device pointers are
+ * HIP device addresses, e.g. from t3d_malloc().  Every entry point returns
+ * T3D_OK (0) or a negative status; t3d_last_error() describes the last failure
+ * on the calling thread.  Results are complete when a call returns (each call
+ * synchronises the problem's stream unless it says otherwise).
+ *
+ * Vector layouts:
+ *   free   -- the reference's compact free-DOF vector: entry i is global DOF
+ *             freeDofIndex[i] (cpp/src/solver.cpp:12-49).
+ *   full   -- the node-major AoS vector v[3*node + c] over every grid node,
+ *             node = (iz*nny + iy)*nnx + ix; fixed DOFs hold 0.  The device
+ *             kernels work on this layout; t3d_free_to_full/t3d_full_to_free
+ *             convert.
+ *   compact element arrays -- indexed by the reference's compact (Morton)
+ *             element id (cpp/src/fea.cpp:289-317).
+ *
+ * Each declaration names the reference function it replaces (file:line in the
+ * reference tree).
+ */
+#ifndef T3D_CAPI_H
+#define T3D_CAPI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define T3D_OK 0
+#define T3D_ERR_HIP (-1)        /* a HIP runtime call failed */
+#define T3D_ERR_ARG (-2)        /* bad argument / size */
+#define T3D_ERR_STATE (-3)      /* object not set up for this call */
+#define T3D_ERR_NOT_SPD (-4)    /* coarsest matrix not SPD (diagonal fallback used) */
+
+typedef struct t3d_problem t3d_problem;
+
+/* ---- runtime / memory helpers (no reference counterpart: device plumbing) ---- */
+const char* t3d_last_error(void);
+int t3d_device_info(char* name, size_t name_len, int* num_cus, size_t* total_mem);
+int t3d_malloc(void** ptr, size_t bytes);
+int t3d_free(void* ptr);
+int t3d_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int t3d_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int t3d_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes);
+int t3d_memset(void* dst_dev, int value, size_t bytes);
+int t3d_synchronize(void);
+
+/* ---- problem (model + boundary conditions + device mirrors) ----
+ * Replaces CreateVoxelFEAmodel_Cuboid (cpp/include/fea.hpp:38, cpp/src/fea.cpp:181-358)
+ * followed by ApplyBoundaryConditions (cpp/include/fea.hpp:39, cpp/src/fea.cpp:360-447):
+ * builds the padded voxel model on the host and uploads its device mirror. */
+int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out);
+int t3d_problem_destroy(t3d_problem* p);
+/* sizes[0..9] = resX resY resZ numElements numNodes numDOFs numFree nnx nny nnz */
+int t3d_problem_sizes(const t3d_problem* p, int64_t* sizes);
+/* Copies the host model (integer maps of fea.cpp:181-447 and Ke of fea.cpp:117-179).
+ * Any pointer may be NULL.  Sizes: eleMapBack[ne], eNodMat[8ne], elemColor[ne],
+ * Ke[576], F[3nN] (AoS), isFree[3nN], freeDofIndex[nF], eDofFreeMat[24ne]. */
+int t3d_problem_model(const t3d_problem* p, int32_t* eleMapBack, int32_t* eNodMat, int32_t* elemColor,
+                      double* Ke, double* F, uint8_t* isFree, int32_t* freeDofIndex, int32_t* eDofFreeMat);
+
+/* restrict_to_free / scatter_from_free (cpp/include/solver.hpp:8-9, cpp/src/solver.cpp:12-49)
+ * between the compact free vector and the full device layout (device pointers). */
+int t3d_free_to_full(t3d_problem* p, const double* free_dev, double* full_dev);
+int t3d_full_to_free(t3d_problem* p, const double* full_dev, double* free_dev);
+/* restrict_to_free(pb.F) of driver.cpp:75 in the full layout (device pointer) */
+int t3d_load_vector(t3d_problem* p, double* b_full_dev);
+
+/* Element moduli eleMod[e] (compact order, device pointer) used by every
+ * operator until the next call; scattered to the structured cell grid. */
+int t3d_set_modulus(t3d_problem* p, const double* E_dev);
+
+/* K_times_u_finest (cpp/include/fea.hpp:42,45; cpp/src/fea.cpp:763-849):
+ * y = K_ff u on the full layout in FP64, bit-identical to the reference's
+ * operation order (colour-ordered accumulation, sequential 24-term rows, no FMA). */
+int t3d_kxu_f64(t3d_problem* p, const double* u_full_dev, double* y_full_dev);
+/* Legacy host-vector form of K_times_u_finest: E[ne], uFree[nF] -> yFree[nF] (host pointers). */
+int t3d_kxu_free_host(t3d_problem* p, const double* E_host, const double* uFree_host, double* yFree_host);
+
+/* parallel_dot (cpp/src/solver.cpp:77-84) with the canonical reduction tree:
+ * result to *out_host. */
+int t3d_dot_f64(t3d_problem* p, const double* a_dev, const double* b_dev, int64_t n, double* out_host);
+
+/* ---- timing (no reference counterpart): average device time of `reps`
+ * back-to-back launches of an operator, measured with HIP events on the
+ * problem's stream.  op: 0 = t3d_kxu_f64. */
+int t3d_time_op(t3d_problem* p, int op, const void* in_dev, void* out_dev, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
