@@ -61,6 +61,16 @@ def lib():
         "t3d_kxu_free_host": (C.c_int, [vp, vp, vp, vp]),
         "t3d_dot_f64": (C.c_int, [vp, vp, vp, C.c_int64, C.POINTER(C.c_double)]),
         "t3d_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double)]),
+        "t3d_mg_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.POINTER(vp)]),
+        "t3d_mg_destroy": (C.c_int, [vp]),
+        "t3d_mg_setup": (C.c_int, [vp, vp]),
+        "t3d_mg_apply": (C.c_int, [vp, vp, vp]),
+        "t3d_mg_info": (C.c_int, [vp, C.POINTER(C.c_int), vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "t3d_mg_mask": (C.c_int, [vp, C.c_int, vp]),
+        "t3d_mg_diag": (C.c_int, [vp, C.c_int, vp]),
+        "t3d_mg_coarsest": (C.c_int, [vp, vp, vp]),
+        "t3d_pcg": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int)]),
+        "t3d_pcg_history": (C.c_int, [vp, C.c_int, C.c_int, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -183,6 +193,71 @@ class Problem:
         ms = C.c_double()
         check(lib().t3d_time_op(self.h, op, din.ptr, dout.ptr, reps, C.byref(ms)), "t3d_time_op")
         return ms.value
+
+
+class MG:
+    """build_static_once + make_diagonal_preconditioner_from_static
+    (cpp/src/multigrid/preconditioner.cpp:19-266) on the device."""
+
+    def __init__(self, prob, nonDyadic=1, maxLevels=5, weight=0.6, Emin=1e-6, NlimitDofs=1000):
+        self.prob = prob
+        h = C.c_void_p()
+        check(lib().t3d_mg_create(prob.h, nonDyadic, maxLevels, weight, Emin, NlimitDofs, C.byref(h)),
+              "t3d_mg_create")
+        self.h = h
+        nl = C.c_int()
+        dims = (C.c_int * 32)()
+        nc = C.c_int()
+        ok = C.c_int()
+        check(lib().t3d_mg_info(self.h, C.byref(nl), dims, C.byref(nc), C.byref(ok)), "t3d_mg_info")
+        self.nlev = nl.value
+        self.dims = [tuple(dims[4 * l:4 * l + 4]) for l in range(self.nlev)]
+        self.nN = [(d[0] + 1) * (d[1] + 1) * (d[2] + 1) for d in self.dims]
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_mg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        nl = C.c_int()
+        nc = C.c_int()
+        ok = C.c_int()
+        check(lib().t3d_mg_info(self.h, C.byref(nl), None, C.byref(nc), C.byref(ok)), "t3d_mg_info")
+        return nc.value, ok.value
+
+    def setup(self, dE):
+        check(lib().t3d_mg_setup(self.h, dE.ptr), "t3d_mg_setup")
+
+    def apply(self, dr, dz):
+        check(lib().t3d_mg_apply(self.h, dr.ptr, dz.ptr), "t3d_mg_apply")
+
+    def mask(self, level):
+        out = np.zeros(3 * self.nN[level], np.uint8)
+        check(lib().t3d_mg_mask(self.h, level, _ptr(out)), "t3d_mg_mask")
+        return out
+
+    def diag(self, level):
+        out = np.zeros(3 * self.nN[level])
+        check(lib().t3d_mg_diag(self.h, level, _ptr(out)), "t3d_mg_diag")
+        return out
+
+    def coarsest(self):
+        N = 3 * self.nN[-1]
+        Kc = np.zeros(N * N)
+        L = np.zeros(N * N)
+        check(lib().t3d_mg_coarsest(self.h, _ptr(Kc), _ptr(L)), "t3d_mg_coarsest")
+        return Kc.reshape(N, N), L.reshape(N, N)
+
+
+def pcg(prob, M, db, dx, tol, maxIt):
+    """PCG_free (cpp/src/solver.cpp:87-217) on device vectors (full layout); returns iterations."""
+    it = C.c_int()
+    check(lib().t3d_pcg(prob.h, M.h if M is not None else None, db.ptr, dx.ptr, tol, maxIt, C.byref(it)), "t3d_pcg")
+    return it.value
 
 
 def device_info():

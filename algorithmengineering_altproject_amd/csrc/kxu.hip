@@ -86,13 +86,139 @@ __global__ void __launch_bounds__(256) k_kxu_f64_node(Grid g, const double* __re
     y[3 * n + 2] = (fixed & 4u) ? 0.0 : acc2;
 }
 
-int launch_kxu_f64(t3d_problem* p, const double* u, double* y)
+// ---------------------------------------------------------------------------
+// v2: LDS-tiled node-centric FP64 operator with wave-uniform sub-lattices.
+//
+// A workgroup owns a 16x8x4 node tile.  Wave w (of 8) owns the 64 nodes whose
+// coordinate parities are (w&1, (w>>1)&1, (w>>2)&1).  For a fixed colour the
+// incident cell offset -- and hence the node's local index a and the three Ke
+// rows it uses -- depend only on the parities, so they are uniform across the
+// wave: Ke is read through the scalar cache into SGPR operands while u and E
+// come from LDS.  Summation order per DOF is colour 0..7 as in the reference.
+// ---------------------------------------------------------------------------
+// Each thread owns NZ nodes of its sub-lattice stacked 4 apart in z, so one
+// scalar Ke load feeds NZ multiply-adds.
+template <int NZ>
+struct KxuTile {
+    static constexpr int TX = 16, TY = 8, TZ = 4 * NZ;
+    static constexpr int UX = TX + 2, UY = TY + 2, UZ = TZ + 2;   // node tile + halo
+    static constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1;   // cell tile + halo
+    static constexpr int UN = UX * UY * UZ;
+    static constexpr int EN = EX * EY * EZ;
+};
+
+template <int NZ>
+__global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __restrict__ Ke,
+                                                     const double* __restrict__ Ecell,
+                                                     const uint8_t* __restrict__ nodeFixed,
+                                                     const double* __restrict__ u, double* __restrict__ y,
+                                                     const int* __restrict__ done)
 {
-    const int B = 256;
-    hipLaunchKernelGGL(k_kxu_f64_node, dim3(unsigned((p->g.nN + B - 1) / B)), dim3(B), 0, p->stream, p->g, p->Ke,
-                       p->Ecell, p->nodeFixed, u, y);
+    if (done && *done) return;
+    using T = KxuTile<NZ>;
+    __shared__ double sU[3 * T::UN];   // SoA: component c at sU[c*UN + idx]
+    __shared__ double sE[T::EN];
+    const int x0 = blockIdx.x * T::TX, y0 = blockIdx.y * T::TY, z0 = blockIdx.z * T::TZ;
+    const int tid = threadIdx.x;
+    // cooperative loads (zero outside the grid)
+    for (int i = tid; i < T::UN; i += 512) {
+        const int lx = i % T::UX, ly = (i / T::UX) % T::UY, lz = i / (T::UX * T::UY);
+        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        if (X >= 0 && Y >= 0 && Z >= 0 && X < g.nnx && Y < g.nny && Z < g.nnz) {
+            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
+            v0 = u[3 * n]; v1 = u[3 * n + 1]; v2 = u[3 * n + 2];
+        }
+        sU[i] = v0; sU[T::UN + i] = v1; sU[2 * T::UN + i] = v2;
+    }
+    for (int i = tid; i < T::EN; i += 512) {
+        const int lx = i % T::EX, ly = (i / T::EX) % T::EY, lz = i / (T::EX * T::EY);
+        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
+        double e = 0.0;
+        if (X >= 0 && Y >= 0 && Z >= 0 && X < g.resX && Y < g.resY && Z < g.resZ)
+            e = Ecell[(int64_t(Z) * g.resY + Y) * g.resX + X];
+        sE[i] = e;
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int px = w & 1, py = (w >> 1) & 1, pz = (w >> 2) & 1;
+    const int lx = 2 * (lane & 7) + px, ly = 2 * ((lane >> 3) & 3) + py, lz = 2 * (lane >> 5) + pz;
+    double acc[NZ][3];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+#pragma unroll 1
+    for (int col = 0; col < 8; ++col) {
+        const int dx = (px ^ col) & 1;
+        const int dy = (py ^ (col >> 1) ^ 1) & 1;
+        const int dz = (pz ^ (col >> 2)) & 1;
+        const int cxl = lx - dx, cyl = ly - dy, czl = lz - dz;   // incident cell, tile coords
+        const int a = __builtin_amdgcn_readfirstlane(local_index(dx, dy, dz));
+        const double* __restrict__ K0 = Ke + (3 * a) * 24;
+        double E[NZ], f[NZ][3];
+        int ub[NZ];
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+            E[k] = sE[((czl + 4 * k + 1) * T::EY + (cyl + 1)) * T::EX + (cxl + 1)];
+            ub[k] = ((czl + 4 * k + 1) * T::UY + (cyl + 1)) * T::UX + (cxl + 1);
+            f[k][0] = f[k][1] = f[k][2] = 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int ox = (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0;
+            const int oy = ((b & 3) <= 1) ? 1 : 0;
+            const int oz = b >> 2;
+            const int off = (oz * T::UY + oy) * T::UX + ox;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int j = 3 * b + c;
+                const double k0 = K0[j], k1 = K0[24 + j], k2 = K0[48 + j];
+#pragma unroll
+                for (int k = 0; k < NZ; ++k) {
+                    const double ue = sU[c * T::UN + ub[k] + off];
+                    f[k][0] = f[k][0] + k0 * ue;
+                    f[k][1] = f[k][1] + k1 * ue;
+                    f[k][2] = f[k][2] + k2 * ue;
+                }
+            }
+        }
+        // elements with E <= 1e-16 (and void cells, E = 0) contribute nothing (fea.cpp:743-744)
+#pragma unroll
+        for (int k = 0; k < NZ; ++k)
+            if (!(E[k] <= 1.0e-16)) {
+                acc[k][0] = acc[k][0] + f[k][0] * E[k];
+                acc[k][1] = acc[k][1] + f[k][1] * E[k];
+                acc[k][2] = acc[k][2] + f[k][2] * E[k];
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        const int X = x0 + lx, Y = y0 + ly, Z = z0 + lz + 4 * k;
+        if (X < g.nnx && Y < g.nny && Z < g.nnz) {
+            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
+            const unsigned fixed = nodeFixed[n];
+            y[3 * n + 0] = (fixed & 1u) ? 0.0 : acc[k][0];
+            y[3 * n + 1] = (fixed & 2u) ? 0.0 : acc[k][1];
+            y[3 * n + 2] = (fixed & 4u) ? 0.0 : acc[k][2];
+        }
+    }
+}
+
+template <int NZ>
+static int launch_kxu_lds(t3d_problem* p, const double* u, double* y, const int* done = nullptr)
+{
+    using T = KxuTile<NZ>;
+    const Grid& g = p->g;
+    dim3 grid(unsigned((g.nnx + T::TX - 1) / T::TX), unsigned((g.nny + T::TY - 1) / T::TY),
+              unsigned((g.nnz + T::TZ - 1) / T::TZ));
+    hipLaunchKernelGGL(k_kxu_f64_lds<NZ>, grid, dim3(512), 0, p->stream, g, p->Ke, p->Ecell, p->nodeFixed, u, y, done);
     T3D_HIP(hipGetLastError());
     return T3D_OK;
+}
+
+int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done)
+{
+    return launch_kxu_lds<1>(p, u, y, done);
 }
 
 }  // namespace t3d
@@ -156,6 +282,13 @@ int t3d_time_op(t3d_problem* p, int op, const void* in, void* out, int reps, dou
     for (int r = 0; r < reps && rc == T3D_OK; ++r) {
         switch (op) {
             case 0: rc = launch_kxu_f64(p, (const double*)in, (double*)out); break;
+            case 101: rc = launch_kxu_lds<2>(p, (const double*)in, (double*)out); break;
+            case 100: {
+                const int B = 256;
+                hipLaunchKernelGGL(k_kxu_f64_node, dim3(unsigned((p->g.nN + B - 1) / B)), dim3(B), 0, p->stream,
+                                   p->g, p->Ke, p->Ecell, p->nodeFixed, (const double*)in, (double*)out);
+                break;
+            }
             default: set_error("t3d_time_op: unknown op"); rc = T3D_ERR_ARG;
         }
     }

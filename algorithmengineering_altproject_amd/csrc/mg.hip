@@ -1,0 +1,546 @@
+// mg.hip -- geometric multigrid preconditioner on the device.
+//
+// Reference: make_diagonal_preconditioner_from_static and its V-cycle closure
+// (cpp/src/multigrid/preconditioner.cpp:38-266), MG_BuildDiagonals
+// (cpp/src/multigrid/diagonal.cpp:11-160), MG_Restrict_nodes_Vec3_Inner /
+// MG_Prolongate_nodes_Vec3_Inner (cpp/src/multigrid/transfers.cpp:378-533),
+// chol_solve_lower (cpp/src/multigrid/coarsest.cpp:92-122).
+//
+// All vectors use the full node-major AoS layout of their level.  Every
+// accumulation keeps the reference's sequential order, so results are
+// bit-identical to the reference's arithmetic:
+//   * finest diagonal: per DOF, incident cells in colour order 0..7;
+//   * coarse diagonals: per coarse DOF, incident coarse cells in colour order,
+//     each contributing its (span+1)^3 embedded fine nodes in (iz,iy,ix) order;
+//   * restriction: per coarse node, the clamped (2s-1)^3 stencil in (iz,iy,ix)
+//     order, scaled by 1/wSum;
+//   * prolongation: 8 trilinear terms in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "mg_host.hpp"
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+struct LevelDev {
+    int resX, resY, resZ;   // cells
+    int nnx, nny, nnz;      // nodes
+    int64_t nN;
+};
+
+static LevelDev level_dev(const MGLevelDims& d)
+{
+    LevelDev L;
+    L.resX = d.resX; L.resY = d.resY; L.resZ = d.resZ;
+    L.nnx = d.resX + 1; L.nny = d.resY + 1; L.nnz = d.resZ + 1;
+    L.nN = d.nN;
+    return L;
+}
+
+// local index a of the corner at cell-offset (dx,dy,dz); dy = 1 is the upper node row
+__device__ __forceinline__ int corner_index(int dx, int dy, int dz)
+{
+    return (dy ? (dx ? 1 : 0) : (dx ? 2 : 3)) + 4 * dz;
+}
+
+// ---------------------------------------------------------------------------
+// finest diagonal (diagonal.cpp:11-63 + BC at preconditioner level, :147-152)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_diag_finest(Grid g, const double* __restrict__ Ke,
+                                                     const double* __restrict__ Ecell,
+                                                     const uint8_t* __restrict__ fixedBits,
+                                                     double* __restrict__ D)
+{
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= g.nN) return;
+    const int ix = int(n % g.nnx), iy = int((n / g.nnx) % g.nny), iz = int(n / (int64_t(g.nnx) * g.nny));
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int col = 0; col < 8; ++col) {
+        const int dx = (ix ^ col) & 1, dy = (iy ^ (col >> 1) ^ 1) & 1, dz = (iz ^ (col >> 2)) & 1;
+        const int cx = ix - dx, ty = iy - dy, cz = iz - dz;
+        if (cx < 0 || ty < 0 || cz < 0 || cx >= g.resX || ty >= g.resY || cz >= g.resZ) continue;
+        const double E = Ecell[(int64_t(cz) * g.resY + ty) * g.resX + cx];
+        const int a = corner_index(dx, dy, dz);
+        const int l = 3 * a;
+        // void cells hold E = 0 and add +0.0, which leaves the sum unchanged
+        d0 = d0 + Ke[(l + 0) * 25] * E;
+        d1 = d1 + Ke[(l + 1) * 25] * E;
+        d2 = d2 + Ke[(l + 2) * 25] * E;
+    }
+    const unsigned f = fixedBits[n];
+    d0 = (d0 > 0.0) ? d0 : 1.0;
+    d1 = (d1 > 0.0) ? d1 : 1.0;
+    d2 = (d2 > 0.0) ? d2 : 1.0;
+    D[3 * n + 0] = (f & 1u) ? 1.0 : d0;
+    D[3 * n + 1] = (f & 2u) ? 1.0 : d1;
+    D[3 * n + 2] = (f & 4u) ? 1.0 : d2;
+}
+
+// ---------------------------------------------------------------------------
+// coarse diagonal by W^2-weighted averaging (diagonal.cpp:66-138)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_diag_coarsen(LevelDev Lf, LevelDev Lc, int span,
+                                                      const double* __restrict__ W,
+                                                      const double* __restrict__ Df,
+                                                      const uint8_t* __restrict__ fmask,
+                                                      const uint8_t* __restrict__ cmask,
+                                                      double* __restrict__ Dc)
+{
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lc.nN) return;
+    const int X = int(n % Lc.nnx), Y = int((n / Lc.nnx) % Lc.nny), Z = int(n / (int64_t(Lc.nnx) * Lc.nny));
+    const unsigned cm = cmask[n];
+    const int grid = span + 1;
+    double acc[3] = {0.0, 0.0, 0.0}, ws[3] = {0.0, 0.0, 0.0};
+    for (int col = 0; col < 8; ++col) {
+        const int cex = X - ((X ^ col) & 1);
+        const int cez = Z - ((Z ^ col) & 1);
+        // element rows cey = resY-1-cty with cty in {Y-1, Y}
+        int cey = Lc.resY - Y;                    // cty = Y-1
+        if ((cey & 1) != ((col >> 1) & 1)) cey = Lc.resY - 1 - Y;   // cty = Y
+        if (cex < 0 || cez < 0 || cey < 0 || cex >= Lc.resX || cez >= Lc.resZ || cey >= Lc.resY) continue;
+        const int cty = Lc.resY - 1 - cey;
+        const int a = corner_index(X - cex, Y - cty, Z - cez);
+        const int fx0 = cex * span, fy0 = (Lc.resY - cey) * span, fz0 = cez * span;
+        for (int iz = 0; iz <= span; ++iz)
+            for (int iy = 0; iy <= span; ++iy)
+                for (int ix = 0; ix <= span; ++ix) {
+                    const int64_t fn = (int64_t(fz0 + iz) * Lf.nny + (fy0 - iy)) * Lf.nnx + (fx0 + ix);
+                    const double w = W[((iz * grid + iy) * grid + ix) * 8 + a];
+                    const double w2 = w * w;
+                    const unsigned fm = fmask[fn];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if ((cm >> c) & 1u) continue;
+                        if ((fm >> c) & 1u) continue;
+                        acc[c] = acc[c] + w2 * Df[3 * fn + c];
+                        ws[c] = ws[c] + w2;
+                    }
+                }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double v;
+        if ((cm >> c) & 1u) v = 1.0;
+        else if (ws[c] > 0) {
+            v = acc[c] / ws[c];
+            if (!(v > 0.0)) v = 1.0;
+        } else v = 1.0;
+        Dc[3 * n + c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// restriction fine -> coarse, coarse mask applied; when Dc != nullptr the
+// level is the coarsest one and the diagonal solve x = r / max(1e-30, D) is
+// fused (preconditioner.cpp:195-227)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double hat_w(int i, int span, double inv)
+{
+    const int d = i - (span - 1);
+    return 1.0 - double(d < 0 ? -d : d) * inv;
+}
+
+__global__ void __launch_bounds__(256) k_restrict(LevelDev Lf, LevelDev Lc, int span,
+                                                  const double* __restrict__ rf, const uint8_t* __restrict__ cmask,
+                                                  double* __restrict__ rc, const double* __restrict__ Dc,
+                                                  double* __restrict__ xc, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lc.nN) return;
+    const int cx = int(n % Lc.nnx), cy = int((n / Lc.nnx) % Lc.nny), cz = int(n / (int64_t(Lc.nnx) * Lc.nny));
+    const int wSize = 2 * span - 1;
+    const double inv = 1.0 / span;
+    const int fx0 = cx * span, fy0 = cy * span, fz0 = cz * span;
+    const int iz0 = (fz0 - span + 1 < 0) ? (span - 1 - fz0) : 0;
+    const int iz1 = (fz0 + span - 1 >= Lf.nnz) ? (wSize - (fz0 + span - Lf.nnz)) : wSize;
+    const int iy0 = (fy0 - span + 1 < 0) ? (span - 1 - fy0) : 0;
+    const int iy1 = (fy0 + span - 1 >= Lf.nny) ? (wSize - (fy0 + span - Lf.nny)) : wSize;
+    const int ix0 = (fx0 - span + 1 < 0) ? (span - 1 - fx0) : 0;
+    const int ix1 = (fx0 + span - 1 >= Lf.nnx) ? (wSize - (fx0 + span - Lf.nnx)) : wSize;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, wsum = 0.0;
+    for (int iz = iz0; iz < iz1; ++iz) {
+        const double wz = hat_w(iz, span, inv);
+        const int64_t zo = int64_t(fz0 + iz - (span - 1)) * Lf.nny * Lf.nnx;
+        for (int iy = iy0; iy < iy1; ++iy) {
+            const double wzy = wz * hat_w(iy, span, inv);
+            const int64_t yo = zo + int64_t(fy0 + iy - (span - 1)) * Lf.nnx;
+            for (int ix = ix0; ix < ix1; ++ix) {
+                const double w = wzy * hat_w(ix, span, inv);
+                const int64_t fi = 3 * (yo + (fx0 + ix - (span - 1)));
+                v0 = v0 + rf[fi + 0] * w;
+                v1 = v1 + rf[fi + 1] * w;
+                v2 = v2 + rf[fi + 2] * w;
+                wsum = wsum + w;
+            }
+        }
+    }
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (wsum > 1e-12) {
+        const double invW = 1.0 / wsum;
+        r0 = v0 * invW; r1 = v1 * invW; r2 = v2 * invW;
+    }
+    const unsigned m = cmask[n];
+    if (m & 1u) r0 = 0.0;
+    if (m & 2u) r1 = 0.0;
+    if (m & 4u) r2 = 0.0;
+    rc[3 * n + 0] = r0; rc[3 * n + 1] = r1; rc[3 * n + 2] = r2;
+    if (Dc) {
+        const double a0 = Dc[3 * n + 0], a1 = Dc[3 * n + 1], a2 = Dc[3 * n + 2];
+        xc[3 * n + 0] = (m & 1u) ? 0.0 : r0 / (1.0e-30 < a0 ? a0 : 1.0e-30);
+        xc[3 * n + 1] = (m & 2u) ? 0.0 : r1 / (1.0e-30 < a1 ? a1 : 1.0e-30);
+        xc[3 * n + 2] = (m & 4u) ? 0.0 : r2 / (1.0e-30 < a2 ? a2 : 1.0e-30);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// prolongation + additive correction + post-smoothing + mask
+// (transfers.cpp:378-445, preconditioner.cpp:230-253):
+//   t = (w*r)/max(1e-30,D);  x = (t + P(xc)) + t;  masked -> 0
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prolong_smooth(LevelDev Lf, LevelDev Lc, int span,
+                                                        const double* __restrict__ xc,
+                                                        const double* __restrict__ rf,
+                                                        const double* __restrict__ Df,
+                                                        const uint8_t* __restrict__ fmask, double weight,
+                                                        double* __restrict__ xf, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lf.nN) return;
+    const int fx = int(n % Lf.nnx), fy = int((n / Lf.nnx) % Lf.nny), fz = int(n / (int64_t(Lf.nnx) * Lf.nny));
+    int cx = fx / span, cy = fy / span, cz = fz / span;
+    int rx = fx % span, ry = fy % span, rz = fz % span;
+    if (cx >= Lc.resX) { cx = Lc.resX - 1; rx = span; }
+    if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
+    if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
+    const double inv = 1.0 / span;
+    const double wx1 = double(rx) * inv, wx0 = 1.0 - wx1;
+    const double wy1 = double(ry) * inv, wy0 = 1.0 - wy1;
+    const double wz1 = double(rz) * inv, wz0 = 1.0 - wz1;
+    const int64_t b00 = (int64_t(Lc.nnx) * Lc.nny * cz + int64_t(Lc.nnx) * cy + cx);
+    const int64_t b01 = b00 + 1, b10 = b00 + Lc.nnx, b11 = b10 + 1;
+    const int64_t s = int64_t(Lc.nnx) * Lc.nny;
+    const int64_t n00 = b00 + s, n01 = n00 + 1, n10 = n00 + Lc.nnx, n11 = n10 + 1;
+    const unsigned m = fmask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = xc[3 * b00 + c] * wx0 * wy0 * wz0 + xc[3 * b01 + c] * wx1 * wy0 * wz0 +
+                         xc[3 * b10 + c] * wx0 * wy1 * wz0 + xc[3 * b11 + c] * wx1 * wy1 * wz0 +
+                         xc[3 * n00 + c] * wx0 * wy0 * wz1 + xc[3 * n01 + c] * wx1 * wy0 * wz1 +
+                         xc[3 * n10 + c] * wx0 * wy1 * wz1 + xc[3 * n11 + c] * wx1 * wy1 * wz1;
+        double out = 0.0;
+        if (!((m >> c) & 1u)) {
+            const double d = Df[3 * n + c];
+            const double t = weight * rf[3 * n + c] / (1.0e-30 < d ? d : 1.0e-30);
+            out = (t + v) + t;
+        }
+        xf[3 * n + c] = out;
+    }
+}
+
+// single-level hierarchy: z = mask ? 0 : r / max(1e-30, D)
+__global__ void k_diag_solve(int64_t nN, const double* __restrict__ r, const double* __restrict__ D,
+                             const uint8_t* __restrict__ mask, double* __restrict__ x, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= nN) return;
+    const unsigned m = mask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double d = D[3 * n + c];
+        x[3 * n + c] = ((m >> c) & 1u) ? 0.0 : r[3 * n + c] / (1.0e-30 < d ? d : 1.0e-30);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// coarsest Cholesky solve L L^T x = b, one workgroup, one row per thread.
+// Forward: the row-form sum x_i = (b_i - sum_{k<i} L_ik x_k) / L_ii is evaluated
+// as a column sweep that subtracts the k-th term from every row i > k at step k,
+// so each row still subtracts its terms in increasing k (coarsest.cpp:101-108).
+// Backward: the reference's saxpy form (coarsest.cpp:113-121), row k receives
+// the updates of i = N-1 .. k+1 in that order.  Mask applied at the end.
+// ---------------------------------------------------------------------------
+constexpr int CHOL_MAX = 1024;
+__global__ void __launch_bounds__(CHOL_MAX) k_chol_solve(int N, const double* __restrict__ L,
+                                                         const double* __restrict__ LT,
+                                                         const double* __restrict__ b,
+                                                         const uint8_t* __restrict__ mask,
+                                                         double* __restrict__ x, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    __shared__ double xk;
+    const int i = threadIdx.x;
+    const bool act = i < N;
+    double s = act ? b[i] : 0.0;
+    // forward
+    for (int k0 = 0; k0 < N; k0 += 8) {
+        double l8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) l8[q] = (act && k0 + q < i) ? L[size_t(i) * N + k0 + q] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + q;
+            if (k >= N) break;
+            if (i == k) {
+                s = s / L[size_t(k) * N + k];
+                xk = s;
+            }
+            __syncthreads();
+            const double v = xk;
+            if (act && i > k) s = s - l8[q] * v;
+            __syncthreads();
+        }
+    }
+    // backward: s holds y_i
+    for (int k0 = N - 1; k0 >= 0; k0 -= 8) {
+        double l8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 - q;
+            l8[q] = (act && k >= 0 && k > i) ? LT[size_t(i) * N + k] : 0.0;   // L[k][i]
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 - q;
+            if (k < 0) break;
+            if (i == k) {
+                s = s / L[size_t(k) * N + k];
+                xk = s;
+            }
+            __syncthreads();
+            const double v = xk;
+            if (act && i < k) s = s - l8[q] * v;
+            __syncthreads();
+        }
+    }
+    if (act) {
+        const unsigned m = mask[i / 3];
+        x[i] = ((m >> (i % 3)) & 1u) ? 0.0 : s;
+    }
+}
+
+}  // namespace t3d
+
+using namespace t3d;
+
+struct t3d_mg {
+    t3d_problem* p = nullptr;
+    HostMG H;
+    int nlev = 0;
+    LevelDev ld[8];
+    uint8_t* mask[8] = {};
+    double* D[8] = {};
+    double* r[8] = {};
+    double* x[8] = {};
+    double* W[8] = {};
+    double* L = nullptr;
+    double* LT = nullptr;
+    int Nc = 0;
+    int cholOk = 1;
+    bool setupDone = false;
+    std::vector<double> Kc;   // last assembled coarsest matrix (host, for inspection)
+};
+
+namespace t3d {
+
+int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
+{
+    t3d_problem* p = g->p;
+    hipStream_t st = p->stream;
+    const int L = g->nlev - 1;
+    const int B = 256;
+    if (L == 0) {
+        hipLaunchKernelGGL(k_diag_solve, dim3(unsigned((g->ld[0].nN + B - 1) / B)), dim3(B), 0, st, g->ld[0].nN, r0,
+                           g->D[0], g->mask[0], z, done);
+        T3D_HIP(hipGetLastError());
+        return T3D_OK;
+    }
+    const bool chol = g->Nc > 0;
+    for (int l = 0; l < L; ++l) {
+        const double* rf = (l == 0) ? r0 : g->r[l];
+        const bool last = (l + 1 == L);
+        hipLaunchKernelGGL(k_restrict, dim3(unsigned((g->ld[l + 1].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
+                           g->ld[l + 1], g->H.lv[l + 1].span, rf, g->mask[l + 1], g->r[l + 1],
+                           (last && !chol) ? g->D[L] : nullptr, g->x[L], done);
+        T3D_HIP(hipGetLastError());
+    }
+    if (chol) {
+        hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(CHOL_MAX), 0, st, g->Nc, g->L, g->LT, g->r[L], g->mask[L],
+                           g->x[L], done);
+        T3D_HIP(hipGetLastError());
+    }
+    for (int l = L - 1; l >= 0; --l) {
+        const double* rf = (l == 0) ? r0 : g->r[l];
+        double* xo = (l == 0) ? z : g->x[l];
+        hipLaunchKernelGGL(k_prolong_smooth, dim3(unsigned((g->ld[l].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
+                           g->ld[l + 1], g->H.lv[l + 1].span, g->x[l + 1], rf, g->D[l], g->mask[l], g->H.weight, xo,
+                           done);
+        T3D_HIP(hipGetLastError());
+    }
+    return T3D_OK;
+}
+
+}  // namespace t3d
+
+extern "C" {
+
+int t3d_mg_destroy(t3d_mg* g)
+{
+    if (!g) return T3D_OK;
+    for (int l = 0; l < 8; ++l) {
+        if (g->mask[l]) (void)hipFree(g->mask[l]);
+        if (g->D[l]) (void)hipFree(g->D[l]);
+        if (g->r[l]) (void)hipFree(g->r[l]);
+        if (g->x[l]) (void)hipFree(g->x[l]);
+        if (g->W[l]) (void)hipFree(g->W[l]);
+    }
+    if (g->L) (void)hipFree(g->L);
+    if (g->LT) (void)hipFree(g->LT);
+    delete g;
+    return T3D_OK;
+}
+
+int t3d_mg_create(t3d_problem* p, int nonDyadic, int maxLevels, float weight, float Emin, int NlimitDofs,
+                  t3d_mg** out)
+{
+    T3D_CHECK(p && out && maxLevels >= 1 && maxLevels <= 8 && NlimitDofs <= CHOL_MAX, T3D_ERR_ARG,
+              "t3d_mg_create: bad argument (maxLevels 1..8, NlimitDofs <= 1024)");
+    *out = nullptr;
+    t3d_mg* g = new t3d_mg();
+    g->p = p;
+    mg_build_static(p->host, nonDyadic != 0, maxLevels, weight, Emin, NlimitDofs, g->H);
+    g->nlev = int(g->H.lv.size());
+    auto fail = [&](const char* m) {
+        set_error(m);
+        t3d_mg_destroy(g);
+        return T3D_ERR_HIP;
+    };
+    for (int l = 0; l < g->nlev; ++l) {
+        g->ld[l] = level_dev(g->H.lv[l]);
+        const int64_t nN = g->H.lv[l].nN;
+        std::vector<uint8_t> bits(nN);
+        for (int64_t n = 0; n < nN; ++n)
+            bits[n] = uint8_t(g->H.masks[l][3 * n] | (g->H.masks[l][3 * n + 1] << 1) | (g->H.masks[l][3 * n + 2] << 2));
+        if (hipMalloc(&g->mask[l], nN) != hipSuccess ||
+            hipMemcpy(g->mask[l], bits.data(), nN, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMalloc(&g->D[l], sizeof(double) * 3 * nN) != hipSuccess)
+            return fail("t3d_mg_create: device allocation failed");
+        if (l > 0) {
+            const auto& W = g->H.W[l];
+            if (hipMalloc(&g->r[l], sizeof(double) * 3 * nN) != hipSuccess ||
+                hipMalloc(&g->x[l], sizeof(double) * 3 * nN) != hipSuccess ||
+                hipMalloc(&g->W[l], sizeof(double) * W.size()) != hipSuccess ||
+                hipMemcpy(g->W[l], W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail("t3d_mg_create: device allocation failed");
+        }
+    }
+    *out = g;
+    return T3D_OK;
+}
+
+// level dims: out[4*l..] = resX resY resZ span; *nlev, *Nc (0 = diagonal coarsest)
+int t3d_mg_info(const t3d_mg* g, int* nlev, int* dims, int* Nc, int* cholOk)
+{
+    T3D_CHECK(g, T3D_ERR_ARG, "t3d_mg_info: null");
+    if (nlev) *nlev = g->nlev;
+    if (dims)
+        for (int l = 0; l < g->nlev; ++l) {
+            dims[4 * l] = g->H.lv[l].resX; dims[4 * l + 1] = g->H.lv[l].resY;
+            dims[4 * l + 2] = g->H.lv[l].resZ; dims[4 * l + 3] = g->H.lv[l].span;
+        }
+    if (Nc) *Nc = g->Nc;
+    if (cholOk) *cholOk = g->cholOk;
+    return T3D_OK;
+}
+
+int t3d_mg_mask(const t3d_mg* g, int level, uint8_t* out)
+{
+    T3D_CHECK(g && out && level >= 0 && level < g->nlev, T3D_ERR_ARG, "t3d_mg_mask: bad argument");
+    std::copy(g->H.masks[level].begin(), g->H.masks[level].end(), out);
+    return T3D_OK;
+}
+
+int t3d_mg_diag(const t3d_mg* g, int level, double* out)
+{
+    T3D_CHECK(g && out && level >= 0 && level < g->nlev, T3D_ERR_ARG, "t3d_mg_diag: bad argument");
+    T3D_CHECK(g->setupDone, T3D_ERR_STATE, "t3d_mg_diag: t3d_mg_setup not called");
+    T3D_HIP(hipMemcpy(out, g->D[level], sizeof(double) * 3 * g->H.lv[level].nN, hipMemcpyDeviceToHost));
+    return T3D_OK;
+}
+
+// coarsest matrix before factorisation (N x N) and factor (N x N, if SPD)
+int t3d_mg_coarsest(const t3d_mg* g, double* Kc, double* Lf)
+{
+    T3D_CHECK(g && g->setupDone, T3D_ERR_STATE, "t3d_mg_coarsest: not set up");
+    const int N = int(3 * g->H.lv.back().nN);
+    if (Kc && !g->Kc.empty()) std::copy(g->Kc.begin(), g->Kc.end(), Kc);
+    if (Lf && g->Nc > 0) T3D_HIP(hipMemcpy(Lf, g->L, sizeof(double) * size_t(N) * N, hipMemcpyDeviceToHost));
+    return T3D_OK;
+}
+
+int t3d_mg_setup(t3d_mg* g, const double* E_dev)
+{
+    T3D_CHECK(g && E_dev, T3D_ERR_ARG, "t3d_mg_setup: null argument");
+    t3d_problem* p = g->p;
+    int rc = t3d_set_modulus(p, E_dev);
+    if (rc) return rc;
+    hipStream_t st = p->stream;
+    const int B = 256;
+    hipLaunchKernelGGL(k_diag_finest, dim3(unsigned((p->g.nN + B - 1) / B)), dim3(B), 0, st, p->g, p->Ke, p->Ecell,
+                       g->mask[0], g->D[0]);
+    T3D_HIP(hipGetLastError());
+    for (int l = 1; l < g->nlev; ++l) {
+        hipLaunchKernelGGL(k_diag_coarsen, dim3(unsigned((g->ld[l].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l - 1],
+                           g->ld[l], g->H.lv[l].span, g->W[l], g->D[l - 1], g->mask[l - 1], g->mask[l], g->D[l]);
+        T3D_HIP(hipGetLastError());
+    }
+    // coarsest direct solve (preconditioner.cpp:48-79): assembled and factored on the host
+    g->Nc = 0;
+    g->cholOk = 1;
+    g->Kc.clear();
+    const int L = g->nlev - 1;
+    const int N = int(3 * g->H.lv[L].nN);
+    if (g->nlev > 1 && N <= g->H.NlimitDofs) {
+        std::vector<double> E(p->ne);
+        T3D_HIP(hipMemcpyAsync(E.data(), E_dev, sizeof(double) * p->ne, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipStreamSynchronize(st));
+        std::vector<double> A;
+        mg_assemble_coarsest(p->host, g->H, E.data(), A);
+        g->Kc = A;
+        if (chol_factor(A, N)) {
+            std::vector<double> AT(size_t(N) * N);
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) AT[size_t(i) * N + j] = A[size_t(j) * N + i];
+            if (!g->L) {
+                T3D_HIP(hipMalloc(&g->L, sizeof(double) * size_t(N) * N));
+                T3D_HIP(hipMalloc(&g->LT, sizeof(double) * size_t(N) * N));
+            }
+            T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
+            T3D_HIP(hipMemcpyAsync(g->LT, AT.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
+            g->Nc = N;
+        } else {
+            g->cholOk = 0;   // diagonal fallback, as the reference (preconditioner.cpp:76-77)
+        }
+    }
+    T3D_HIP(hipStreamSynchronize(st));
+    g->setupDone = true;
+    return T3D_OK;
+}
+
+int t3d_mg_apply(t3d_mg* g, const double* r_dev, double* z_dev)
+{
+    T3D_CHECK(g && r_dev && z_dev, T3D_ERR_ARG, "t3d_mg_apply: null argument");
+    T3D_CHECK(g->setupDone, T3D_ERR_STATE, "t3d_mg_apply: t3d_mg_setup not called");
+    int rc = mg_apply_dev(g, r_dev, z_dev, nullptr);
+    if (rc) return rc;
+    T3D_HIP(hipStreamSynchronize(g->p->stream));
+    return T3D_OK;
+}
+
+}  // extern "C"

@@ -40,8 +40,10 @@ __device__ __forceinline__ double block_tree_256(double s, double* sh)
 }
 
 __global__ void __launch_bounds__(256) k_canon_dot_tiles(const double* __restrict__ a, const double* __restrict__ b,
-                                                         int64_t n, double* __restrict__ out)
+                                                         int64_t n, double* __restrict__ out,
+                                                         const int* __restrict__ done)
 {
+    if (done && *done) return;
     __shared__ double sh[CANON_LANES];
     const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
     double s = 0.0;
@@ -55,8 +57,9 @@ __global__ void __launch_bounds__(256) k_canon_dot_tiles(const double* __restric
 }
 
 __global__ void __launch_bounds__(256) k_canon_sum_tiles(const double* __restrict__ v, int64_t n,
-                                                         double* __restrict__ out)
+                                                         double* __restrict__ out, const int* __restrict__ done)
 {
+    if (done && *done) return;
     __shared__ double sh[CANON_LANES];
     const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
     double s = 0.0;
@@ -67,6 +70,12 @@ __global__ void __launch_bounds__(256) k_canon_sum_tiles(const double* __restric
     }
     const double r = block_tree_256(s, sh);
     if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+__global__ void k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    *dst = *src;
 }
 
 int ensure_red(t3d_problem* p, int64_t items)
@@ -82,23 +91,25 @@ int ensure_red(t3d_problem* p, int64_t items)
 }
 
 // reduces p->red[0..ntiles) (first-level tile partials) to one value at dst
-int canon_finish(t3d_problem* p, int64_t ntiles, double* dst)
+int canon_finish(t3d_problem* p, int64_t ntiles, double* dst, const int* done)
 {
     double* src = p->red;
     double* alt = p->red + ntiles;
     int64_t m = ntiles;
     while (m > 1) {
         const int64_t m2 = (m + CANON_TILE - 1) / CANON_TILE;
-        hipLaunchKernelGGL(k_canon_sum_tiles, dim3(unsigned(m2)), dim3(CANON_LANES), 0, p->stream, src, m, alt);
+        hipLaunchKernelGGL(k_canon_sum_tiles, dim3(unsigned(m2)), dim3(CANON_LANES), 0, p->stream, src, m, alt,
+                           done);
         T3D_HIP(hipGetLastError());
         std::swap(src, alt);
         m = m2;
     }
-    T3D_HIP(hipMemcpyAsync(dst, src, sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+    hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, p->stream, src, dst, done);
+    T3D_HIP(hipGetLastError());
     return T3D_OK;
 }
 
-int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst)
+int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst, const int* done)
 {
     if (n <= 0) {
         T3D_HIP(hipMemsetAsync(dst, 0, sizeof(double), p->stream));
@@ -107,9 +118,10 @@ int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, d
     int rc = ensure_red(p, n);
     if (rc) return rc;
     const int64_t tiles = (n + CANON_TILE - 1) / CANON_TILE;
-    hipLaunchKernelGGL(k_canon_dot_tiles, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, p->stream, a, b, n, p->red);
+    hipLaunchKernelGGL(k_canon_dot_tiles, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, p->stream, a, b, n, p->red,
+                       done);
     T3D_HIP(hipGetLastError());
-    return canon_finish(p, tiles, dst);
+    return canon_finish(p, tiles, dst, done);
 }
 
 // ---------------------------------------------------------------------------
@@ -204,9 +216,11 @@ int t3d_problem_destroy(t3d_problem* p)
 {
     if (!p) return T3D_OK;
     void* bufs[] = {p->Ke, p->nodeFixed, p->compactToCell, p->cellToCompact, p->freeDofIndex,
-                    p->Ecell, p->EcellF, p->bFull, p->red, p->scal};
+                    p->Ecell, p->EcellF, p->bFull, p->red, p->scal, p->wr, p->wz, p->wp, p->wAp, p->pcgSc,
+                    p->pcgFlags};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (p->hostFlags) (void)hipHostFree(p->hostFlags);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return T3D_OK;

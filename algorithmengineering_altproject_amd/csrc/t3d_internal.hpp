@@ -45,6 +45,7 @@ struct Grid {
 }  // namespace t3d
 
 // device-side problem context
+struct t3d_mg;
 struct t3d_problem {
     t3d::HostModel host;
     t3d::Grid g;
@@ -64,11 +65,21 @@ struct t3d_problem {
     double* red = nullptr;              // partial sums
     int64_t redCap = 0;
     double* scal = nullptr;             // device scalars
+    // PCG workspace (full layout), allocated on first use
+    double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wAp = nullptr;
+    double* pcgSc = nullptr;            // per-iteration scalars
+    int pcgScCap = 0;
+    int* pcgFlags = nullptr;            // [0] done, [1] iterations
+    int* hostFlags = nullptr;           // pinned mirror
 };
 
 namespace t3d {
 int ensure_red(t3d_problem* p, int64_t items);
 // canonical sum of n items produced by kernel `first` into p->red; final value to dst (device)
-int canon_finish(t3d_problem* p, int64_t ntiles, double* dst);
-int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst);
+int canon_finish(t3d_problem* p, int64_t ntiles, double* dst, const int* done = nullptr);
+int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst,
+                  const int* done = nullptr);
+int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
+__global__ void k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ done);
+int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done);
 }  // namespace t3d

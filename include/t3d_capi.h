@@ -36,6 +36,7 @@ extern "C" {
 #define T3D_ERR_NOT_SPD (-4)    /* coarsest matrix not SPD (diagonal fallback used) */
 
 typedef struct t3d_problem t3d_problem;
+typedef struct t3d_mg t3d_mg;
 
 /* ---- runtime / memory helpers (no reference counterpart: device plumbing) ---- */
 const char* t3d_last_error(void);
@@ -83,6 +84,39 @@ int t3d_kxu_free_host(t3d_problem* p, const double* E_host, const double* uFree_
 /* parallel_dot (cpp/src/solver.cpp:77-84) with the canonical reduction tree:
  * result to *out_host. */
 int t3d_dot_f64(t3d_problem* p, const double* a_dev, const double* b_dev, int64_t n, double* out_host);
+
+/* ---- multigrid preconditioner ----
+ * build_static_once (cpp/include/multigrid/multigrid.hpp:65, cpp/src/multigrid/preconditioner.cpp:19-34):
+ * level hierarchy (hierarchy.cpp:84-138) and fixed-DOF masks (masks.cpp:11-37).
+ * weight/Emin are the reference's float parameters (MGPrecondConfig.weight,
+ * GlobalParams.youngsModulusMin); NlimitDofs <= 1024. */
+int t3d_mg_create(t3d_problem* p, int nonDyadic, int maxLevels, float weight, float Emin, int NlimitDofs,
+                  t3d_mg** out);
+int t3d_mg_destroy(t3d_mg* g);
+/* make_diagonal_preconditioner_from_static (cpp/include/multigrid/multigrid.hpp:66-68,
+ * cpp/src/multigrid/preconditioner.cpp:38-79): per-level Jacobi diagonals on the
+ * device (diagonal.cpp:11-160) and, when the coarsest level has <= NlimitDofs
+ * DOFs, the Galerkin coarsest matrix (coarsest.cpp:138-323) assembled on the
+ * host in canonical element order and Cholesky-factored (coarsest.cpp:13-90).
+ * E_dev: element moduli, compact order; also becomes the problem's modulus. */
+int t3d_mg_setup(t3d_mg* g, const double* E_dev);
+/* The V-cycle closure M(r, z) (cpp/src/multigrid/preconditioner.cpp:88-265), full layout. */
+int t3d_mg_apply(t3d_mg* g, const double* r_full_dev, double* z_full_dev);
+/* inspection: dims[4*l..] = resX resY resZ span; Nc = coarsest Cholesky size (0 = diagonal) */
+int t3d_mg_info(const t3d_mg* g, int* nlev, int* dims, int* Nc, int* cholOk);
+int t3d_mg_mask(const t3d_mg* g, int level, uint8_t* out_host);
+int t3d_mg_diag(const t3d_mg* g, int level, double* out_host);
+int t3d_mg_coarsest(const t3d_mg* g, double* Kc_host, double* L_host);
+
+/* ---- PCG ----
+ * PCG_free (cpp/include/solver.hpp:22-25, cpp/src/solver.cpp:87-217) on the full
+ * layout with the problem's current modulus (t3d_set_modulus / t3d_mg_setup);
+ * M = NULL runs unpreconditioned CG.  x is the initial guess and the result;
+ * *iters = iterations (maxIt when not converged), as the reference returns. */
+int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b_full_dev, double* x_full_dev, double tol, int maxIt,
+            int* iters);
+/* ||r_{k+1}||^2 of the last solve for k < n (n <= maxIt of that solve) */
+int t3d_pcg_history(t3d_problem* p, int maxIt, int n, double* rn2_host);
 
 /* ---- timing (no reference counterpart): average device time of `reps`
  * back-to-back launches of an operator, measured with HIP events on the
