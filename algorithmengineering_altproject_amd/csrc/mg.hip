@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_diag_coarsen(LevelDev Lf, LevelDev Lc, 
     double acc[3] = {0.0, 0.0, 0.0}, ws[3] = {0.0, 0.0, 0.0};
     for (int col = 0; col < 8; ++col) {
         const int cex = X - ((X ^ col) & 1);
-        const int cez = Z - ((Z ^ col) & 1);
+        const int cez = Z - ((Z ^ (col >> 2)) & 1);
         // element rows cey = resY-1-cty with cty in {Y-1, Y}
         int cey = Lc.resY - Y;                    // cty = Y-1
         if ((cey & 1) != ((col >> 1) & 1)) cey = Lc.resY - 1 - Y;   // cty = Y
