@@ -71,6 +71,18 @@ def lib():
         "t3d_mg_coarsest": (C.c_int, [vp, vp, vp]),
         "t3d_pcg": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int)]),
         "t3d_pcg_history": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+        "t3d_compliance": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
+        "t3d_filter_create": (C.c_int, [vp, C.c_float, C.POINTER(vp)]),
+        "t3d_filter_destroy": (C.c_int, [vp]),
+        "t3d_filter_info": (C.c_int, [vp, vp, vp]),
+        "t3d_filter_apply": (C.c_int, [vp, vp, vp, C.POINTER(C.c_int)]),
+        "t3d_simp": (C.c_int, [vp, vp, C.c_float, C.c_float, vp]),
+        "t3d_oc_update": (C.c_int, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
+        "t3d_seq_sum_f32": (C.c_int, [vp, vp, C.c_int64, C.POINTER(C.c_float)]),
+        "t3d_params_default": (None, [vp]),
+        "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
+        "t3d_top3d_run": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -251,6 +263,109 @@ class MG:
         L = np.zeros(N * N)
         check(lib().t3d_mg_coarsest(self.h, _ptr(Kc), _ptr(L)), "t3d_mg_coarsest")
         return Kc.reshape(N, N), L.reshape(N, N)
+
+
+class Filter:
+    """SetupPDEFilter / ApplyPDEFilter (cpp/src/filter.cpp:9-275) on the device."""
+
+    def __init__(self, prob, rMin):
+        self.prob = prob
+        h = C.c_void_p()
+        check(lib().t3d_filter_create(prob.h, rMin, C.byref(h)), "t3d_filter_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_filter_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self):
+        k = np.zeros(64)
+        d = np.zeros(self.prob.nN)
+        check(lib().t3d_filter_info(self.h, _ptr(k), _ptr(d)), "t3d_filter_info")
+        return k, d
+
+    def apply(self, dsrc, ddst):
+        it = C.c_int()
+        check(lib().t3d_filter_apply(self.h, dsrc.ptr, ddst.ptr, C.byref(it)), "t3d_filter_apply")
+        return it.value
+
+
+def compliance(prob, dE, dU, dce):
+    c = C.c_double()
+    check(lib().t3d_compliance(prob.h, dE.ptr, dU.ptr, dce.ptr, C.byref(c)), "t3d_compliance")
+    return c.value
+
+
+def simp(prob, dx, E0, Emin, dE):
+    check(lib().t3d_simp(prob.h, dx.ptr, E0, Emin, dE.ptr), "t3d_simp")
+
+
+def oc_update(prob, dx, ddc, V0, move, dxnew):
+    out = np.zeros(4, np.float32)
+    check(lib().t3d_oc_update(prob.h, dx.ptr, ddc.ptr, V0, move, dxnew.ptr, _ptr(out)), "t3d_oc_update")
+    return dict(change=np.float32(out[0]), sharpness=np.float32(out[1]), volFrac=np.float32(out[2]),
+                steps=int(out[3]))
+
+
+def fp_probe(op, a, b=None):
+    """Device sqrtf / float division / double sqrt of host arrays (diagnostics)."""
+    da = DeviceArray.from_host(a)
+    db = DeviceArray.from_host(b) if b is not None else None
+    out = DeviceArray(a.size, a.dtype)
+    check(lib().t3d_fp_probe(op, a.size, da.ptr, db.ptr if db is not None else None, out.ptr), "t3d_fp_probe")
+    return out.download()
+
+
+def seq_sum(prob, dv, n=None):
+    out = C.c_float()
+    check(lib().t3d_seq_sum_f32(prob.h, dv.ptr, dv.n if n is None else n, C.byref(out)), "t3d_seq_sum_f32")
+    return np.float32(out.value)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("E0", C.c_float), ("Emin", C.c_float), ("nu", C.c_float), ("penal", C.c_float), ("cgTol", C.c_float),
+        ("cgMaxIt", C.c_int), ("V0", C.c_float), ("nLoop", C.c_int), ("rMin", C.c_float),
+        ("mgMaxLevels", C.c_int), ("mgWeight", C.c_float), ("nonDyadic", C.c_int), ("NlimitDofs", C.c_int),
+        ("rebuildEvery", C.c_int), ("precision", C.c_int),
+    ]
+
+
+class LoopRecord(C.Structure):
+    _fields_ = [
+        ("loop", C.c_int), ("pcgIters", C.c_int), ("filterIters", C.c_int), ("bisectSteps", C.c_int),
+        ("C", C.c_double), ("Cdisp", C.c_double),
+        ("volFrac", C.c_float), ("sharpness", C.c_float), ("change", C.c_float), ("fval", C.c_float),
+        ("tSolve", C.c_double), ("tOpti", C.c_double), ("tFilter", C.c_double), ("tTotal", C.c_double),
+    ]
+
+
+def default_params(**kw):
+    p = Params()
+    lib().t3d_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def top3d_run(nely, nelx, nelz, **kw):
+    """TOP3D_XL_GLOBAL (cpp/src/driver.cpp:23-218) for one simulation, on the device.
+    Returns (records, x, Csolid, solidIters)."""
+    P = default_params(**kw)
+    rec = (LoopRecord * max(1, P.nLoop))()
+    # the design has numElements = nelx*nely*nelz entries (padding is void)
+    x = np.zeros(nelx * nely * nelz, np.float32)
+    Cs = C.c_double()
+    si = C.c_int()
+    nl = C.c_int()
+    check(lib().t3d_top3d_run(nely, nelx, nelz, C.byref(P), rec, P.nLoop, _ptr(x), C.byref(Cs), C.byref(si),
+                              C.byref(nl)), "t3d_top3d_run")
+    recs = [{f: getattr(r, f) for f, _ in LoopRecord._fields_} for r in rec[:nl.value]]
+    return recs, x, Cs.value, si.value
 
 
 def pcg(prob, M, db, dx, tol, maxIt):

@@ -1,0 +1,175 @@
+// driver.cpp -- the TOP3D_XL optimisation loop on the device.
+//
+// Reference: TOP3D_XL_GLOBAL (cpp/src/driver.cpp:23-218) for one simulation.
+// Everything between the host's per-loop bookkeeping stays on the device:
+// SIMP moduli, the lazily rebuilt multigrid preconditioner (loop 1 and every
+// `rebuildEvery`-th loop), the warm-started PCG solve, compliance, sensitivities,
+// the PDE filter and the OC update.  Per loop the host reads back only the
+// handful of scalars the loop condition and the log line need.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "t3d_internal.hpp"
+
+using namespace t3d;
+
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+extern "C" {
+
+void t3d_params_default(t3d_params* P)
+{
+    // GlobalParams (core.hpp:26-42), MGPrecondConfig as set by the driver
+    // (driver.cpp:54), rMin = sqrt(3) from the CLI (top3d_cli.cpp:33)
+    P->E0 = 1.0f; P->Emin = 1.0e-6f; P->nu = 0.3f; P->penal = 3.0f; P->cgTol = 1.0e-3f;
+    P->cgMaxIt = 800; P->V0 = 0.12f; P->nLoop = 30; P->rMin = std::sqrt(3.0f);
+    P->mgMaxLevels = 5; P->mgWeight = 0.6f; P->nonDyadic = 1; P->NlimitDofs = 1000;
+    P->rebuildEvery = 5; P->precision = 0;
+}
+
+// from filter.hip
+int t3d_filter_apply_async(t3d_filter* f, const float* src, float* dst);
+int t3d_filter_last_iters(const t3d_filter* f);
+
+int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_record* rec, int recCap,
+                  float* x_out, double* CsolidOut, int* solidIters, int* loopsRun)
+{
+    T3D_CHECK(P && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_top3d_run: bad argument");
+    T3D_CHECK(P->penal == 3.0f, T3D_ERR_ARG, "t3d_top3d_run: only penal = 3 is supported (powf(rho,3) rounding)");
+    T3D_CHECK(P->precision == 0, T3D_ERR_ARG, "t3d_top3d_run: precision must be 0 (FP64 plain path)");
+    if (loopsRun) *loopsRun = 0;
+    t3d_problem* p = nullptr;
+    t3d_mg* M = nullptr;
+    t3d_filter* F = nullptr;
+    int rc = t3d_problem_create_cuboid(nely, nelx, nelz, &p);
+    if (rc) return rc;
+    struct Guard {
+        t3d_problem*& p;
+        t3d_mg*& M;
+        t3d_filter*& F;
+        ~Guard()
+        {
+            t3d_filter_destroy(F);
+            t3d_mg_destroy(M);
+            t3d_problem_destroy(p);
+        }
+    } guard{p, M, F};
+    if ((rc = t3d_filter_create(p, P->rMin, &F))) return rc;
+    if ((rc = t3d_mg_create(p, P->nonDyadic, P->mgMaxLevels, P->mgWeight, P->Emin, P->NlimitDofs, &M))) return rc;
+    const int ne = p->ne;
+    const int64_t nD = 3 * p->g.nN;
+    hipStream_t st = p->stream;
+    DevBuf bx, bxn, bE, bce, bxdc, bdcf, bdc, bu, bb, bsc;
+    T3D_HIP(hipMalloc(&bx.p, sizeof(float) * ne));
+    T3D_HIP(hipMalloc(&bxn.p, sizeof(float) * ne));
+    T3D_HIP(hipMalloc(&bE.p, sizeof(double) * ne));
+    T3D_HIP(hipMalloc(&bce.p, sizeof(double) * ne));
+    T3D_HIP(hipMalloc(&bxdc.p, sizeof(float) * ne));
+    T3D_HIP(hipMalloc(&bdcf.p, sizeof(float) * ne));
+    T3D_HIP(hipMalloc(&bdc.p, sizeof(float) * ne));
+    T3D_HIP(hipMalloc(&bu.p, sizeof(double) * nD));
+    T3D_HIP(hipMalloc(&bb.p, sizeof(double) * nD));
+    T3D_HIP(hipMalloc(&bsc.p, sizeof(double) * 16));
+    float* x = (float*)bx.p;
+    float* xn = (float*)bxn.p;
+    double* E = (double*)bE.p;
+    double* ce = (double*)bce.p;
+    float* xdc = (float*)bxdc.p;
+    float* dcf = (float*)bdcf.p;
+    float* dc = (float*)bdc.p;
+    double* u = (double*)bu.p;
+    double* b = (double*)bb.p;
+    double* sc = (double*)bsc.p;   // [0] Csolid [1] C [2] cObj ; float slots from [8]
+    float* fsc = reinterpret_cast<float*>(sc + 8);
+    const double tol = double(P->cgTol);
+
+    T3D_HIP(hipMemcpyAsync(b, p->bFull, sizeof(double) * nD, hipMemcpyDeviceToDevice, st));
+    if ((rc = launch_fill_f32(p, ne, P->V0, x))) return rc;
+    // solid reference solve (driver.cpp:85-103)
+    if ((rc = launch_fill_f64(p, ne, double(P->E0), E))) return rc;
+    if ((rc = launch_fill_f64(p, nD, 0.0, u))) return rc;
+    if ((rc = t3d_mg_setup(M, E))) return rc;
+    int it0 = 0;
+    if ((rc = t3d_pcg(p, M, b, u, tol, P->cgMaxIt, &it0))) return rc;
+    if ((rc = compliance_dev(p, E, u, ce, sc + 0))) return rc;
+    double Csolid = 0.0;
+    T3D_HIP(hipMemcpyAsync(&Csolid, sc + 0, sizeof(double), hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipStreamSynchronize(st));
+    if (CsolidOut) *CsolidOut = Csolid;
+    if (solidIters) *solidIters = it0;
+
+    int loop = 0;
+    float change = 1.0f, sharp = 1.0f;
+    while (loop < P->nLoop && double(change) > 1e-4 && double(sharp) > 0.01) {
+        const double tIt = now_s();
+        ++loop;
+        if ((rc = launch_simp(p, x, P->E0, P->Emin, E))) return rc;
+        const double tS = now_s();
+        if (loop == 1 || (loop % P->rebuildEvery) == 0) {
+            if ((rc = t3d_mg_setup(M, E))) return rc;
+        } else {
+            if ((rc = t3d_set_modulus(p, E))) return rc;
+        }
+        int its = 0;
+        if ((rc = t3d_pcg(p, M, b, u, tol, P->cgMaxIt, &its))) return rc;
+        const double tSolve = now_s() - tS;
+        const double tO = now_s();
+        if ((rc = compliance_dev(p, E, u, ce, sc + 1))) return rc;
+        if ((rc = cobj_dev(p, E, ce, sc + 0, sc + 2))) return rc;
+        if ((rc = launch_sens(p, x, ce, sc + 0, P->E0, P->Emin, P->penal, xdc))) return rc;
+        T3D_HIP(hipStreamSynchronize(st));
+        const double tF = now_s();
+        if ((rc = t3d_filter_apply_async(F, xdc, dcf))) return rc;
+        T3D_HIP(hipStreamSynchronize(st));
+        const double tFilter = now_s() - tF;
+        if ((rc = launch_dc_post(p, dcf, x, dc))) return rc;
+        if ((rc = launch_oc(p, x, dc, P->V0, 0.2f, xn, fsc))) return rc;
+        double hs[3];
+        float hf[5];
+        T3D_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipMemcpyAsync(hf, fsc, sizeof(float) * 5, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipStreamSynchronize(st));
+        std::swap(x, xn);
+        std::memcpy(&change, &hf[2], sizeof(float));
+        sharp = hf[3];
+        const float volFrac = hf[4];
+        const double tOpti = now_s() - tO;
+        if (rec && loop - 1 < recCap) {
+            t3d_loop_record& R = rec[loop - 1];
+            R.loop = loop;
+            R.pcgIters = its;
+            R.filterIters = t3d_filter_last_iters(F);
+            R.bisectSteps = int(hf[1]);
+            R.C = hs[1];
+            R.Cdisp = (Csolid > 0) ? hs[2] * Csolid : hs[1];
+            R.volFrac = volFrac;
+            R.sharpness = sharp;
+            R.change = change;
+            R.fval = volFrac - P->V0;
+            R.tSolve = tSolve;
+            R.tOpti = tOpti;
+            R.tFilter = tFilter;
+            R.tTotal = now_s() - tIt;
+        }
+    }
+    if (x_out) T3D_HIP(hipMemcpy(x_out, x, sizeof(float) * ne, hipMemcpyDeviceToHost));
+    if (loopsRun) *loopsRun = loop;
+    return T3D_OK;
+}
+
+}  // extern "C"

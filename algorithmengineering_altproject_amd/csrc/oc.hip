@@ -1,0 +1,541 @@
+// oc.hip -- element-wise SIMP / sensitivity / OC kernels, the compliance
+// kernel and the exact sequential float sums of the OC update.
+//
+// Reference: driver.cpp:117-189 (SIMP, dc, filter pre/post scaling, OC
+// bisection, change, sharpness, volume fraction) and ComputeCompliance
+// (fea.cpp:587-677, 851-864).  Float semantics follow the reference exactly:
+// float arithmetic without contraction, correctly rounded sqrt and division
+// (plain sqrtf and '/': clang's HIP default is correctly rounded for both --
+// the __fsqrt_rn intrinsic is the native approximation unless OCML basic
+// rounded operations are enabled, so it is not used),
+// std::clamp / std::max selection rules, and std::accumulate<float> sums
+// reproduced bit-for-bit by the transducer scan of seqsum.hpp.  powf(rho,3)
+// and powf(rho,2) are evaluated correctly rounded.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "seqsum.hpp"
+#include "t3d_internal.hpp"
+
+namespace t3d {
+
+__device__ __forceinline__ float fmax_r(float a, float b) { return (a < b) ? b : a; }          // std::max
+__device__ __forceinline__ float fclamp_r(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
+
+// correctly rounded x^3 and x^2 for binary32 x (exact double-double product, one rounding)
+__device__ __forceinline__ float next_float(float f, bool up)
+{
+    unsigned b = __float_as_uint(f);
+    b = up ? b + 1u : (b == 0u ? 0x80000001u : b - 1u);
+    return __uint_as_float(b);
+}
+__device__ __forceinline__ float cr_cube(float x)
+{
+    const double d = double(x);
+    const double p2 = d * d;
+    const double hi = p2 * d;
+    const double lo = __fma_rn(p2, d, -hi);
+    const float f = float(hi);
+    const double df = double(f);
+    if (lo == 0.0 || df == hi) return f;
+    const float g = next_float(f, hi > df);
+    const double mid = 0.5 * (df + double(g));
+    if (hi != mid) return f;
+    return ((lo > 0.0) == (double(g) > df)) ? g : f;
+}
+__device__ __forceinline__ float cr_square(float x)
+{
+    const double d = double(x);
+    return float(d * d);
+}
+
+// ---------------------------------------------------------------------------
+// SIMP (driver.cpp:117-121): E = (double)(Emin + rho^3 * (E0 - Emin)), float ops
+// ---------------------------------------------------------------------------
+__global__ void k_simp(int ne, const float* __restrict__ x, float E0, float Emin, double* __restrict__ E)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const float rho = fclamp_r(x[e], 0.0f, 1.0f);
+    const float pw = cr_cube(rho);
+    const float v = __fadd_rn(Emin, __fmul_rn(pw, __fsub_rn(E0, Emin)));
+    E[e] = double(v);
+}
+
+__global__ void k_fill_f64(int64_t n, double v, double* __restrict__ a)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_f32(int64_t n, float v, float* __restrict__ a)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// sensitivities and the filter input (driver.cpp:151-163):
+//   dc = -(penal * rho^2 * (E0-Emin)) * (float)(ce/Csolid) ; xdc = x*dc
+// ---------------------------------------------------------------------------
+__global__ void k_sens_xdc(int ne, const float* __restrict__ x, const double* __restrict__ ce,
+                           const double* __restrict__ Csolid, float E0, float Emin, float penal,
+                           float* __restrict__ xdc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const double Cs = *Csolid;
+    const float rho = fclamp_r(x[e], 0.0f, 1.0f);
+    const float dE = __fmul_rn(__fmul_rn(penal, cr_square(rho)), __fsub_rn(E0, Emin));
+    const float ceN = (Cs > 0) ? float(ce[e] / Cs) : float(ce[e]);
+    const float dc = __fmul_rn(-dE, ceN);
+    xdc[e] = __fmul_rn(x[e], dc);
+}
+
+// dc = filt / max(1e-3, x) (driver.cpp:165)
+__global__ void k_dc_post(int ne, const float* __restrict__ filt, const float* __restrict__ x, float* __restrict__ dc)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    dc[e] = ((filt[e]) / (fmax_r(1.0e-3f, x[e])));
+}
+
+// OC candidate density for multiplier lmid (driver.cpp:175-180)
+__device__ __forceinline__ float oc_xnew(float x, float dc, float lmid, float move)
+{
+    const float val = sqrtf(fmax_r(1e-30f, ((-dc) / (lmid))));
+    float xe = fclamp_r(__fmul_rn(x, val), __fsub_rn(x, move), __fadd_rn(x, move));
+    xe = fclamp_r(xe, 0.0f, 1.0f);
+    return fmax_r(1.0e-3f, xe);
+}
+
+// ---------------------------------------------------------------------------
+// block-wide exact sequential float sum (1024 threads, see seqsum.hpp)
+// ---------------------------------------------------------------------------
+constexpr int SS_THREADS = 1024;
+constexpr int SS_PER = 8;
+constexpr int SS_CHUNK = SS_THREADS * SS_PER;
+constexpr int SS_PREFIX = 1024;
+
+struct SeqShared {
+    seqsum::Tr waveTot[SS_THREADS / 64];
+    unsigned long long cross;
+    float s;
+    long long i0;
+};
+
+__device__ __forceinline__ seqsum::Tr shfl_up_tr(const seqsum::Tr& t, int d)
+{
+    seqsum::Tr o;
+    o.r0 = __shfl_up((long long)t.r0, d, 64);
+    o.r1 = __shfl_up((long long)t.r1, d, 64);
+    return o;
+}
+
+template <class Gen>
+__device__ float block_seq_sum(int64_t n, const Gen& gen, SeqShared& sh)
+{
+    using namespace seqsum;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        float s = 0.0f;
+        int64_t i = 0;
+        for (; i < n && (i < SS_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, gen(i));
+        sh.s = s;
+        sh.i0 = i;
+    }
+    __syncthreads();
+    while (sh.i0 < n) {
+        const int64_t i0 = sh.i0;
+        const float s = sh.s;
+        int64_t m;
+        int ue;
+        decompose(s, m, ue);
+        float v[SS_PER];
+        Tr F = tr_identity();
+#pragma unroll
+        for (int k = 0; k < SS_PER; ++k) {
+            const int64_t idx = i0 + int64_t(tid) * SS_PER + k;
+            v[k] = (idx < n) ? gen(idx) : 0.0f;
+            F = tr_compose(F, tr_elem(v[k], ue));
+        }
+        // inclusive wave scan
+        Tr inc = F;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Tr o = shfl_up_tr(inc, d);
+            if (lane >= d) inc = tr_compose(o, inc);
+        }
+        if (lane == 63) sh.waveTot[wave] = inc;
+        if (tid == 0) sh.cross = ~0ull;
+        __syncthreads();
+        if (tid == 0) {
+            Tr acc = tr_identity();
+            for (int w = 0; w < SS_THREADS / 64; ++w) {
+                const Tr t = sh.waveTot[w];
+                sh.waveTot[w] = acc;   // exclusive wave prefix
+                acc = tr_compose(acc, t);
+            }
+            // chunk total kept in the cross-free state update below
+            const int64_t mt = m + tr_apply(acc, m);
+            sh.s = (mt < M_TOP) ? ldexpf(float(mt), ue) : s;   // provisional; replaced on a crossing
+        }
+        __syncthreads();
+        Tr excl = shfl_up_tr(inc, 1);
+        if (lane == 0) excl = tr_identity();
+        const Tr P = tr_compose(sh.waveTot[wave], excl);
+        int64_t mm = m + tr_apply(P, m);
+        unsigned long long myCross = ~0ull;
+        int64_t mmc = 0;
+        float vc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < SS_PER; ++k) {
+            const int64_t idx = i0 + int64_t(tid) * SS_PER + k;
+            if (idx < n && myCross == ~0ull) {
+                const int64_t r = tr_apply(tr_elem(v[k], ue), mm);
+                if (mm + r >= M_TOP) {
+                    myCross = (unsigned long long)idx;
+                    mmc = mm;
+                    vc = v[k];
+                } else {
+                    mm += r;
+                }
+            }
+        }
+        if (myCross != ~0ull) atomicMin(&sh.cross, myCross);
+        __syncthreads();
+        const unsigned long long cr = sh.cross;
+        if (cr == ~0ull) {
+            if (tid == 0) sh.i0 = i0 + SS_CHUNK;
+        } else if (myCross == cr) {
+            sh.s = __fadd_rn(ldexpf(float(mmc), ue), vc);
+            sh.i0 = int64_t(cr) + 1;
+        }
+        __syncthreads();
+    }
+    const float res = sh.s;
+    __syncthreads();
+    return res;
+}
+
+struct GenXnew {
+    const float* x;
+    const float* dc;
+    float lmid, move;
+    __device__ float operator()(int64_t i) const { return oc_xnew(x[i], dc[i], lmid, move); }
+};
+struct GenX {
+    const float* x;
+    __device__ float operator()(int64_t i) const { return x[i]; }
+};
+struct GenX1mX {
+    const float* x;
+    __device__ float operator()(int64_t i) const { return __fmul_rn(x[i], __fsub_rn(1.0f, x[i])); }
+};
+
+// OC bisection (driver.cpp:170-183) in one persistent workgroup: out[0] = last
+// lmid, out[1] = bisection steps
+__global__ void __launch_bounds__(SS_THREADS) k_oc_bisect(int ne, const float* __restrict__ x,
+                                                          const float* __restrict__ dc, float V0, float move,
+                                                          float* __restrict__ out)
+{
+    __shared__ SeqShared sh;
+    float l1 = 0.0f, l2 = 1e9f, lmid = 0.0f;
+    int steps = 0;
+    while (double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6) {
+        lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
+        GenXnew gx{x, dc, lmid, move};
+        const float vol = ((block_seq_sum(ne, gx, sh)) / (float(ne)));
+        if (__fsub_rn(vol, V0) > 0) l1 = lmid;
+        else l2 = lmid;
+        ++steps;
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lmid;
+        out[1] = float(steps);
+    }
+}
+
+// xnew for the final multiplier and change = max|xnew - x| (driver.cpp:184)
+__global__ void k_oc_apply(int ne, const float* __restrict__ x, const float* __restrict__ dc,
+                           const float* __restrict__ lm, float move, float* __restrict__ xnew,
+                           unsigned* __restrict__ changeBits)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    float d = 0.0f;
+    if (e < ne) {
+        const float v = oc_xnew(x[e], dc[e], lm[0], move);
+        xnew[e] = v;
+        d = fabsf(__fsub_rn(v, x[e]));
+    }
+    // max of non-negative floats = max of their bit patterns
+    unsigned b = __float_as_uint(d);
+    for (int w = 32; w >= 1; w >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, w, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(changeBits, b);
+}
+
+// sharpness = 4*sum(x(1-x))/ne and volume fraction sum(x)/ne (driver.cpp:188-201)
+__global__ void __launch_bounds__(SS_THREADS) k_oc_stats(int ne, const float* __restrict__ x,
+                                                         float* __restrict__ out)
+{
+    __shared__ SeqShared sh;
+    GenX1mX g1{x};
+    const float s1 = block_seq_sum(ne, g1, sh);
+    GenX g2{x};
+    const float s2 = block_seq_sum(ne, g2, sh);
+    if (threadIdx.x == 0) {
+        out[0] = ((__fmul_rn(4.0f, s1)) / (float(ne)));
+        out[1] = ((s2) / (float(ne)));
+    }
+}
+
+// generic sequential float sum of an array (for tests)
+__global__ void __launch_bounds__(SS_THREADS) k_seq_sum(int64_t n, const float* __restrict__ v, float* __restrict__ out)
+{
+    __shared__ SeqShared sh;
+    GenX g{v};
+    const float s = block_seq_sum(n, g, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ---------------------------------------------------------------------------
+// compliance (fea.cpp:587-677): ce[e] = sum_i ue_i * (sum_j K_ij ue_j), 0 when
+// E <= 1e-16; canonical tile sum of E*ce in compact element order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_compliance(Grid g, int ne, const int32_t* __restrict__ c2cell,
+                                                    const double* __restrict__ Ke, const double* __restrict__ E,
+                                                    const double* __restrict__ U, double* __restrict__ ce,
+                                                    double* __restrict__ partial)
+{
+    __shared__ double sh[CANON_LANES];
+    __shared__ double sK[576];
+    for (int i = threadIdx.x; i < 576; i += blockDim.x) sK[i] = Ke[i];
+    __syncthreads();
+    const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
+    double s = 0.0;
+    for (int k = 0; k < CANON_PER_LANE; ++k) {
+        const int64_t e = base + int64_t(k) * CANON_LANES;
+        if (e >= ne) continue;
+        const double Ee = E[e];
+        double item = 0.0;
+        if (Ee <= 1.0e-16) {
+            ce[e] = 0.0;
+        } else {
+            const int cell = c2cell[e];
+            const int cx = cell % g.resX, ty = (cell / g.resX) % g.resY, cz = cell / (g.resX * g.resY);
+            double ue[24];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const int b = a & 3;
+                const int ox = (b == 1 || b == 2) ? 1 : 0, oy = (b <= 1) ? 1 : 0, oz = a >> 2;
+                const int64_t nn = (int64_t(cz + oz) * g.nny + (ty + oy)) * g.nnx + (cx + ox);
+                ue[3 * a] = U[3 * nn];
+                ue[3 * a + 1] = U[3 * nn + 1];
+                ue[3 * a + 2] = U[3 * nn + 2];
+            }
+            double c = 0.0;
+            for (int i = 0; i < 24; ++i) {
+                double rd = 0.0;
+#pragma unroll
+                for (int j = 0; j < 24; ++j) rd = rd + sK[i * 24 + j] * ue[j];
+                c = c + ue[i] * rd;
+            }
+            ce[e] = c;
+            item = Ee * c;
+        }
+        s = s + item;
+    }
+    const int l = threadIdx.x;
+    sh[l] = s;
+    __syncthreads();
+    for (int w = CANON_LANES / 2; w >= 64; w >>= 1) {
+        if (l < w) sh[l] = sh[l] + sh[l + w];
+        __syncthreads();
+    }
+    if (l < 64) {
+        double v = sh[l];
+#pragma unroll
+        for (int w = 32; w >= 1; w >>= 1) v = v + __shfl_down(v, w, 64);
+        if (l == 0) partial[blockIdx.x] = v;
+    }
+}
+
+// canonical tile partials of E[e] * (ce[e] / Csolid) (driver.cpp:144-150)
+__global__ void __launch_bounds__(256) k_cobj_tiles(int ne, const double* __restrict__ E,
+                                                    const double* __restrict__ ce, const double* __restrict__ Csolid,
+                                                    double* __restrict__ partial)
+{
+    __shared__ double sh[CANON_LANES];
+    const double Cs = *Csolid;
+    const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < CANON_PER_LANE; ++k) {
+        const int64_t e = base + int64_t(k) * CANON_LANES;
+        if (e < ne) s = s + ((Cs > 0) ? E[e] * (ce[e] / Cs) : E[e] * ce[e]);
+    }
+    const int l = threadIdx.x;
+    sh[l] = s;
+    __syncthreads();
+    for (int w = CANON_LANES / 2; w >= 64; w >>= 1) {
+        if (l < w) sh[l] = sh[l] + sh[l + w];
+        __syncthreads();
+    }
+    if (l < 64) {
+        double v = sh[l];
+#pragma unroll
+        for (int w = 32; w >= 1; w >>= 1) v = v + __shfl_down(v, w, 64);
+        if (l == 0) partial[blockIdx.x] = v;
+    }
+}
+
+int compliance_dev(t3d_problem* p, const double* E, const double* U, double* ce, double* Cdst)
+{
+    int rc = ensure_red(p, p->ne);
+    if (rc) return rc;
+    const int64_t tiles = (p->ne + CANON_TILE - 1) / CANON_TILE;
+    hipLaunchKernelGGL(k_compliance, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, p->stream, p->g, p->ne,
+                       p->compactToCell, p->Ke, E, U, ce, p->red);
+    T3D_HIP(hipGetLastError());
+    return canon_finish(p, tiles, Cdst);
+}
+
+int cobj_dev(t3d_problem* p, const double* E, const double* ce, const double* Csolid, double* dst)
+{
+    int rc = ensure_red(p, p->ne);
+    if (rc) return rc;
+    const int64_t tiles = (p->ne + CANON_TILE - 1) / CANON_TILE;
+    hipLaunchKernelGGL(k_cobj_tiles, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, p->stream, p->ne, E, ce, Csolid,
+                       p->red);
+    T3D_HIP(hipGetLastError());
+    return canon_finish(p, tiles, dst);
+}
+
+int launch_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E)
+{
+    hipLaunchKernelGGL(k_simp, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, p->stream, p->ne, x, E0, Emin, E);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+int launch_fill_f64(t3d_problem* p, int64_t n, double v, double* a)
+{
+    hipLaunchKernelGGL(k_fill_f64, dim3(unsigned((n + 255) / 256)), dim3(256), 0, p->stream, n, v, a);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+int launch_fill_f32(t3d_problem* p, int64_t n, float v, float* a)
+{
+    hipLaunchKernelGGL(k_fill_f32, dim3(unsigned((n + 255) / 256)), dim3(256), 0, p->stream, n, v, a);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+int launch_sens(t3d_problem* p, const float* x, const double* ce, const double* Csolid, float E0, float Emin,
+                float penal, float* xdc)
+{
+    hipLaunchKernelGGL(k_sens_xdc, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, p->stream, p->ne, x, ce, Csolid,
+                       E0, Emin, penal, xdc);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+int launch_dc_post(t3d_problem* p, const float* filt, const float* x, float* dc)
+{
+    hipLaunchKernelGGL(k_dc_post, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, p->stream, p->ne, filt, x, dc);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+// scal: [0] lmid [1] steps [2] changeBits [3] sharpness [4] volFrac (float slots)
+int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal)
+{
+    hipStream_t st = p->stream;
+    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(SS_THREADS), 0, st, p->ne, x, dc, V0, move, scal);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipMemsetAsync(scal + 2, 0, sizeof(float), st));
+    hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, scal, move,
+                       xnew, reinterpret_cast<unsigned*>(scal + 2));
+    T3D_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_oc_stats, dim3(1), dim3(SS_THREADS), 0, st, p->ne, xnew, scal + 3);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+}  // namespace t3d
+
+using namespace t3d;
+
+extern "C" {
+
+int t3d_compliance(t3d_problem* p, const double* E, const double* U, double* ce, double* C_host)
+{
+    T3D_CHECK(p && E && U && ce, T3D_ERR_ARG, "t3d_compliance: null argument");
+    int rc = compliance_dev(p, E, U, ce, p->scal);
+    if (rc) return rc;
+    if (C_host) T3D_HIP(hipMemcpyAsync(C_host, p->scal, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    return T3D_OK;
+}
+
+int t3d_seq_sum_f32(t3d_problem* p, const float* v, int64_t n, float* out_host)
+{
+    T3D_CHECK(p && v && out_host && n >= 0, T3D_ERR_ARG, "t3d_seq_sum_f32: bad argument");
+    float* d = reinterpret_cast<float*>(p->scal + 8);
+    hipLaunchKernelGGL(k_seq_sum, dim3(1), dim3(SS_THREADS), 0, p->stream, n, v, d);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipMemcpyAsync(out_host, d, sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    return T3D_OK;
+}
+
+int t3d_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E)
+{
+    T3D_CHECK(p && x && E, T3D_ERR_ARG, "t3d_simp: null argument");
+    int rc = launch_simp(p, x, E0, Emin, E);
+    if (rc) return rc;
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    return T3D_OK;
+}
+
+// OC update: x, dc (post-filter sensitivities) -> xnew; out[0] change, out[1]
+// sharpness, out[2] volume fraction, out[3] bisection steps (host)
+int t3d_oc_update(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* out)
+{
+    T3D_CHECK(p && x && dc && xnew, T3D_ERR_ARG, "t3d_oc_update: null argument");
+    float* sc = reinterpret_cast<float*>(p->scal + 16);
+    int rc = launch_oc(p, x, dc, V0, move, xnew, sc);
+    if (rc) return rc;
+    float h[5];
+    T3D_HIP(hipMemcpyAsync(h, sc, sizeof(float) * 5, hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    if (out) {
+        out[0] = h[2];   // bit pattern of a non-negative float
+        out[1] = h[3];
+        out[2] = h[4];
+        out[3] = h[1];
+    }
+    return T3D_OK;
+}
+
+}  // extern "C"
+
+namespace t3d {
+// device floating-point semantics probe: op 0 sqrtf(a), 1 a/b (float), 2 sqrt(a) (double)
+__global__ void k_fp_probe(int op, int64_t n, const void* __restrict__ a, const void* __restrict__ b,
+                           void* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op == 0) ((float*)out)[i] = sqrtf(((const float*)a)[i]);
+    else if (op == 1) ((float*)out)[i] = ((const float*)a)[i] / ((const float*)b)[i];
+    else ((double*)out)[i] = sqrt(((const double*)a)[i]);
+}
+}  // namespace t3d
+
+extern "C" int t3d_fp_probe(int op, int64_t n, const void* a, const void* b, void* out)
+{
+    T3D_CHECK(op >= 0 && op <= 2 && n >= 0 && a && out && (op != 1 || b), T3D_ERR_ARG, "t3d_fp_probe: bad argument");
+    hipLaunchKernelGGL(t3d::k_fp_probe, dim3(unsigned((n + 255) / 256)), dim3(256), 0, 0, op, n, a, b, out);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipDeviceSynchronize());
+    return T3D_OK;
+}

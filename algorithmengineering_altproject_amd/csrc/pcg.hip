@@ -38,7 +38,7 @@ __global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restri
 __global__ void k_pcg_stop(const double* __restrict__ normb2, double tol, double* __restrict__ stop,
                            int* __restrict__ flags)
 {
-    const double nb = __dsqrt_rn(*normb2);
+    const double nb = sqrt(*normb2);
     *stop = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
     flags[0] = 0;
     flags[1] = -1;
@@ -87,7 +87,7 @@ __global__ void k_pcg_check(const double* __restrict__ rn2, const double* __rest
                             int* __restrict__ flags)
 {
     if (flags[0]) return;
-    if (__dsqrt_rn(*rn2) < *stop) {
+    if (sqrt(*rn2) < *stop) {
         flags[0] = 1;
         flags[1] = it + 1;
     }

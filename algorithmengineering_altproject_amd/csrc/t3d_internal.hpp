@@ -82,4 +82,13 @@ int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, d
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
 __global__ void k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ done);
 int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done);
+int compliance_dev(t3d_problem* p, const double* E, const double* U, double* ce, double* Cdst);
+int cobj_dev(t3d_problem* p, const double* E, const double* ce, const double* Csolid, double* dst);
+int launch_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E);
+int launch_fill_f64(t3d_problem* p, int64_t n, double v, double* a);
+int launch_fill_f32(t3d_problem* p, int64_t n, float v, float* a);
+int launch_sens(t3d_problem* p, const float* x, const double* ce, const double* Csolid, float E0, float Emin,
+                float penal, float* xdc);
+int launch_dc_post(t3d_problem* p, const float* filt, const float* x, float* dc);
+int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal);
 }  // namespace t3d

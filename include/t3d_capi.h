@@ -37,6 +37,7 @@ extern "C" {
 
 typedef struct t3d_problem t3d_problem;
 typedef struct t3d_mg t3d_mg;
+typedef struct t3d_filter t3d_filter;
 
 /* ---- runtime / memory helpers (no reference counterpart: device plumbing) ---- */
 const char* t3d_last_error(void);
@@ -117,6 +118,67 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b_full_dev, double* x_full_
             int* iters);
 /* ||r_{k+1}||^2 of the last solve for k < n (n <= maxIt of that solve) */
 int t3d_pcg_history(t3d_problem* p, int maxIt, int n, double* rn2_host);
+
+/* ---- compliance ----
+ * ComputeCompliance (cpp/include/fea.hpp:47, cpp/src/fea.cpp:587-677,851-864):
+ * ce[e] = ue^T Ke ue (0 when E <= 1e-16), returns sum E*ce (canonical tree).
+ * E: compact element moduli; U: full layout displacements; ce: compact [ne]. */
+int t3d_compliance(t3d_problem* p, const double* E_dev, const double* U_full_dev, double* ce_dev, double* C_host);
+
+/* ---- PDE filter ----
+ * SetupPDEFilter (cpp/include/filter.hpp:17, cpp/src/filter.cpp:9-75) */
+int t3d_filter_create(t3d_problem* p, float rMin, t3d_filter** out);
+int t3d_filter_destroy(t3d_filter* f);
+/* kernel64: the 8x8 Helmholtz kernel; diag_host: [nN] Jacobi preconditioner */
+int t3d_filter_info(const t3d_filter* f, double* kernel64, double* diag_host);
+/* ApplyPDEFilter (cpp/include/filter.hpp:28, cpp/src/filter.cpp:175-275) on
+ * float element arrays (compact order, device); keeps the warm-start state.
+ * *iters = CG iterations, -1 for the early exit before the loop. */
+int t3d_filter_apply(t3d_filter* f, const float* src_dev, float* dst_dev, int* iters);
+
+/* ---- SIMP / OC (inline in TOP3D_XL_GLOBAL, cpp/src/driver.cpp:117-189) ---- */
+/* E[e] = Emin + rho^3 (E0 - Emin) in float, rho = clamp(x,0,1) (driver.cpp:117-121) */
+int t3d_simp(t3d_problem* p, const float* x_dev, float E0, float Emin, double* E_dev);
+/* OC bisection + update (driver.cpp:169-189): dc = post-filter sensitivities;
+ * out_host[0] = change, [1] = sharpness, [2] = volume fraction, [3] = bisection steps */
+int t3d_oc_update(t3d_problem* p, const float* x_dev, const float* dc_dev, float V0, float move, float* xnew_dev,
+                  float* out_host);
+/* std::accumulate<float> of v[0..n) reproduced exactly (driver.cpp:181,188,201) */
+int t3d_seq_sum_f32(t3d_problem* p, const float* v_dev, int64_t n, float* out_host);
+
+/* ---- the optimisation loop ----
+ * TOP3D_XL_GLOBAL (cpp/include/driver.hpp:4, cpp/src/driver.cpp:23-218) for one
+ * simulation, parameters explicit (defaults = the reference's GlobalParams,
+ * MGPrecondConfig and CLI values). */
+typedef struct t3d_params {
+    float E0, Emin, nu, penal, cgTol;
+    int cgMaxIt;
+    float V0;
+    int nLoop;
+    float rMin;
+    int mgMaxLevels;
+    float mgWeight;
+    int nonDyadic;
+    int NlimitDofs;
+    int rebuildEvery;
+    int precision;   /* 0 = FP64 plain path */
+} t3d_params;
+typedef struct t3d_loop_record {
+    int loop, pcgIters, filterIters, bisectSteps;
+    double C, Cdisp;
+    float volFrac, sharpness, change, fval;
+    double tSolve, tOpti, tFilter, tTotal;   /* host wall-clock seconds */
+} t3d_loop_record;
+void t3d_params_default(t3d_params* p);
+/* Runs the solid reference solve and up to P->nLoop OC loops; rec[recCap]
+ * receives one record per loop, x_out_host[ne] (optional) the final design. */
+int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_record* rec, int recCap,
+                  float* x_out_host, double* Csolid, int* solidIters, int* loopsRun);
+
+/* ---- diagnostics (no reference counterpart): device floating-point semantics
+ * the plain path relies on.  op 0: out = sqrtf(a) (float), 1: out = a / b
+ * (float), 2: out = sqrt(a) (double); all must be correctly rounded. */
+int t3d_fp_probe(int op, int64_t n, const void* a_dev, const void* b_dev, void* out_dev);
 
 /* ---- timing (no reference counterpart): average device time of `reps`
  * back-to-back launches of an operator, measured with HIP events on the
