@@ -216,6 +216,17 @@ static int launch_kxu_lds(t3d_problem* p, const double* u, double* y, const int*
     return T3D_OK;
 }
 
+__global__ void __launch_bounds__(256) k_stream_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b)
+{
+    const int64_t i = 2 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+    if (i + 1 < n) {
+        const double2 v = *reinterpret_cast<const double2*>(a + i);
+        *reinterpret_cast<double2*>(b + i) = v;
+    } else if (i < n) {
+        b[i] = a[i];
+    }
+}
+
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done)
 {
     return launch_kxu_lds<1>(p, u, y, done);
@@ -283,6 +294,12 @@ int t3d_time_op(t3d_problem* p, int op, const void* in, void* out, int reps, dou
         switch (op) {
             case 0: rc = launch_kxu_f64(p, (const double*)in, (double*)out); break;
             case 101: rc = launch_kxu_lds<2>(p, (const double*)in, (double*)out); break;
+            case 1: {   // streaming copy of 3*nN doubles: bandwidth / counter calibration
+                const int64_t n = 3 * p->g.nN;
+                hipLaunchKernelGGL(k_stream_copy, dim3(unsigned((n / 2 + 255) / 256)), dim3(256), 0, p->stream, n,
+                                   (const double*)in, (double*)out);
+                break;
+            }
             case 100: {
                 const int B = 256;
                 hipLaunchKernelGGL(k_kxu_f64_node, dim3(unsigned((p->g.nN + B - 1) / B)), dim3(B), 0, p->stream,

@@ -182,7 +182,8 @@ int t3d_fp_probe(int op, int64_t n, const void* a_dev, const void* b_dev, void* 
 
 /* ---- timing (no reference counterpart): average device time of `reps`
  * back-to-back launches of an operator, measured with HIP events on the
- * problem's stream.  op: 0 = t3d_kxu_f64. */
+ * problem's stream.  op: 0 = t3d_kxu_f64 (in = u, out = y, full layout);
+ * 1 = streaming copy of 3*numNodes doubles (bandwidth / counter calibration). */
 int t3d_time_op(t3d_problem* p, int op, const void* in_dev, void* out_dev, int reps, double* avg_ms);
 
 #ifdef __cplusplus
