@@ -80,6 +80,10 @@ def lib():
         "t3d_oc_update": (C.c_int, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
         "t3d_seq_sum_f32": (C.c_int, [vp, vp, C.c_int64, C.POINTER(C.c_float)]),
         "t3d_params_default": (None, [vp]),
+        "t3d_mixed_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
+        "t3d_mixed_destroy": (C.c_int, [vp]),
+        "t3d_mixed_setup": (C.c_int, [vp]),
+        "t3d_pcg_mixed": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
         "t3d_top3d_run": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -366,6 +370,33 @@ def top3d_run(nely, nelx, nelz, **kw):
                               C.byref(nl)), "t3d_top3d_run")
     recs = [{f: getattr(r, f) for f, _ in LoopRecord._fields_} for r in rec[:nl.value]]
     return recs, x, Cs.value, si.value
+
+
+class Mixed:
+    """Mixed-precision MG-PCG state: FP32 K*u and V-cycle, FP64 outer loop."""
+
+    def __init__(self, prob, M):
+        self.prob, self.M = prob, M
+        h = C.c_void_p()
+        check(lib().t3d_mixed_create(prob.h, M.h, C.byref(h)), "t3d_mixed_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_mixed_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def setup(self):
+        check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
+
+    def pcg(self, db, dx, tol, maxIt, trueEvery=50, flexible=True):
+        it = C.c_int()
+        check(lib().t3d_pcg_mixed(self.h, db.ptr, dx.ptr, tol, maxIt, trueEvery, int(flexible), C.byref(it)),
+              "t3d_pcg_mixed")
+        return it.value
 
 
 def pcg(prob, M, db, dx, tol, maxIt):

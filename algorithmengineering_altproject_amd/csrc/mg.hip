@@ -388,6 +388,38 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
 
 }  // namespace t3d
 
+namespace t3d {
+struct MGView {
+    int nlev;
+    int span[8];
+    int resX[8], resY[8], resZ[8];
+    int64_t nN[8];
+    const uint8_t* mask[8];
+    const double* D[8];
+    const double* L;
+    const double* LT;
+    int Nc;
+    double weight;
+};
+int mg_view(const t3d_mg* g, MGView& v)
+{
+    T3D_CHECK(g && g->setupDone, T3D_ERR_STATE, "multigrid object not set up");
+    v.nlev = g->nlev;
+    for (int l = 0; l < g->nlev; ++l) {
+        v.span[l] = g->H.lv[l].span;
+        v.resX[l] = g->H.lv[l].resX; v.resY[l] = g->H.lv[l].resY; v.resZ[l] = g->H.lv[l].resZ;
+        v.nN[l] = g->H.lv[l].nN;
+        v.mask[l] = g->mask[l];
+        v.D[l] = g->D[l];
+    }
+    v.L = g->L;
+    v.LT = g->LT;
+    v.Nc = g->Nc;
+    v.weight = g->H.weight;
+    return T3D_OK;
+}
+}  // namespace t3d
+
 extern "C" {
 
 int t3d_mg_destroy(t3d_mg* g)

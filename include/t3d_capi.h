@@ -38,6 +38,7 @@ extern "C" {
 typedef struct t3d_problem t3d_problem;
 typedef struct t3d_mg t3d_mg;
 typedef struct t3d_filter t3d_filter;
+typedef struct t3d_mixed t3d_mixed;
 
 /* ---- runtime / memory helpers (no reference counterpart: device plumbing) ---- */
 const char* t3d_last_error(void);
@@ -118,6 +119,20 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b_full_dev, double* x_full_
             int* iters);
 /* ||r_{k+1}||^2 of the last solve for k < n (n <= maxIt of that solve) */
 int t3d_pcg_history(t3d_problem* p, int maxIt, int n, double* rn2_host);
+
+/* ---- mixed precision (the chosen "precision" form; no single reference
+ * counterpart: it replaces PCG_free + the V-cycle closure, cpp/src/solver.cpp:87-217,
+ * cpp/src/multigrid/preconditioner.cpp:88-265, with FP32 K*u and V-cycle inside a
+ * flexible FP64 outer loop; FP64 dots, alpha/beta, x, r, residual replacement
+ * and coarsest Cholesky) ---- */
+int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out);
+int t3d_mixed_destroy(t3d_mixed* s);
+/* refresh the FP32 copies of the multigrid diagonals after t3d_mg_setup */
+int t3d_mixed_setup(t3d_mixed* s);
+/* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
+ * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old */
+int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, double tol, int maxIt, int trueEvery,
+                  int flexible, int* iters);
 
 /* ---- compliance ----
  * ComputeCompliance (cpp/include/fea.hpp:47, cpp/src/fea.cpp:587-677,851-864):
