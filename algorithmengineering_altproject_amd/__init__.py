@@ -83,6 +83,8 @@ def lib():
         "t3d_mixed_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
         "t3d_mixed_destroy": (C.c_int, [vp]),
         "t3d_mixed_setup": (C.c_int, [vp]),
+        "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
+        "t3d_mixed_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
         "t3d_pcg_mixed": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
         "t3d_top3d_run": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(C.c_double),
@@ -391,6 +393,15 @@ class Mixed:
 
     def setup(self):
         check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
+
+    def kxu_f32(self, du, dy, form=0):
+        """y = K u in FP32 (device float arrays, full layout); form 0 Walsh, 1 node gather."""
+        check(lib().t3d_kxu_f32(self.h, form, du.ptr, dy.ptr), "t3d_kxu_f32")
+
+    def time_kxu(self, du, dy, reps, form=0):
+        ms = C.c_float()
+        check(lib().t3d_mixed_time_op(self.h, form, du.ptr, dy.ptr, reps, C.byref(ms)), "t3d_mixed_time_op")
+        return ms.value
 
     def pcg(self, db, dx, tol, maxIt, trueEvery=50, flexible=True):
         it = C.c_int()

@@ -13,7 +13,14 @@
 //     -alpha (z_new.Ap)/(r.z)_old, which tolerates the FP32 preconditioner
 //     being only approximately linear;
 //   * every `trueEvery` iterations the residual is recomputed as b - K x with
-//     the FP64 operator (residual replacement; the direction is kept).
+//     the FP64 operator (residual replacement; the direction is kept), and a
+//     solve only ends when that true residual passes the tolerance.
+//
+// Reductions: every block of the producing kernel writes one FP64 partial
+// (a fixed tree over its threads); a one-block finishing kernel sums the
+// partials in block order and derives the scalars of the iteration (alpha, the
+// convergence flag, beta), so every result is deterministic.  The kernel
+// boundary orders the partials; no device-wide fence per block is needed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,177 +30,440 @@
 
 namespace t3d {
 
-// ---------------------------------------------------------------------------
-// FP32 operator with fused direction update and p.Ap partials
-// ---------------------------------------------------------------------------
-namespace mx {
-constexpr int TX = 16, TY = 8, TZ = 4;
-constexpr int UX = TX + 2, UY = TY + 2, UZ = TZ + 2;
-constexpr int EX = TX + 1, EY = TY + 1, EZ = TZ + 1;
-constexpr int UN = UX * UY * UZ;
-constexpr int EN = EX * EY * EZ;
-}  // namespace mx
-
-__device__ __forceinline__ int mx_corner(int dx, int dy, int dz) { return (dy ? (dx ? 1 : 0) : (dx ? 2 : 3)) + 4 * dz; }
-
-// p_cur = z + beta*p_prev (z == nullptr: p_cur = p_prev); y = K p_cur;
-// partial[block] = sum over the tile's nodes of p_cur.y (FP64)
-__global__ void __launch_bounds__(512) k_kxu_f32_fused(Grid g, const float* __restrict__ Ke,
-                                                       const float* __restrict__ Ecell,
-                                                       const uint8_t* __restrict__ nodeFixed,
-                                                       const float* __restrict__ z, const float* __restrict__ pprev,
-                                                       const double* __restrict__ beta, float* __restrict__ pcur,
-                                                       float* __restrict__ y, double* __restrict__ partial,
-                                                       const int* __restrict__ done)
-{
-    using namespace mx;
-    if (done && *done) return;
-    __shared__ float sU[3 * UN];
-    __shared__ float sE[EN];
-    __shared__ double sRed[8];
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * TZ;
-    const int tid = threadIdx.x;
-    const float b = (z && beta) ? float(*beta) : 0.0f;
-    for (int i = tid; i < UN; i += 512) {
-        const int lx = i % UX, ly = (i / UX) % UY, lz = i / (UX * UY);
-        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
-        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-        if (X >= 0 && Y >= 0 && Z >= 0 && X < g.nnx && Y < g.nny && Z < g.nnz) {
-            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-            v0 = pprev[3 * n]; v1 = pprev[3 * n + 1]; v2 = pprev[3 * n + 2];
-            if (z) {
-                v0 = fmaf(b, v0, z[3 * n]);
-                v1 = fmaf(b, v1, z[3 * n + 1]);
-                v2 = fmaf(b, v2, z[3 * n + 2]);
-                // the tile interior owns the write of p_cur
-                if (pcur && lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && lz >= 1 && lz <= TZ) {
-                    pcur[3 * n] = v0; pcur[3 * n + 1] = v1; pcur[3 * n + 2] = v2;
-                }
-            }
-        }
-        sU[i] = v0; sU[UN + i] = v1; sU[2 * UN + i] = v2;
-    }
-    for (int i = tid; i < EN; i += 512) {
-        const int lx = i % EX, ly = (i / EX) % EY, lz = i / (EX * EY);
-        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
-        float e = 0.f;
-        if (X >= 0 && Y >= 0 && Z >= 0 && X < g.resX && Y < g.resY && Z < g.resZ)
-            e = Ecell[(int64_t(Z) * g.resY + Y) * g.resX + X];
-        sE[i] = e;
-    }
-    __syncthreads();
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int lane = tid & 63;
-    const int px = w & 1, py = (w >> 1) & 1, pz = (w >> 2) & 1;
-    const int lx = 2 * (lane & 7) + px, ly = 2 * ((lane >> 3) & 3) + py, lz = 2 * (lane >> 5) + pz;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll 1
-    for (int col = 0; col < 8; ++col) {
-        const int dx = (px ^ col) & 1, dy = (py ^ (col >> 1) ^ 1) & 1, dz = (pz ^ (col >> 2)) & 1;
-        const int cxl = lx - dx, cyl = ly - dy, czl = lz - dz;
-        const float E = sE[((czl + 1) * EY + (cyl + 1)) * EX + (cxl + 1)];
-        const int a = __builtin_amdgcn_readfirstlane(mx_corner(dx, dy, dz));
-        const float* __restrict__ K0 = Ke + (3 * a) * 24;
-        const int ub = ((czl + 1) * UY + (cyl + 1)) * UX + (cxl + 1);
-        float f0 = 0.f, f1 = 0.f, f2 = 0.f;
-#pragma unroll
-        for (int bb = 0; bb < 8; ++bb) {
-            const int ox = (bb == 1 || bb == 2 || bb == 5 || bb == 6) ? 1 : 0;
-            const int oy = ((bb & 3) <= 1) ? 1 : 0;
-            const int oz = bb >> 2;
-            const int idx = ub + (oz * UY + oy) * UX + ox;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float ue = sU[c * UN + idx];
-                const int j = 3 * bb + c;
-                f0 = fmaf(K0[j], ue, f0);
-                f1 = fmaf(K0[24 + j], ue, f1);
-                f2 = fmaf(K0[48 + j], ue, f2);
-            }
-        }
-        const float Eu = (E <= 1.0e-16f) ? 0.f : E;
-        a0 = fmaf(f0, Eu, a0);
-        a1 = fmaf(f1, Eu, a1);
-        a2 = fmaf(f2, Eu, a2);
-    }
-    const int X = x0 + lx, Y = y0 + ly, Z = z0 + lz;
-    double dotp = 0.0;
-    if (X < g.nnx && Y < g.nny && Z < g.nnz) {
-        const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-        const unsigned fixed = nodeFixed[n];
-        const float y0v = (fixed & 1u) ? 0.f : a0, y1v = (fixed & 2u) ? 0.f : a1, y2v = (fixed & 4u) ? 0.f : a2;
-        y[3 * n] = y0v; y[3 * n + 1] = y1v; y[3 * n + 2] = y2v;
-        const int ui = ((lz + 1) * UY + (ly + 1)) * UX + (lx + 1);
-        dotp = double(sU[ui]) * y0v + double(sU[UN + ui]) * y1v + double(sU[2 * UN + ui]) * y2v;
-    }
-    if (partial) {
-        for (int o = 32; o >= 1; o >>= 1) dotp += __shfl_xor(dotp, o, 64);
-        if (lane == 0) sRed[w] = dotp;
-        __syncthreads();
-        if (tid == 0) {
-            double s = 0.0;
-            for (int k = 0; k < 8; ++k) s += sRed[k];
-            partial[(int64_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
-        }
-    }
-}
+// scalar slots of the mixed solver
+enum {
+    S_NORMB2 = 0, S_STOP = 1, S_RZ = 2, S_ZAP = 4, S_PAP = 5, S_ALPHA = 6, S_BETA = 7, S_RN2 = 8, S_RZNEW = 9
+};
 
 // ---------------------------------------------------------------------------
-// deterministic block sums for the mixed solver (fixed tree per block, blocks
-// reduced in index order)
+// block reductions
 // ---------------------------------------------------------------------------
+// sum over a 256-thread block; the result is valid in thread 0; sh[4]
 __device__ __forceinline__ double block_sum_256(double s, double* sh)
 {
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    return t;
+}
+
+// sum of partial[0..m) in a fixed order by a 1024-thread block (valid in thread 0)
+__device__ __forceinline__ double sum_partials_1024(const double* __restrict__ partial, unsigned m, double* sh)
+{
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < m; i += 1024) s += partial[i];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
     double t = 0.0;
     if (threadIdx.x == 0)
-        for (int k = 0; k < int(blockDim.x >> 6); ++k) t += sh[k];
-    return t;   // valid in thread 0
+        for (int k = 0; k < 16; ++k) t += sh[k];
+    return t;
 }
 
-// x += alpha p32 ; r -= alpha Ap32 ; partial r.r
-__global__ void __launch_bounds__(256) k_mx_update_xr(int64_t n, const double* __restrict__ rz,
-                                                      const double* __restrict__ denom,
-                                                      const float* __restrict__ p, const float* __restrict__ Ap,
-                                                      double* __restrict__ x, double* __restrict__ r,
-                                                      double* __restrict__ alphaOut, double* __restrict__ partial,
-                                                      const int* __restrict__ done)
+__device__ __forceinline__ unsigned linear_block()
+{
+    return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 operator.  Tile of 32 x 8 x NZ nodes per 256-thread block; the
+// (NZ+2)-plane node halo and the (NZ+1)-plane cell halo are staged in LDS.
+// Each thread owns one (x, y) node column and sweeps it in z with a rolling
+// register window of the 3x3x3 neighbourhood, so every LDS value is read once
+// per column; the 8 incident cells and Ke's corner rows are compile-time
+// indices (Ke through scalar loads).
+// Fused form: the staged values are p = z + beta*p_prev (the tile interior
+// writes p), and the block's p.(K p) partial feeds alpha = rz / max(1e-30, p.Ap).
+// ---------------------------------------------------------------------------
+namespace kx {
+constexpr int TX = 32, TY = 8;
+template <int NZ>
+struct Tile {
+    static constexpr int UX = TX + 2, UY = TY + 2, UZ = NZ + 2, UN = UX * UY * UZ;
+    static constexpr int EX = TX + 1, EY = TY + 1, EZ = NZ + 1, EN = EX * EY * EZ;
+};
+}  // namespace kx
+
+__device__ __forceinline__ constexpr int corner_of(int dx, int dy, int dz)
+{
+    return (dy ? (dx ? 1 : 0) : (dx ? 2 : 3)) + 4 * dz;
+}
+
+template <int NZ, bool FUSED>
+__global__ void __launch_bounds__(256) k_kx32(Grid g, const float* __restrict__ Ke, const float* __restrict__ Ecell,
+                                              const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
+                                              const float* __restrict__ z, float* __restrict__ pcur,
+                                              float* __restrict__ y, double* __restrict__ partial,
+                                              const double* __restrict__ S, const int* __restrict__ done)
+{
+    using T = kx::Tile<NZ>;
+    using kx::TX;
+    using kx::TY;
+    if (FUSED && *done) return;
+    __shared__ float sU[3 * T::UN];
+    __shared__ float sE[T::EN];
+    __shared__ double sh[4];
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * NZ;
+    const int tid = threadIdx.x;
+    const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
+    // stage the node halo: rows of UX nodes are contiguous 3*UX floats in memory
+    constexpr int ROW = 3 * T::UX;
+    for (int f = tid; f < ROW * T::UY * T::UZ; f += 256) {
+        const int row = f / ROW;
+        const int k = f - row * ROW;
+        const int lx = k / 3, c = k - 3 * lx;
+        const int ly = row % T::UY, lz = row / T::UY;
+        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
+        float v = 0.f;
+        if (X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny && Z >= 0 && Z < g.nnz) {
+            const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
+            v = u[gi];
+            if (FUSED) {
+                v = fmaf(beta, v, z[gi]);
+                if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && lz >= 1 && lz <= NZ) pcur[gi] = v;
+            }
+        }
+        sU[c * T::UN + row * T::UX + lx] = v;
+    }
+    // cell halo; cells with E <= 1e-16 (and void cells) contribute nothing (fea.cpp:743-744)
+    for (int i = tid; i < T::EN; i += 256) {
+        const int lx = i % T::EX, ly = (i / T::EX) % T::EY, lz = i / (T::EX * T::EY);
+        const int X = x0 - 1 + lx, Y = y0 - 1 + ly, Z = z0 - 1 + lz;
+        float e = 0.f;
+        if (X >= 0 && X < g.resX && Y >= 0 && Y < g.resY && Z >= 0 && Z < g.resZ) {
+            e = Ecell[(int64_t(Z) * g.resY + Y) * g.resX + X];
+            if (e <= 1.0e-16f) e = 0.f;
+        }
+        sE[i] = e;
+    }
+    __syncthreads();
+    const int tx = tid % TX, ty = tid / TX;
+    const int X = x0 + tx, Y = y0 + ty;
+    const int nz = min(NZ, g.nnz - z0);
+    double dot = 0.0;
+    if (X < g.nnx && Y < g.nny) {
+        float w[3][3][3][3];   // [z][y][x][component] around the current node
+        float e[2][2][2];      // [dz][dy][dx]: cell (X-dx, Y-dy, Z-dz)
+        auto plane = [&](int L, float (&P)[3][3][3]) {
+#pragma unroll
+            for (int iy = 0; iy < 3; ++iy)
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) P[iy][ix][c] = sU[c * T::UN + (L * T::UY + ty + iy) * T::UX + tx + ix];
+        };
+        auto eplane = [&](int L, float (&P)[2][2]) {
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) P[dy][dx] = sE[(L * T::EY + ty + 1 - dy) * T::EX + tx + 1 - dx];
+        };
+        plane(0, w[1]);
+        plane(1, w[2]);
+        eplane(0, e[0]);
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+            if (j >= nz) break;
+#pragma unroll
+            for (int a = 0; a < 27; ++a) {
+                (&w[0][0][0][0])[a] = (&w[1][0][0][0])[a];
+                (&w[1][0][0][0])[a] = (&w[2][0][0][0])[a];
+            }
+            plane(j + 2, w[2]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) (&e[1][0][0])[a] = (&e[0][0][0])[a];
+            eplane(j + 1, e[0]);
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int a = corner_of(dx, dy, dz);
+                        float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+#pragma unroll
+                        for (int b = 0; b < 8; ++b) {
+                            const int ox = (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0;
+                            const int oy = ((b & 3) <= 1) ? 1 : 0;
+                            const int oz = b >> 2;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                const float uv = w[oz - dz + 1][oy - dy + 1][ox - dx + 1][c];
+                                const int jj = 3 * b + c;
+                                f0 = fmaf(Ke[(3 * a) * 24 + jj], uv, f0);
+                                f1 = fmaf(Ke[(3 * a + 1) * 24 + jj], uv, f1);
+                                f2 = fmaf(Ke[(3 * a + 2) * 24 + jj], uv, f2);
+                            }
+                        }
+                        const float E = e[dz][dy][dx];
+                        a0 = fmaf(E, f0, a0);
+                        a1 = fmaf(E, f1, a1);
+                        a2 = fmaf(E, f2, a2);
+                    }
+            const int64_t n = (int64_t(z0 + j) * g.nny + Y) * g.nnx + X;
+            const unsigned fx = nodeFixed[n];
+            const float y0v = (fx & 1u) ? 0.f : a0, y1v = (fx & 2u) ? 0.f : a1, y2v = (fx & 4u) ? 0.f : a2;
+            y[3 * n] = y0v;
+            y[3 * n + 1] = y1v;
+            y[3 * n + 2] = y2v;
+            if (FUSED) {
+                dot = fma(double(w[1][1][1][0]), double(y0v), dot);
+                dot = fma(double(w[1][1][1][1]), double(y1v), dot);
+                dot = fma(double(w[1][1][1][2]), double(y2v), dot);
+            }
+        }
+    }
+    if (FUSED) {
+        const double t = block_sum_256(dot, sh);
+        if (tid == 0) partial[linear_block()] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 operator, Walsh form.  In the Walsh-Hadamard basis of a voxel's eight
+// nodes, H[m][b] = (-1)^(m . o(b)) with o(b) the node's offset in the voxel,
+// the element matrix K^ = (H x I3) Ke (H x I3)^T has 45 nonzeros (the constant
+// modes carry no energy and the linear, bilinear and trilinear modes decouple
+// under the 2x2x2 Gauss rule), and Ke u = (H x I3)^T (K^/64) (H x I3) u.
+// Per voxel that is 45 FMAs plus butterflies instead of 576 FMAs.
+//
+// A 256-thread block owns 31 x 7 node columns and sweeps NZ node planes in z;
+// thread (tx, ty) owns the voxel column at (x0-1+tx, y0-1+ty).  The 2D Walsh
+// transform of a node plane's 2x2 face is shared by the voxel below and the
+// voxel above; the z-transform pairs consecutive faces.  Each voxel's face
+// forces are accumulated per node plane in Walsh space (top face of the voxel
+// below + bottom face of the voxel above), transformed back to the four
+// corners, and summed over the four voxel columns that share a node through
+// LDS, in a fixed order.
+// ---------------------------------------------------------------------------
+namespace wk {
+constexpr int CX = 32, CY = 8;           // voxel columns (threads)
+constexpr int NX = CX - 1, NY = CY - 1;  // owned node columns
+constexpr int PX = CX + 1, PY = CY + 1;  // staged node plane
+constexpr int PN = PX * PY;
+constexpr int NNZ = 45;
+// nonzero structure of K^, index (4 mz + 2 my + mx) * 3 + c; verified on the
+// host against the element matrix when the operator is created
+__device__ constexpr unsigned char OUT[NNZ] = {3,  3,  3,  4,  4,  5,  5,  6,  6,  7,  7,  7,  8,  8,  9,
+                                               9,  10, 10, 11, 11, 11, 12, 12, 13, 13, 14, 14, 14, 15, 15,
+                                               16, 16, 16, 17, 17, 18, 18, 18, 19, 19, 20, 20, 21, 22, 23};
+__device__ constexpr unsigned char IN[NNZ] = {3,  7,  14, 4,  6,  5,  12, 4,  6,  3,  7,  14, 8,  13, 9,
+                                              20, 10, 17, 11, 16, 18, 5,  12, 8,  13, 3,  7,  14, 15, 19,
+                                              11, 16, 18, 10, 17, 11, 16, 18, 15, 19, 9,  20, 21, 22, 23};
+}  // namespace wk
+
+// 2D Walsh butterfly of the four corners a[oy][ox] -> F[my][mx] (self-inverse up to 4)
+__device__ __forceinline__ void walsh2(float a00, float a10, float a01, float a11, float F[2][2])
+{
+    const float s0 = a00 + a10, d0 = a00 - a10, s1 = a01 + a11, d1 = a01 - a11;
+    F[0][0] = s0 + s1;
+    F[0][1] = d0 + d1;
+    F[1][0] = s0 - s1;
+    F[1][1] = d0 - d1;
+}
+
+template <int NZ, bool FUSED>
+__global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
+                                               const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
+                                               const float* __restrict__ z, float* __restrict__ pcur,
+                                               float* __restrict__ y, double* __restrict__ partial,
+                                               const double* __restrict__ S, const int* __restrict__ done)
+{
+    using namespace wk;
+    if (FUSED && *done) return;
+    __shared__ float sP[3][3][PN];     // three staged node planes, [buffer][component][node]
+    __shared__ float sC[12][CX * CY];  // face-corner forces [(2 oy + ox) * 3 + c][thread]
+    __shared__ double sh[4];
+    const int tid = threadIdx.x;
+    const int tx = tid % CX, ty = tid / CX;
+    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
+    const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
+    const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
+    const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
+    // stage node plane Z into buffer q (rows of PX nodes are 3*PX contiguous floats)
+    auto stage = [&](int Z, int q) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        const bool own = Z >= z0 && Z < z0 + NZ;
+        for (int f = tid; f < 3 * PN; f += CX * CY) {
+            const int row = f / (3 * PX);
+            const int k = f - row * (3 * PX);
+            const int lx = k / 3, c = k - 3 * lx;
+            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+            float v = 0.f;
+            if (zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
+                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
+                v = u[gi];
+                if (FUSED) {
+                    v = fmaf(beta, v, z[gi]);
+                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) pcur[gi] = v;
+                }
+            }
+            sP[q][c][row * PX + lx] = v;
+        }
+    };
+    // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
+    auto face = [&](int q, float F[3][2][2]) {
+        const int b = ty * PX + tx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            walsh2(sP[q][c][b], sP[q][c][b + 1], sP[q][c][b + PX], sP[q][c][b + PX + 1], F[c]);
+    };
+    float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
+    stage(z0 - 1, 0);
+    __syncthreads();
+    face(0, Fp);
+    double dot = 0.0;
+    const int nsteps = min(NZ, g.nnz - z0) + 1;
+    for (int s = 0; s < nsteps; ++s) {
+        const int qn = (s + 1) % 3, qo = s % 3;
+        stage(z0 + s, qn);
+        __syncthreads();
+        face(qn, Fn);
+        // voxel (cx, cy, cz), cz = z0 - 1 + s
+        const int cz = z0 - 1 + s;
+        float E = 0.f;
+        if (colIn && cz >= 0 && cz < g.resZ) {
+            E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
+            if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
+        }
+        float uh[24], vh[24];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int my = 0; my < 2; ++my)
+#pragma unroll
+                for (int mx = 0; mx < 2; ++mx) {
+                    uh[(2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] + Fn[c][my][mx]);
+                    uh[(4 + 2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] - Fn[c][my][mx]);
+                }
+#pragma unroll
+        for (int i = 0; i < 24; ++i) vh[i] = 0.f;
+#pragma unroll
+        for (int k = 0; k < NNZ; ++k) vh[OUT[k]] = fmaf(KW[k], uh[IN[k]], vh[OUT[k]]);
+        // faces: bottom (oz = 0) = vh[mz=0] + vh[mz=1] closes plane cz; top carries to cz + 1
+        float A[3][2][2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int my = 0; my < 2; ++my)
+#pragma unroll
+                for (int mx = 0; mx < 2; ++mx) {
+                    const float v0 = vh[(2 * my + mx) * 3 + c], v1 = vh[(4 + 2 * my + mx) * 3 + c];
+                    A[c][my][mx] = carry[c][my][mx] + (v0 + v1);
+                    carry[c][my][mx] = v0 - v1;
+                }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
+        if (s == 0) continue;   // plane z0 - 1 belongs to the block below
+        // plane Z = cz: corner forces of this voxel column, then the node gather
+        const int Z = cz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float C4[2][2];
+            walsh2(A[c][0][0], A[c][0][1], A[c][1][0], A[c][1][1], C4);
+            // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
+            sC[0 * 3 + c][tid] = C4[0][0];
+            sC[1 * 3 + c][tid] = C4[0][1];
+            sC[2 * 3 + c][tid] = C4[1][0];
+            sC[3 * 3 + c][tid] = C4[1][1];
+        }
+        __syncthreads();
+        const int X = x0 + tx, Y = y0 + ty;
+        if (tx < NX && ty < NY && X < g.nnx && Y < g.nny) {
+            // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)
+            float yv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float c00 = sC[3 * 3 + c][ty * CX + tx];              // column (X-1, Y-1), corner (1,1)
+                const float c10 = sC[2 * 3 + c][ty * CX + tx + 1];          // column (X,   Y-1), corner (0,1)
+                const float c01 = sC[1 * 3 + c][(ty + 1) * CX + tx];        // column (X-1, Y),   corner (1,0)
+                const float c11 = sC[0 * 3 + c][(ty + 1) * CX + tx + 1];    // column (X,   Y),   corner (0,0)
+                yv[c] = (c00 + c10) + (c01 + c11);
+            }
+            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
+            const unsigned fx = nodeFixed[n];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float v = ((fx >> c) & 1u) ? 0.f : yv[c];
+                y[3 * n + c] = v;
+                if (FUSED) dot = fma(double(sP[qo][c][(ty + 1) * PX + tx + 1]), double(v), dot);
+            }
+        }
+    }
+    if (FUSED) {
+        const double t = block_sum_256(dot, sh);
+        if (tid == 0) partial[linear_block()] = t;
+    }
+}
+
+// alpha = rz / max(1e-30, p.Ap)
+__global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
+                                                    double* __restrict__ S, const int* __restrict__ done)
+{
+    if (*done) return;
+    __shared__ double sh[16];
+    const double pAp = sum_partials_1024(partial, m, sh);
+    if (threadIdx.x == 0) {
+        S[S_PAP] = pAp;
+        S[S_ALPHA] = S[S_RZ] / (1.0e-30 < pAp ? pAp : 1.0e-30);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 vector kernels (grid-stride, fixed grid; the last block finishes)
+// ---------------------------------------------------------------------------
+constexpr int VB = 2048;   // blocks of the vector kernels
+
+// rn2 = r.r ; if check: the convergence test of iteration it
+__global__ void __launch_bounds__(1024) k_fin_rn2(const double* __restrict__ partial, unsigned m,
+                                                  double* __restrict__ S, int* __restrict__ flags, int it, int check,
+                                                  int guard)
+{
+    if (guard && flags[0]) return;
+    __shared__ double sh[16];
+    const double rn2 = sum_partials_1024(partial, m, sh);
+    if (threadIdx.x == 0) {
+        S[S_RN2] = rn2;
+        if (check && !flags[0] && sqrt(rn2) < S[S_STOP]) {
+            flags[0] = 1;
+            flags[1] = it + 1;
+        }
+    }
+}
+
+// x += alpha p ; r -= alpha Ap ; rn2 = r.r (+ convergence test)
+__global__ void __launch_bounds__(256) k_mx_update_xr(int64_t n, const float* __restrict__ p,
+                                                      const float* __restrict__ Ap, double* __restrict__ x,
+                                                      double* __restrict__ r, double* __restrict__ partial,
+                                                      const double* __restrict__ S, const int* __restrict__ done)
 {
     if (*done) return;
     __shared__ double sh[4];
-    const double d = *denom;
-    const double alpha = *rz / (1.0e-30 < d ? d : 1.0e-30);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *alphaOut = alpha;
+    const double alpha = S[S_ALPHA];
     double s = 0.0;
-    const int64_t base = int64_t(blockIdx.x) * blockDim.x * 4 + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int64_t i = base + int64_t(k) * blockDim.x;
-        if (i < n) {
-            x[i] = fma(alpha, double(p[i]), x[i]);
-            const double v = fma(-alpha, double(Ap[i]), r[i]);
-            r[i] = v;
-            s = fma(v, v, s);
-        }
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        x[i] = fma(alpha, double(p[i]), x[i]);
+        const double v = fma(-alpha, double(Ap[i]), r[i]);
+        r[i] = v;
+        s = fma(v, v, s);
     }
     const double t = block_sum_256(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(256) k_mx_sum_partials(int64_t m, const double* __restrict__ in,
-                                                         double* __restrict__ out, const int* __restrict__ done)
-{
-    if (done && *done) return;
-    __shared__ double sh[4];
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += in[i];
-    const double t = block_sum_256(s, sh);
-    if (threadIdx.x == 0) *out = t;
-}
-
-// r = b - Ap (FP64) ; partial r.r
+// r = b - Ap (FP64) ; rn2 (+ convergence test)
 __global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const double* __restrict__ b,
                                                           const double* __restrict__ Ap, double* __restrict__ r,
                                                           double* __restrict__ partial, const int* __restrict__ done)
@@ -201,22 +471,33 @@ __global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const doubl
     if (done && *done) return;
     __shared__ double sh[4];
     double s = 0.0;
-    const int64_t base = int64_t(blockIdx.x) * blockDim.x * 4 + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int64_t i = base + int64_t(k) * blockDim.x;
-        if (i < n) {
-            const double v = b[i] - Ap[i];
-            r[i] = v;
-            s = fma(v, v, s);
-        }
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double v = b[i] - Ap[i];
+        r[i] = v;
+        s = fma(v, v, s);
     }
     const double t = block_sum_256(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+__global__ void k_mx_init(double tol, double* __restrict__ S, int* __restrict__ flags)
+{
+    const double nb = sqrt(S[S_NORMB2]);
+    S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
+    S[S_BETA] = 0.0;
+    flags[0] = 0;
+    flags[1] = -1;
+}
+
+__global__ void k_mx_clear_done(int* __restrict__ flags)
+{
+    flags[0] = 0;
+    flags[1] = -1;
+}
+
 // ---------------------------------------------------------------------------
-// FP32 V-cycle kernels
+// FP32 V-cycle kernels.  3D launches (x in blocks of 64 threads, y and z in
+// the grid) so node coordinates need no division.
 // ---------------------------------------------------------------------------
 struct LevelF {
     int resX, resY, resZ;
@@ -224,57 +505,89 @@ struct LevelF {
     int64_t nN;
 };
 
+// restriction pass 1 (x): t[(fz*nny+fy)*cnx+cx] = sum_dx W(dx) rf[fz,fy,cx*s+dx] (unnormalised)
 template <class TIn>
-__global__ void __launch_bounds__(256) k_restrict_f32(LevelF Lf, LevelF Lc, int span, const TIn* __restrict__ rf,
-                                                      const uint8_t* __restrict__ cmask, float* __restrict__ rc,
-                                                      const float* __restrict__ Dc, float* __restrict__ xc,
-                                                      double* __restrict__ rcd, const int* __restrict__ done)
+__global__ void __launch_bounds__(64) k_restrict_x(LevelF Lf, int cnx, int span, const TIn* __restrict__ rf,
+                                                   float* __restrict__ t, const int* __restrict__ done)
 {
     if (done && *done) return;
-    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= Lc.nN) return;
-    const int cx = int(n % Lc.nnx), cy = int((n / Lc.nnx) % Lc.nny), cz = int(n / (int64_t(Lc.nnx) * Lc.nny));
-    const int wSize = 2 * span - 1;
+    const int cx = blockIdx.x * 64 + threadIdx.x;
+    if (cx >= cnx) return;
+    const int fy = blockIdx.y, fz = blockIdx.z;
     const float inv = 1.0f / float(span);
-    const int fx0 = cx * span, fy0 = cy * span, fz0 = cz * span;
-    const int iz0 = (fz0 - span + 1 < 0) ? (span - 1 - fz0) : 0;
-    const int iz1 = (fz0 + span - 1 >= Lf.nnz) ? (wSize - (fz0 + span - Lf.nnz)) : wSize;
-    const int iy0 = (fy0 - span + 1 < 0) ? (span - 1 - fy0) : 0;
-    const int iy1 = (fy0 + span - 1 >= Lf.nny) ? (wSize - (fy0 + span - Lf.nny)) : wSize;
-    const int ix0 = (fx0 - span + 1 < 0) ? (span - 1 - fx0) : 0;
-    const int ix1 = (fx0 + span - 1 >= Lf.nnx) ? (wSize - (fx0 + span - Lf.nnx)) : wSize;
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, ws = 0.f;
-    for (int iz = iz0; iz < iz1; ++iz) {
-        const int dz = iz - (span - 1);
+    const int64_t row = (int64_t(fz) * Lf.nny + fy) * Lf.nnx;
+    const int f0 = cx * span;
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+    for (int d = -(span - 1); d <= span - 1; ++d) {
+        const int fx = f0 + d;
+        if (fx < 0 || fx >= Lf.nnx) continue;
+        const float w = 1.0f - float(d < 0 ? -d : d) * inv;
+        const int64_t i = 3 * (row + fx);
+        v0 = fmaf(w, float(rf[i]), v0);
+        v1 = fmaf(w, float(rf[i + 1]), v1);
+        v2 = fmaf(w, float(rf[i + 2]), v2);
+    }
+    const int64_t o = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
+    t[o] = v0;
+    t[o + 1] = v1;
+    t[o + 2] = v2;
+}
+
+// restriction pass 2 (y, z), normalisation by the in-range weight sum, coarse
+// mask; at the coarsest level optionally the diagonal solve xc = rc / D or an
+// FP64 copy of rc for the Cholesky solve (transfers.cpp:449-533)
+__global__ void __launch_bounds__(64) k_restrict_yz(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
+                                                    const uint8_t* __restrict__ cmask, float* __restrict__ rc,
+                                                    const float* __restrict__ Dc, float* __restrict__ xc,
+                                                    double* __restrict__ rcd, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int cx = blockIdx.x * 64 + threadIdx.x;
+    if (cx >= Lc.nnx) return;
+    const int cy = blockIdx.y, cz = blockIdx.z;
+    const float inv = 1.0f / float(span);
+    const int cnx = Lc.nnx;
+    float wxs = 0.f;
+    for (int d = -(span - 1); d <= span - 1; ++d) {
+        const int fx = cx * span + d;
+        if (fx >= 0 && fx < Lf.nnx) wxs += 1.0f - float(d < 0 ? -d : d) * inv;
+    }
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, wyz = 0.f;
+    for (int dz = -(span - 1); dz <= span - 1; ++dz) {
+        const int fz = cz * span + dz;
+        if (fz < 0 || fz >= Lf.nnz) continue;
         const float wz = 1.0f - float(dz < 0 ? -dz : dz) * inv;
-        const int64_t zo = int64_t(fz0 + dz) * Lf.nny * Lf.nnx;
-        for (int iy = iy0; iy < iy1; ++iy) {
-            const int dy = iy - (span - 1);
-            const float wzy = wz * (1.0f - float(dy < 0 ? -dy : dy) * inv);
-            const int64_t yo = zo + int64_t(fy0 + dy) * Lf.nnx;
-            for (int ix = ix0; ix < ix1; ++ix) {
-                const int dx = ix - (span - 1);
-                const float w = wzy * (1.0f - float(dx < 0 ? -dx : dx) * inv);
-                const int64_t fi = 3 * (yo + (fx0 + dx));
-                v0 = fmaf(float(rf[fi]), w, v0);
-                v1 = fmaf(float(rf[fi + 1]), w, v1);
-                v2 = fmaf(float(rf[fi + 2]), w, v2);
-                ws += w;
-            }
+        for (int dy = -(span - 1); dy <= span - 1; ++dy) {
+            const int fy = cy * span + dy;
+            if (fy < 0 || fy >= Lf.nny) continue;
+            const float w = wz * (1.0f - float(dy < 0 ? -dy : dy) * inv);
+            const int64_t i = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
+            v0 = fmaf(w, t[i], v0);
+            v1 = fmaf(w, t[i + 1], v1);
+            v2 = fmaf(w, t[i + 2], v2);
+            wyz += w;
         }
     }
+    const float ws = wxs * wyz;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
     if (ws > 1e-12f) {
         const float iw = 1.0f / ws;
-        r0 = v0 * iw; r1 = v1 * iw; r2 = v2 * iw;
+        r0 = v0 * iw;
+        r1 = v1 * iw;
+        r2 = v2 * iw;
     }
+    const int64_t n = (int64_t(cz) * Lc.nny + cy) * Lc.nnx + cx;
     const unsigned m = cmask[n];
     if (m & 1u) r0 = 0.f;
     if (m & 2u) r1 = 0.f;
     if (m & 4u) r2 = 0.f;
-    rc[3 * n] = r0; rc[3 * n + 1] = r1; rc[3 * n + 2] = r2;
-    if (rcd) {   // FP64 copy for the coarsest Cholesky solve
-        rcd[3 * n] = r0; rcd[3 * n + 1] = r1; rcd[3 * n + 2] = r2;
+    rc[3 * n] = r0;
+    rc[3 * n + 1] = r1;
+    rc[3 * n + 2] = r2;
+    if (rcd) {
+        rcd[3 * n] = r0;
+        rcd[3 * n + 1] = r1;
+        rcd[3 * n + 2] = r2;
     }
     if (Dc) {
         xc[3 * n] = (m & 1u) ? 0.f : r0 / fmaxf(1.0e-30f, Dc[3 * n]);
@@ -283,63 +596,133 @@ __global__ void __launch_bounds__(256) k_restrict_f32(LevelF Lf, LevelF Lc, int 
     }
 }
 
-// x_f = mask ? 0 : 2t + P(x_c), t = w r/D ; at the finest level also the
-// partials of r.z and z.Ap (FP64)
-template <class TIn, bool FINEST>
-__global__ void __launch_bounds__(256) k_prolong_smooth_f32(LevelF Lf, LevelF Lc, int span,
-                                                            const float* __restrict__ xc, const TIn* __restrict__ rf,
-                                                            const float* __restrict__ Df,
-                                                            const uint8_t* __restrict__ fmask, float weight,
-                                                            float* __restrict__ xf, const float* __restrict__ Ap,
-                                                            double* __restrict__ partRZ, double* __restrict__ partZAp,
-                                                            const int* __restrict__ done)
-{
-    if (done && *done) return;
-    __shared__ double sh[4];
-    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    double srz = 0.0, szap = 0.0;
-    if (n < Lf.nN) {
-        const int fx = int(n % Lf.nnx), fy = int((n / Lf.nnx) % Lf.nny), fz = int(n / (int64_t(Lf.nnx) * Lf.nny));
-        int cx = fx / span, cy = fy / span, cz = fz / span;
-        int rx = fx % span, ry = fy % span, rz = fz % span;
+// trilinear prolongation of xc at fine node (fx, fy, fz), clamped at the
+// upper faces (transfers.cpp:378-445)
+struct Prolong {
+    int cx, cy, cz;
+    float wx0, wx1, wy0, wy1, wz0, wz1;
+    __device__ Prolong(const LevelF& Lc, int span, int fx, int fy, int fz)
+    {
+        cx = fx / span;
+        cy = fy / span;
+        cz = fz / span;
+        int rx = fx - cx * span, ry = fy - cy * span, rz = fz - cz * span;
         if (cx >= Lc.resX) { cx = Lc.resX - 1; rx = span; }
         if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
         if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
         const float inv = 1.0f / float(span);
-        const float wx1 = float(rx) * inv, wx0 = 1.f - wx1;
-        const float wy1 = float(ry) * inv, wy0 = 1.f - wy1;
-        const float wz1 = float(rz) * inv, wz0 = 1.f - wz1;
+        wx1 = float(rx) * inv; wx0 = 1.f - wx1;
+        wy1 = float(ry) * inv; wy0 = 1.f - wy1;
+        wz1 = float(rz) * inv; wz0 = 1.f - wz1;
+    }
+    __device__ void eval(const LevelF& Lc, const float* __restrict__ xc, float out[3]) const
+    {
         const int64_t s = int64_t(Lc.nnx) * Lc.nny;
         const int64_t b00 = s * cz + int64_t(Lc.nnx) * cy + cx;
         const int64_t b10 = b00 + Lc.nnx, n00 = b00 + s, n10 = n00 + Lc.nnx;
-        const unsigned m = fmask[n];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float lo = wy0 * (wx0 * xc[3 * b00 + c] + wx1 * xc[3 * b00 + 3 + c]) +
                              wy1 * (wx0 * xc[3 * b10 + c] + wx1 * xc[3 * b10 + 3 + c]);
             const float hi = wy0 * (wx0 * xc[3 * n00 + c] + wx1 * xc[3 * n00 + 3 + c]) +
                              wy1 * (wx0 * xc[3 * n10 + c] + wx1 * xc[3 * n10 + 3 + c]);
-            const float v = wz0 * lo + wz1 * hi;
-            float out = 0.f;
-            if (!((m >> c) & 1u)) {
-                const float t = weight * float(rf[3 * n + c]) / fmaxf(1.0e-30f, Df[3 * n + c]);
-                out = 2.f * t + v;
-            }
-            xf[3 * n + c] = out;
-            if (FINEST) {
-                srz = fma(double(rf[3 * n + c]), double(out), srz);
+            out[c] = wz0 * lo + wz1 * hi;
+        }
+    }
+};
+
+// coarse levels: x_f = mask ? 0 : 2t + P(x_c), t = w r/D  (transfers.cpp:378-445,
+// preconditioner.cpp:230-253)
+__global__ void __launch_bounds__(64) k_prolong_smooth32(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
+                                                         const float* __restrict__ rf, const float* __restrict__ Df,
+                                                         const uint8_t* __restrict__ fmask, float weight,
+                                                         float* __restrict__ xf, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int fx = blockIdx.x * 64 + threadIdx.x;
+    if (fx >= Lf.nnx) return;
+    const int fy = blockIdx.y, fz = blockIdx.z;
+    const int64_t n = (int64_t(fz) * Lf.nny + fy) * Lf.nnx + fx;
+    const Prolong P(Lc, span, fx, fy, fz);
+    float v[3];
+    P.eval(Lc, xc, v);
+    const unsigned m = fmask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float out = 0.f;
+        if (!((m >> c) & 1u)) {
+            const float t = weight * rf[3 * n + c] / fmaxf(1.0e-30f, Df[3 * n + c]);
+            out = 2.f * t + v[c];
+        }
+        xf[3 * n + c] = out;
+    }
+}
+
+// finest level: z = mask ? 0 : 2t + P(x_c) from the FP64 residual, with the
+// partials of r.z and z.Ap; the last block derives beta (or, on the first
+// V-cycle of a solve, only rz).  Bricks of 32 x 8 nodes x PZ planes.
+constexpr int PZ = 16;
+__global__ void __launch_bounds__(256) k_prolong_finest(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
+                                                        const double* __restrict__ rf, const float* __restrict__ Df,
+                                                        const uint8_t* __restrict__ fmask, float weight,
+                                                        float* __restrict__ xf, const float* __restrict__ Ap,
+                                                        double* __restrict__ partial, const int* __restrict__ done)
+{
+    if (*done) return;
+    __shared__ double sh[4];
+    const int fx = blockIdx.x * 32 + (threadIdx.x & 31);
+    const int fy = blockIdx.y * 8 + (threadIdx.x >> 5);
+    double srz = 0.0, szap = 0.0;
+    if (fx < Lf.nnx && fy < Lf.nny) {
+        const int zEnd = min(Lf.nnz, int(blockIdx.z + 1) * PZ);
+        for (int fz = blockIdx.z * PZ; fz < zEnd; ++fz) {
+            const int64_t n = (int64_t(fz) * Lf.nny + fy) * Lf.nnx + fx;
+            const Prolong P(Lc, span, fx, fy, fz);
+            float v[3];
+            P.eval(Lc, xc, v);
+            const unsigned m = fmask[n];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double r = rf[3 * n + c];
+                float out = 0.f;
+                if (!((m >> c) & 1u)) {
+                    const float t = weight * float(r) / fmaxf(1.0e-30f, Df[3 * n + c]);
+                    out = 2.f * t + v[c];
+                }
+                xf[3 * n + c] = out;
+                srz = fma(r, double(out), srz);
                 szap = fma(double(out), double(Ap[3 * n + c]), szap);
             }
         }
     }
-    if (FINEST) {
-        const double a = block_sum_256(srz, sh);
-        __syncthreads();
-        const double b = block_sum_256(szap, sh);
-        if (threadIdx.x == 0) {
-            partRZ[blockIdx.x] = a;
-            partZAp[blockIdx.x] = b;
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned bid = linear_block();
+    const double a = block_sum_256(srz, sh);
+    if (threadIdx.x == 0) partial[bid] = a;
+    const double b = block_sum_256(szap, sh);
+    if (threadIdx.x == 0) partial[nb + bid] = b;
+}
+
+// rz_new = r.z, zAp = z.Ap -> beta (0 on the first V-cycle of a solve); rz = rz_new
+__global__ void __launch_bounds__(1024) k_fin_beta(const double* __restrict__ partial, unsigned m,
+                                                   double* __restrict__ S, int flexible, int first,
+                                                   const int* __restrict__ done)
+{
+    if (*done) return;
+    __shared__ double sh[16];
+    const double rz = sum_partials_1024(partial, m, sh);
+    const double zAp = sum_partials_1024(partial + m, m, sh);
+    if (threadIdx.x == 0) {
+        if (first) {
+            S[S_BETA] = 0.0;
+        } else {
+            const double o = S[S_RZ];
+            const double num = flexible ? -S[S_ALPHA] * zAp : rz;
+            S[S_BETA] = num / (1.0e-30 < o ? o : 1.0e-30);
         }
+        S[S_RZ] = rz;
+        S[S_RZNEW] = rz;
+        S[S_ZAP] = zAp;
     }
 }
 
@@ -347,37 +730,6 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) b[i] = float(a[i]);
-}
-
-// flexible: beta = -alpha (z.Ap) / max(1e-30, rz_old) (Polak-Ribiere);
-// standard: beta = rz_new / max(1e-30, rz_old)
-__global__ void k_mx_beta(const double* __restrict__ alpha, const double* __restrict__ zAp,
-                          const double* __restrict__ rzNew, const double* __restrict__ rzOld, int flexible,
-                          double* __restrict__ beta, const int* __restrict__ done)
-{
-    if (*done) return;
-    const double o = *rzOld;
-    const double num = flexible ? -(*alpha) * (*zAp) : *rzNew;
-    *beta = num / (1.0e-30 < o ? o : 1.0e-30);
-}
-
-__global__ void k_mx_check(const double* __restrict__ rn2, const double* __restrict__ stop, int it,
-                           int* __restrict__ flags)
-{
-    if (flags[0]) return;
-    if (sqrt(*rn2) < *stop) {
-        flags[0] = 1;
-        flags[1] = it + 1;
-    }
-}
-
-__global__ void k_mx_set_int(int* __restrict__ p, int v) { *p = v; }
-__global__ void k_mx_set_double(double* __restrict__ p, double v) { *p = v; }
-
-__global__ void k_pcg_stop_mixed(const double* __restrict__ normb2, double tol, double* __restrict__ stop)
-{
-    const double nb = sqrt(*normb2);
-    *stop = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
 }
 
 }  // namespace t3d
@@ -388,19 +740,19 @@ using namespace t3d;
 struct t3d_mixed {
     t3d_problem* p = nullptr;
     t3d_mg* M = nullptr;
-    float* Ke32 = nullptr;
+    float* Ke32 = nullptr;     // element matrix (node-gather form)
+    float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
     float* D32[8] = {};
     float* r32[8] = {};
     float* x32[8] = {};
+    float* tmp = nullptr;      // restriction pass-1 buffer
     double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
     double* xc64 = nullptr;
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
-    double* part = nullptr;    // partial sums
-    double* part2 = nullptr;
+    double* part = nullptr;    // block partials
     double* sc = nullptr;      // scalars
-    int* iflags = nullptr;     // [0] done [1] iters [2] restart
+    int* iflags = nullptr;     // [0] done [1] iterations
     int hostFlags[4];
-    int64_t nTilesK = 0, nBlk = 0;
     int nlev = 0;
 };
 
@@ -433,12 +785,101 @@ static LevelF levf(const MGView& v, int l)
     return L;
 }
 
+namespace t3d {
+
+constexpr int KX_NZ = 8;
+constexpr int KW_NZ = 16;
+
+static dim3 kx_grid(const Grid& g, int nz)
+{
+    return dim3(unsigned((g.nnx + kx::TX - 1) / kx::TX), unsigned((g.nny + kx::TY - 1) / kx::TY),
+                unsigned((g.nnz + nz - 1) / nz));
+}
+
+static dim3 kw_grid(const Grid& g)
+{
+    return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + wk::NY - 1) / wk::NY),
+                unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
+}
+
+// y = K u in FP32; form 0: Walsh form, 1: node-gather form
+static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
+{
+    t3d_problem* p = s->p;
+    if (form == 0)
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
+                           nullptr, y, nullptr, nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((k_kx32<KX_NZ, false>), kx_grid(p->g, KX_NZ), dim3(256), 0, p->stream, p->g,
+                           (const float*)s->Ke32, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
+                           nullptr, y, nullptr, nullptr, nullptr);
+}
+
+
+static dim3 finest_grid(const LevelF& L)
+{
+    return dim3(unsigned((L.nnx + 31) / 32), unsigned((L.nny + 7) / 8), unsigned((L.nnz + PZ - 1) / PZ));
+}
+
+// z32 = M32(r); the finest prolongation derives rz and beta (see k_prolong_finest)
+static int mixed_vcycle(t3d_mixed* s, const MGView& v, const double* r, int flexible, int first, const int* done)
+{
+    t3d_problem* p = s->p;
+    hipStream_t st = p->stream;
+    const int L = v.nlev - 1;
+    const bool chol = v.Nc > 0;
+    T3D_CHECK(L >= 1, T3D_ERR_STATE, "mixed V-cycle needs at least two levels");
+    for (int l = 0; l < L; ++l) {
+        const bool last = (l + 1 == L);
+        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
+        const int span = v.span[l + 1];
+        const float* Dc = (last && !chol) ? s->D32[L] : nullptr;
+        double* rcd = (last && chol) ? s->rc64 : nullptr;
+        const dim3 g1(unsigned((Lc.nnx + 63) / 64), unsigned(Lf.nny), unsigned(Lf.nnz));
+        if (l == 0)
+            hipLaunchKernelGGL(k_restrict_x<double>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, r, s->tmp, done);
+        else
+            hipLaunchKernelGGL(k_restrict_x<float>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, (const float*)s->r32[l],
+                               s->tmp, done);
+        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
+        hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
+                           s->r32[l + 1], Dc, s->x32[L], rcd, done);
+        T3D_HIP(hipGetLastError());
+    }
+    if (chol) {
+        hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, st, v.Nc, v.L, v.LT, s->rc64, v.mask[L], s->xc64,
+                           done);
+        const int64_t nc = 3 * v.nN[L];
+        hipLaunchKernelGGL(k_d2f, dim3(unsigned((nc + 255) / 256)), dim3(256), 0, st, nc, s->xc64, s->x32[L]);
+        T3D_HIP(hipGetLastError());
+    }
+    const float w = float(v.weight);
+    for (int l = L - 1; l >= 1; --l) {
+        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
+        const dim3 gp(unsigned((Lf.nnx + 63) / 64), unsigned(Lf.nny), unsigned(Lf.nnz));
+        hipLaunchKernelGGL(k_prolong_smooth32, gp, dim3(64), 0, st, Lf, Lc, v.span[l + 1], (const float*)s->x32[l + 1],
+                           (const float*)s->r32[l], (const float*)s->D32[l], v.mask[l], w, s->x32[l], done);
+        T3D_HIP(hipGetLastError());
+    }
+    const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
+    const dim3 fg = finest_grid(L0);
+    hipLaunchKernelGGL(k_prolong_finest, fg, dim3(256), 0, st, L0, L1, v.span[1], (const float*)s->x32[1], r,
+                       (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap, s->part, done);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, fg.x * fg.y * fg.z, s->sc,
+                       flexible, first, done);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+}  // namespace t3d
+
 extern "C" {
 
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->part2, s->sc, s->iflags};
+    void* b[] = {s->Ke32, s->KW, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 8; ++l) {
@@ -457,6 +898,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     MGView v;
     int rc = mg_view(M, v);
     if (rc) return rc;
+    T3D_CHECK(v.nlev >= 2, T3D_ERR_STATE, "t3d_mixed_create: the hierarchy needs at least two levels");
     t3d_mixed* s = new t3d_mixed();
     s->p = p;
     s->M = M;
@@ -468,19 +910,64 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     };
     float K32[576];
     for (int i = 0; i < 576; ++i) K32[i] = float(p->host.Ke[i]);
+    // Walsh form: K^ = (H x I3) Ke (H x I3)^T; check its sparsity against the kernel's pattern
+    float KW[wk::NNZ];
+    {
+        static const int off[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0},
+                                      {0, 1, 1}, {1, 1, 1}, {1, 0, 1}, {0, 0, 1}};   // o(b), fea.cpp node order
+        double Hm[8][8];
+        for (int m = 0; m < 8; ++m)
+            for (int b = 0; b < 8; ++b) {
+                const int dotp = (m & 1) * off[b][0] + ((m >> 1) & 1) * off[b][1] + ((m >> 2) & 1) * off[b][2];
+                Hm[m][b] = (dotp & 1) ? -1.0 : 1.0;
+            }
+        static double Kh[24][24];
+        double mx = 0.0;
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double acc = 0.0;
+                for (int b = 0; b < 8; ++b)
+                    for (int bb = 0; bb < 8; ++bb)
+                        acc += Hm[i / 3][b] * p->host.Ke[(3 * b + i % 3) * 24 + 3 * bb + j % 3] * Hm[j / 3][bb];
+                Kh[i][j] = acc;
+                mx = std::max(mx, std::fabs(acc));
+            }
+        const unsigned char outIdx[wk::NNZ] = {3,  3,  3,  4,  4,  5,  5,  6,  6,  7,  7,  7,  8,  8,  9,
+                                               9,  10, 10, 11, 11, 11, 12, 12, 13, 13, 14, 14, 14, 15, 15,
+                                               16, 16, 16, 17, 17, 18, 18, 18, 19, 19, 20, 20, 21, 22, 23};
+        const unsigned char inIdx[wk::NNZ] = {3,  7,  14, 4,  6,  5,  12, 4,  6,  3,  7,  14, 8,  13, 9,
+                                              20, 10, 17, 11, 16, 18, 5,  12, 8,  13, 3,  7,  14, 15, 19,
+                                              11, 16, 18, 10, 17, 11, 16, 18, 15, 19, 9,  20, 21, 22, 23};
+        bool inPat[24][24] = {};
+        for (int k = 0; k < wk::NNZ; ++k) {
+            inPat[outIdx[k]][inIdx[k]] = true;
+            KW[k] = float(Kh[outIdx[k]][inIdx[k]] / 64.0);
+        }
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j)
+                T3D_CHECK(inPat[i][j] || std::fabs(Kh[i][j]) <= 1e-12 * mx, T3D_ERR_STATE,
+                          "t3d_mixed_create: element matrix does not have the Walsh-form structure");
+    }
     const int64_t nD = 3 * p->g.nN;
-    const Grid& g = p->g;
-    s->nTilesK = int64_t((g.nnx + mx::TX - 1) / mx::TX) * ((g.nny + mx::TY - 1) / mx::TY) *
-                 ((g.nnz + mx::TZ - 1) / mx::TZ);
-    s->nBlk = (nD + 1023) / 1024;
-    const int64_t npart = std::max<int64_t>(std::max<int64_t>(s->nTilesK, s->nBlk), (g.nN + 255) / 256) + 16;
+    const dim3 kg = kw_grid(p->g);
+    const LevelF L0 = levf(v, 0);
+    const dim3 fg = finest_grid(L0);
+    const int64_t npart = std::max<int64_t>({int64_t(kg.x) * kg.y * kg.z, 2 * int64_t(fg.x) * fg.y * fg.z, VB}) + 16;
+    int64_t ntmp = 0;
+    for (int l = 0; l + 1 < v.nlev; ++l) {
+        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
+        ntmp = std::max<int64_t>(ntmp, 3 * int64_t(Lc.nnx) * Lf.nny * Lf.nnz);
+    }
     if (hipMalloc(&s->Ke32, sizeof(float) * 576) != hipSuccess ||
         hipMemcpy(s->Ke32, K32, sizeof(float) * 576, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->KW, sizeof(float) * wk::NNZ) != hipSuccess ||
+        hipMemcpy(s->KW, KW, sizeof(float) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->pA, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->pB, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->Ap, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->z, sizeof(float) * nD) != hipSuccess ||
+        hipMalloc(&s->tmp, sizeof(float) * ntmp) != hipSuccess ||
         hipMalloc(&s->part, sizeof(double) * npart) != hipSuccess ||
-        hipMalloc(&s->part2, sizeof(double) * npart) != hipSuccess ||
-        hipMalloc(&s->sc, sizeof(double) * 64) != hipSuccess || hipMalloc(&s->iflags, sizeof(int) * 4) != hipSuccess)
+        hipMalloc(&s->sc, sizeof(double) * 64) != hipSuccess || hipMalloc(&s->iflags, sizeof(int) * 4) != hipSuccess ||
+        hipMemset(s->iflags, 0, sizeof(int) * 4) != hipSuccess)
         return fail();
     for (int l = 0; l < v.nlev; ++l) {
         if (hipMalloc(&s->D32[l], sizeof(float) * 3 * v.nN[l]) != hipSuccess) return fail();
@@ -511,76 +998,39 @@ int t3d_mixed_setup(t3d_mixed* s)
     return T3D_OK;
 }
 
-}  // extern "C"
-
-namespace t3d {
-
-// z32 = M32(r), with r.z and z.Ap partials at the finest level
-static int mixed_vcycle(t3d_mixed* s, const MGView& v, const double* r, const float* Ap, double* rzOut,
-                        double* zApOut, const int* done)
+// y = K u in FP32 (full layout, fixed DOFs masked), the modulus last set with
+// t3d_set_modulus; the single-precision counterpart of t3d_kxu_f64.
+// form 0: Walsh form (used by the solver), 1: node-gather form
+int t3d_kxu_f32(t3d_mixed* s, int form, const float* u, float* y)
 {
-    t3d_problem* p = s->p;
-    hipStream_t st = p->stream;
-    const int L = v.nlev - 1;
-    const int B = 256;
-    const bool chol = v.Nc > 0;
-    T3D_CHECK(L >= 1, T3D_ERR_STATE, "mixed V-cycle needs at least two levels");
-    for (int l = 0; l < L; ++l) {
-        const bool last = (l + 1 == L);
-        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
-        const unsigned nb = unsigned((Lc.nN + B - 1) / B);
-        const float* Dc = (last && !chol) ? s->D32[L] : nullptr;
-        double* rcd = (last && chol) ? s->rc64 : nullptr;
-        if (l == 0)
-            hipLaunchKernelGGL(k_restrict_f32<double>, dim3(nb), dim3(B), 0, st, Lf, Lc, v.span[1], r, v.mask[1],
-                               s->r32[1], Dc, s->x32[L], rcd, done);
-        else
-            hipLaunchKernelGGL(k_restrict_f32<float>, dim3(nb), dim3(B), 0, st, Lf, Lc, v.span[l + 1],
-                               (const float*)s->r32[l], v.mask[l + 1], s->r32[l + 1], Dc, s->x32[L], rcd, done);
-        T3D_HIP(hipGetLastError());
-    }
-    if (chol) {
-        hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, st, v.Nc, v.L, v.LT, s->rc64, v.mask[L], s->xc64,
-                           done);
-        const int64_t nc = 3 * v.nN[L];
-        hipLaunchKernelGGL(k_d2f, dim3(unsigned((nc + 255) / 256)), dim3(256), 0, st, nc, s->xc64, s->x32[L]);
-        T3D_HIP(hipGetLastError());
-    }
-    const float w = float(v.weight);
-    for (int l = L - 1; l >= 0; --l) {
-        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
-        const unsigned nb = unsigned((Lf.nN + B - 1) / B);
-        if (l == 0) {
-            hipLaunchKernelGGL((k_prolong_smooth_f32<double, true>), dim3(nb), dim3(B), 0, st, Lf, Lc, v.span[1],
-                               s->x32[1], r, s->D32[0], v.mask[0], w, s->z, Ap, s->part, s->part2, done);
-            T3D_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nb), s->part, rzOut, done);
-            hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nb), s->part2, zApOut, done);
-        } else {
-            hipLaunchKernelGGL((k_prolong_smooth_f32<float, false>), dim3(nb), dim3(B), 0, st, Lf, Lc,
-                               v.span[l + 1], s->x32[l + 1], (const float*)s->r32[l], s->D32[l], v.mask[l], w,
-                               s->x32[l], nullptr, nullptr, nullptr, done);
-        }
-        T3D_HIP(hipGetLastError());
-    }
-    return T3D_OK;
-}
-
-// end of an iteration: z = M r (rz_new -> S[9], z.Ap -> S[4]); beta -> S[7]; rz = rz_new
-static int mixed_tail(t3d_mixed* s, const MGView& v, const double* r, double* S, int flexible, const int* flags)
-{
-    hipStream_t st = s->p->stream;
-    int rc = mixed_vcycle(s, v, r, s->Ap, S + 9, S + 4, flags);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_mx_beta, dim3(1), dim3(1), 0, st, S + 6, S + 4, S + 9, S + 2, flexible, S + 7, flags);
-    hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, st, S + 9, S + 2, flags);
+    T3D_CHECK(s && u && y && (form == 0 || form == 1), T3D_ERR_ARG, "t3d_kxu_f32: bad argument");
+    launch_kx32(s, form, u, y);
     T3D_HIP(hipGetLastError());
+    T3D_HIP(hipStreamSynchronize(s->p->stream));
     return T3D_OK;
 }
 
-}  // namespace t3d
-
-extern "C" {
+// HIP-event timing of the FP32 operator of the given form (op 0 Walsh, 1 node
+// gather) on the problem's stream: average milliseconds per launch
+int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
+{
+    T3D_CHECK(s && u && y && reps > 0 && ms && (op == 0 || op == 1), T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    t3d_problem* p = s->p;
+    hipEvent_t e0, e1;
+    T3D_HIP(hipEventCreate(&e0));
+    T3D_HIP(hipEventCreate(&e1));
+    T3D_HIP(hipEventRecord(e0, p->stream));
+    for (int i = 0; i < reps; ++i) launch_kx32(s, op, u, y);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipEventRecord(e1, p->stream));
+    T3D_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    T3D_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / float(reps);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return T3D_OK;
+}
 
 // Mixed-precision MG-PCG on the full layout (b, x FP64, device).  The problem's
 // modulus and the multigrid object (after t3d_mg_setup + t3d_mixed_setup) are
@@ -596,9 +1046,6 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     hipStream_t st = p->stream;
     const Grid& g = p->g;
     const int64_t n = 3 * g.nN;
-    const unsigned nblk = unsigned((n + 1023) / 1024);
-    double* r = nullptr;
-    double* Ap64 = nullptr;
     // FP64 work vectors of the problem (shared with the plain PCG)
     if (!p->wr) {
         T3D_HIP(hipMalloc(&p->wr, sizeof(double) * n));
@@ -608,25 +1055,21 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         T3D_HIP(hipMalloc(&p->pcgFlags, sizeof(int) * 4));
         T3D_HIP(hipHostMalloc(&p->hostFlags, sizeof(int) * 4, hipHostMallocDefault));
     }
-    r = p->wr;
-    Ap64 = p->wAp;
-    double* S = s->sc;   // [0] normb2 [1] stop [2] rz [3] rzNew [4] zAp [5] denom [6] alpha [7] beta [8] rn2
+    double* r = p->wr;
+    double* Ap64 = p->wAp;
+    double* S = s->sc;
     int* flags = s->iflags;
-    dim3 kgrid(unsigned((g.nnx + mx::TX - 1) / mx::TX), unsigned((g.nny + mx::TY - 1) / mx::TY),
-               unsigned((g.nnz + mx::TZ - 1) / mx::TZ));
-    const int64_t ntk = s->nTilesK;
-    // r = b - K64 x ; rn2 ; z = M r ; rz
+    const dim3 kg = kw_grid(g);
+    // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
+    if ((rc = canon_dot_dev(p, b, b, n, S + S_NORMB2))) return rc;
+    hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(1), 0, st, tol, S, flags);
     if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
-    hipLaunchKernelGGL(k_mx_set_int, dim3(1), dim3(1), 0, st, flags, 0);
-    hipLaunchKernelGGL(k_mx_set_int, dim3(1), dim3(1), 0, st, flags + 1, -1);
-    hipLaunchKernelGGL(k_mx_true_residual, dim3(nblk), dim3(256), 0, st, n, b, Ap64, r, s->part, flags);
-    hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nblk), s->part, S + 8, flags);
+    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
+                       nullptr);
+    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, -1, 0, 0);
     T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * n, st));
-    if ((rc = mixed_vcycle(s, v, r, s->Ap, S + 2, S + 4, flags))) return rc;
-    if ((rc = canon_dot_dev(p, b, b, n, S + 0))) return rc;
-    hipLaunchKernelGGL(k_pcg_stop_mixed, dim3(1), dim3(1), 0, st, S + 0, tol, S + 1);
     T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * n, st));
-    hipLaunchKernelGGL(k_mx_set_double, dim3(1), dim3(1), 0, st, S + 7, 0.0);   // beta_0 = 0
+    if ((rc = mixed_vcycle(s, v, r, flexible, 1, flags))) return rc;
     // iteration it reads p_prev and writes p_cur; the two buffers alternate
     auto pPrevOf = [&](int it) { return (it & 1) ? s->pB : s->pA; };
     auto pCurOf = [&](int it) { return (it & 1) ? s->pA : s->pB; };
@@ -636,23 +1079,28 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     while (it0 < maxIt) {
         const int it1 = std::min(maxIt, it0 + CHUNK);
         for (int it = it0; it < it1; ++it) {
-            // p_cur = z + beta p_prev ; Ap = K32 p_cur ; denom = p_cur.Ap
-            hipLaunchKernelGGL(k_kxu_f32_fused, kgrid, dim3(512), 0, st, g, s->Ke32, p->EcellF, p->nodeFixed,
-                               s->z, pPrevOf(it), S + 7, pCurOf(it), s->Ap, s->part, flags);
-            hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, ntk, s->part, S + 5, flags);
-            // alpha = rz / denom ; x += alpha p ; r -= alpha Ap ; rn2
-            hipLaunchKernelGGL(k_mx_update_xr, dim3(nblk), dim3(256), 0, st, n, S + 2, S + 5, pCurOf(it), s->Ap, x,
-                               r, S + 6, s->part, flags);
-            hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nblk), s->part, S + 8, flags);
-            if (trueEvery > 0 && ((it + 1) % trueEvery) == 0) {
+            // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
+            hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kg, dim3(256), 0, st, g, (const float*)s->KW,
+                               (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, (const float*)pPrevOf(it),
+                               (const float*)s->z, pCurOf(it), s->Ap, s->part, (const double*)S, (const int*)flags);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, kg.x * kg.y * kg.z, S,
+                               (const int*)flags);
+            const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
+            // x += alpha p ; r -= alpha Ap ; rn2 ; convergence test
+            hipLaunchKernelGGL(k_mx_update_xr, dim3(VB), dim3(256), 0, st, n, (const float*)pCurOf(it),
+                               (const float*)s->Ap, x, r, s->part, (const double*)S, (const int*)flags);
+            hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, it,
+                               replace ? 0 : 1, 1);
+            if (replace) {
                 // residual replacement with the FP64 operator (the direction is kept)
                 if ((rc = launch_kxu_f64(p, x, Ap64, flags))) return rc;
-                hipLaunchKernelGGL(k_mx_true_residual, dim3(nblk), dim3(256), 0, st, n, b, Ap64, r, s->part, flags);
-                hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nblk), s->part, S + 8,
-                                   flags);
+                hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r,
+                                   s->part, (const int*)flags);
+                hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
+                                   it, 1, 1);
             }
-            hipLaunchKernelGGL(k_mx_check, dim3(1), dim3(1), 0, st, S + 8, S + 1, it, flags);
-            if ((rc = mixed_tail(s, v, r, S, flexible, flags))) return rc;
+            // z = M r ; beta ; rz
+            if ((rc = mixed_vcycle(s, v, r, flexible, 0, flags))) return rc;
         }
         T3D_HIP(hipMemcpyAsync(s->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
@@ -660,23 +1108,24 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             it0 = it1;
             continue;
         }
-        // the recursive residual passed the test at iteration itDone: confirm
-        // with the true residual b - K x (FP64); otherwise continue from it
+        // the residual passed the test at iteration itDone: confirm with the
+        // true residual b - K x (FP64); otherwise continue from there
         const int itDone = s->hostFlags[1] - 1;
         if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
-        hipLaunchKernelGGL(k_mx_true_residual, dim3(nblk), dim3(256), 0, st, n, b, Ap64, r, s->part, nullptr);
-        hipLaunchKernelGGL(k_mx_sum_partials, dim3(1), dim3(256), 0, st, int64_t(nblk), s->part, S + 8, nullptr);
+        hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
+                           nullptr);
+        hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
+                           itDone, 0, 0);
         double hs[2];
-        T3D_HIP(hipMemcpyAsync(hs, S + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-        T3D_HIP(hipMemcpyAsync(hs + 1, S + 8, sizeof(double), hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipMemcpyAsync(hs, S + S_STOP, sizeof(double), hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipMemcpyAsync(hs + 1, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
         if (std::sqrt(hs[1]) < hs[0]) {
             result = itDone + 1;
             break;
         }
-        hipLaunchKernelGGL(k_mx_set_int, dim3(1), dim3(1), 0, st, flags, 0);
-        hipLaunchKernelGGL(k_mx_set_int, dim3(1), dim3(1), 0, st, flags + 1, -1);
-        if ((rc = mixed_tail(s, v, r, S, flexible, flags))) return rc;
+        hipLaunchKernelGGL(k_mx_clear_done, dim3(1), dim3(1), 0, st, flags);
+        if ((rc = mixed_vcycle(s, v, r, flexible, 0, flags))) return rc;
         it0 = itDone + 1;
     }
     T3D_HIP(hipStreamSynchronize(st));

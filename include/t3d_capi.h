@@ -130,9 +130,19 @@ int t3d_mixed_destroy(t3d_mixed* s);
 /* refresh the FP32 copies of the multigrid diagonals after t3d_mg_setup */
 int t3d_mixed_setup(t3d_mixed* s);
 /* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
- * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old */
+ * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old.
+ * Convergence is reported only when the true residual ||b - K x|| (FP64
+ * operator) is below tol * max(1e-30, ||b||). */
 int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, double tol, int maxIt, int trueEvery,
                   int flexible, int* iters);
+/* y = K u in FP32 on the full layout (K_times_u_finest, cpp/src/fea.cpp:763-837,
+ * in single precision with FMA; fixed DOFs of y are 0).  form 0: Walsh form
+ * (45-nonzero element matrix in the voxel's Walsh-Hadamard basis; used by the
+ * solver), form 1: node gather with the full 24x24 element matrix */
+int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_dev);
+/* average milliseconds per launch of the FP32 operator of form op over reps
+ * launches, HIP events on the problem's stream */
+int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 
 /* ---- compliance ----
  * ComputeCompliance (cpp/include/fea.hpp:47, cpp/src/fea.cpp:587-677,851-864):
