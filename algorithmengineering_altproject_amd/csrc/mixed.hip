@@ -8,7 +8,10 @@
 //     p = z + beta*p and the dot p.Ap are fused into the operator kernel;
 //   * the V-cycle (restriction, Jacobi smoothing, prolongation) in FP32 with
 //     FP32 diagonals; the coarsest Cholesky solve stays FP64;
-//   * FP64 x, r, dot products, alpha and beta; beta either the reference's
+//   * the recursive residual r in FP32 and the x updates accumulated in an
+//     FP32 increment dx that is added into the FP64 x at every residual
+//     replacement and at the end;
+//   * FP64 dot products, alpha and beta; beta either the reference's
 //     (r.z)_new/(r.z)_old or the flexible (Polak-Ribiere) form
 //     -alpha (z_new.Ap)/(r.z)_old, which tolerates the FP32 preconditioner
 //     being only approximately linear;
@@ -286,7 +289,64 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
     const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
-    // stage node plane Z into buffer q (rows of PX nodes are 3*PX contiguous floats)
+    // Node planes are staged in LDS; rows of PX nodes are 3*PX contiguous
+    // floats in memory.  The plain operator fetches a plane into registers one
+    // step ahead (fetch) and writes it to LDS after the arithmetic (commit), so
+    // the loads are in flight meanwhile; the fused form (two inputs) stages in
+    // place (stage), which keeps its register count at 6 waves per SIMD.
+    constexpr bool AHEAD = !FUSED;
+    constexpr int SPT = (3 * PN + CX * CY - 1) / (CX * CY);
+    float fu[SPT], fz[SPT];
+    auto fetch = [&](int Z) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const int f = tid + j * CX * CY;
+            const int row = f / (3 * PX);
+            const int k = f - row * (3 * PX);
+            const int lx = k / 3, c = k - 3 * lx;
+            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+            fu[j] = 0.f;
+            fz[j] = 0.f;
+            if (f < 3 * PN && zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
+                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
+                fu[j] = u[gi];
+                if (FUSED) fz[j] = z[gi];
+            }
+        }
+    };
+    auto commit = [&](int Z, int q) {
+        const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const int f = tid + j * CX * CY;
+            if (f >= 3 * PN) continue;
+            const int row = f / (3 * PX);
+            const int k = f - row * (3 * PX);
+            const int lx = k / 3, c = k - 3 * lx;
+            float v = fu[j];
+            if (FUSED) {
+                v = fmaf(beta, v, fz[j]);
+                const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+                if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY && X < g.nnx && Y < g.nny)
+                    pcur[3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c] = v;
+            }
+            sP[q][c][row * PX + lx] = v;
+        }
+    };
+    // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
+    auto face = [&](int q, float F[3][2][2]) {
+        const int b = ty * PX + tx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            walsh2(sP[q][c][b], sP[q][c][b + 1], sP[q][c][b + PX], sP[q][c][b + PX + 1], F[c]);
+    };
+    float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
+    // direct staging (fused form): load, combine and store in one pass
     auto stage = [&](int Z, int q) {
         const bool zin = Z >= 0 && Z < g.nnz;
         const bool own = Z >= z0 && Z < z0 + NZ;
@@ -307,35 +367,36 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
             sP[q][c][row * PX + lx] = v;
         }
     };
-    // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
-    auto face = [&](int q, float F[3][2][2]) {
-        const int b = ty * PX + tx;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            walsh2(sP[q][c][b], sP[q][c][b + 1], sP[q][c][b + PX], sP[q][c][b + PX + 1], F[c]);
+    auto modulus = [&](int cz) {
+        float E = 0.f;
+        if (colIn && cz >= 0 && cz < g.resZ) {
+            E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
+            if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
+        }
+        return E;
     };
-    float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
-    stage(z0 - 1, 0);
+    if (AHEAD) {
+        fetch(z0 - 1);
+        commit(z0 - 1, 0);
+        fetch(z0);
+        commit(z0, 1);
+    } else {
+        stage(z0 - 1, 0);
+        stage(z0, 1);
+    }
+    float Enext = modulus(z0 - 1);
     __syncthreads();
     face(0, Fp);
     double dot = 0.0;
     const int nsteps = min(NZ, g.nnz - z0) + 1;
     for (int s = 0; s < nsteps; ++s) {
         const int qn = (s + 1) % 3, qo = s % 3;
-        stage(z0 + s, qn);
-        __syncthreads();
-        face(qn, Fn);
-        // voxel (cx, cy, cz), cz = z0 - 1 + s
+        const bool more = s + 1 < nsteps;
+        if (AHEAD && more) fetch(z0 + s + 1);   // plane of the next step, in flight during this one
+        const float E = Enext;         // voxel (cx, cy, cz), cz = z0 - 1 + s
         const int cz = z0 - 1 + s;
-        float E = 0.f;
-        if (colIn && cz >= 0 && cz < g.resZ) {
-            E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
-            if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
-        }
+        if (more) Enext = modulus(cz + 1);
+        face(qn, Fn);
         float uh[24], vh[24];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -366,7 +427,7 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
-        if (s == 0) continue;   // plane z0 - 1 belongs to the block below
+        if (s > 0) {   // plane z0 - 1 belongs to the block below
         // plane Z = cz: corner forces of this voxel column, then the node gather
         const int Z = cz;
 #pragma unroll
@@ -401,6 +462,14 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
                 if (FUSED) dot = fma(double(sP[qo][c][(ty + 1) * PX + tx + 1]), double(v), dot);
             }
         }
+        }
+        if (more) {
+            if (AHEAD)
+                commit(z0 + s + 1, (s + 2) % 3);
+            else
+                stage(z0 + s + 1, (s + 2) % 3);
+        }
+        __syncthreads();
     }
     if (FUSED) {
         const double t = block_sum_256(dot, sh);
@@ -443,29 +512,9 @@ __global__ void __launch_bounds__(1024) k_fin_rn2(const double* __restrict__ par
     }
 }
 
-// x += alpha p ; r -= alpha Ap ; rn2 = r.r (+ convergence test)
-__global__ void __launch_bounds__(256) k_mx_update_xr(int64_t n, const float* __restrict__ p,
-                                                      const float* __restrict__ Ap, double* __restrict__ x,
-                                                      double* __restrict__ r, double* __restrict__ partial,
-                                                      const double* __restrict__ S, const int* __restrict__ done)
-{
-    if (*done) return;
-    __shared__ double sh[4];
-    const double alpha = S[S_ALPHA];
-    double s = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
-        x[i] = fma(alpha, double(p[i]), x[i]);
-        const double v = fma(-alpha, double(Ap[i]), r[i]);
-        r[i] = v;
-        s = fma(v, v, s);
-    }
-    const double t = block_sum_256(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
-// r = b - Ap (FP64) ; rn2 (+ convergence test)
+// r = b - Ap (FP64 arithmetic, stored in FP32) ; partial of the FP64 r.r
 __global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const double* __restrict__ b,
-                                                          const double* __restrict__ Ap, double* __restrict__ r,
+                                                          const double* __restrict__ Ap, float* __restrict__ r,
                                                           double* __restrict__ partial, const int* __restrict__ done)
 {
     if (done && *done) return;
@@ -473,11 +522,22 @@ __global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const doubl
     double s = 0.0;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
         const double v = b[i] - Ap[i];
-        r[i] = v;
+        r[i] = float(v);
         s = fma(v, v, s);
     }
     const double t = block_sum_256(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// x += dx ; dx = 0
+__global__ void __launch_bounds__(256) k_mx_fold(int64_t n, double* __restrict__ x, float* __restrict__ dx,
+                                                 const int* __restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        x[i] += double(dx[i]);
+        dx[i] = 0.f;
+    }
 }
 
 __global__ void k_mx_init(double tol, double* __restrict__ S, int* __restrict__ flags)
@@ -590,9 +650,9 @@ __global__ void __launch_bounds__(64) k_restrict_yz(LevelF Lf, LevelF Lc, int sp
         rcd[3 * n + 2] = r2;
     }
     if (Dc) {
-        xc[3 * n] = (m & 1u) ? 0.f : r0 / fmaxf(1.0e-30f, Dc[3 * n]);
-        xc[3 * n + 1] = (m & 2u) ? 0.f : r1 / fmaxf(1.0e-30f, Dc[3 * n + 1]);
-        xc[3 * n + 2] = (m & 4u) ? 0.f : r2 / fmaxf(1.0e-30f, Dc[3 * n + 2]);
+        xc[3 * n] = (m & 1u) ? 0.f : r0 * Dc[3 * n];
+        xc[3 * n + 1] = (m & 2u) ? 0.f : r1 * Dc[3 * n + 1];
+        xc[3 * n + 2] = (m & 4u) ? 0.f : r2 * Dc[3 * n + 2];
     }
 }
 
@@ -651,56 +711,170 @@ __global__ void __launch_bounds__(64) k_prolong_smooth32(LevelF Lf, LevelF Lc, i
     for (int c = 0; c < 3; ++c) {
         float out = 0.f;
         if (!((m >> c) & 1u)) {
-            const float t = weight * rf[3 * n + c] / fmaxf(1.0e-30f, Df[3 * n + c]);
+            const float t = weight * rf[3 * n + c] * Df[3 * n + c];
             out = 2.f * t + v[c];
         }
         xf[3 * n + c] = out;
     }
 }
 
-// finest level: z = mask ? 0 : 2t + P(x_c) from the FP64 residual, with the
-// partials of r.z and z.Ap; the last block derives beta (or, on the first
-// V-cycle of a solve, only rz).  Bricks of 32 x 8 nodes x PZ planes.
-constexpr int PZ = 16;
-__global__ void __launch_bounds__(256) k_prolong_finest(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
-                                                        const double* __restrict__ rf, const float* __restrict__ Df,
-                                                        const uint8_t* __restrict__ fmask, float weight,
-                                                        float* __restrict__ xf, const float* __restrict__ Ap,
-                                                        double* __restrict__ partial, const int* __restrict__ done)
+// ---------------------------------------------------------------------------
+// Finest-level row kernels.  A node row (fixed fy, fz) is 3*nnx contiguous
+// values; a block takes RPB consecutive rows, so every load and store is a
+// contiguous run (one value per lane), and the x-direction work of the
+// transfers is done from LDS.
+// ---------------------------------------------------------------------------
+constexpr int RPB = 4;
+
+// x-pass of the restriction for the block's rows staged in sR (FP32):
+// t[row][cx][c] = sum_d W(d) sR[row][cx*span + d][c]
+__device__ __forceinline__ void restrict_x_rows(const float* sR, int L, int nnx, int nr, int64_t row0, int cnx,
+                                                int span, float* __restrict__ t)
+{
+    const int Lc = 3 * cnx;
+    const float inv = 1.0f / float(span);
+    for (int rr = 0; rr < nr; ++rr)
+        for (int k = threadIdx.x; k < Lc; k += 256) {
+            const int cxx = k / 3, c = k - 3 * cxx;
+            float acc = 0.f;
+            for (int d = -(span - 1); d <= span - 1; ++d) {
+                const int fx = cxx * span + d;
+                if (fx < 0 || fx >= nnx) continue;
+                acc = fmaf(1.0f - float(d < 0 ? -d : d) * inv, sR[rr * L + 3 * fx + c], acc);
+            }
+            t[(row0 + rr) * Lc + k] = acc;
+        }
+}
+
+// dx += alpha p ; r -= alpha Ap (FP32 storage, FP64 alpha) ; partial r.r ;
+// optionally (t != nullptr) the x-pass of the restriction of the new r
+__global__ void __launch_bounds__(256) k_xr_rows(int nnx, int NR, const float* __restrict__ p,
+                                                 const float* __restrict__ Ap, float* __restrict__ dx,
+                                                 float* __restrict__ r, double* __restrict__ partial,
+                                                 const double* __restrict__ S, int cnx, int span,
+                                                 float* __restrict__ t, const int* __restrict__ done)
 {
     if (*done) return;
+    extern __shared__ float sR[];
     __shared__ double sh[4];
-    const int fx = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int fy = blockIdx.y * 8 + (threadIdx.x >> 5);
-    double srz = 0.0, szap = 0.0;
-    if (fx < Lf.nnx && fy < Lf.nny) {
-        const int zEnd = min(Lf.nnz, int(blockIdx.z + 1) * PZ);
-        for (int fz = blockIdx.z * PZ; fz < zEnd; ++fz) {
-            const int64_t n = (int64_t(fz) * Lf.nny + fy) * Lf.nnx + fx;
-            const Prolong P(Lc, span, fx, fy, fz);
-            float v[3];
-            P.eval(Lc, xc, v);
-            const unsigned m = fmask[n];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double r = rf[3 * n + c];
-                float out = 0.f;
-                if (!((m >> c) & 1u)) {
-                    const float t = weight * float(r) / fmaxf(1.0e-30f, Df[3 * n + c]);
-                    out = 2.f * t + v[c];
-                }
-                xf[3 * n + c] = out;
-                srz = fma(r, double(out), srz);
-                szap = fma(double(out), double(Ap[3 * n + c]), szap);
-            }
+    const int L = 3 * nnx;
+    const int64_t row0 = int64_t(blockIdx.x) * RPB;
+    const int nr = min(RPB, int(NR - row0));
+    const double alpha = S[S_ALPHA];
+    const float alphaf = float(alpha);
+    double s = 0.0;
+    // the block's rows are one contiguous range starting at an even index
+    // (RPB is even), so float2 accesses are aligned
+    const int64_t base = row0 * L;
+    const int ne = nr * L;
+    for (int e = 2 * threadIdx.x; e < ne; e += 512) {
+        const int64_t i = base + e;
+        if (e + 1 < ne) {
+            const float2 pp = *reinterpret_cast<const float2*>(p + i);
+            const float2 aa = *reinterpret_cast<const float2*>(Ap + i);
+            float2 xx = *reinterpret_cast<const float2*>(dx + i);
+            float2 rv = *reinterpret_cast<const float2*>(r + i);
+            xx.x = fmaf(alphaf, pp.x, xx.x);
+            xx.y = fmaf(alphaf, pp.y, xx.y);
+            const double v0 = fma(-alpha, double(aa.x), double(rv.x));
+            const double v1 = fma(-alpha, double(aa.y), double(rv.y));
+            rv.x = float(v0);
+            rv.y = float(v1);
+            *reinterpret_cast<float2*>(dx + i) = xx;
+            *reinterpret_cast<float2*>(r + i) = rv;
+            s = fma(v0, v0, s);
+            s = fma(v1, v1, s);
+            sR[e] = rv.x;
+            sR[e + 1] = rv.y;
+        } else {
+            dx[i] = fmaf(alphaf, p[i], dx[i]);
+            const double v = fma(-alpha, double(Ap[i]), double(r[i]));
+            r[i] = float(v);
+            s = fma(v, v, s);
+            sR[e] = float(v);
         }
     }
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned bid = linear_block();
+    const double tot = block_sum_256(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    if (t) {
+        __syncthreads();
+        restrict_x_rows(sR, L, nnx, nr, row0, cnx, span, t);
+    }
+}
+
+// x-pass of the restriction of the finest-level residual (first V-cycle of a
+// solve and after a residual replacement)
+__global__ void __launch_bounds__(256) k_rx_rows(int nnx, int NR, const float* __restrict__ r, int cnx, int span,
+                                                 float* __restrict__ t, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    extern __shared__ float sR[];
+    const int L = 3 * nnx;
+    const int64_t row0 = int64_t(blockIdx.x) * RPB;
+    const int nr = min(RPB, int(NR - row0));
+    for (int rr = 0; rr < nr; ++rr)
+        for (int k = threadIdx.x; k < L; k += 256) sR[rr * L + k] = r[(row0 + rr) * L + k];
+    __syncthreads();
+    restrict_x_rows(sR, L, nnx, nr, row0, cnx, span, t);
+}
+
+// finest level: z = mask ? 0 : 2t + P(x_c), t = w r/D,
+// with the partials of r.z and z.Ap (two per block).  The y/z part of the
+// trilinear prolongation is evaluated once per row and coarse x in LDS.
+__global__ void __launch_bounds__(256) k_pro_rows(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
+                                                  const float* __restrict__ rf, const float* __restrict__ Df,
+                                                  const uint8_t* __restrict__ fmask, float weight,
+                                                  float* __restrict__ xf, const float* __restrict__ Ap,
+                                                  double* __restrict__ partial, const int* __restrict__ done)
+{
+    if (*done) return;
+    extern __shared__ float sQ[];   // [RPB][3 * Lc.nnx]
+    __shared__ double sh[4];
+    const int NR = Lf.nny * Lf.nnz;
+    const int L = 3 * Lf.nnx, Lq = 3 * Lc.nnx;
+    const int64_t row0 = int64_t(blockIdx.x) * RPB;
+    const int nr = min(RPB, int(NR - row0));
+    const float inv = 1.0f / float(span);
+    const int64_t cs = int64_t(Lc.nnx) * Lc.nny;
+    for (int rr = 0; rr < nr; ++rr) {
+        const int row = int(row0) + rr;
+        const int fy = row % Lf.nny, fz = row / Lf.nny;
+        int cy = fy / span, cz = fz / span;
+        int ry = fy - cy * span, rz = fz - cz * span;
+        if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
+        if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
+        const float wy1 = float(ry) * inv, wy0 = 1.f - wy1, wz1 = float(rz) * inv, wz0 = 1.f - wz1;
+        const int64_t b0 = cs * cz + int64_t(Lc.nnx) * cy;
+        for (int k = threadIdx.x; k < Lq; k += 256) {
+            const int64_t q = 3 * b0 + k;
+            const float lo = wy0 * xc[q] + wy1 * xc[q + Lq];
+            const float hi = wy0 * xc[q + 3 * cs] + wy1 * xc[q + 3 * cs + Lq];
+            sQ[rr * Lq + k] = wz0 * lo + wz1 * hi;
+        }
+    }
+    __syncthreads();
+    double srz = 0.0, szap = 0.0;
+    for (int rr = 0; rr < nr; ++rr) {
+        const int64_t nbase = (row0 + rr) * Lf.nnx;
+        for (int k = threadIdx.x; k < L; k += 256) {
+            const int xx = k / 3, c = k - 3 * xx;
+            int cxx = xx / span, rx = xx - cxx * span;
+            if (cxx >= Lc.resX) { cxx = Lc.resX - 1; rx = span; }
+            const float wx1 = float(rx) * inv;
+            const float v = (1.f - wx1) * sQ[rr * Lq + 3 * cxx + c] + wx1 * sQ[rr * Lq + 3 * cxx + 3 + c];
+            const int64_t i = 3 * nbase + k;
+            const float r = rf[i];
+            float out = 0.f;
+            if (!((fmask[nbase + xx] >> c) & 1u)) out = 2.f * (weight * r * Df[i]) + v;
+            xf[i] = out;
+            srz = fma(double(r), double(out), srz);
+            szap = fma(double(out), double(Ap[i]), szap);
+        }
+    }
     const double a = block_sum_256(srz, sh);
-    if (threadIdx.x == 0) partial[bid] = a;
+    if (threadIdx.x == 0) partial[blockIdx.x] = a;
     const double b = block_sum_256(szap, sh);
-    if (threadIdx.x == 0) partial[nb + bid] = b;
+    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = b;
 }
 
 // rz_new = r.z, zAp = z.Ap -> beta (0 on the first V-cycle of a solve); rz = rz_new
@@ -726,6 +900,14 @@ __global__ void __launch_bounds__(1024) k_fin_beta(const double* __restrict__ pa
     }
 }
 
+// FP32 inverse Jacobi diagonal 1 / max(1e-30, D) (the smoother's and the
+// coarsest diagonal solve's divisor, preconditioner.cpp:230-253)
+__global__ void k_dinv(int64_t n, const double* __restrict__ D, float* __restrict__ Dinv)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) Dinv[i] = float(1.0 / (1.0e-30 < D[i] ? D[i] : 1.0e-30));
+}
+
 __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -742,13 +924,15 @@ struct t3d_mixed {
     t3d_mg* M = nullptr;
     float* Ke32 = nullptr;     // element matrix (node-gather form)
     float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
-    float* D32[8] = {};
+    float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
     float* r32[8] = {};
     float* x32[8] = {};
     float* tmp = nullptr;      // restriction pass-1 buffer
     double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
     double* xc64 = nullptr;
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
+    float* r = nullptr;        // recursive residual (FP32)
+    float* dx = nullptr;       // FP32 increment of x since the last fold
     double* part = nullptr;    // block partials
     double* sc = nullptr;      // scalars
     int* iflags = nullptr;     // [0] done [1] iterations
@@ -817,13 +1001,14 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 }
 
 
-static dim3 finest_grid(const LevelF& L)
-{
-    return dim3(unsigned((L.nnx + 31) / 32), unsigned((L.nny + 7) / 8), unsigned((L.nnz + PZ - 1) / PZ));
-}
 
-// z32 = M32(r); the finest prolongation derives rz and beta (see k_prolong_finest)
-static int mixed_vcycle(t3d_mixed* s, const MGView& v, const double* r, int flexible, int first, const int* done)
+static unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz + RPB - 1) / RPB); }
+
+// z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
+// x-pass of the finest restriction is already in s->tmp (fused into the x, r
+// update)
+static int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
+                        const int* done)
 {
     t3d_problem* p = s->p;
     hipStream_t st = p->stream;
@@ -836,12 +1021,15 @@ static int mixed_vcycle(t3d_mixed* s, const MGView& v, const double* r, int flex
         const int span = v.span[l + 1];
         const float* Dc = (last && !chol) ? s->D32[L] : nullptr;
         double* rcd = (last && chol) ? s->rc64 : nullptr;
-        const dim3 g1(unsigned((Lc.nnx + 63) / 64), unsigned(Lf.nny), unsigned(Lf.nnz));
-        if (l == 0)
-            hipLaunchKernelGGL(k_restrict_x<double>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, r, s->tmp, done);
-        else
+        if (l == 0) {
+            if (!tmpReady)
+                hipLaunchKernelGGL(k_rx_rows, dim3(row_blocks(Lf)), dim3(256), sizeof(float) * RPB * 3 * Lf.nnx, st,
+                                   Lf.nnx, Lf.nny * Lf.nnz, r, Lc.nnx, span, s->tmp, done);
+        } else {
+            const dim3 g1(unsigned((Lc.nnx + 63) / 64), unsigned(Lf.nny), unsigned(Lf.nnz));
             hipLaunchKernelGGL(k_restrict_x<float>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, (const float*)s->r32[l],
                                s->tmp, done);
+        }
         const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
         hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
                            s->r32[l + 1], Dc, s->x32[L], rcd, done);
@@ -863,11 +1051,12 @@ static int mixed_vcycle(t3d_mixed* s, const MGView& v, const double* r, int flex
         T3D_HIP(hipGetLastError());
     }
     const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
-    const dim3 fg = finest_grid(L0);
-    hipLaunchKernelGGL(k_prolong_finest, fg, dim3(256), 0, st, L0, L1, v.span[1], (const float*)s->x32[1], r,
-                       (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap, s->part, done);
-    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, fg.x * fg.y * fg.z, s->sc,
-                       flexible, first, done);
+    const unsigned nb = row_blocks(L0);
+    hipLaunchKernelGGL(k_pro_rows, dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0, L1, v.span[1],
+                       (const float*)s->x32[1], r, (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap,
+                       s->part, done);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, nb, s->sc, flexible, first,
+                       done);
     T3D_HIP(hipGetLastError());
     return T3D_OK;
 }
@@ -879,7 +1068,7 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
+    void* b[] = {s->Ke32, s->KW, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 8; ++l) {
@@ -951,8 +1140,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     const int64_t nD = 3 * p->g.nN;
     const dim3 kg = kw_grid(p->g);
     const LevelF L0 = levf(v, 0);
-    const dim3 fg = finest_grid(L0);
-    const int64_t npart = std::max<int64_t>({int64_t(kg.x) * kg.y * kg.z, 2 * int64_t(fg.x) * fg.y * fg.z, VB}) + 16;
+    const int64_t npart = std::max<int64_t>({int64_t(kg.x) * kg.y * kg.z, 2 * int64_t(row_blocks(L0)), VB}) + 16;
     int64_t ntmp = 0;
     for (int l = 0; l + 1 < v.nlev; ++l) {
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
@@ -964,6 +1152,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         hipMemcpy(s->KW, KW, sizeof(float) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->pA, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->pB, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->Ap, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->z, sizeof(float) * nD) != hipSuccess ||
+        hipMalloc(&s->r, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->dx, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->tmp, sizeof(float) * ntmp) != hipSuccess ||
         hipMalloc(&s->part, sizeof(double) * npart) != hipSuccess ||
         hipMalloc(&s->sc, sizeof(double) * 64) != hipSuccess || hipMalloc(&s->iflags, sizeof(int) * 4) != hipSuccess ||
@@ -982,7 +1171,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     return T3D_OK;
 }
 
-// refresh the FP32 diagonals after t3d_mg_setup
+// refresh the FP32 inverse diagonals after t3d_mg_setup
 int t3d_mixed_setup(t3d_mixed* s)
 {
     T3D_CHECK(s, T3D_ERR_ARG, "t3d_mixed_setup: null");
@@ -991,7 +1180,7 @@ int t3d_mixed_setup(t3d_mixed* s)
     if (rc) return rc;
     for (int l = 0; l < v.nlev; ++l) {
         const int64_t n = 3 * v.nN[l];
-        hipLaunchKernelGGL(k_d2f, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s->p->stream, n, v.D[l], s->D32[l]);
+        hipLaunchKernelGGL(k_dinv, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s->p->stream, n, v.D[l], s->D32[l]);
         T3D_HIP(hipGetLastError());
     }
     T3D_HIP(hipStreamSynchronize(s->p->stream));
@@ -1046,20 +1235,17 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     hipStream_t st = p->stream;
     const Grid& g = p->g;
     const int64_t n = 3 * g.nN;
-    // FP64 work vectors of the problem (shared with the plain PCG)
-    if (!p->wr) {
-        T3D_HIP(hipMalloc(&p->wr, sizeof(double) * n));
-        T3D_HIP(hipMalloc(&p->wz, sizeof(double) * n));
-        T3D_HIP(hipMalloc(&p->wp, sizeof(double) * n));
-        T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * n));
-        T3D_HIP(hipMalloc(&p->pcgFlags, sizeof(int) * 4));
-        T3D_HIP(hipHostMalloc(&p->hostFlags, sizeof(int) * 4, hipHostMallocDefault));
-    }
-    double* r = p->wr;
+    // FP64 K x of the residual replacement (the problem's work vector, shared
+    // with the plain PCG)
+    if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * n));
+    float* r = s->r;
     double* Ap64 = p->wAp;
+    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     double* S = s->sc;
     int* flags = s->iflags;
     const dim3 kg = kw_grid(g);
+    const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
+    const unsigned xrb = row_blocks(L0);
     // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
     if ((rc = canon_dot_dev(p, b, b, n, S + S_NORMB2))) return rc;
     hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(1), 0, st, tol, S, flags);
@@ -1069,7 +1255,8 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, -1, 0, 0);
     T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * n, st));
     T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * n, st));
-    if ((rc = mixed_vcycle(s, v, r, flexible, 1, flags))) return rc;
+    T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * n, st));
+    if ((rc = mixed_vcycle(s, v, r, flexible, 1, false, flags))) return rc;
     // iteration it reads p_prev and writes p_cur; the two buffers alternate
     auto pPrevOf = [&](int it) { return (it & 1) ? s->pB : s->pA; };
     auto pCurOf = [&](int it) { return (it & 1) ? s->pA : s->pB; };
@@ -1086,13 +1273,16 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, kg.x * kg.y * kg.z, S,
                                (const int*)flags);
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
-            // x += alpha p ; r -= alpha Ap ; rn2 ; convergence test
-            hipLaunchKernelGGL(k_mx_update_xr, dim3(VB), dim3(256), 0, st, n, (const float*)pCurOf(it),
-                               (const float*)s->Ap, x, r, s->part, (const double*)S, (const int*)flags);
-            hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, it,
+            // dx += alpha p ; r -= alpha Ap ; rn2 ; convergence test
+            hipLaunchKernelGGL(k_xr_rows, dim3(xrb), dim3(256), sizeof(float) * RPB * 3 * L0.nnx, st, L0.nnx,
+                               L0.nny * L0.nnz, (const float*)pCurOf(it), (const float*)s->Ap, s->dx, r, s->part,
+                               (const double*)S, L1.nnx, v.span[1], replace ? nullptr : s->tmp, (const int*)flags);
+            hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, xrb, S, flags, it,
                                replace ? 0 : 1, 1);
             if (replace) {
-                // residual replacement with the FP64 operator (the direction is kept)
+                // x += dx, then residual replacement with the FP64 operator (the
+                // direction is kept)
+                hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, (const int*)flags);
                 if ((rc = launch_kxu_f64(p, x, Ap64, flags))) return rc;
                 hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r,
                                    s->part, (const int*)flags);
@@ -1100,7 +1290,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
                                    it, 1, 1);
             }
             // z = M r ; beta ; rz
-            if ((rc = mixed_vcycle(s, v, r, flexible, 0, flags))) return rc;
+            if ((rc = mixed_vcycle(s, v, r, flexible, 0, !replace, flags))) return rc;
         }
         T3D_HIP(hipMemcpyAsync(s->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
@@ -1111,6 +1301,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         // the residual passed the test at iteration itDone: confirm with the
         // true residual b - K x (FP64); otherwise continue from there
         const int itDone = s->hostFlags[1] - 1;
+        hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, nullptr);
         if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
         hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
                            nullptr);
@@ -1125,9 +1316,11 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             break;
         }
         hipLaunchKernelGGL(k_mx_clear_done, dim3(1), dim3(1), 0, st, flags);
-        if ((rc = mixed_vcycle(s, v, r, flexible, 0, flags))) return rc;
+        if ((rc = mixed_vcycle(s, v, r, flexible, 0, false, flags))) return rc;
         it0 = itDone + 1;
     }
+    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, nullptr);
+    T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(st));
     if (iters) *iters = result;
     return T3D_OK;

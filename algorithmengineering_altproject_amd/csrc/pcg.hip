@@ -107,14 +107,13 @@ __global__ void k_pcg_update_p(int64_t n, const double* __restrict__ rz_new, con
 static int pcg_alloc(t3d_problem* p, int maxIt)
 {
     const size_t nD = size_t(3 * p->g.nN);
-    if (!p->wr) {
-        T3D_HIP(hipMalloc(&p->wr, sizeof(double) * nD));
-        T3D_HIP(hipMalloc(&p->wz, sizeof(double) * nD));
-        T3D_HIP(hipMalloc(&p->wp, sizeof(double) * nD));
-        T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
-        T3D_HIP(hipMalloc(&p->pcgFlags, sizeof(int) * 4));
-        T3D_HIP(hipHostMalloc(&p->hostFlags, sizeof(int) * 4, hipHostMallocDefault));
-    }
+    // each work vector on first use (the mixed-precision solver shares wAp)
+    if (!p->wr) T3D_HIP(hipMalloc(&p->wr, sizeof(double) * nD));
+    if (!p->wz) T3D_HIP(hipMalloc(&p->wz, sizeof(double) * nD));
+    if (!p->wp) T3D_HIP(hipMalloc(&p->wp, sizeof(double) * nD));
+    if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
+    if (!p->pcgFlags) T3D_HIP(hipMalloc(&p->pcgFlags, sizeof(int) * 4));
+    if (!p->hostFlags) T3D_HIP(hipHostMalloc(&p->hostFlags, sizeof(int) * 4, hipHostMallocDefault));
     const int need = 3 * (maxIt + 1) + 8;
     if (need > p->pcgScCap) {
         if (p->pcgSc) T3D_HIP(hipFree(p->pcgSc));
