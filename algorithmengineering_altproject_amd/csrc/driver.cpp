@@ -167,6 +167,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
             R.tTotal = now_s() - tIt;
         }
     }
+    T3D_HIP(hipDeviceSynchronize());
     if (x_out) T3D_HIP(hipMemcpy(x_out, x, sizeof(float) * ne, hipMemcpyDeviceToHost));
     if (loopsRun) *loopsRun = loop;
     return T3D_OK;

@@ -391,6 +391,8 @@ int t3d_filter_create(t3d_problem* p, float rMin, t3d_filter** out)
         hipMalloc(&f->sc, sizeof(double) * 512) != hipSuccess || hipMalloc(&f->flags, sizeof(int) * 4) != hipSuccess)
         return fail();
     if (ensure_red(p, int64_t(nN))) return fail();
+    // the set-up copies and fills above ran on the null stream
+    T3D_HIP(hipDeviceSynchronize());
     *out = f;
     return T3D_OK;
 }
@@ -399,6 +401,7 @@ int t3d_filter_info(const t3d_filter* f, double* kernel64, double* diag_host)
 {
     T3D_CHECK(f, T3D_ERR_ARG, "t3d_filter_info: null");
     if (kernel64) std::memcpy(kernel64, f->kernel, sizeof(double) * 64);
+    T3D_HIP(hipDeviceSynchronize());
     if (diag_host) T3D_HIP(hipMemcpy(diag_host, f->diag, sizeof(double) * f->p->g.nN, hipMemcpyDeviceToHost));
     return T3D_OK;
 }

@@ -118,7 +118,8 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
     using T = KxuTile<NZ>;
     __shared__ double sU[3 * T::UN];   // SoA: component c at sU[c*UN + idx]
     __shared__ double sE[T::EN];
-    const int x0 = blockIdx.x * T::TX, y0 = blockIdx.y * T::TY, z0 = blockIdx.z * T::TZ;
+    const TileId tile = xcd_tile();
+    const int x0 = tile.x * T::TX, y0 = tile.y * T::TY, z0 = tile.z * T::TZ;
     const int tid = threadIdx.x;
     // cooperative loads (zero outside the grid)
     for (int i = tid; i < T::UN; i += 512) {
@@ -271,6 +272,7 @@ int t3d_kxu_free_host(t3d_problem* p, const double* E_h, const double* uF_h, dou
         set_error("t3d_kxu_free_host: upload failed");
         return T3D_ERR_HIP;
     }
+    (void)hipDeviceSynchronize();   // the uploads ran on the null stream
     if ((rc = t3d_set_modulus(p, E)) == T3D_OK && (rc = t3d_free_to_full(p, uf, u)) == T3D_OK &&
         (rc = t3d_kxu_f64(p, u, y)) == T3D_OK && (rc = t3d_full_to_free(p, y, uf)) == T3D_OK) {
         if (hipMemcpy(yF_h, uf, sizeof(double) * p->nF, hipMemcpyDeviceToHost) != hipSuccess) {

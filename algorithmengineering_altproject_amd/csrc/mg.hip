@@ -472,6 +472,8 @@ int t3d_mg_create(t3d_problem* p, int nonDyadic, int maxLevels, float weight, fl
                 return fail("t3d_mg_create: device allocation failed");
         }
     }
+    // the set-up copies and fills above ran on the null stream
+    T3D_HIP(hipDeviceSynchronize());
     *out = g;
     return T3D_OK;
 }
@@ -502,6 +504,7 @@ int t3d_mg_diag(const t3d_mg* g, int level, double* out)
 {
     T3D_CHECK(g && out && level >= 0 && level < g->nlev, T3D_ERR_ARG, "t3d_mg_diag: bad argument");
     T3D_CHECK(g->setupDone, T3D_ERR_STATE, "t3d_mg_diag: t3d_mg_setup not called");
+    T3D_HIP(hipDeviceSynchronize());
     T3D_HIP(hipMemcpy(out, g->D[level], sizeof(double) * 3 * g->H.lv[level].nN, hipMemcpyDeviceToHost));
     return T3D_OK;
 }
@@ -512,6 +515,7 @@ int t3d_mg_coarsest(const t3d_mg* g, double* Kc, double* Lf)
     T3D_CHECK(g && g->setupDone, T3D_ERR_STATE, "t3d_mg_coarsest: not set up");
     const int N = int(3 * g->H.lv.back().nN);
     if (Kc && !g->Kc.empty()) std::copy(g->Kc.begin(), g->Kc.end(), Kc);
+    T3D_HIP(hipDeviceSynchronize());
     if (Lf && g->Nc > 0) T3D_HIP(hipMemcpy(Lf, g->L, sizeof(double) * size_t(N) * N, hipMemcpyDeviceToHost));
     return T3D_OK;
 }
@@ -555,6 +559,7 @@ int t3d_mg_setup(t3d_mg* g, const double* E_dev)
             }
             T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
             T3D_HIP(hipMemcpyAsync(g->LT, AT.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
+            T3D_HIP(hipStreamSynchronize(st));   // A and AT are host buffers of this scope
             g->Nc = N;
         } else {
             g->cholOk = 0;   // diagonal fallback, as the reference (preconditioner.cpp:76-77)

@@ -285,7 +285,8 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
     __shared__ double sh[4];
     const int tid = threadIdx.x;
     const int tx = tid % CX, ty = tid / CX;
-    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
+    const TileId tile = xcd_tile();
+    const int x0 = tile.x * NX, y0 = tile.y * NY, z0 = tile.z * NZ;
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
     const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
@@ -1153,9 +1154,11 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         hipMalloc(&s->pA, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->pB, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->Ap, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->z, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->r, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->dx, sizeof(float) * nD) != hipSuccess ||
+        hipMemset(s->z, 0, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->tmp, sizeof(float) * ntmp) != hipSuccess ||
         hipMalloc(&s->part, sizeof(double) * npart) != hipSuccess ||
-        hipMalloc(&s->sc, sizeof(double) * 64) != hipSuccess || hipMalloc(&s->iflags, sizeof(int) * 4) != hipSuccess ||
+        hipMalloc(&s->sc, sizeof(double) * 64) != hipSuccess || hipMemset(s->sc, 0, sizeof(double) * 64) != hipSuccess ||
+        hipMalloc(&s->iflags, sizeof(int) * 4) != hipSuccess ||
         hipMemset(s->iflags, 0, sizeof(int) * 4) != hipSuccess)
         return fail();
     for (int l = 0; l < v.nlev; ++l) {
@@ -1167,6 +1170,8 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     const int64_t nc = 3 * v.nN[v.nlev - 1];
     if (hipMalloc(&s->rc64, sizeof(double) * nc) != hipSuccess || hipMalloc(&s->xc64, sizeof(double) * nc) != hipSuccess)
         return fail();
+    // the set-up copies and fills above ran on the null stream
+    T3D_HIP(hipDeviceSynchronize());
     *out = s;
     return T3D_OK;
 }
@@ -1199,17 +1204,27 @@ int t3d_kxu_f32(t3d_mixed* s, int form, const float* u, float* y)
     return T3D_OK;
 }
 
-// HIP-event timing of the FP32 operator of the given form (op 0 Walsh, 1 node
-// gather) on the problem's stream: average milliseconds per launch
+// HIP-event timing on the problem's stream, average milliseconds per launch:
+// op 0 / 1: the FP32 operator y = K u of form 0 (Walsh) / 1 (node gather);
+// op 2: the solver's fused Walsh-form kernel (p = z + beta u staged and
+// written to the solver's p buffer, y = K p, block partials of p.y), with the
+// solver's z buffer and beta, as launched once per PCG iteration
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
 {
-    T3D_CHECK(s && u && y && reps > 0 && ms && (op == 0 || op == 1), T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 2, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
     t3d_problem* p = s->p;
     hipEvent_t e0, e1;
     T3D_HIP(hipEventCreate(&e0));
     T3D_HIP(hipEventCreate(&e1));
     T3D_HIP(hipEventRecord(e0, p->stream));
-    for (int i = 0; i < reps; ++i) launch_kx32(s, op, u, y);
+    for (int i = 0; i < reps; ++i) {
+        if (op < 2)
+            launch_kx32(s, op, u, y);
+        else
+            hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                               (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
+                               (const float*)s->z, s->pB, y, s->part, (const double*)s->sc, (const int*)s->iflags);
+    }
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipEventRecord(e1, p->stream));
     T3D_HIP(hipEventSynchronize(e1));

@@ -203,6 +203,7 @@ int t3d_pcg_history(t3d_problem* p, int maxIt, int n, double* rn2_host)
 {
     T3D_CHECK(p && p->pcgSc && rn2_host && n <= maxIt, T3D_ERR_ARG, "t3d_pcg_history: bad argument");
     const double* rn2 = p->pcgSc + 8 + (maxIt + 1) + maxIt;
+    T3D_HIP(hipDeviceSynchronize());
     T3D_HIP(hipMemcpy(rn2_host, rn2, sizeof(double) * n, hipMemcpyDeviceToHost));
     return T3D_OK;
 }

@@ -186,24 +186,34 @@ int t3d_free(void* ptr)
     if (ptr) T3D_HIP(hipFree(ptr));
     return T3D_OK;
 }
+// The copy/fill helpers complete before they return: the problem streams are
+// non-blocking and do not order against work on the null stream, and a
+// null-stream hipMemset or pageable H2D copy can still be in flight when
+// hipMemset/hipMemcpy return.
 int t3d_memcpy_h2d(void* dst, const void* src, size_t bytes)
 {
     T3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    T3D_HIP(hipDeviceSynchronize());
     return T3D_OK;
 }
 int t3d_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {
+    T3D_HIP(hipDeviceSynchronize());
     T3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return T3D_OK;
 }
 int t3d_memcpy_d2d(void* dst, const void* src, size_t bytes)
 {
+    T3D_HIP(hipDeviceSynchronize());
     T3D_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    T3D_HIP(hipDeviceSynchronize());
     return T3D_OK;
 }
 int t3d_memset(void* dst, int value, size_t bytes)
 {
+    T3D_HIP(hipDeviceSynchronize());
     T3D_HIP(hipMemset(dst, value, bytes));
+    T3D_HIP(hipDeviceSynchronize());
     return T3D_OK;
 }
 int t3d_synchronize(void)
@@ -289,6 +299,8 @@ int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out)
         set_error("device allocation failed");
         return fail(T3D_ERR_HIP);
     }
+    // the set-up copies and fills above ran on the null stream
+    T3D_HIP(hipDeviceSynchronize());
     *out = p;
     return T3D_OK;
 }

@@ -42,6 +42,29 @@ struct Grid {
     int64_t ncell;
 };
 
+// XCD-aware tile order for 3D tiled kernels.  Workgroups are dispatched to
+// the 8 XCDs round-robin (hardware block b runs on XCD b % 8), and each XCD
+// has its own L2.  This maps hardware block b to a logical tile so that XCD k
+// works through the k-th contiguous range of tiles (x fastest, then y, z):
+// neighbouring tiles, which re-read each other's halos, share an L2.
+constexpr unsigned NUM_XCD = 8;
+struct TileId {
+    unsigned x, y, z;
+};
+__device__ __forceinline__ TileId xcd_tile()
+{
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const unsigned q = nb / NUM_XCD, rem = nb % NUM_XCD;
+    const unsigned k = b % NUM_XCD, j = b / NUM_XCD;
+    const unsigned l = k * q + (k < rem ? k : rem) + j;   // a bijection of [0, nb)
+    TileId t;
+    t.x = l % gridDim.x;
+    t.y = (l / gridDim.x) % gridDim.y;
+    t.z = l / (gridDim.x * gridDim.y);
+    return t;
+}
+
 }  // namespace t3d
 
 // device-side problem context

@@ -140,8 +140,9 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, do
  * (45-nonzero element matrix in the voxel's Walsh-Hadamard basis; used by the
  * solver), form 1: node gather with the full 24x24 element matrix */
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_dev);
-/* average milliseconds per launch of the FP32 operator of form op over reps
- * launches, HIP events on the problem's stream */
+/* average milliseconds per launch over reps launches, HIP events on the
+ * problem's stream: op 0/1 the FP32 operator of form 0/1; op 2 the solver's
+ * fused Walsh-form kernel (p = z + beta*u, y = K p, p.y partials) */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 
 /* ---- compliance ----

@@ -120,3 +120,23 @@ def test_mixed_residual_replacement_off(t3d):
     x.zero()
     it = mx.pcg(db, x, 1e-4, 3000, trueEvery=0)
     assert it < 3000 and _true_rel_residual(p, db, x) < 1e-4
+
+
+def test_repeated_solves_identical(t3d):
+    """Zeroing x through the C ABI and solving again reproduces the first solve
+    bit for bit (FP64 plain and mixed): the host helpers complete before they
+    return, so no fill of x can overlap the next solve on the problem stream."""
+    p, dE, M, mx, db = _setup(t3d, (48, 96, 48), 0.12)
+    x = t3d.DeviceArray(p.nDOF)
+    runs64, runsm = [], []
+    for _ in range(3):
+        x.zero()
+        it = t3d.pcg(p, M, db, x, 1e-5, 3000)
+        runs64.append((it, x.download()))
+        x.zero()
+        it = mx.pcg(db, x, 1e-5, 3000)
+        runsm.append((it, x.download()))
+    for runs in (runs64, runsm):
+        for it, xs in runs[1:]:
+            assert it == runs[0][0]
+            assert np.array_equal(xs, runs[0][1])
