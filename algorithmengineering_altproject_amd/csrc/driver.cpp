@@ -51,26 +51,45 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
 {
     T3D_CHECK(P && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_top3d_run: bad argument");
     T3D_CHECK(P->penal == 3.0f, T3D_ERR_ARG, "t3d_top3d_run: only penal = 3 is supported (powf(rho,3) rounding)");
-    T3D_CHECK(P->precision == 0, T3D_ERR_ARG, "t3d_top3d_run: precision must be 0 (FP64 plain path)");
+    T3D_CHECK(P->precision == 0 || P->precision == 1, T3D_ERR_ARG,
+              "t3d_top3d_run: precision must be 0 (FP64 plain path) or 1 (mixed-precision solve)");
     if (loopsRun) *loopsRun = 0;
     t3d_problem* p = nullptr;
     t3d_mg* M = nullptr;
     t3d_filter* F = nullptr;
+    t3d_mixed* MX = nullptr;
     int rc = t3d_problem_create_cuboid(nely, nelx, nelz, &p);
     if (rc) return rc;
     struct Guard {
         t3d_problem*& p;
         t3d_mg*& M;
         t3d_filter*& F;
+        t3d_mixed*& MX;
         ~Guard()
         {
+            t3d_mixed_destroy(MX);
             t3d_filter_destroy(F);
             t3d_mg_destroy(M);
             t3d_problem_destroy(p);
         }
-    } guard{p, M, F};
+    } guard{p, M, F, MX};
     if ((rc = t3d_filter_create(p, P->rMin, &F))) return rc;
     if ((rc = t3d_mg_create(p, P->nonDyadic, P->mgMaxLevels, P->mgWeight, P->Emin, P->NlimitDofs, &M))) return rc;
+    // precision 1: the solves run in the mixed-precision PCG (FP32 operator and
+    // V-cycle, flexible FP64 outer loop, true-residual stopping); everything
+    // else is the plain path
+    const bool mixed = P->precision == 1;
+    const double* b_for_solve = nullptr;   // the load vector, set below
+    auto setup = [&](const double* Ed) {
+        int r = t3d_mg_setup(M, Ed);
+        if (!r && mixed && !MX) r = t3d_mixed_create(p, M, &MX);   // needs the set-up hierarchy
+        if (!r && mixed) r = t3d_mixed_setup(MX);
+        return r;
+    };
+    auto solve = [&](double* ud, int* its) {
+        return mixed ? t3d_pcg_mixed(MX, b_for_solve, ud, double(P->cgTol), P->cgMaxIt, 50, 1, its)
+                     : t3d_pcg(p, M, b_for_solve, ud, double(P->cgTol), P->cgMaxIt, its);
+    };
     const int ne = p->ne;
     const int64_t nD = 3 * p->g.nN;
     hipStream_t st = p->stream;
@@ -96,16 +115,16 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     double* b = (double*)bb.p;
     double* sc = (double*)bsc.p;   // [0] Csolid [1] C [2] cObj ; float slots from [8]
     float* fsc = reinterpret_cast<float*>(sc + 8);
-    const double tol = double(P->cgTol);
 
     T3D_HIP(hipMemcpyAsync(b, p->bFull, sizeof(double) * nD, hipMemcpyDeviceToDevice, st));
+    b_for_solve = b;
     if ((rc = launch_fill_f32(p, ne, P->V0, x))) return rc;
     // solid reference solve (driver.cpp:85-103)
     if ((rc = launch_fill_f64(p, ne, double(P->E0), E))) return rc;
     if ((rc = launch_fill_f64(p, nD, 0.0, u))) return rc;
-    if ((rc = t3d_mg_setup(M, E))) return rc;
+    if ((rc = setup(E))) return rc;
     int it0 = 0;
-    if ((rc = t3d_pcg(p, M, b, u, tol, P->cgMaxIt, &it0))) return rc;
+    if ((rc = solve(u, &it0))) return rc;
     if ((rc = compliance_dev(p, E, u, ce, sc + 0))) return rc;
     double Csolid = 0.0;
     T3D_HIP(hipMemcpyAsync(&Csolid, sc + 0, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -121,12 +140,12 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         if ((rc = launch_simp(p, x, P->E0, P->Emin, E))) return rc;
         const double tS = now_s();
         if (loop == 1 || (loop % P->rebuildEvery) == 0) {
-            if ((rc = t3d_mg_setup(M, E))) return rc;
+            if ((rc = setup(E))) return rc;
         } else {
             if ((rc = t3d_set_modulus(p, E))) return rc;
         }
         int its = 0;
-        if ((rc = t3d_pcg(p, M, b, u, tol, P->cgMaxIt, &its))) return rc;
+        if ((rc = solve(u, &its))) return rc;
         const double tSolve = now_s() - tS;
         const double tO = now_s();
         if ((rc = compliance_dev(p, E, u, ce, sc + 1))) return rc;

@@ -187,7 +187,7 @@ typedef struct t3d_params {
     int nonDyadic;
     int NlimitDofs;
     int rebuildEvery;
-    int precision;   /* 0 = FP64 plain path */
+    int precision;   /* 0 = FP64 plain path, 1 = mixed-precision solves (t3d_pcg_mixed) */
 } t3d_params;
 typedef struct t3d_loop_record {
     int loop, pcgIters, filterIters, bisectSteps;
