@@ -84,6 +84,7 @@ def lib():
         "t3d_mixed_destroy": (C.c_int, [vp]),
         "t3d_mixed_setup": (C.c_int, [vp]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
+        "t3d_mixed_set_operator": (C.c_int, [vp, C.c_int]),
         "t3d_mixed_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
         "t3d_pcg_mixed": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
@@ -394,8 +395,12 @@ class Mixed:
     def setup(self):
         check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
 
+    def set_operator(self, form):
+        """Operator of the solver: 0 Walsh form, 2 MFMA split-bf16."""
+        check(lib().t3d_mixed_set_operator(self.h, form), "t3d_mixed_set_operator")
+
     def kxu_f32(self, du, dy, form=0):
-        """y = K u in FP32 (device float arrays, full layout); form 0 Walsh, 1 node gather."""
+        """y = K u in FP32 (device float arrays, full layout); form 0 Walsh, 1 node gather, 2 MFMA."""
         check(lib().t3d_kxu_f32(self.h, form, du.ptr, dy.ptr), "t3d_kxu_f32")
 
     def time_kxu(self, du, dy, reps, form=0):

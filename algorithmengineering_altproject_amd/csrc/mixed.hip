@@ -28,6 +28,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "t3d_internal.hpp"
 
@@ -472,6 +474,183 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
             else
                 stage(z0 + s + 1, (s + 2) % 3);
         }
+        __syncthreads();
+    }
+    if (FUSED) {
+        const double t = block_sum_256(dot, sh);
+        if (tid == 0) partial[linear_block()] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 operator, MFMA form (split-BF16 x3).  The voxel contraction
+// F (24 x voxels) = Ke (24x24) U (24 x voxels) runs on the matrix cores as
+// v_mfma_f32_32x32x16_bf16 with compensated products: Ke = Kh + Kl and
+// U = Uh + Ul (bf16, round to nearest even), F ~ Kh Uh + Kh Ul + Kl Uh
+// accumulated in FP32 (relative error ~2^-16 per product instead of FP32's
+// 2^-24; the outer FP64 loop's true-residual replacement absorbs it).
+// Block and z-sweep as the Walsh form: 256 threads own 31 x 7 node columns; a
+// wave takes two rows of 32 voxel columns (two 32-column MFMA tiles), lane l
+// feeds voxel column l % 32 with the DOF half l / 32 of each 16-wide K step.
+// Operand layouts of the instruction (gfx950): A[i][k]: lane l holds i = l%32,
+// k = 8 (l/32) + e; B[k][j]: k = 8 (l/32) + e, j = l%32; D[i][j]: lane l
+// register r holds i = 8 (r/4) + 4 (l/32) + r%4, j = l%32.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+// node offset o(b) of local node b (fea.cpp node order)
+__device__ __forceinline__ constexpr int off_x(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__device__ __forceinline__ constexpr int off_y(int b) { return ((b & 3) <= 1) ? 1 : 0; }
+__device__ __forceinline__ constexpr int off_z(int b) { return b >> 2; }
+
+template <int NZ, bool FUSED>
+__global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __restrict__ KA,
+                                               const float* __restrict__ Ecell, const uint8_t* __restrict__ nodeFixed,
+                                               const float* __restrict__ u, const float* __restrict__ z,
+                                               float* __restrict__ pcur, float* __restrict__ y,
+                                               double* __restrict__ partial, const double* __restrict__ S,
+                                               const int* __restrict__ done)
+{
+    using namespace wk;
+    if (FUSED && *done) return;
+    __shared__ float sP[3][3][PN];        // three staged node planes, [buffer][component][node]
+    __shared__ float sBot[12][CX * CY];   // bottom-face forces of the current voxel layer
+    __shared__ float sTop[2][12][CX * CY];  // top-face forces, by layer parity
+    __shared__ double sh[4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int jx = lane & 31, kh = lane >> 5;
+    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
+    const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
+    // A operands (element matrix, split hi/lo), per K step: [ks][hl][lane][8]
+    bf16x8 Ah[2], Al[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        u16x8 h, l;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            h[e] = KA[((ks * 2 + 0) * 64 + lane) * 8 + e];
+            l[e] = KA[((ks * 2 + 1) * 64 + lane) * 8 + e];
+        }
+        Ah[ks] = __builtin_bit_cast(bf16x8, h);
+        Al[ks] = __builtin_bit_cast(bf16x8, l);
+    }
+    auto stage = [&](int Z, int q) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        const bool own = Z >= z0 && Z < z0 + NZ;
+        for (int f = tid; f < 3 * PN; f += CX * CY) {
+            const int row = f / (3 * PX);
+            const int k = f - row * (3 * PX);
+            const int lx = k / 3, c = k - 3 * lx;
+            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+            float v = 0.f;
+            if (zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
+                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
+                v = u[gi];
+                if (FUSED) {
+                    v = fmaf(beta, v, z[gi]);
+                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) pcur[gi] = v;
+                }
+            }
+            sP[q][c][row * PX + lx] = v;
+        }
+    };
+    stage(z0 - 1, 0);
+    stage(z0, 1);
+    __syncthreads();
+    double dot = 0.0;
+    const int nsteps = min(NZ, g.nnz - z0) + 1;
+    for (int s = 0; s < nsteps; ++s) {
+        const int qb = s % 3, qt = (s + 1) % 3;   // planes cz (bottom faces) and cz + 1 (top faces)
+        const int cz = z0 - 1 + s;
+        const int par = cz & 1;
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int ty = 2 * wv + nb;               // voxel row of this MFMA tile
+            const int cx = x0 - 1 + jx, cy = y0 - 1 + ty;
+            float E = 0.f;
+            if (cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY && cz >= 0 && cz < g.resZ) {
+                E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
+                if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
+            }
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                u16x8 bh, bl;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    // DOF k = 16 ks + 8 kh + e of the voxel = 3 b + c (zero padding beyond 24)
+                    float v = 0.f;
+                    if (ks == 0) {
+                        const int k0 = 8 * kh + e, k1 = k0;   // kh = 0: DOFs 0..7, kh = 1: DOFs 8..15
+                        const int b = k1 / 3, c = k1 - 3 * b;
+                        const int q = (b >> 2) ? qt : qb;
+                        v = sP[q][c][(ty + off_y(b)) * PX + jx + off_x(b)];
+                        (void)k0;
+                    } else if (kh == 0) {
+                        const int k1 = 16 + e;                    // DOFs 16..23
+                        const int b = k1 / 3, c = k1 - 3 * b;
+                        v = sP[qt][c][(ty + off_y(b)) * PX + jx + off_x(b)];
+                    }
+                    const unsigned short h = bf16_rne(v);
+                    bh[e] = h;
+                    bl[e] = bf16_rne(v - bf16_val(h));
+                }
+                const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah[ks], Bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah[ks], Bl, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al[ks], Bh, acc, 0, 0, 0);
+            }
+            // force rows i = 8 (r/4) + 4 kh + r%4 of voxel column (jx, ty), scaled by E
+            const int col = ty * CX + jx;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = 8 * (r / 4) + 4 * kh + (r % 4);
+                if (i >= 24) continue;
+                const int b = i / 3, c = i - 3 * b;
+                const int slot = (2 * off_y(b) + off_x(b)) * 3 + c;
+                const float f = E * acc[r];
+                if (b < 4)
+                    sBot[slot][col] = f;
+                else
+                    sTop[par][slot][col] = f;
+            }
+        }
+        __syncthreads();
+        if (s > 0) {   // plane cz: bottom faces of layer cz + top faces of layer cz - 1
+            const int tx = tid % CX, tyn = tid / CX;
+            const int X = x0 + tx, Y = y0 + tyn, Z = cz;
+            if (tx < NX && tyn < NY && X < g.nnx && Y < g.nny) {
+                const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
+                const unsigned fx = nodeFixed[n];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    // node (X, Y) is corner (ox, oy) of voxel column (X - ox, Y - oy)
+                    float acc2 = 0.f;
+#pragma unroll
+                    for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+                        for (int ox = 0; ox < 2; ++ox) {
+                            const int colv = (tyn + 1 - oy) * CX + (tx + 1 - ox);
+                            const int slot = (2 * oy + ox) * 3 + c;
+                            acc2 += sBot[slot][colv] + sTop[par ^ 1][slot][colv];
+                        }
+                    const float v = ((fx >> c) & 1u) ? 0.f : acc2;
+                    y[3 * n + c] = v;
+                    if (FUSED) dot = fma(double(sP[qb][c][(tyn + 1) * PX + tx + 1]), double(v), dot);
+                }
+            }
+        }
+        if (s + 1 < nsteps) stage(z0 + s + 1, (s + 2) % 3);
         __syncthreads();
     }
     if (FUSED) {
@@ -927,6 +1106,8 @@ struct t3d_mixed {
     t3d_mg* M = nullptr;
     float* Ke32 = nullptr;     // element matrix (node-gather form)
     float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
+    unsigned short* KA = nullptr;   // element matrix as split-bf16 MFMA A operands
+    int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA split-bf16
     float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
     float* r32[8] = {};
     float* x32[8] = {};
@@ -989,11 +1170,15 @@ static dim3 kw_grid(const Grid& g)
                 unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
 }
 
-// y = K u in FP32; form 0: Walsh form, 1: node-gather form
+// y = K u in FP32; form 0: Walsh form, 1: node-gather form, 2: MFMA split-bf16
 static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 {
     t3d_problem* p = s->p;
-    if (form == 0)
+    if (form == 2)
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
+                           nullptr, nullptr, y, nullptr, nullptr, nullptr);
+    else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
                            nullptr, y, nullptr, nullptr, nullptr);
@@ -1006,6 +1191,20 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 
 
 static unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz + RPB - 1) / RPB); }
+
+// the solver's fused operator launch: p = z + beta p_prev, Ap = K p, p.Ap partials
+static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* Ap)
+{
+    t3d_problem* p = s->p;
+    if (s->form == 2)
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
+                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+    else
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev,
+                           (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+}
 
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
 // x-pass of the finest restriction is already in s->tmp (fused into the x, r
@@ -1071,7 +1270,7 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
+    void* b[] = {s->Ke32, s->KW, s->KA, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 8; ++l) {
@@ -1149,7 +1348,35 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
         ntmp = std::max<int64_t>(ntmp, 3 * int64_t(Lc.nnx) * Lf.nny * Lf.nnz);
     }
+    // MFMA A operands: A[i][k] = Ke[i][k] (24x24 padded to 32x32), split into
+    // bf16 hi + lo (round to nearest even), in the instruction's lane layout
+    // [K step][hi/lo][lane][8]: lane l holds row l%32, columns 16 ks + 8 (l/32) + e
+    std::vector<unsigned short> KAh(2 * 2 * 64 * 8);
+    {
+        auto rne = [](float x) {
+            unsigned u;
+            std::memcpy(&u, &x, 4);
+            return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+        };
+        auto val = [](unsigned short h) {
+            const unsigned u = unsigned(h) << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        };
+        for (int ks = 0; ks < 2; ++ks)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) {
+                    const int i = l % 32, k = 16 * ks + 8 * (l / 32) + e;
+                    const float a = (i < 24 && k < 24) ? float(p->host.Ke[i * 24 + k]) : 0.0f;
+                    const unsigned short h = rne(a);
+                    KAh[((ks * 2 + 0) * 64 + l) * 8 + e] = h;
+                    KAh[((ks * 2 + 1) * 64 + l) * 8 + e] = rne(a - val(h));
+                }
+    }
     if (hipMalloc(&s->Ke32, sizeof(float) * 576) != hipSuccess ||
+        hipMalloc(&s->KA, sizeof(unsigned short) * KAh.size()) != hipSuccess ||
+        hipMemcpy(s->KA, KAh.data(), sizeof(unsigned short) * KAh.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->Ke32, K32, sizeof(float) * 576, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->KW, sizeof(float) * wk::NNZ) != hipSuccess ||
         hipMemcpy(s->KW, KW, sizeof(float) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1199,33 +1426,39 @@ int t3d_mixed_setup(t3d_mixed* s)
 // form 0: Walsh form (used by the solver), 1: node-gather form
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u, float* y)
 {
-    T3D_CHECK(s && u && y && (form == 0 || form == 1), T3D_ERR_ARG, "t3d_kxu_f32: bad argument");
+    T3D_CHECK(s && u && y && form >= 0 && form <= 2, T3D_ERR_ARG, "t3d_kxu_f32: bad argument");
     launch_kx32(s, form, u, y);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(s->p->stream));
     return T3D_OK;
 }
 
+// selects the operator of the solver: 0 Walsh form (default), 2 MFMA split-bf16
+int t3d_mixed_set_operator(t3d_mixed* s, int form)
+{
+    T3D_CHECK(s && (form == 0 || form == 2), T3D_ERR_ARG, "t3d_mixed_set_operator: form must be 0 or 2");
+    s->form = form;
+    return T3D_OK;
+}
+
 // HIP-event timing on the problem's stream, average milliseconds per launch:
-// op 0 / 1: the FP32 operator y = K u of form 0 (Walsh) / 1 (node gather);
-// op 2: the solver's fused Walsh-form kernel (p = z + beta u staged and
-// written to the solver's p buffer, y = K p, block partials of p.y), with the
-// solver's z buffer and beta, as launched once per PCG iteration
+// op 0 / 1 / 2: the FP32 operator y = K u of form 0 (Walsh) / 1 (node gather) /
+// 2 (MFMA split-bf16); op 3: the solver's fused kernel of the selected
+// operator (p = z + beta u staged and written to the solver's p buffer,
+// y = K p, block partials of p.y), as launched once per PCG iteration
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
 {
-    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 2, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 3, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
     t3d_problem* p = s->p;
     hipEvent_t e0, e1;
     T3D_HIP(hipEventCreate(&e0));
     T3D_HIP(hipEventCreate(&e1));
     T3D_HIP(hipEventRecord(e0, p->stream));
     for (int i = 0; i < reps; ++i) {
-        if (op < 2)
+        if (op < 3)
             launch_kx32(s, op, u, y);
         else
-            hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                               (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
-                               (const float*)s->z, s->pB, y, s->part, (const double*)s->sc, (const int*)s->iflags);
+            launch_fused(s, u, s->pB, y);
     }
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipEventRecord(e1, p->stream));
@@ -1284,9 +1517,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         const int it1 = std::min(maxIt, it0 + CHUNK);
         for (int it = it0; it < it1; ++it) {
             // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
-            hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kg, dim3(256), 0, st, g, (const float*)s->KW,
-                               (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, (const float*)pPrevOf(it),
-                               (const float*)s->z, pCurOf(it), s->Ap, s->part, (const double*)S, (const int*)flags);
+            launch_fused(s, pPrevOf(it), pCurOf(it), s->Ap);
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, kg.x * kg.y * kg.z, S,
                                (const int*)flags);
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;

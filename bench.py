@@ -145,8 +145,8 @@ def main():
     y32 = t3d.DeviceArray(p.nDOF, np.float32)
     mx.time_kxu(u32, y32, 3, 0)
     kxu32_ms = mx.time_kxu(u32, y32, args.kxu_reps, 0)
-    mx.time_kxu(u32, y32, 3, 2)
-    fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 2)
+    mx.time_kxu(u32, y32, 3, 3)
+    fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 3)
     b64 = 8 * (2 * p.nF + p.ne)       # read u, write y (free DOFs), read E
     b32 = 4 * (2 * p.nF + p.ne)
     bfused = 4 * (4 * p.nF + p.ne)    # read p_prev, z; write p, y; read E
