@@ -396,11 +396,12 @@ class Mixed:
         check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
 
     def set_operator(self, form):
-        """Operator of the solver: 0 Walsh form, 2 MFMA split-bf16."""
+        """Operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6."""
         check(lib().t3d_mixed_set_operator(self.h, form), "t3d_mixed_set_operator")
 
     def kxu_f32(self, du, dy, form=0):
-        """y = K u in FP32 (device float arrays, full layout); form 0 Walsh, 1 node gather, 2 MFMA."""
+        """y = K u in FP32 (device float arrays, full layout); form 0 Walsh, 1 node gather,
+        2 MFMA bf16 x3, 3 MFMA bf16 x6."""
         check(lib().t3d_kxu_f32(self.h, form, du.ptr, dy.ptr), "t3d_kxu_f32")
 
     def time_kxu(self, du, dy, reps, form=0):

@@ -483,18 +483,22 @@ __global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// FP32 operator, MFMA form (split-BF16 x3).  The voxel contraction
+// FP32 operator, MFMA form (split bf16).  The voxel contraction
 // F (24 x voxels) = Ke (24x24) U (24 x voxels) runs on the matrix cores as
-// v_mfma_f32_32x32x16_bf16 with compensated products: Ke = Kh + Kl and
-// U = Uh + Ul (bf16, round to nearest even), F ~ Kh Uh + Kh Ul + Kl Uh
-// accumulated in FP32 (relative error ~2^-16 per product instead of FP32's
-// 2^-24; the outer FP64 loop's true-residual replacement absorbs it).
+// v_mfma_f32_32x32x16_bf16 with compensated products accumulated in FP32:
+//   PIECES = 2 (bf16 x3): Ke = Kh + Kl, U = Uh + Ul (round to nearest even),
+//     F ~ Kh Uh + Kh Ul + Kl Uh  (16-bit operands, ~4e-6 relative in K u);
+//   PIECES = 3 (bf16 x6): three pieces each, F ~ KhUh + KhUm + KmUh + KhUl +
+//     KmUm + KlUh  (24-bit operands, FP32-level accuracy).
+// DOFs are ordered component-major in the matrix dimensions (k = 8 c + b for
+// component c of local node b), so in the instruction's operand layouts
+// (gfx950: A[i][k]: lane l holds i = l%32, k = 8 (l/32) + e; B[k][j]: k =
+// 8 (l/32) + e, j = l%32; D[i][j]: lane l register r holds i = 8 (r/4) +
+// 4 (l/32) + r%4, j = l%32) lane l feeds the 8 nodes of one component of voxel
+// column l % 32 and receives component r/4 of the bottom-face nodes (l < 32) or
+// the top-face nodes (l >= 32).
 // Block and z-sweep as the Walsh form: 256 threads own 31 x 7 node columns; a
-// wave takes two rows of 32 voxel columns (two 32-column MFMA tiles), lane l
-// feeds voxel column l % 32 with the DOF half l / 32 of each 16-wide K step.
-// Operand layouts of the instruction (gfx950): A[i][k]: lane l holds i = l%32,
-// k = 8 (l/32) + e; B[k][j]: k = 8 (l/32) + e, j = l%32; D[i][j]: lane l
-// register r holds i = 8 (r/4) + 4 (l/32) + r%4, j = l%32.
+// wave takes two rows of 32 voxel columns (two 32-column MFMA tiles).
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
@@ -510,9 +514,8 @@ __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_f
 // node offset o(b) of local node b (fea.cpp node order)
 __device__ __forceinline__ constexpr int off_x(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
 __device__ __forceinline__ constexpr int off_y(int b) { return ((b & 3) <= 1) ? 1 : 0; }
-__device__ __forceinline__ constexpr int off_z(int b) { return b >> 2; }
 
-template <int NZ, bool FUSED>
+template <int NZ, bool FUSED, int PIECES>
 __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __restrict__ KA,
                                                const float* __restrict__ Ecell, const uint8_t* __restrict__ nodeFixed,
                                                const float* __restrict__ u, const float* __restrict__ z,
@@ -522,8 +525,8 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
 {
     using namespace wk;
     if (FUSED && *done) return;
-    __shared__ float sP[3][3][PN];        // three staged node planes, [buffer][component][node]
-    __shared__ float sBot[12][CX * CY];   // bottom-face forces of the current voxel layer
+    __shared__ float sP[3][3][PN];          // three staged node planes, [buffer][component][node]
+    __shared__ float sBot[12][CX * CY];     // bottom-face forces of the current voxel layer
     __shared__ float sTop[2][12][CX * CY];  // top-face forces, by layer parity
     __shared__ double sh[4];
     const int tid = threadIdx.x;
@@ -531,19 +534,17 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
     const int jx = lane & 31, kh = lane >> 5;
     const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
-    // A operands (element matrix, split hi/lo), per K step: [ks][hl][lane][8]
-    bf16x8 Ah[2], Al[2];
+    // A operands (element matrix pieces), [K step][piece][lane][8]
+    bf16x8 A[2][PIECES];
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-        u16x8 h, l;
+    for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            h[e] = KA[((ks * 2 + 0) * 64 + lane) * 8 + e];
-            l[e] = KA[((ks * 2 + 1) * 64 + lane) * 8 + e];
+        for (int pc = 0; pc < PIECES; ++pc) {
+            u16x8 h;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) h[e] = KA[((ks * PIECES + pc) * 64 + lane) * 8 + e];
+            A[ks][pc] = __builtin_bit_cast(bf16x8, h);
         }
-        Ah[ks] = __builtin_bit_cast(bf16x8, h);
-        Al[ks] = __builtin_bit_cast(bf16x8, l);
-    }
     auto stage = [&](int Z, int q) {
         const bool zin = Z >= 0 && Z < g.nnz;
         const bool own = Z >= z0 && Z < z0 + NZ;
@@ -585,44 +586,40 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
             f32x16 acc = {};
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks) {
-                u16x8 bh, bl;
+                // component of this lane's 8 K entries: ks 0 -> kh, ks 1 -> 2 (lanes 32..63: padding)
+                const int c = ks == 0 ? kh : 2;
+                const bool live = ks == 0 || kh == 0;
+                u16x8 B[PIECES];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    // DOF k = 16 ks + 8 kh + e of the voxel = 3 b + c (zero padding beyond 24)
+                for (int b = 0; b < 8; ++b) {
                     float v = 0.f;
-                    if (ks == 0) {
-                        const int k0 = 8 * kh + e, k1 = k0;   // kh = 0: DOFs 0..7, kh = 1: DOFs 8..15
-                        const int b = k1 / 3, c = k1 - 3 * b;
-                        const int q = (b >> 2) ? qt : qb;
-                        v = sP[q][c][(ty + off_y(b)) * PX + jx + off_x(b)];
-                        (void)k0;
-                    } else if (kh == 0) {
-                        const int k1 = 16 + e;                    // DOFs 16..23
-                        const int b = k1 / 3, c = k1 - 3 * b;
-                        v = sP[qt][c][(ty + off_y(b)) * PX + jx + off_x(b)];
-                    }
+                    if (live) v = sP[(b >> 2) ? qt : qb][c][(ty + off_y(b)) * PX + jx + off_x(b)];
                     const unsigned short h = bf16_rne(v);
-                    bh[e] = h;
-                    bl[e] = bf16_rne(v - bf16_val(h));
+                    B[0][b] = h;
+                    const float r1 = v - bf16_val(h);
+                    const unsigned short m = bf16_rne(r1);
+                    B[1][b] = m;
+                    if (PIECES == 3) B[PIECES - 1][b] = bf16_rne(r1 - bf16_val(m));
                 }
-                const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah[ks], Bh, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah[ks], Bl, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al[ks], Bh, acc, 0, 0, 0);
+                const bf16x8 B0 = __builtin_bit_cast(bf16x8, B[0]), B1 = __builtin_bit_cast(bf16x8, B[1]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B1, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B0, acc, 0, 0, 0);
+                if (PIECES == 3) {
+                    const bf16x8 B2 = __builtin_bit_cast(bf16x8, B[PIECES - 1]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B2, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B1, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][PIECES - 1], B0, acc, 0, 0, 0);
+                }
             }
-            // force rows i = 8 (r/4) + 4 kh + r%4 of voxel column (jx, ty), scaled by E
+            // registers r < 12: component r/4 of node 4 kh + r%4 of voxel column (jx, ty)
             const int col = ty * CX + jx;
+            float* dst = kh ? &sTop[par][0][0] : &sBot[0][0];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = 8 * (r / 4) + 4 * kh + (r % 4);
-                if (i >= 24) continue;
-                const int b = i / 3, c = i - 3 * b;
-                const int slot = (2 * off_y(b) + off_x(b)) * 3 + c;
-                const float f = E * acc[r];
-                if (b < 4)
-                    sBot[slot][col] = f;
-                else
-                    sTop[par][slot][col] = f;
+            for (int r = 0; r < 12; ++r) {
+                const int c = r / 4, bq = r % 4;   // face node bq: corner (off_x, off_y)
+                const int slot = (2 * off_y(bq) + off_x(bq)) * 3 + c;
+                dst[slot * (CX * CY) + col] = E * acc[r];
             }
         }
         __syncthreads();
@@ -1106,8 +1103,9 @@ struct t3d_mixed {
     t3d_mg* M = nullptr;
     float* Ke32 = nullptr;     // element matrix (node-gather form)
     float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
-    unsigned short* KA = nullptr;   // element matrix as split-bf16 MFMA A operands
-    int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA split-bf16
+    unsigned short* KA2 = nullptr;  // element matrix as 2-piece bf16 MFMA A operands
+    unsigned short* KA3 = nullptr;  // ... and as 3-piece
+    int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6
     float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
     float* r32[8] = {};
     float* x32[8] = {};
@@ -1170,13 +1168,18 @@ static dim3 kw_grid(const Grid& g)
                 unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
 }
 
-// y = K u in FP32; form 0: Walsh form, 1: node-gather form, 2: MFMA split-bf16
+// y = K u in FP32; form 0: Walsh form, 1: node-gather form, 2: MFMA bf16 x3,
+// 3: MFMA bf16 x6
 static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 {
     t3d_problem* p = s->p;
     if (form == 2)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
+                           nullptr, nullptr, y, nullptr, nullptr, nullptr);
+    else if (form == 3)
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
                            nullptr, nullptr, y, nullptr, nullptr, nullptr);
     else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
@@ -1197,8 +1200,12 @@ static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* A
 {
     t3d_problem* p = s->p;
     if (s->form == 2)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
+                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+    else if (s->form == 3)
+        hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+                           (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
                            pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
     else
         hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
@@ -1270,7 +1277,7 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->KA, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
+    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 8; ++l) {
@@ -1348,10 +1355,12 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
         ntmp = std::max<int64_t>(ntmp, 3 * int64_t(Lc.nnx) * Lf.nny * Lf.nnz);
     }
-    // MFMA A operands: A[i][k] = Ke[i][k] (24x24 padded to 32x32), split into
-    // bf16 hi + lo (round to nearest even), in the instruction's lane layout
-    // [K step][hi/lo][lane][8]: lane l holds row l%32, columns 16 ks + 8 (l/32) + e
-    std::vector<unsigned short> KAh(2 * 2 * 64 * 8);
+    // MFMA A operands: A[i][k] = Ke[3 b_i + c_i][3 b_k + c_k] for the
+    // component-major indices i = 8 c_i + b_i, k = 8 c_k + b_k (24x24 padded to
+    // 32x32), split into 2 or 3 bf16 pieces (round to nearest even), in the
+    // instruction's lane layout [K step][piece][lane][8]: lane l holds row l%32,
+    // columns 16 ks + 8 (l/32) + e
+    std::vector<unsigned short> KAh2(2 * 2 * 64 * 8), KAh3(2 * 3 * 64 * 8);
     {
         auto rne = [](float x) {
             unsigned u;
@@ -1368,15 +1377,26 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
             for (int l = 0; l < 64; ++l)
                 for (int e = 0; e < 8; ++e) {
                     const int i = l % 32, k = 16 * ks + 8 * (l / 32) + e;
-                    const float a = (i < 24 && k < 24) ? float(p->host.Ke[i * 24 + k]) : 0.0f;
+                    float a = 0.0f;
+                    if (i < 24 && k < 24) {
+                        const int ci = i / 8, bi = i % 8, ck = k / 8, bk = k % 8;
+                        a = float(p->host.Ke[(3 * bi + ci) * 24 + 3 * bk + ck]);
+                    }
                     const unsigned short h = rne(a);
-                    KAh[((ks * 2 + 0) * 64 + l) * 8 + e] = h;
-                    KAh[((ks * 2 + 1) * 64 + l) * 8 + e] = rne(a - val(h));
+                    const float r1 = a - val(h);
+                    const unsigned short m = rne(r1);
+                    KAh2[((ks * 2 + 0) * 64 + l) * 8 + e] = h;
+                    KAh2[((ks * 2 + 1) * 64 + l) * 8 + e] = m;
+                    KAh3[((ks * 3 + 0) * 64 + l) * 8 + e] = h;
+                    KAh3[((ks * 3 + 1) * 64 + l) * 8 + e] = m;
+                    KAh3[((ks * 3 + 2) * 64 + l) * 8 + e] = rne(r1 - val(m));
                 }
     }
     if (hipMalloc(&s->Ke32, sizeof(float) * 576) != hipSuccess ||
-        hipMalloc(&s->KA, sizeof(unsigned short) * KAh.size()) != hipSuccess ||
-        hipMemcpy(s->KA, KAh.data(), sizeof(unsigned short) * KAh.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->KA2, sizeof(unsigned short) * KAh2.size()) != hipSuccess ||
+        hipMemcpy(s->KA2, KAh2.data(), sizeof(unsigned short) * KAh2.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->KA3, sizeof(unsigned short) * KAh3.size()) != hipSuccess ||
+        hipMemcpy(s->KA3, KAh3.data(), sizeof(unsigned short) * KAh3.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(s->Ke32, K32, sizeof(float) * 576, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->KW, sizeof(float) * wk::NNZ) != hipSuccess ||
         hipMemcpy(s->KW, KW, sizeof(float) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1426,36 +1446,38 @@ int t3d_mixed_setup(t3d_mixed* s)
 // form 0: Walsh form (used by the solver), 1: node-gather form
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u, float* y)
 {
-    T3D_CHECK(s && u && y && form >= 0 && form <= 2, T3D_ERR_ARG, "t3d_kxu_f32: bad argument");
+    T3D_CHECK(s && u && y && form >= 0 && form <= 3, T3D_ERR_ARG, "t3d_kxu_f32: bad argument");
     launch_kx32(s, form, u, y);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(s->p->stream));
     return T3D_OK;
 }
 
-// selects the operator of the solver: 0 Walsh form (default), 2 MFMA split-bf16
+// selects the operator of the solver: 0 Walsh form (default), 2 MFMA bf16 x3,
+// 3 MFMA bf16 x6
 int t3d_mixed_set_operator(t3d_mixed* s, int form)
 {
-    T3D_CHECK(s && (form == 0 || form == 2), T3D_ERR_ARG, "t3d_mixed_set_operator: form must be 0 or 2");
+    T3D_CHECK(s && (form == 0 || form == 2 || form == 3), T3D_ERR_ARG,
+              "t3d_mixed_set_operator: form must be 0, 2 or 3");
     s->form = form;
     return T3D_OK;
 }
 
 // HIP-event timing on the problem's stream, average milliseconds per launch:
-// op 0 / 1 / 2: the FP32 operator y = K u of form 0 (Walsh) / 1 (node gather) /
-// 2 (MFMA split-bf16); op 3: the solver's fused kernel of the selected
-// operator (p = z + beta u staged and written to the solver's p buffer,
-// y = K p, block partials of p.y), as launched once per PCG iteration
+// op 0..3: the FP32 operator y = K u of form 0 (Walsh) / 1 (node gather) /
+// 2 (MFMA bf16 x3) / 3 (MFMA bf16 x6); op 4: the solver's fused kernel of the
+// selected operator (p = z + beta u staged and written to the solver's p
+// buffer, y = K p, block partials of p.y), as launched once per PCG iteration
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
 {
-    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 3, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 4, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
     t3d_problem* p = s->p;
     hipEvent_t e0, e1;
     T3D_HIP(hipEventCreate(&e0));
     T3D_HIP(hipEventCreate(&e1));
     T3D_HIP(hipEventRecord(e0, p->stream));
     for (int i = 0; i < reps; ++i) {
-        if (op < 3)
+        if (op < 4)
             launch_kx32(s, op, u, y);
         else
             launch_fused(s, u, s->pB, y);

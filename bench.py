@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--tight-tol", type=float, default=1e-5)
     ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--true-every", type=int, default=50, help="residual replacement period of the mixed solver")
+    ap.add_argument("--design", choices=["uniform", "random-filtered"], default="uniform",
+                    help="uniform rho = 0.12, or rho ~ U[1e-3, 1] (seeded) PDE-filtered with rmin = 2")
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--operator", choices=["walsh", "mfma3", "mfma6"], default="walsh",
+                    help="FP32 operator of the mixed solve: Walsh form (FMA), MFMA bf16 x3 or MFMA bf16 x6")
     ap.add_argument("--kxu-reps", type=int, default=50)
     ap.add_argument("--kxu-only", action="store_true", help="only the K*u timing blocks (for counter runs)")
     ap.add_argument("--copy-only", action="store_true", help="only the streaming-copy calibration kernel")
@@ -96,6 +101,15 @@ def cpu_baseline(args, E, iters):
     }
 
 
+def mfma_flops(p, products):
+    """bf16 MFMA flops the split-bf16 operator issues per apply: per 32 voxels of
+    a tile row, 2 K steps x `products` products of a 32x32x16 MFMA (2*32*32*16 flop)."""
+    nnx, nny, nnz = p.nnx, p.nny, p.nnz
+    tiles = -(-nnx // 31) * -(-nny // 7) * -(-nnz // 16)
+    steps_per_tile = 17                      # 16 node planes + the voxel layer below
+    return tiles * steps_per_tile * 8 * 2 * products * 2 * 32 * 32 * 16
+
+
 def true_residual(t3d, p, db, dx):
     dy = t3d.DeviceArray(p.nDOF)
     p.kxu_f64(dx, dy)
@@ -116,8 +130,14 @@ def main():
     p = t3d.Problem(args.nely, args.nelx, args.nelz)
     log("problem %dx%dx%d: %d elements, %d free DOFs (setup %.1fs)" % (args.nely, args.nelx, args.nelz, p.ne, p.nF,
                                                                          time.time() - t0))
-    # SIMP moduli of the uniform design rho = V0 (driver.cpp:117-121), Emin = 1e-9
-    dx = t3d.DeviceArray.from_host(np.full(p.ne, np.float32(0.12), np.float32))
+    # SIMP moduli (driver.cpp:117-121), Emin = 1e-9, of the uniform design rho = V0
+    # or of seeded random densities smoothed by the PDE filter (rmin = 2)
+    if args.design == "uniform":
+        dx = t3d.DeviceArray.from_host(np.full(p.ne, np.float32(0.12), np.float32))
+    else:
+        raw = t3d.DeviceArray.from_host(np.random.default_rng(args.seed).uniform(1e-3, 1.0, p.ne).astype(np.float32))
+        dx = t3d.DeviceArray(p.ne, np.float32)
+        t3d.Filter(p, 2.0).apply(raw, dx)
     dE = t3d.DeviceArray(p.ne)
     t3d.simp(p, dx, 1.0, float(np.float32(1e-9)), dE)
     p.set_modulus(dE)
@@ -134,6 +154,8 @@ def main():
     M.setup(dE)
     mx = t3d.Mixed(p, M)
     mx.setup()
+    form = {"walsh": 0, "mfma3": 2, "mfma6": 3}[args.operator]
+    mx.set_operator(form)
     log("MG: levels %s, coarsest Cholesky N=%d (setup %.2fs)" % (M.dims, M.info()[0], time.time() - t0))
 
     # ---- K*u blocks: HIP events around back-to-back launches on the problem stream
@@ -145,15 +167,24 @@ def main():
     y32 = t3d.DeviceArray(p.nDOF, np.float32)
     mx.time_kxu(u32, y32, 3, 0)
     kxu32_ms = mx.time_kxu(u32, y32, args.kxu_reps, 0)
+    mx.time_kxu(u32, y32, 3, 2)
+    kxum_ms = mx.time_kxu(u32, y32, args.kxu_reps, 2)
     mx.time_kxu(u32, y32, 3, 3)
-    fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 3)
+    kxum6_ms = mx.time_kxu(u32, y32, args.kxu_reps, 3)
+    mx.time_kxu(u32, y32, 3, 4)
+    fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 4)
     b64 = 8 * (2 * p.nF + p.ne)       # read u, write y (free DOFs), read E
     b32 = 4 * (2 * p.nF + p.ne)
     bfused = 4 * (4 * p.nF + p.ne)    # read p_prev, z; write p, y; read E
     log("K*u FP64 (plain, bit-exact): %.4f ms/apply, %.0f GB/s algorithmic, %.2f TFLOP/s"
         % (kxu64_ms, b64 / kxu64_ms / 1e6, 1152 * p.ne / kxu64_ms / 1e9))
     log("K*u FP32 Walsh form: %.4f ms/apply, %.0f GB/s algorithmic" % (kxu32_ms, b32 / kxu32_ms / 1e6))
-    log("K*u FP32 fused (solver kernel): %.4f ms/launch, %.0f GB/s algorithmic" % (fused_ms, bfused / fused_ms / 1e6))
+    log("K*u FP32 MFMA bf16 x3: %.4f ms/apply, %.0f GB/s algorithmic, %.1f TFLOP/s bf16 MFMA issued"
+        % (kxum_ms, b32 / kxum_ms / 1e6, mfma_flops(p, 3) / kxum_ms / 1e9))
+    log("K*u FP32 MFMA bf16 x6: %.4f ms/apply, %.0f GB/s algorithmic, %.1f TFLOP/s bf16 MFMA issued"
+        % (kxum6_ms, b32 / kxum6_ms / 1e6, mfma_flops(p, 6) / kxum6_ms / 1e9))
+    log("K*u FP32 fused (solver kernel, %s): %.4f ms/launch, %.0f GB/s algorithmic"
+        % (args.operator, fused_ms, bfused / fused_ms / 1e6))
     if args.kxu_only:
         return
 
@@ -220,7 +251,9 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log("CPU baseline skipped: %r" % (e,))
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "kxu_f32_fused_192x384x192_traffic.json")
+    tpath = os.path.join(REPO, "profiles", "kxu_f32_%s_%dx%dx%d_traffic.json"
+                         % ("fused" if form == 0 else "mfma%d_fused" % (3 if form == 2 else 6), args.nely,
+                            args.nelx, args.nelz))
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -236,12 +269,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (uniform rho=0.12 SIMP moduli, reference cantilever BCs/load)",
+        "data": ("synthetic (uniform rho=0.12 SIMP moduli, reference cantilever BCs/load)" if args.design == "uniform"
+                 else "synthetic (rho ~ U[1e-3,1], seed %d, PDE-filtered rmin=2; SIMP moduli; reference "
+                      "cantilever BCs/load)" % args.seed),
         "config": {
             "workload": "MG-PCG solve from x0=0 to true residual < %.0e, cuboid %dx%dx%d (nely x nelx x nelz), "
-                        "%d free DOFs; mixed precision: FP32 Walsh-form K*u and FP32 V-cycle, FP64 dots/alpha/"
+                        "%d free DOFs; mixed precision: FP32 %s K*u and FP32 V-cycle, FP64 dots/alpha/"
                         "beta/true residual, flexible beta, residual replacement every %d iterations"
-                        % (args.tol, args.nely, args.nelx, args.nelz, p.nF, args.true_every),
+                        % (args.tol, args.nely, args.nelx, args.nelz, p.nF,
+                           {0: "Walsh-form", 2: "MFMA bf16 x3", 3: "MFMA bf16 x6"}[form], args.true_every),
             "tol": args.tol,
             "max_iters": args.maxit,
             "pcg_iters": it_m,
@@ -256,9 +292,15 @@ def main():
             "kxu_fp64_gbps": b64 / kxu64_ms / 1e6,
             "kxu_fp32_walsh_ms": kxu32_ms,
             "kxu_fp32_walsh_gbps": b32 / kxu32_ms / 1e6,
+            "kxu_fp32_mfma_bf16x3_ms": kxum_ms,
+            "kxu_fp32_mfma_bf16x3_bf16_tflops_issued": mfma_flops(p, 3) / kxum_ms / 1e9,
+            "kxu_fp32_mfma_bf16x6_ms": kxum6_ms,
+            "kxu_fp32_mfma_bf16x6_bf16_tflops_issued": mfma_flops(p, 6) / kxum6_ms / 1e9,
         },
         "roofline": {
-            "kernel": "fused FP32 Walsh-form K*u of the mixed solve (k_kxw32<16,true>)",
+            "kernel": ("fused FP32 Walsh-form K*u of the mixed solve (k_kxw32<16,true>)" if form == 0
+                       else "fused FP32 MFMA bf16 x%d K*u of the mixed solve (k_kxm32<16,true,%d>)"
+                       % (3 if form == 2 else 6, form)),
             "bound": "hbm",
             "achieved": bfused / fused_ms / 1e6,
             "peak": HBM_PEAK_GBPS,

@@ -139,14 +139,14 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, do
  * in single precision; fixed DOFs of y are 0).  form 0: Walsh form (45-nonzero
  * element matrix in the voxel's Walsh-Hadamard basis, FMA), form 1: node gather
  * with the full 24x24 element matrix (FMA), form 2: the 24x24 x 24xN voxel
- * contraction on MFMA as split-bf16 x3 products accumulated in FP32 */
+ * contraction on MFMA as split-bf16 x3 products (2-piece operands) accumulated
+ * in FP32, form 3: the same with 3-piece operands and 6 products (bf16 x6) */
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_dev);
-/* operator of t3d_pcg_mixed: 0 Walsh form (default), 2 MFMA split-bf16 */
+/* operator of t3d_pcg_mixed: 0 Walsh form (default), 2 MFMA bf16 x3, 3 MFMA bf16 x6 */
 int t3d_mixed_set_operator(t3d_mixed* s, int form);
 /* average milliseconds per launch over reps launches, HIP events on the
- * problem's stream: op 0/1/2 the FP32 operator of form 0/1/2; op 3 the
- * solver's fused kernel of the selected operator (p = z + beta*u, y = K p,
- * p.y partials) */
+ * problem's stream: op 0..3 the FP32 operator of form 0..3; op 4 the solver's
+ * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials) */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 
 /* ---- compliance ----
