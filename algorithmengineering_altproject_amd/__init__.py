@@ -84,6 +84,14 @@ def lib():
         "t3d_mixed_destroy": (C.c_int, [vp]),
         "t3d_mixed_setup": (C.c_int, [vp]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
+        "t3d_batch_create": (C.c_int, [C.c_int, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
+        "t3d_batch_destroy": (C.c_int, [vp]),
+        "t3d_batch_problem": (C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p)]),
+        "t3d_batch_mg": (C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p)]),
+        "t3d_batch_setup": (C.c_int, [vp, vp]),
+        "t3d_batch_set_tiles": (C.c_int, [vp, C.c_int, C.c_int]),
+        "t3d_batch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "t3d_batch_solve": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "t3d_mixed_set_operator": (C.c_int, [vp, C.c_int]),
         "t3d_mixed_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
         "t3d_pcg_mixed": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
@@ -162,9 +170,23 @@ class Problem:
         (self.resX, self.resY, self.resZ, self.ne, self.nN, self.nDOF, self.nF,
          self.nnx, self.nny, self.nnz) = [int(v) for v in s]
 
+    @classmethod
+    def from_handle(cls, h):
+        """A view of a problem owned elsewhere (e.g. by a Batch); not destroyed by Python."""
+        obj = cls.__new__(cls)
+        obj.h = None
+        obj._view = h
+        s = (C.c_int64 * 10)()
+        check(lib().t3d_problem_sizes(h, s), "t3d_problem_sizes")
+        (obj.resX, obj.resY, obj.resZ, obj.ne, obj.nN, obj.nDOF, obj.nF,
+         obj.nnx, obj.nny, obj.nnz) = [int(v) for v in s]
+        obj.h = h
+        obj._owned = False
+        return obj
+
     def __del__(self):
         try:
-            if self.h:
+            if self.h and getattr(self, "_owned", True):
                 lib().t3d_problem_destroy(self.h)
                 self.h = None
         except Exception:
@@ -414,6 +436,54 @@ class Mixed:
         check(lib().t3d_pcg_mixed(self.h, db.ptr, dx.ptr, tol, maxIt, trueEvery, int(flexible), C.byref(it)),
               "t3d_pcg_mixed")
         return it.value
+
+
+class Batch:
+    """Many problems solved together by the batched mixed-precision MG-PCG
+    (t3d_batch_*): dims = [(nely, nelx, nelz), ...]."""
+
+    def __init__(self, dims, Emin=1e-9):
+        n = len(dims)
+        ny = np.array([d[0] for d in dims], np.int32)
+        nx = np.array([d[1] for d in dims], np.int32)
+        nz = np.array([d[2] for d in dims], np.int32)
+        h = C.c_void_p()
+        check(lib().t3d_batch_create(n, _ptr(ny), _ptr(nx), _ptr(nz), Emin, C.byref(h)), "t3d_batch_create")
+        self.h = h
+        self.n = n
+        self.problems = []
+        for i in range(n):
+            ph = C.c_void_p()
+            check(lib().t3d_batch_problem(self.h, i, C.byref(ph)), "t3d_batch_problem")
+            self.problems.append(Problem.from_handle(ph))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def setup(self, dEs):
+        ptrs = (C.c_void_p * self.n)(*[d.ptr.value for d in dEs])
+        check(lib().t3d_batch_setup(self.h, ptrs), "t3d_batch_setup")
+
+    def set_tiles(self, kwNZ, rowsPerBlock):
+        check(lib().t3d_batch_set_tiles(self.h, kwNZ, rowsPerBlock), "t3d_batch_set_tiles")
+
+    def info(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib().t3d_batch_info(self.h, C.byref(a), C.byref(b)), "t3d_batch_info")
+        return a.value, b.value
+
+    def solve(self, dbs, dxs, tol, maxIt, trueEvery=50, flexible=True, useGraph=True):
+        bp = (C.c_void_p * self.n)(*[d.ptr.value for d in dbs])
+        xp = (C.c_void_p * self.n)(*[d.ptr.value for d in dxs])
+        it = np.zeros(self.n, np.int32)
+        check(lib().t3d_batch_solve(self.h, bp, xp, tol, maxIt, trueEvery, int(flexible), int(useGraph), _ptr(it)),
+              "t3d_batch_solve")
+        return it
 
 
 def pcg(prob, M, db, dx, tol, maxIt):

@@ -1,0 +1,710 @@
+// batch.hip -- many independent problems of uneven sizes solved together with
+// the mixed-precision MG-PCG (the batching form; the single-problem solver is
+// mixed.hip).
+//
+// Every kernel of an iteration is launched once for the whole batch: a
+// device-side block map sends block b to (problem, block of that problem),
+// and each problem's geometry and buffers come from a device descriptor.  The
+// problems iterate in lockstep; a converged problem's blocks return at once
+// (its done flag), so one launch covers every active problem.  The
+// per-iteration launch sequence is captured once per p-buffer parity into a
+// hipGraph and replayed; residual-replacement iterations (every `trueEvery`)
+// and the true-residual confirmation of a converged problem run outside the
+// graphs with the single-problem kernels.  The Walsh-form tile depth and the
+// rows per block of the row kernels are chosen per batch by timing the
+// candidates on the batch's own data (autotuner), unless fixed by the caller.
+//
+// Reference: PCG_free (cpp/src/solver.cpp:87-217) with the V-cycle
+// (cpp/src/multigrid/preconditioner.cpp:88-265) per problem; the batch has no
+// reference counterpart (the reference runs simulation_count problems one
+// after another, cpp/src/driver.cpp:23-218).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "chol.hpp"
+#include "mixed_internal.hpp"
+
+namespace t3d {
+
+struct BLevel {
+    LevelF L;
+    int span;               // restriction span into this level
+    const uint8_t* mask;
+    const float* Dinv;
+    float* r;
+    float* x;
+};
+
+// device descriptor of one problem of the batch
+struct BProb {
+    Grid g;
+    int nlev;
+    BLevel lv[8];
+    float weight;
+    int Nc;
+    const double* L;
+    const double* LT;
+    double* rc64;
+    double* xc64;
+    const float* Ecell;
+    const uint8_t* nodeFixed;
+    float *pA, *pB, *Ap, *z, *r, *dx, *tmp;
+    double* part;
+    double* S;
+    int* flags;
+    unsigned kgx, kgy;   // Walsh-form tile grid (x, y) at the current tile depth
+    unsigned kwTiles;    // tiles of the Walsh-form operator
+    unsigned rowBlocks;  // blocks of the row kernels at the current rows per block
+};
+
+struct BMap {
+    int prob;
+    int local;
+};
+
+// ---- batched kernels: one block of a map entry per block ----
+template <int NZ>
+__global__ void __launch_bounds__(256) kb_kxw32(const BProb* __restrict__ P, const BMap* __restrict__ map,
+                                               const float* __restrict__ KW, int parity)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    const unsigned loc = unsigned(m.local);
+    const unsigned tx = loc % q.kgx, ty = (loc / q.kgx) % q.kgy, tz = loc / (q.kgx * q.kgy);
+    const float* pprev = parity ? q.pB : q.pA;
+    float* pcur = parity ? q.pA : q.pB;
+    kxw32_body<NZ, true>(q.g, KW, q.Ecell, q.nodeFixed, pprev, q.z, pcur, q.Ap, q.part, q.S, q.flags, tx, ty, tz,
+                         loc, 0);
+}
+
+__global__ void __launch_bounds__(1024) kb_fin_alpha(const BProb* __restrict__ P)
+{
+    const BProb& q = P[blockIdx.x];
+    fin_alpha_body(q.part, q.kwTiles, q.S, q.flags, 0, 0, 0, 0, 0);
+}
+
+// rn2 ; the convergence test of iteration *itDev when check
+__global__ void __launch_bounds__(1024) kb_fin_rn2(const BProb* __restrict__ P, const int* __restrict__ itDev,
+                                                   int check)
+{
+    const BProb& q = P[blockIdx.x];
+    fin_rn2_body(q.part, q.rowBlocks, q.S, q.flags, *itDev, check, 1, 0, 0, 0, 0, 0);
+}
+
+__global__ void kb_tick(int* __restrict__ itDev) { *itDev += 1; }
+
+template <int R>
+__global__ void __launch_bounds__(256) kb_xr_rows(const BProb* __restrict__ P, const BMap* __restrict__ map,
+                                                  int parity, int withRestrict)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    const float* pcur = parity ? q.pA : q.pB;
+    xr_rows_body<R>(q.g.nnx, q.g.nny * q.g.nnz, pcur, q.Ap, q.dx, q.r, q.part, q.S, q.lv[1].L.nnx, q.lv[1].span,
+                    withRestrict ? q.tmp : nullptr, q.flags, unsigned(m.local), 0, 0, unsigned(m.local), 0);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) kb_rx_rows(const BProb* __restrict__ P, const BMap* __restrict__ map)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    rx_rows_body<R>(q.g.nnx, q.g.nny * q.g.nnz, q.r, q.lv[1].L.nnx, q.lv[1].span, q.tmp, q.flags,
+                    unsigned(m.local), 0, 0, 0, 0);
+}
+
+// coarse-level x-pass of the restriction from level l; local = (fz * nny + fy) * ceil(cnx/64) + bx
+__global__ void __launch_bounds__(64) kb_restrict_x(const BProb* __restrict__ P, const BMap* __restrict__ map, int l)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    const LevelF& Lf = q.lv[l].L;
+    const int cnx = q.lv[l + 1].L.nnx;
+    const unsigned gx = unsigned((cnx + 63) / 64);
+    const unsigned loc = unsigned(m.local);
+    const unsigned bx = loc % gx, fy = (loc / gx) % unsigned(Lf.nny), fz = loc / (gx * unsigned(Lf.nny));
+    restrict_x_body<float>(Lf, cnx, q.lv[l + 1].span, (const float*)q.lv[l].r, q.tmp, q.flags, bx, fy, fz, 0, 0);
+}
+
+// y/z pass of the restriction into level l + 1 (+ coarsest diagonal solve or FP64 copy)
+__global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P, const BMap* __restrict__ map, int l)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    const LevelF& Lc = q.lv[l + 1].L;
+    const unsigned gx = unsigned((Lc.nnx + 63) / 64);
+    const unsigned loc = unsigned(m.local);
+    const unsigned bx = loc % gx, cy = (loc / gx) % unsigned(Lc.nny), cz = loc / (gx * unsigned(Lc.nny));
+    const int L = q.nlev - 1;
+    const bool last = (l + 1 == L);
+    const bool chol = q.Nc > 0;
+    restrict_yz_body(q.lv[l].L, Lc, q.lv[l + 1].span, q.tmp, q.lv[l + 1].mask, q.lv[l + 1].r,
+                     (last && !chol) ? q.lv[L].Dinv : nullptr, q.lv[L].x, (last && chol) ? q.rc64 : nullptr, q.flags,
+                     bx, cy, cz, 0, 0);
+}
+
+// coarsest Cholesky solve and its FP32 copy; one block per listed problem
+__global__ void __launch_bounds__(CHOL_MAX) kb_chol(const BProb* __restrict__ P, const int* __restrict__ list)
+{
+    const BProb& q = P[list[blockIdx.x]];
+    if (*q.flags) return;
+    chol_solve_body(q.Nc, q.L, q.LT, q.rc64, q.lv[q.nlev - 1].mask, q.xc64);
+    __syncthreads();
+    float* xc = q.lv[q.nlev - 1].x;
+    for (int i = threadIdx.x; i < q.Nc; i += blockDim.x) xc[i] = float(q.xc64[i]);
+}
+
+// coarse-level prolongation + smoothing into level l (l >= 1)
+__global__ void __launch_bounds__(64) kb_prolong(const BProb* __restrict__ P, const BMap* __restrict__ map, int l)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    const LevelF& Lf = q.lv[l].L;
+    const unsigned gx = unsigned((Lf.nnx + 63) / 64);
+    const unsigned loc = unsigned(m.local);
+    const unsigned bx = loc % gx, fy = (loc / gx) % unsigned(Lf.nny), fz = loc / (gx * unsigned(Lf.nny));
+    prolong_smooth32_body(Lf, q.lv[l + 1].L, q.lv[l + 1].span, q.lv[l + 1].x, q.lv[l].r, q.lv[l].Dinv, q.lv[l].mask,
+                          q.weight, q.lv[l].x, q.flags, bx, fy, fz, 0, 0);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) kb_pro_rows(const BProb* __restrict__ P, const BMap* __restrict__ map)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    pro_rows_body<R>(q.lv[0].L, q.lv[1].L, q.lv[1].span, q.lv[1].x, q.r,
+                     q.lv[0].Dinv, q.lv[0].mask, q.weight, q.z, q.Ap, q.part, q.flags, unsigned(m.local), 0, 0, 0,
+                     q.rowBlocks);
+}
+
+__global__ void __launch_bounds__(1024) kb_fin_beta(const BProb* __restrict__ P, int flexible, int first)
+{
+    const BProb& q = P[blockIdx.x];
+    fin_beta_body(q.part, q.rowBlocks, q.S, flexible, first, q.flags, 0, 0, 0, 0, 0);
+}
+
+// per-problem initial state of a solve: stop, beta, flags
+__global__ void kb_init(const BProb* __restrict__ P, double tol)
+{
+    const BProb& q = P[blockIdx.x];
+    if (threadIdx.x == 0) {
+        const double nb = sqrt(q.S[S_NORMB2]);
+        q.S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
+        q.S[S_BETA] = 0.0;
+        q.flags[0] = 0;
+        q.flags[1] = -1;
+    }
+}
+
+}  // namespace t3d
+
+using namespace t3d;
+
+struct t3d_batch {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    std::vector<t3d_problem*> P;
+    std::vector<t3d_mg*> M;
+    std::vector<t3d_mixed*> X;
+    std::vector<hipStream_t> ownStreams;   // the problems' own streams, restored at destroy
+    BProb* dP = nullptr;
+    std::vector<BProb> hP;
+    // block maps
+    BMap* mapKW = nullptr;
+    unsigned nKW = 0;
+    BMap* mapRows = nullptr;
+    unsigned nRows = 0;
+    BMap* mapRX[8] = {};
+    unsigned nRX[8] = {};
+    BMap* mapYZ[8] = {};
+    unsigned nYZ[8] = {};
+    BMap* mapPro[8] = {};
+    unsigned nPro[8] = {};
+    int* cholList = nullptr;
+    int nChol = 0;
+    int maxLev = 0;
+    int* itDev = nullptr;
+    // tiles
+    int kwNZ = 16, rpb = 4;
+    bool tuned = false, fixedTiles = false;
+    // graphs per parity
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool setupDone = false;
+};
+
+namespace t3d {
+
+static void free_maps(t3d_batch* B)
+{
+    auto fr = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    fr(B->mapKW);
+    fr(B->mapRows);
+    for (int l = 0; l < 8; ++l) {
+        fr(B->mapRX[l]);
+        fr(B->mapYZ[l]);
+        fr(B->mapPro[l]);
+        B->mapRX[l] = B->mapYZ[l] = B->mapPro[l] = nullptr;
+        B->nRX[l] = B->nYZ[l] = B->nPro[l] = 0;
+    }
+    fr(B->cholList);
+    B->mapKW = B->mapRows = nullptr;
+    B->cholList = nullptr;
+    for (int k = 0; k < 2; ++k)
+        if (B->graph[k]) {
+            (void)hipGraphExecDestroy(B->graph[k]);
+            B->graph[k] = nullptr;
+        }
+}
+
+static int upload_map(const std::vector<BMap>& h, BMap** d, unsigned* n)
+{
+    *n = unsigned(h.size());
+    if (h.empty()) return T3D_OK;
+    T3D_HIP(hipMalloc(d, sizeof(BMap) * h.size()));
+    T3D_HIP(hipMemcpy(*d, h.data(), sizeof(BMap) * h.size(), hipMemcpyHostToDevice));
+    return T3D_OK;
+}
+
+// descriptors and block maps for the current tile depth and rows per block
+static int batch_build(t3d_batch* B)
+{
+    free_maps(B);
+    B->hP.assign(B->n, BProb{});
+    std::vector<BMap> kw, rows, rx[8], yz[8], pro[8];
+    std::vector<int> chol;
+    B->maxLev = 0;
+    for (int i = 0; i < B->n; ++i) {
+        t3d_problem* p = B->P[i];
+        t3d_mixed* s = B->X[i];
+        MGView v;
+        int rc = mg_view(B->M[i], v);
+        if (rc) return rc;
+        T3D_CHECK(v.nlev >= 2, T3D_ERR_STATE, "t3d_batch: every hierarchy needs at least two levels");
+        BProb& q = B->hP[i];
+        q.g = p->g;
+        q.nlev = v.nlev;
+        for (int l = 0; l < v.nlev; ++l) {
+            q.lv[l].L = levf(v, l);
+            q.lv[l].span = v.span[l];
+            q.lv[l].mask = v.mask[l];
+            q.lv[l].Dinv = s->D32[l];
+            q.lv[l].r = s->r32[l];
+            q.lv[l].x = s->x32[l];
+        }
+        q.weight = float(v.weight);
+        q.Nc = v.Nc;
+        q.L = v.L;
+        q.LT = v.LT;
+        q.rc64 = s->rc64;
+        q.xc64 = s->xc64;
+        q.Ecell = p->EcellF;
+        q.nodeFixed = p->nodeFixed;
+        q.pA = s->pA;
+        q.pB = s->pB;
+        q.Ap = s->Ap;
+        q.z = s->z;
+        q.r = s->r;
+        q.dx = s->dx;
+        q.tmp = s->tmp;
+        q.part = s->part;
+        q.S = s->sc;
+        q.flags = s->iflags;
+        const Grid& g = p->g;
+        q.kgx = unsigned((g.nnx + wk::NX - 1) / wk::NX);
+        q.kgy = unsigned((g.nny + wk::NY - 1) / wk::NY);
+        const unsigned kgz = unsigned((g.nnz + B->kwNZ - 1) / B->kwNZ);
+        q.kwTiles = q.kgx * q.kgy * kgz;
+        q.rowBlocks = unsigned((int64_t(g.nny) * g.nnz + B->rpb - 1) / B->rpb);
+        for (unsigned t = 0; t < q.kwTiles; ++t) kw.push_back({i, int(t)});
+        for (unsigned t = 0; t < q.rowBlocks; ++t) rows.push_back({i, int(t)});
+        const int L = v.nlev - 1;
+        B->maxLev = std::max(B->maxLev, L);
+        for (int l = 0; l < L; ++l) {
+            const LevelF& Lf = q.lv[l].L;
+            const LevelF& Lc = q.lv[l + 1].L;
+            if (l >= 1) {
+                const unsigned nbx = unsigned(int64_t(Lc.nnx + 63) / 64) * unsigned(Lf.nny) * unsigned(Lf.nnz);
+                for (unsigned t = 0; t < nbx; ++t) rx[l].push_back({i, int(t)});
+                const unsigned npb = unsigned((Lf.nnx + 63) / 64) * unsigned(Lf.nny) * unsigned(Lf.nnz);
+                for (unsigned t = 0; t < npb; ++t) pro[l].push_back({i, int(t)});
+            }
+            const unsigned nyz = unsigned((Lc.nnx + 63) / 64) * unsigned(Lc.nny) * unsigned(Lc.nnz);
+            for (unsigned t = 0; t < nyz; ++t) yz[l].push_back({i, int(t)});
+        }
+        if (v.Nc > 0) chol.push_back(i);
+    }
+    T3D_HIP(hipMemcpy(B->dP, B->hP.data(), sizeof(BProb) * B->n, hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = upload_map(kw, &B->mapKW, &B->nKW))) return rc;
+    if ((rc = upload_map(rows, &B->mapRows, &B->nRows))) return rc;
+    for (int l = 0; l < 8; ++l) {
+        if ((rc = upload_map(rx[l], &B->mapRX[l], &B->nRX[l]))) return rc;
+        if ((rc = upload_map(yz[l], &B->mapYZ[l], &B->nYZ[l]))) return rc;
+        if ((rc = upload_map(pro[l], &B->mapPro[l], &B->nPro[l]))) return rc;
+    }
+    B->nChol = int(chol.size());
+    if (B->nChol) {
+        T3D_HIP(hipMalloc(&B->cholList, sizeof(int) * chol.size()));
+        T3D_HIP(hipMemcpy(B->cholList, chol.data(), sizeof(int) * chol.size(), hipMemcpyHostToDevice));
+    }
+    T3D_HIP(hipDeviceSynchronize());
+    return T3D_OK;
+}
+
+static void launch_kw(t3d_batch* B, int parity)
+{
+    const dim3 grid(B->nKW), blk(256);
+    switch (B->kwNZ) {
+    case 8: hipLaunchKernelGGL(kb_kxw32<8>, grid, blk, 0, B->stream, B->dP, B->mapKW, (const float*)B->X[0]->KW, parity); break;
+    case 32: hipLaunchKernelGGL(kb_kxw32<32>, grid, blk, 0, B->stream, B->dP, B->mapKW, (const float*)B->X[0]->KW, parity); break;
+    default: hipLaunchKernelGGL(kb_kxw32<16>, grid, blk, 0, B->stream, B->dP, B->mapKW, (const float*)B->X[0]->KW, parity); break;
+    }
+}
+
+static void launch_xr(t3d_batch* B, int parity, int withRestrict)
+{
+    const dim3 grid(B->nRows), blk(256);
+    // dynamic LDS: rows per block x the widest row of the batch
+    size_t w = 0;
+    for (const BProb& q : B->hP) w = std::max<size_t>(w, size_t(3) * q.g.nnx);
+    const size_t sh = sizeof(float) * w * B->rpb;
+    switch (B->rpb) {
+    case 2: hipLaunchKernelGGL(kb_xr_rows<2>, grid, blk, sh, B->stream, B->dP, B->mapRows, parity, withRestrict); break;
+    case 8: hipLaunchKernelGGL(kb_xr_rows<8>, grid, blk, sh, B->stream, B->dP, B->mapRows, parity, withRestrict); break;
+    default: hipLaunchKernelGGL(kb_xr_rows<4>, grid, blk, sh, B->stream, B->dP, B->mapRows, parity, withRestrict); break;
+    }
+}
+
+static void launch_rx(t3d_batch* B)
+{
+    size_t w = 0;
+    for (const BProb& q : B->hP) w = std::max<size_t>(w, size_t(3) * q.g.nnx);
+    const size_t sh = sizeof(float) * w * B->rpb;
+    const dim3 grid(B->nRows), blk(256);
+    switch (B->rpb) {
+    case 2: hipLaunchKernelGGL(kb_rx_rows<2>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    case 8: hipLaunchKernelGGL(kb_rx_rows<8>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    default: hipLaunchKernelGGL(kb_rx_rows<4>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    }
+}
+
+static void launch_pro(t3d_batch* B)
+{
+    size_t w = 0;
+    for (const BProb& q : B->hP) w = std::max<size_t>(w, size_t(3) * q.lv[1].L.nnx);
+    const size_t sh = sizeof(float) * w * B->rpb;
+    const dim3 grid(B->nRows), blk(256);
+    switch (B->rpb) {
+    case 2: hipLaunchKernelGGL(kb_pro_rows<2>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    case 8: hipLaunchKernelGGL(kb_pro_rows<8>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    default: hipLaunchKernelGGL(kb_pro_rows<4>, grid, blk, sh, B->stream, B->dP, B->mapRows); break;
+    }
+}
+
+// batched V-cycle: z = M32(r), rz and beta per problem
+static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady)
+{
+    hipStream_t st = B->stream;
+    if (!tmpReady) launch_rx(B);
+    for (int l = 0; l < B->maxLev; ++l) {
+        if (l >= 1 && B->nRX[l])
+            hipLaunchKernelGGL(kb_restrict_x, dim3(B->nRX[l]), dim3(64), 0, st, (const BProb*)B->dP,
+                               (const BMap*)B->mapRX[l], l);
+        if (B->nYZ[l])
+            hipLaunchKernelGGL(kb_restrict_yz, dim3(B->nYZ[l]), dim3(64), 0, st, (const BProb*)B->dP,
+                               (const BMap*)B->mapYZ[l], l);
+    }
+    if (B->nChol)
+        hipLaunchKernelGGL(kb_chol, dim3(B->nChol), dim3(CHOL_MAX), 0, st, (const BProb*)B->dP,
+                           (const int*)B->cholList);
+    for (int l = B->maxLev - 1; l >= 1; --l)
+        if (B->nPro[l])
+            hipLaunchKernelGGL(kb_prolong, dim3(B->nPro[l]), dim3(64), 0, st, (const BProb*)B->dP,
+                               (const BMap*)B->mapPro[l], l);
+    launch_pro(B);
+    hipLaunchKernelGGL(kb_fin_beta, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP, flexible, first);
+}
+
+// one lockstep iteration of every active problem (no residual replacement)
+static void batch_iteration(t3d_batch* B, int parity, int flexible)
+{
+    hipStream_t st = B->stream;
+    launch_kw(B, parity);
+    hipLaunchKernelGGL(kb_fin_alpha, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP);
+    launch_xr(B, parity, 1);
+    hipLaunchKernelGGL(kb_fin_rn2, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP, (const int*)B->itDev, 1);
+    batch_vcycle(B, flexible, 0, true);
+    hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->itDev);
+}
+
+}  // namespace t3d
+
+extern "C" {
+
+int t3d_batch_destroy(t3d_batch* B)
+{
+    if (!B) return T3D_OK;
+    (void)hipDeviceSynchronize();
+    free_maps(B);
+    if (B->dP) (void)hipFree(B->dP);
+    if (B->itDev) (void)hipFree(B->itDev);
+    for (int i = 0; i < B->n; ++i) {
+        if (B->P[i]) B->P[i]->stream = B->ownStreams[i];
+        t3d_mixed_destroy(B->X[i]);
+        t3d_mg_destroy(B->M[i]);
+        t3d_problem_destroy(B->P[i]);
+    }
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+    return T3D_OK;
+}
+
+int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, float Emin, t3d_batch** out)
+{
+    T3D_CHECK(n > 0 && n <= 4096 && nely && nelx && nelz && out, T3D_ERR_ARG, "t3d_batch_create: bad argument");
+    *out = nullptr;
+    t3d_batch* B = new t3d_batch();
+    B->n = n;
+    B->P.assign(n, nullptr);
+    B->M.assign(n, nullptr);
+    B->X.assign(n, nullptr);
+    B->ownStreams.assign(n, nullptr);
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete B;
+        set_error("t3d_batch_create: stream creation failed");
+        return T3D_ERR_HIP;
+    }
+    int rc = T3D_OK;
+    for (int i = 0; i < n && !rc; ++i) {
+        rc = t3d_problem_create_cuboid(nely[i], nelx[i], nelz[i], &B->P[i]);
+        if (rc) break;
+        // every problem of the batch works on the batch's stream
+        B->ownStreams[i] = B->P[i]->stream;
+        B->P[i]->stream = B->stream;
+        rc = t3d_mg_create(B->P[i], 1, 5, 0.6f, Emin, 1000, &B->M[i]);
+    }
+    if (!rc && (hipMalloc(&B->dP, sizeof(BProb) * n) != hipSuccess || hipMalloc(&B->itDev, sizeof(int)) != hipSuccess)) {
+        set_error("t3d_batch_create: allocation failed");
+        rc = T3D_ERR_HIP;
+    }
+    if (rc) {
+        t3d_batch_destroy(B);
+        return rc;
+    }
+    *out = B;
+    return T3D_OK;
+}
+
+int t3d_batch_problem(t3d_batch* B, int i, t3d_problem** p)
+{
+    T3D_CHECK(B && p && i >= 0 && i < B->n, T3D_ERR_ARG, "t3d_batch_problem: bad argument");
+    *p = B->P[i];
+    return T3D_OK;
+}
+
+int t3d_batch_mg(t3d_batch* B, int i, t3d_mg** g)
+{
+    T3D_CHECK(B && g && i >= 0 && i < B->n, T3D_ERR_ARG, "t3d_batch_mg: bad argument");
+    *g = B->M[i];
+    return T3D_OK;
+}
+
+// per-problem element moduli (device pointers, compact order): multigrid set-up
+int t3d_batch_setup(t3d_batch* B, const double* const* E_dev)
+{
+    T3D_CHECK(B && E_dev, T3D_ERR_ARG, "t3d_batch_setup: null argument");
+    int rc;
+    for (int i = 0; i < B->n; ++i) {
+        if ((rc = t3d_mg_setup(B->M[i], E_dev[i]))) return rc;
+        if (!B->X[i] && (rc = t3d_mixed_create(B->P[i], B->M[i], &B->X[i]))) return rc;
+        if ((rc = t3d_mixed_setup(B->X[i]))) return rc;
+    }
+    B->setupDone = true;
+    return batch_build(B);
+}
+
+// kwNZ in {8, 16, 32}, rowsPerBlock in {2, 4, 8}; 0, 0 = autotune on the next solve
+int t3d_batch_set_tiles(t3d_batch* B, int kwNZ, int rowsPerBlock)
+{
+    T3D_CHECK(B, T3D_ERR_ARG, "t3d_batch_set_tiles: null");
+    if (kwNZ == 0 && rowsPerBlock == 0) {
+        B->fixedTiles = false;
+        B->tuned = false;
+        return T3D_OK;
+    }
+    T3D_CHECK((kwNZ == 8 || kwNZ == 16 || kwNZ == 32) && (rowsPerBlock == 2 || rowsPerBlock == 4 || rowsPerBlock == 8),
+              T3D_ERR_ARG, "t3d_batch_set_tiles: kwNZ in {8,16,32}, rowsPerBlock in {2,4,8}");
+    B->kwNZ = kwNZ;
+    B->rpb = rowsPerBlock;
+    B->fixedTiles = true;
+    B->tuned = true;
+    return B->setupDone ? batch_build(B) : T3D_OK;
+}
+
+int t3d_batch_info(const t3d_batch* B, int* kwNZ, int* rowsPerBlock)
+{
+    T3D_CHECK(B, T3D_ERR_ARG, "t3d_batch_info: null");
+    if (kwNZ) *kwNZ = B->kwNZ;
+    if (rowsPerBlock) *rowsPerBlock = B->rpb;
+    return T3D_OK;
+}
+
+// Solves K_i x_i = b_i for every problem (device pointers, full layout, FP64;
+// x_i is the initial guess and the result).  iters[i] = iterations of problem
+// i (maxIt when not converged).  useGraph: replay the captured iterations.
+int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, double tol, int maxIt, int trueEvery,
+                    int flexible, int useGraph, int* iters)
+{
+    T3D_CHECK(B && b && x && iters && maxIt >= 0, T3D_ERR_ARG, "t3d_batch_solve: bad argument");
+    T3D_CHECK(B->setupDone, T3D_ERR_STATE, "t3d_batch_solve: t3d_batch_setup not called");
+    hipStream_t st = B->stream;
+    const int n = B->n;
+    int rc;
+    // per-problem start: r = b - K64 x, |b|, V-cycle of the first iteration
+    for (int i = 0; i < n; ++i) {
+        t3d_problem* p = B->P[i];
+        t3d_mixed* s = B->X[i];
+        const int64_t nD = 3 * p->g.nN;
+        if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
+        if ((rc = canon_dot_dev(p, b[i], b[i], nD, s->sc + S_NORMB2))) return rc;
+        if ((rc = launch_kxu_f64(p, x[i], p->wAp))) return rc;
+        if ((rc = mixed_true_residual(s, b[i]))) return rc;
+        T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * nD, st));
+        T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * nD, st));
+        T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * nD, st));
+    }
+    hipLaunchKernelGGL(kb_init, dim3(n), dim3(64), 0, st, (const BProb*)B->dP, tol);
+    T3D_HIP(hipMemsetAsync(B->itDev, 0, sizeof(int), st));
+    batch_vcycle(B, flexible, 1, false);
+    T3D_HIP(hipGetLastError());
+    // autotuner: time the candidate tiles of the Walsh-form kernel and of the
+    // row kernels on this batch (the timed launches rewrite p, Ap, z and the
+    // partials with the values the first iteration recomputes)
+    if (!B->tuned) {
+        hipEvent_t e0, e1;
+        T3D_HIP(hipEventCreate(&e0));
+        T3D_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        int bestNZ = 16, bestR = 4;
+        for (int nz : {8, 16, 32}) {
+            B->kwNZ = nz;
+            if ((rc = batch_build(B))) return rc;
+            launch_kw(B, 0);
+            T3D_HIP(hipEventRecord(e0, st));
+            for (int k = 0; k < 3; ++k) launch_kw(B, 0);
+            T3D_HIP(hipEventRecord(e1, st));
+            T3D_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            T3D_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bestNZ = nz;
+            }
+        }
+        B->kwNZ = bestNZ;
+        best = 1e30f;
+        for (int r : {2, 4, 8}) {
+            B->rpb = r;
+            if ((rc = batch_build(B))) return rc;
+            launch_pro(B);
+            T3D_HIP(hipEventRecord(e0, st));
+            for (int k = 0; k < 3; ++k) launch_pro(B);
+            T3D_HIP(hipEventRecord(e1, st));
+            T3D_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            T3D_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bestR = r;
+            }
+        }
+        B->rpb = bestR;
+        if ((rc = batch_build(B))) return rc;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        B->tuned = true;
+        // the last candidate's pro_rows left its own partial layout: redo the
+        // first V-cycle's finest level with the chosen tiles
+        batch_vcycle(B, flexible, 1, false);
+    }
+    // graphs of one iteration per parity
+    if (useGraph) {
+        for (int par = 0; par < 2; ++par) {
+            if (B->graph[par]) continue;
+            hipGraph_t gr;
+            T3D_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            batch_iteration(B, par, flexible);
+            T3D_HIP(hipStreamEndCapture(st, &gr));
+            T3D_HIP(hipGraphInstantiate(&B->graph[par], gr, nullptr, nullptr, 0));
+            T3D_HIP(hipGraphDestroy(gr));
+        }
+    }
+    std::vector<int> result(n, maxIt), skipped(n, 0), finished(n, 0);
+    std::vector<int> hf(2 * n);
+    const int CHUNK = 32;
+    int it0 = 0;
+    while (it0 < maxIt) {
+        const int it1 = std::min(maxIt, it0 + CHUNK);
+        for (int it = it0; it < it1; ++it) {
+            const int par = it & 1;
+            const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
+            if (!replace) {
+                if (useGraph)
+                    T3D_HIP(hipGraphLaunch(B->graph[par], st));
+                else
+                    batch_iteration(B, par, flexible);
+            } else {
+                launch_kw(B, par);
+                hipLaunchKernelGGL(kb_fin_alpha, dim3(n), dim3(1024), 0, st, (const BProb*)B->dP);
+                launch_xr(B, par, 0);
+                hipLaunchKernelGGL(kb_fin_rn2, dim3(n), dim3(1024), 0, st, (const BProb*)B->dP,
+                                   (const int*)B->itDev, 0);
+                for (int i = 0; i < n; ++i)
+                    if (!finished[i] && (rc = mixed_replace_residual(B->X[i], b[i], x[i], it))) return rc;
+                batch_vcycle(B, flexible, 0, false);
+                hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->itDev);
+            }
+        }
+        T3D_HIP(hipGetLastError());
+        T3D_HIP(hipStreamSynchronize(st));
+        bool any = false, allDone = true;
+        for (int i = 0; i < n; ++i) {
+            if (finished[i]) continue;
+            T3D_HIP(hipMemcpy(&hf[2 * i], B->X[i]->iflags, sizeof(int) * 2, hipMemcpyDeviceToHost));
+            if (hf[2 * i]) any = true;
+        }
+        if (any) {
+            for (int i = 0; i < n; ++i) {
+                if (finished[i] || !hf[2 * i]) continue;
+                const int itDone = hf[2 * i + 1] - 1;   // global iteration index
+                double tr = 0, stop = 0;
+                if ((rc = mixed_confirm(B->X[i], b[i], x[i], &tr, &stop))) return rc;
+                if (std::sqrt(tr) < stop) {
+                    finished[i] = 1;
+                    result[i] = itDone + 1 - skipped[i];
+                } else {
+                    // resume: idle iterations until it1 do not count; align p parity
+                    skipped[i] += it1 - (itDone + 1);
+                    if ((rc = mixed_resume(B->X[i], itDone, it1, flexible))) return rc;
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) allDone = allDone && finished[i];
+        if (allDone) break;
+        it0 = it1;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (!finished[i]) result[i] = maxIt - skipped[i];
+        if ((rc = mixed_fold(B->X[i], x[i]))) return rc;
+        iters[i] = result[i];
+    }
+    T3D_HIP(hipStreamSynchronize(st));
+    return T3D_OK;
+}
+
+}  // extern "C"

@@ -33,71 +33,9 @@
 
 #include "t3d_internal.hpp"
 
+#include "mx_kernels.hpp"
+
 namespace t3d {
-
-// scalar slots of the mixed solver
-enum {
-    S_NORMB2 = 0, S_STOP = 1, S_RZ = 2, S_ZAP = 4, S_PAP = 5, S_ALPHA = 6, S_BETA = 7, S_RN2 = 8, S_RZNEW = 9
-};
-
-// ---------------------------------------------------------------------------
-// block reductions
-// ---------------------------------------------------------------------------
-// sum over a 256-thread block; the result is valid in thread 0; sh[4]
-__device__ __forceinline__ double block_sum_256(double s, double* sh)
-{
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    return t;
-}
-
-// sum of partial[0..m) in a fixed order by a 1024-thread block (valid in thread 0)
-__device__ __forceinline__ double sum_partials_1024(const double* __restrict__ partial, unsigned m, double* sh)
-{
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < m; i += 1024) s += partial[i];
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 16; ++k) t += sh[k];
-    return t;
-}
-
-__device__ __forceinline__ unsigned linear_block()
-{
-    return (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-}
-
-// ---------------------------------------------------------------------------
-// FP32 operator.  Tile of 32 x 8 x NZ nodes per 256-thread block; the
-// (NZ+2)-plane node halo and the (NZ+1)-plane cell halo are staged in LDS.
-// Each thread owns one (x, y) node column and sweeps it in z with a rolling
-// register window of the 3x3x3 neighbourhood, so every LDS value is read once
-// per column; the 8 incident cells and Ke's corner rows are compile-time
-// indices (Ke through scalar loads).
-// Fused form: the staged values are p = z + beta*p_prev (the tile interior
-// writes p), and the block's p.(K p) partial feeds alpha = rz / max(1e-30, p.Ap).
-// ---------------------------------------------------------------------------
-namespace kx {
-constexpr int TX = 32, TY = 8;
-template <int NZ>
-struct Tile {
-    static constexpr int UX = TX + 2, UY = TY + 2, UZ = NZ + 2, UN = UX * UY * UZ;
-    static constexpr int EX = TX + 1, EY = TY + 1, EZ = NZ + 1, EN = EX * EY * EZ;
-};
-}  // namespace kx
-
-__device__ __forceinline__ constexpr int corner_of(int dx, int dy, int dz)
-{
-    return (dy ? (dx ? 1 : 0) : (dx ? 2 : 3)) + 4 * dz;
-}
 
 template <int NZ, bool FUSED>
 __global__ void __launch_bounds__(256) k_kx32(Grid g, const float* __restrict__ Ke, const float* __restrict__ Ecell,
@@ -229,257 +167,51 @@ __global__ void __launch_bounds__(256) k_kx32(Grid g, const float* __restrict__ 
         if (tid == 0) partial[linear_block()] = t;
     }
 }
-
-// ---------------------------------------------------------------------------
-// FP32 operator, Walsh form.  In the Walsh-Hadamard basis of a voxel's eight
-// nodes, H[m][b] = (-1)^(m . o(b)) with o(b) the node's offset in the voxel,
-// the element matrix K^ = (H x I3) Ke (H x I3)^T has 45 nonzeros (the constant
-// modes carry no energy and the linear, bilinear and trilinear modes decouple
-// under the 2x2x2 Gauss rule), and Ke u = (H x I3)^T (K^/64) (H x I3) u.
-// Per voxel that is 45 FMAs plus butterflies instead of 576 FMAs.
-//
-// A 256-thread block owns 31 x 7 node columns and sweeps NZ node planes in z;
-// thread (tx, ty) owns the voxel column at (x0-1+tx, y0-1+ty).  The 2D Walsh
-// transform of a node plane's 2x2 face is shared by the voxel below and the
-// voxel above; the z-transform pairs consecutive faces.  Each voxel's face
-// forces are accumulated per node plane in Walsh space (top face of the voxel
-// below + bottom face of the voxel above), transformed back to the four
-// corners, and summed over the four voxel columns that share a node through
-// LDS, in a fixed order.
-// ---------------------------------------------------------------------------
-namespace wk {
-constexpr int CX = 32, CY = 8;           // voxel columns (threads)
-constexpr int NX = CX - 1, NY = CY - 1;  // owned node columns
-constexpr int PX = CX + 1, PY = CY + 1;  // staged node plane
-constexpr int PN = PX * PY;
-constexpr int NNZ = 45;
-// nonzero structure of K^, index (4 mz + 2 my + mx) * 3 + c; verified on the
-// host against the element matrix when the operator is created
-__device__ constexpr unsigned char OUT[NNZ] = {3,  3,  3,  4,  4,  5,  5,  6,  6,  7,  7,  7,  8,  8,  9,
-                                               9,  10, 10, 11, 11, 11, 12, 12, 13, 13, 14, 14, 14, 15, 15,
-                                               16, 16, 16, 17, 17, 18, 18, 18, 19, 19, 20, 20, 21, 22, 23};
-__device__ constexpr unsigned char IN[NNZ] = {3,  7,  14, 4,  6,  5,  12, 4,  6,  3,  7,  14, 8,  13, 9,
-                                              20, 10, 17, 11, 16, 18, 5,  12, 8,  13, 3,  7,  14, 15, 19,
-                                              11, 16, 18, 10, 17, 11, 16, 18, 15, 19, 9,  20, 21, 22, 23};
-}  // namespace wk
-
-// 2D Walsh butterfly of the four corners a[oy][ox] -> F[my][mx] (self-inverse up to 4)
-__device__ __forceinline__ void walsh2(float a00, float a10, float a01, float a11, float F[2][2])
+// r = b - Ap (FP64 arithmetic, stored in FP32) ; partial of the FP64 r.r
+__global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const double* __restrict__ b,
+                                                          const double* __restrict__ Ap, float* __restrict__ r,
+                                                          double* __restrict__ partial, const int* __restrict__ done)
 {
-    const float s0 = a00 + a10, d0 = a00 - a10, s1 = a01 + a11, d1 = a01 - a11;
-    F[0][0] = s0 + s1;
-    F[0][1] = d0 + d1;
-    F[1][0] = s0 - s1;
-    F[1][1] = d0 - d1;
-}
-
-template <int NZ, bool FUSED>
-__global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
-                                               const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
-                                               const float* __restrict__ z, float* __restrict__ pcur,
-                                               float* __restrict__ y, double* __restrict__ partial,
-                                               const double* __restrict__ S, const int* __restrict__ done)
-{
-    using namespace wk;
-    if (FUSED && *done) return;
-    __shared__ float sP[3][3][PN];     // three staged node planes, [buffer][component][node]
-    __shared__ float sC[12][CX * CY];  // face-corner forces [(2 oy + ox) * 3 + c][thread]
+    if (done && *done) return;
     __shared__ double sh[4];
-    const int tid = threadIdx.x;
-    const int tx = tid % CX, ty = tid / CX;
-    // hardware tile order: measured faster here than the XCD-grouped order
-    // (which halves the L2 misses but leaves z-neighbours, which share halo
-    // planes, far apart in time)
-    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
-    const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
-    const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
-    const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
-    // Node planes are staged in LDS; rows of PX nodes are 3*PX contiguous
-    // floats in memory.  The plain operator fetches a plane into registers one
-    // step ahead (fetch) and writes it to LDS after the arithmetic (commit), so
-    // the loads are in flight meanwhile; the fused form (two inputs) stages in
-    // place (stage), which keeps its register count at 6 waves per SIMD.
-    constexpr bool AHEAD = !FUSED;
-    constexpr int SPT = (3 * PN + CX * CY - 1) / (CX * CY);
-    float fu[SPT], fz[SPT];
-    auto fetch = [&](int Z) {
-        const bool zin = Z >= 0 && Z < g.nnz;
-#pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const int f = tid + j * CX * CY;
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-            fu[j] = 0.f;
-            fz[j] = 0.f;
-            if (f < 3 * PN && zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
-                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
-                fu[j] = u[gi];
-                if (FUSED) fz[j] = z[gi];
-            }
-        }
-    };
-    auto commit = [&](int Z, int q) {
-        const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
-#pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const int f = tid + j * CX * CY;
-            if (f >= 3 * PN) continue;
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            float v = fu[j];
-            if (FUSED) {
-                v = fmaf(beta, v, fz[j]);
-                const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-                if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY && X < g.nnx && Y < g.nny)
-                    pcur[3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c] = v;
-            }
-            sP[q][c][row * PX + lx] = v;
-        }
-    };
-    // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
-    auto face = [&](int q, float F[3][2][2]) {
-        const int b = ty * PX + tx;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            walsh2(sP[q][c][b], sP[q][c][b + 1], sP[q][c][b + PX], sP[q][c][b + PX + 1], F[c]);
-    };
-    float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
-    // direct staging (fused form): load, combine and store in one pass
-    auto stage = [&](int Z, int q) {
-        const bool zin = Z >= 0 && Z < g.nnz;
-        const bool own = Z >= z0 && Z < z0 + NZ;
-        for (int f = tid; f < 3 * PN; f += CX * CY) {
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-            float v = 0.f;
-            if (zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
-                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
-                v = u[gi];
-                if (FUSED) {
-                    v = fmaf(beta, v, z[gi]);
-                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) pcur[gi] = v;
-                }
-            }
-            sP[q][c][row * PX + lx] = v;
-        }
-    };
-    auto modulus = [&](int cz) {
-        float E = 0.f;
-        if (colIn && cz >= 0 && cz < g.resZ) {
-            E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
-            if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
-        }
-        return E;
-    };
-    if (AHEAD) {
-        fetch(z0 - 1);
-        commit(z0 - 1, 0);
-        fetch(z0);
-        commit(z0, 1);
-    } else {
-        stage(z0 - 1, 0);
-        stage(z0, 1);
+    double s = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        const double v = b[i] - Ap[i];
+        r[i] = float(v);
+        s = fma(v, v, s);
     }
-    float Enext = modulus(z0 - 1);
-    __syncthreads();
-    face(0, Fp);
-    double dot = 0.0;
-    const int nsteps = min(NZ, g.nnz - z0) + 1;
-    for (int s = 0; s < nsteps; ++s) {
-        const int qn = (s + 1) % 3, qo = s % 3;
-        const bool more = s + 1 < nsteps;
-        if (AHEAD && more) fetch(z0 + s + 1);   // plane of the next step, in flight during this one
-        const float E = Enext;         // voxel (cx, cy, cz), cz = z0 - 1 + s
-        const int cz = z0 - 1 + s;
-        if (more) Enext = modulus(cz + 1);
-        face(qn, Fn);
-        float uh[24], vh[24];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int my = 0; my < 2; ++my)
-#pragma unroll
-                for (int mx = 0; mx < 2; ++mx) {
-                    uh[(2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] + Fn[c][my][mx]);
-                    uh[(4 + 2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] - Fn[c][my][mx]);
-                }
-#pragma unroll
-        for (int i = 0; i < 24; ++i) vh[i] = 0.f;
-#pragma unroll
-        for (int k = 0; k < NNZ; ++k) vh[OUT[k]] = fmaf(KW[k], uh[IN[k]], vh[OUT[k]]);
-        // faces: bottom (oz = 0) = vh[mz=0] + vh[mz=1] closes plane cz; top carries to cz + 1
-        float A[3][2][2];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int my = 0; my < 2; ++my)
-#pragma unroll
-                for (int mx = 0; mx < 2; ++mx) {
-                    const float v0 = vh[(2 * my + mx) * 3 + c], v1 = vh[(4 + 2 * my + mx) * 3 + c];
-                    A[c][my][mx] = carry[c][my][mx] + (v0 + v1);
-                    carry[c][my][mx] = v0 - v1;
-                }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
-        if (s > 0) {   // plane z0 - 1 belongs to the block below
-        // plane Z = cz: corner forces of this voxel column, then the node gather
-        const int Z = cz;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float C4[2][2];
-            walsh2(A[c][0][0], A[c][0][1], A[c][1][0], A[c][1][1], C4);
-            // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
-            sC[0 * 3 + c][tid] = C4[0][0];
-            sC[1 * 3 + c][tid] = C4[0][1];
-            sC[2 * 3 + c][tid] = C4[1][0];
-            sC[3 * 3 + c][tid] = C4[1][1];
-        }
-        __syncthreads();
-        const int X = x0 + tx, Y = y0 + ty;
-        if (tx < NX && ty < NY && X < g.nnx && Y < g.nny) {
-            // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)
-            float yv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float c00 = sC[3 * 3 + c][ty * CX + tx];              // column (X-1, Y-1), corner (1,1)
-                const float c10 = sC[2 * 3 + c][ty * CX + tx + 1];          // column (X,   Y-1), corner (0,1)
-                const float c01 = sC[1 * 3 + c][(ty + 1) * CX + tx];        // column (X-1, Y),   corner (1,0)
-                const float c11 = sC[0 * 3 + c][(ty + 1) * CX + tx + 1];    // column (X,   Y),   corner (0,0)
-                yv[c] = (c00 + c10) + (c01 + c11);
-            }
-            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-            const unsigned fx = nodeFixed[n];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float v = ((fx >> c) & 1u) ? 0.f : yv[c];
-                y[3 * n + c] = v;
-                if (FUSED) dot = fma(double(sP[qo][c][(ty + 1) * PX + tx + 1]), double(v), dot);
-            }
-        }
-        }
-        if (more) {
-            if (AHEAD)
-                commit(z0 + s + 1, (s + 2) % 3);
-            else
-                stage(z0 + s + 1, (s + 2) % 3);
-        }
-        __syncthreads();
+    const double t = block_sum_256(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+// x += dx ; dx = 0
+__global__ void __launch_bounds__(256) k_mx_fold(int64_t n, double* __restrict__ x, float* __restrict__ dx,
+                                                 const int* __restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+        x[i] += double(dx[i]);
+        dx[i] = 0.f;
     }
-    if (FUSED) {
-        const double t = block_sum_256(dot, sh);
-        if (tid == 0) partial[linear_block()] = t;
-    }
+}
+__global__ void k_mx_init(double tol, double* __restrict__ S, int* __restrict__ flags)
+{
+    const double nb = sqrt(S[S_NORMB2]);
+    S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
+    S[S_BETA] = 0.0;
+    flags[0] = 0;
+    flags[1] = -1;
+}
+__global__ void k_mx_clear_done(int* __restrict__ flags)
+{
+    flags[0] = 0;
+    flags[1] = -1;
+}
+// FP32 inverse Jacobi diagonal 1 / max(1e-30, D) (the smoother's and the
+// coarsest diagonal solve's divisor, preconditioner.cpp:230-253)
+__global__ void k_dinv(int64_t n, const double* __restrict__ D, float* __restrict__ Dinv)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) Dinv[i] = float(1.0 / (1.0e-30 < D[i] ? D[i] : 1.0e-30));
 }
 
 // ---------------------------------------------------------------------------
@@ -656,505 +388,97 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
     }
 }
 
-// alpha = rz / max(1e-30, p.Ap)
+template <int NZ, bool FUSED>
+__global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
+                                               const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
+                                               const float* __restrict__ z, float* __restrict__ pcur,
+                                               float* __restrict__ y, double* __restrict__ partial,
+                                               const double* __restrict__ S, const int* __restrict__ done)
+{
+    kxw32_body<NZ, FUSED>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+}
+
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
                                                     double* __restrict__ S, const int* __restrict__ done)
 {
-    if (*done) return;
-    __shared__ double sh[16];
-    const double pAp = sum_partials_1024(partial, m, sh);
-    if (threadIdx.x == 0) {
-        S[S_PAP] = pAp;
-        S[S_ALPHA] = S[S_RZ] / (1.0e-30 < pAp ? pAp : 1.0e-30);
-    }
+    fin_alpha_body(partial, m, S, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// ---------------------------------------------------------------------------
-// FP64 vector kernels (grid-stride, fixed grid; the last block finishes)
-// ---------------------------------------------------------------------------
-constexpr int VB = 2048;   // blocks of the vector kernels
-
-// rn2 = r.r ; if check: the convergence test of iteration it
 __global__ void __launch_bounds__(1024) k_fin_rn2(const double* __restrict__ partial, unsigned m,
                                                   double* __restrict__ S, int* __restrict__ flags, int it, int check,
                                                   int guard)
 {
-    if (guard && flags[0]) return;
-    __shared__ double sh[16];
-    const double rn2 = sum_partials_1024(partial, m, sh);
-    if (threadIdx.x == 0) {
-        S[S_RN2] = rn2;
-        if (check && !flags[0] && sqrt(rn2) < S[S_STOP]) {
-            flags[0] = 1;
-            flags[1] = it + 1;
-        }
-    }
+    fin_rn2_body(partial, m, S, flags, it, check, guard, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// r = b - Ap (FP64 arithmetic, stored in FP32) ; partial of the FP64 r.r
-__global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const double* __restrict__ b,
-                                                          const double* __restrict__ Ap, float* __restrict__ r,
-                                                          double* __restrict__ partial, const int* __restrict__ done)
+__global__ void __launch_bounds__(1024) k_fin_beta(const double* __restrict__ partial, unsigned m,
+                                                   double* __restrict__ S, int flexible, int first,
+                                                   const int* __restrict__ done)
 {
-    if (done && *done) return;
-    __shared__ double sh[4];
-    double s = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
-        const double v = b[i] - Ap[i];
-        r[i] = float(v);
-        s = fma(v, v, s);
-    }
-    const double t = block_sum_256(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    fin_beta_body(partial, m, S, flexible, first, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// x += dx ; dx = 0
-__global__ void __launch_bounds__(256) k_mx_fold(int64_t n, double* __restrict__ x, float* __restrict__ dx,
-                                                 const int* __restrict__ done)
-{
-    if (done && *done) return;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
-        x[i] += double(dx[i]);
-        dx[i] = 0.f;
-    }
-}
-
-__global__ void k_mx_init(double tol, double* __restrict__ S, int* __restrict__ flags)
-{
-    const double nb = sqrt(S[S_NORMB2]);
-    S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
-    S[S_BETA] = 0.0;
-    flags[0] = 0;
-    flags[1] = -1;
-}
-
-__global__ void k_mx_clear_done(int* __restrict__ flags)
-{
-    flags[0] = 0;
-    flags[1] = -1;
-}
-
-// ---------------------------------------------------------------------------
-// FP32 V-cycle kernels.  3D launches (x in blocks of 64 threads, y and z in
-// the grid) so node coordinates need no division.
-// ---------------------------------------------------------------------------
-struct LevelF {
-    int resX, resY, resZ;
-    int nnx, nny, nnz;
-    int64_t nN;
-};
-
-// restriction pass 1 (x): t[(fz*nny+fy)*cnx+cx] = sum_dx W(dx) rf[fz,fy,cx*s+dx] (unnormalised)
 template <class TIn>
 __global__ void __launch_bounds__(64) k_restrict_x(LevelF Lf, int cnx, int span, const TIn* __restrict__ rf,
                                                    float* __restrict__ t, const int* __restrict__ done)
 {
-    if (done && *done) return;
-    const int cx = blockIdx.x * 64 + threadIdx.x;
-    if (cx >= cnx) return;
-    const int fy = blockIdx.y, fz = blockIdx.z;
-    const float inv = 1.0f / float(span);
-    const int64_t row = (int64_t(fz) * Lf.nny + fy) * Lf.nnx;
-    const int f0 = cx * span;
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-    for (int d = -(span - 1); d <= span - 1; ++d) {
-        const int fx = f0 + d;
-        if (fx < 0 || fx >= Lf.nnx) continue;
-        const float w = 1.0f - float(d < 0 ? -d : d) * inv;
-        const int64_t i = 3 * (row + fx);
-        v0 = fmaf(w, float(rf[i]), v0);
-        v1 = fmaf(w, float(rf[i + 1]), v1);
-        v2 = fmaf(w, float(rf[i + 2]), v2);
-    }
-    const int64_t o = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
-    t[o] = v0;
-    t[o + 1] = v1;
-    t[o + 2] = v2;
+    restrict_x_body<TIn>(Lf, cnx, span, rf, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// restriction pass 2 (y, z), normalisation by the in-range weight sum, coarse
-// mask; at the coarsest level optionally the diagonal solve xc = rc / D or an
-// FP64 copy of rc for the Cholesky solve (transfers.cpp:449-533)
 __global__ void __launch_bounds__(64) k_restrict_yz(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
                                                     const float* __restrict__ Dc, float* __restrict__ xc,
                                                     double* __restrict__ rcd, const int* __restrict__ done)
 {
-    if (done && *done) return;
-    const int cx = blockIdx.x * 64 + threadIdx.x;
-    if (cx >= Lc.nnx) return;
-    const int cy = blockIdx.y, cz = blockIdx.z;
-    const float inv = 1.0f / float(span);
-    const int cnx = Lc.nnx;
-    float wxs = 0.f;
-    for (int d = -(span - 1); d <= span - 1; ++d) {
-        const int fx = cx * span + d;
-        if (fx >= 0 && fx < Lf.nnx) wxs += 1.0f - float(d < 0 ? -d : d) * inv;
-    }
-    float v0 = 0.f, v1 = 0.f, v2 = 0.f, wyz = 0.f;
-    for (int dz = -(span - 1); dz <= span - 1; ++dz) {
-        const int fz = cz * span + dz;
-        if (fz < 0 || fz >= Lf.nnz) continue;
-        const float wz = 1.0f - float(dz < 0 ? -dz : dz) * inv;
-        for (int dy = -(span - 1); dy <= span - 1; ++dy) {
-            const int fy = cy * span + dy;
-            if (fy < 0 || fy >= Lf.nny) continue;
-            const float w = wz * (1.0f - float(dy < 0 ? -dy : dy) * inv);
-            const int64_t i = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
-            v0 = fmaf(w, t[i], v0);
-            v1 = fmaf(w, t[i + 1], v1);
-            v2 = fmaf(w, t[i + 2], v2);
-            wyz += w;
-        }
-    }
-    const float ws = wxs * wyz;
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    if (ws > 1e-12f) {
-        const float iw = 1.0f / ws;
-        r0 = v0 * iw;
-        r1 = v1 * iw;
-        r2 = v2 * iw;
-    }
-    const int64_t n = (int64_t(cz) * Lc.nny + cy) * Lc.nnx + cx;
-    const unsigned m = cmask[n];
-    if (m & 1u) r0 = 0.f;
-    if (m & 2u) r1 = 0.f;
-    if (m & 4u) r2 = 0.f;
-    rc[3 * n] = r0;
-    rc[3 * n + 1] = r1;
-    rc[3 * n + 2] = r2;
-    if (rcd) {
-        rcd[3 * n] = r0;
-        rcd[3 * n + 1] = r1;
-        rcd[3 * n + 2] = r2;
-    }
-    if (Dc) {
-        xc[3 * n] = (m & 1u) ? 0.f : r0 * Dc[3 * n];
-        xc[3 * n + 1] = (m & 2u) ? 0.f : r1 * Dc[3 * n + 1];
-        xc[3 * n + 2] = (m & 4u) ? 0.f : r2 * Dc[3 * n + 2];
-    }
+    restrict_yz_body(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// trilinear prolongation of xc at fine node (fx, fy, fz), clamped at the
-// upper faces (transfers.cpp:378-445)
-struct Prolong {
-    int cx, cy, cz;
-    float wx0, wx1, wy0, wy1, wz0, wz1;
-    __device__ Prolong(const LevelF& Lc, int span, int fx, int fy, int fz)
-    {
-        cx = fx / span;
-        cy = fy / span;
-        cz = fz / span;
-        int rx = fx - cx * span, ry = fy - cy * span, rz = fz - cz * span;
-        if (cx >= Lc.resX) { cx = Lc.resX - 1; rx = span; }
-        if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
-        if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
-        const float inv = 1.0f / float(span);
-        wx1 = float(rx) * inv; wx0 = 1.f - wx1;
-        wy1 = float(ry) * inv; wy0 = 1.f - wy1;
-        wz1 = float(rz) * inv; wz0 = 1.f - wz1;
-    }
-    __device__ void eval(const LevelF& Lc, const float* __restrict__ xc, float out[3]) const
-    {
-        const int64_t s = int64_t(Lc.nnx) * Lc.nny;
-        const int64_t b00 = s * cz + int64_t(Lc.nnx) * cy + cx;
-        const int64_t b10 = b00 + Lc.nnx, n00 = b00 + s, n10 = n00 + Lc.nnx;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float lo = wy0 * (wx0 * xc[3 * b00 + c] + wx1 * xc[3 * b00 + 3 + c]) +
-                             wy1 * (wx0 * xc[3 * b10 + c] + wx1 * xc[3 * b10 + 3 + c]);
-            const float hi = wy0 * (wx0 * xc[3 * n00 + c] + wx1 * xc[3 * n00 + 3 + c]) +
-                             wy1 * (wx0 * xc[3 * n10 + c] + wx1 * xc[3 * n10 + 3 + c]);
-            out[c] = wz0 * lo + wz1 * hi;
-        }
-    }
-};
-
-// coarse levels: x_f = mask ? 0 : 2t + P(x_c), t = w r/D  (transfers.cpp:378-445,
-// preconditioner.cpp:230-253)
 __global__ void __launch_bounds__(64) k_prolong_smooth32(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                          const float* __restrict__ rf, const float* __restrict__ Df,
                                                          const uint8_t* __restrict__ fmask, float weight,
                                                          float* __restrict__ xf, const int* __restrict__ done)
 {
-    if (done && *done) return;
-    const int fx = blockIdx.x * 64 + threadIdx.x;
-    if (fx >= Lf.nnx) return;
-    const int fy = blockIdx.y, fz = blockIdx.z;
-    const int64_t n = (int64_t(fz) * Lf.nny + fy) * Lf.nnx + fx;
-    const Prolong P(Lc, span, fx, fy, fz);
-    float v[3];
-    P.eval(Lc, xc, v);
-    const unsigned m = fmask[n];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float out = 0.f;
-        if (!((m >> c) & 1u)) {
-            const float t = weight * rf[3 * n + c] * Df[3 * n + c];
-            out = 2.f * t + v[c];
-        }
-        xf[3 * n + c] = out;
-    }
+    prolong_smooth32_body(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// ---------------------------------------------------------------------------
-// Finest-level row kernels.  A node row (fixed fy, fz) is 3*nnx contiguous
-// values; a block takes RPB consecutive rows, so every load and store is a
-// contiguous run (one value per lane), and the x-direction work of the
-// transfers is done from LDS.
-// ---------------------------------------------------------------------------
-constexpr int RPB = 4;
-
-// x-pass of the restriction for the block's rows staged in sR (FP32):
-// t[row][cx][c] = sum_d W(d) sR[row][cx*span + d][c]
-__device__ __forceinline__ void restrict_x_rows(const float* sR, int L, int nnx, int nr, int64_t row0, int cnx,
-                                                int span, float* __restrict__ t)
-{
-    const int Lc = 3 * cnx;
-    const float inv = 1.0f / float(span);
-    for (int rr = 0; rr < nr; ++rr)
-        for (int k = threadIdx.x; k < Lc; k += 256) {
-            const int cxx = k / 3, c = k - 3 * cxx;
-            float acc = 0.f;
-            for (int d = -(span - 1); d <= span - 1; ++d) {
-                const int fx = cxx * span + d;
-                if (fx < 0 || fx >= nnx) continue;
-                acc = fmaf(1.0f - float(d < 0 ? -d : d) * inv, sR[rr * L + 3 * fx + c], acc);
-            }
-            t[(row0 + rr) * Lc + k] = acc;
-        }
-}
-
-// dx += alpha p ; r -= alpha Ap (FP32 storage, FP64 alpha) ; partial r.r ;
-// optionally (t != nullptr) the x-pass of the restriction of the new r
 __global__ void __launch_bounds__(256) k_xr_rows(int nnx, int NR, const float* __restrict__ p,
                                                  const float* __restrict__ Ap, float* __restrict__ dx,
                                                  float* __restrict__ r, double* __restrict__ partial,
                                                  const double* __restrict__ S, int cnx, int span,
                                                  float* __restrict__ t, const int* __restrict__ done)
 {
-    if (*done) return;
-    extern __shared__ float sR[];
-    __shared__ double sh[4];
-    const int L = 3 * nnx;
-    const int64_t row0 = int64_t(blockIdx.x) * RPB;
-    const int nr = min(RPB, int(NR - row0));
-    const double alpha = S[S_ALPHA];
-    const float alphaf = float(alpha);
-    double s = 0.0;
-    // the block's rows are one contiguous range starting at an even index
-    // (RPB is even), so float2 accesses are aligned
-    const int64_t base = row0 * L;
-    const int ne = nr * L;
-    for (int e = 2 * threadIdx.x; e < ne; e += 512) {
-        const int64_t i = base + e;
-        if (e + 1 < ne) {
-            const float2 pp = *reinterpret_cast<const float2*>(p + i);
-            const float2 aa = *reinterpret_cast<const float2*>(Ap + i);
-            float2 xx = *reinterpret_cast<const float2*>(dx + i);
-            float2 rv = *reinterpret_cast<const float2*>(r + i);
-            xx.x = fmaf(alphaf, pp.x, xx.x);
-            xx.y = fmaf(alphaf, pp.y, xx.y);
-            const double v0 = fma(-alpha, double(aa.x), double(rv.x));
-            const double v1 = fma(-alpha, double(aa.y), double(rv.y));
-            rv.x = float(v0);
-            rv.y = float(v1);
-            *reinterpret_cast<float2*>(dx + i) = xx;
-            *reinterpret_cast<float2*>(r + i) = rv;
-            s = fma(v0, v0, s);
-            s = fma(v1, v1, s);
-            sR[e] = rv.x;
-            sR[e + 1] = rv.y;
-        } else {
-            dx[i] = fmaf(alphaf, p[i], dx[i]);
-            const double v = fma(-alpha, double(Ap[i]), double(r[i]));
-            r[i] = float(v);
-            s = fma(v, v, s);
-            sR[e] = float(v);
-        }
-    }
-    const double tot = block_sum_256(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-    if (t) {
-        __syncthreads();
-        restrict_x_rows(sR, L, nnx, nr, row0, cnx, span, t);
-    }
+    xr_rows_body<RPB>(nnx, NR, p, Ap, dx, r, partial, S, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// x-pass of the restriction of the finest-level residual (first V-cycle of a
-// solve and after a residual replacement)
 __global__ void __launch_bounds__(256) k_rx_rows(int nnx, int NR, const float* __restrict__ r, int cnx, int span,
                                                  float* __restrict__ t, const int* __restrict__ done)
 {
-    if (done && *done) return;
-    extern __shared__ float sR[];
-    const int L = 3 * nnx;
-    const int64_t row0 = int64_t(blockIdx.x) * RPB;
-    const int nr = min(RPB, int(NR - row0));
-    for (int rr = 0; rr < nr; ++rr)
-        for (int k = threadIdx.x; k < L; k += 256) sR[rr * L + k] = r[(row0 + rr) * L + k];
-    __syncthreads();
-    restrict_x_rows(sR, L, nnx, nr, row0, cnx, span, t);
+    rx_rows_body<RPB>(nnx, NR, r, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// finest level: z = mask ? 0 : 2t + P(x_c), t = w r/D,
-// with the partials of r.z and z.Ap (two per block).  The y/z part of the
-// trilinear prolongation is evaluated once per row and coarse x in LDS.
 __global__ void __launch_bounds__(256) k_pro_rows(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                   const float* __restrict__ rf, const float* __restrict__ Df,
                                                   const uint8_t* __restrict__ fmask, float weight,
                                                   float* __restrict__ xf, const float* __restrict__ Ap,
                                                   double* __restrict__ partial, const int* __restrict__ done)
 {
-    if (*done) return;
-    extern __shared__ float sQ[];   // [RPB][3 * Lc.nnx]
-    __shared__ double sh[4];
-    const int NR = Lf.nny * Lf.nnz;
-    const int L = 3 * Lf.nnx, Lq = 3 * Lc.nnx;
-    const int64_t row0 = int64_t(blockIdx.x) * RPB;
-    const int nr = min(RPB, int(NR - row0));
-    const float inv = 1.0f / float(span);
-    const int64_t cs = int64_t(Lc.nnx) * Lc.nny;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int row = int(row0) + rr;
-        const int fy = row % Lf.nny, fz = row / Lf.nny;
-        int cy = fy / span, cz = fz / span;
-        int ry = fy - cy * span, rz = fz - cz * span;
-        if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
-        if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
-        const float wy1 = float(ry) * inv, wy0 = 1.f - wy1, wz1 = float(rz) * inv, wz0 = 1.f - wz1;
-        const int64_t b0 = cs * cz + int64_t(Lc.nnx) * cy;
-        for (int k = threadIdx.x; k < Lq; k += 256) {
-            const int64_t q = 3 * b0 + k;
-            const float lo = wy0 * xc[q] + wy1 * xc[q + Lq];
-            const float hi = wy0 * xc[q + 3 * cs] + wy1 * xc[q + 3 * cs + Lq];
-            sQ[rr * Lq + k] = wz0 * lo + wz1 * hi;
-        }
-    }
-    __syncthreads();
-    double srz = 0.0, szap = 0.0;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int64_t nbase = (row0 + rr) * Lf.nnx;
-        for (int k = threadIdx.x; k < L; k += 256) {
-            const int xx = k / 3, c = k - 3 * xx;
-            int cxx = xx / span, rx = xx - cxx * span;
-            if (cxx >= Lc.resX) { cxx = Lc.resX - 1; rx = span; }
-            const float wx1 = float(rx) * inv;
-            const float v = (1.f - wx1) * sQ[rr * Lq + 3 * cxx + c] + wx1 * sQ[rr * Lq + 3 * cxx + 3 + c];
-            const int64_t i = 3 * nbase + k;
-            const float r = rf[i];
-            float out = 0.f;
-            if (!((fmask[nbase + xx] >> c) & 1u)) out = 2.f * (weight * r * Df[i]) + v;
-            xf[i] = out;
-            srz = fma(double(r), double(out), srz);
-            szap = fma(double(out), double(Ap[i]), szap);
-        }
-    }
-    const double a = block_sum_256(srz, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = a;
-    const double b = block_sum_256(szap, sh);
-    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = b;
-}
-
-// rz_new = r.z, zAp = z.Ap -> beta (0 on the first V-cycle of a solve); rz = rz_new
-__global__ void __launch_bounds__(1024) k_fin_beta(const double* __restrict__ partial, unsigned m,
-                                                   double* __restrict__ S, int flexible, int first,
-                                                   const int* __restrict__ done)
-{
-    if (*done) return;
-    __shared__ double sh[16];
-    const double rz = sum_partials_1024(partial, m, sh);
-    const double zAp = sum_partials_1024(partial + m, m, sh);
-    if (threadIdx.x == 0) {
-        if (first) {
-            S[S_BETA] = 0.0;
-        } else {
-            const double o = S[S_RZ];
-            const double num = flexible ? -S[S_ALPHA] * zAp : rz;
-            S[S_BETA] = num / (1.0e-30 < o ? o : 1.0e-30);
-        }
-        S[S_RZ] = rz;
-        S[S_RZNEW] = rz;
-        S[S_ZAP] = zAp;
-    }
-}
-
-// FP32 inverse Jacobi diagonal 1 / max(1e-30, D) (the smoother's and the
-// coarsest diagonal solve's divisor, preconditioner.cpp:230-253)
-__global__ void k_dinv(int64_t n, const double* __restrict__ D, float* __restrict__ Dinv)
-{
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) Dinv[i] = float(1.0 / (1.0e-30 < D[i] ? D[i] : 1.0e-30));
+    pro_rows_body<RPB>(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, Ap, partial, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
 __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = float(a[i]);
+    d2f_body(n, a, b, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
 }  // namespace t3d
 
 using namespace t3d;
 
-// mixed-precision state attached to a multigrid object
-struct t3d_mixed {
-    t3d_problem* p = nullptr;
-    t3d_mg* M = nullptr;
-    float* Ke32 = nullptr;     // element matrix (node-gather form)
-    float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
-    unsigned short* KA2 = nullptr;  // element matrix as 2-piece bf16 MFMA A operands
-    unsigned short* KA3 = nullptr;  // ... and as 3-piece
-    int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6
-    float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
-    float* r32[8] = {};
-    float* x32[8] = {};
-    float* tmp = nullptr;      // restriction pass-1 buffer
-    double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
-    double* xc64 = nullptr;
-    float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
-    float* r = nullptr;        // recursive residual (FP32)
-    float* dx = nullptr;       // FP32 increment of x since the last fold
-    double* part = nullptr;    // block partials
-    double* sc = nullptr;      // scalars
-    int* iflags = nullptr;     // [0] done [1] iterations
-    int hostFlags[4];
-    int nlev = 0;
-};
-
-// internal accessors of the multigrid object (mg.hip)
-namespace t3d {
-struct MGView {
-    int nlev;
-    int span[8];
-    int resX[8], resY[8], resZ[8];
-    int64_t nN[8];
-    const uint8_t* mask[8];
-    const double* D[8];
-    const double* L;
-    const double* LT;
-    int Nc;
-    double weight;
-};
-int mg_view(const t3d_mg* g, MGView& v);
-__global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,
-                             const double* __restrict__ b, const uint8_t* __restrict__ mask, double* __restrict__ x,
-                             const int* __restrict__ done);
-}  // namespace t3d
-
-static LevelF levf(const MGView& v, int l)
-{
-    LevelF L;
-    L.resX = v.resX[l]; L.resY = v.resY[l]; L.resZ = v.resZ[l];
-    L.nnx = v.resX[l] + 1; L.nny = v.resY[l] + 1; L.nnz = v.resZ[l] + 1;
-    L.nN = v.nN[l];
-    return L;
-}
+#include "mixed_internal.hpp"
 
 namespace t3d {
 
 constexpr int KX_NZ = 8;
-constexpr int KW_NZ = 16;
 
 static dim3 kx_grid(const Grid& g, int nz)
 {
@@ -1162,7 +486,7 @@ static dim3 kx_grid(const Grid& g, int nz)
                 unsigned((g.nnz + nz - 1) / nz));
 }
 
-static dim3 kw_grid(const Grid& g)
+dim3 kw_grid(const Grid& g)
 {
     return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + wk::NY - 1) / wk::NY),
                 unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
@@ -1193,7 +517,7 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 
 
 
-static unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz + RPB - 1) / RPB); }
+unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz + RPB - 1) / RPB); }
 
 // the solver's fused operator launch: p = z + beta p_prev, Ap = K p, p.Ap partials
 static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* Ap)
@@ -1216,7 +540,7 @@ static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* A
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
 // x-pass of the finest restriction is already in s->tmp (fused into the x, r
 // update)
-static int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
+int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
                         const int* done)
 {
     t3d_problem* p = s->p;
@@ -1266,6 +590,87 @@ static int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexi
                        s->part, done);
     hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, nb, s->sc, flexible, first,
                        done);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+}  // namespace t3d
+
+namespace t3d {
+
+// helpers of the batched solver (batch.hip), one problem each, on its stream
+
+// r = b - (K64 x, already in p->wAp) in FP32 ; rn2 (no convergence test)
+int mixed_true_residual(t3d_mixed* s, const double* b)
+{
+    t3d_problem* p = s->p;
+    const int64_t n = 3 * p->g.nN;
+    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
+                       s->part, nullptr);
+    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, p->stream, (const double*)s->part, unsigned(VB), s->sc,
+                       s->iflags, -1, 0, 0);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+// the residual replacement of iteration it, skipped when the problem is done:
+// x += dx, r = b - K64 x, rn2 and the convergence test
+int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it)
+{
+    t3d_problem* p = s->p;
+    const int64_t n = 3 * p->g.nN;
+    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
+    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, p->stream, n, x, s->dx, (const int*)s->iflags);
+    int rc = launch_kxu_f64(p, x, p->wAp, s->iflags);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
+                       s->part, (const int*)s->iflags);
+    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, p->stream, (const double*)s->part, unsigned(VB), s->sc,
+                       s->iflags, it, 1, 1);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+// x += dx ; r = b - K64 x ; returns the FP64 r.r and the stopping threshold
+int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double* stop)
+{
+    t3d_problem* p = s->p;
+    int rc = mixed_fold(s, x);
+    if (rc) return rc;
+    if ((rc = launch_kxu_f64(p, x, p->wAp))) return rc;
+    if ((rc = mixed_true_residual(s, b))) return rc;
+    T3D_HIP(hipMemcpyAsync(stop, s->sc + S_STOP, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipMemcpyAsync(rn2, s->sc + S_RN2, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    return T3D_OK;
+}
+
+// x += dx ; dx = 0 (unconditional)
+int mixed_fold(t3d_mixed* s, double* x)
+{
+    t3d_problem* p = s->p;
+    const int64_t n = 3 * p->g.nN;
+    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
+    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, p->stream, n, x, s->dx, nullptr);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
+// continue a problem whose convergence was not confirmed: clear its flag, run
+// the V-cycle of iteration itDone with the replaced residual, and place the
+// direction of iteration itDone where the iteration itNext reads p_prev
+int mixed_resume(t3d_mixed* s, int itDone, int itNext, int flexible)
+{
+    t3d_problem* p = s->p;
+    MGView v;
+    int rc = mg_view(s->M, v);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mx_clear_done, dim3(1), dim3(1), 0, p->stream, s->iflags);
+    if ((rc = mixed_vcycle(s, v, s->r, flexible, 0, false, s->iflags))) return rc;
+    float* last = (itDone & 1) ? s->pA : s->pB;   // p_cur of iteration itDone
+    float* next = (itNext & 1) ? s->pB : s->pA;   // p_prev of iteration itNext
+    if (last != next)
+        T3D_HIP(hipMemcpyAsync(next, last, sizeof(float) * 3 * p->g.nN, hipMemcpyDeviceToDevice, p->stream));
     T3D_HIP(hipGetLastError());
     return T3D_OK;
 }
@@ -1349,7 +754,11 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     const int64_t nD = 3 * p->g.nN;
     const dim3 kg = kw_grid(p->g);
     const LevelF L0 = levf(v, 0);
-    const int64_t npart = std::max<int64_t>({int64_t(kg.x) * kg.y * kg.z, 2 * int64_t(row_blocks(L0)), VB}) + 16;
+    // partials: the largest layout any caller uses (Walsh tiles down to 8 planes
+    // deep, row kernels down to 2 rows per block, the vector kernels)
+    const int64_t kwMax = int64_t(kg.x) * kg.y * ((p->g.nnz + 7) / 8);
+    const int64_t rowsMax = (int64_t(L0.nny) * L0.nnz + 1) / 2;
+    const int64_t npart = std::max<int64_t>({kwMax, 2 * rowsMax, VB}) + 16;
     int64_t ntmp = 0;
     for (int l = 0; l + 1 < v.nlev; ++l) {
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);

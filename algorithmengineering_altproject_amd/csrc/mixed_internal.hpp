@@ -1,0 +1,74 @@
+// mixed_internal.hpp -- the mixed-precision solver's state and the helpers
+// shared with the batched solver (batch.hip).
+#pragma once
+#include "mx_kernels.hpp"
+
+// mixed-precision state attached to a multigrid object
+struct t3d_mixed {
+    t3d_problem* p = nullptr;
+    t3d_mg* M = nullptr;
+    float* Ke32 = nullptr;     // element matrix (node-gather form)
+    float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
+    unsigned short* KA2 = nullptr;  // element matrix as 2-piece bf16 MFMA A operands
+    unsigned short* KA3 = nullptr;  // ... and as 3-piece
+    int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6
+    float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
+    float* r32[8] = {};
+    float* x32[8] = {};
+    float* tmp = nullptr;      // restriction pass-1 buffer
+    double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
+    double* xc64 = nullptr;
+    float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
+    float* r = nullptr;        // recursive residual (FP32)
+    float* dx = nullptr;       // FP32 increment of x since the last fold
+    double* part = nullptr;    // block partials
+    double* sc = nullptr;      // scalars
+    int* iflags = nullptr;     // [0] done [1] iterations
+    int hostFlags[4];
+    int nlev = 0;
+};
+
+// internal accessors of the multigrid object (mg.hip)
+namespace t3d {
+struct MGView {
+    int nlev;
+    int span[8];
+    int resX[8], resY[8], resZ[8];
+    int64_t nN[8];
+    const uint8_t* mask[8];
+    const double* D[8];
+    const double* L;
+    const double* LT;
+    int Nc;
+    double weight;
+};
+int mg_view(const t3d_mg* g, MGView& v);
+__global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,
+                             const double* __restrict__ b, const uint8_t* __restrict__ mask, double* __restrict__ x,
+                             const int* __restrict__ done);
+}  // namespace t3d
+
+inline t3d::LevelF levf(const t3d::MGView& v, int l)
+{
+    t3d::LevelF L;
+    L.resX = v.resX[l]; L.resY = v.resY[l]; L.resZ = v.resZ[l];
+    L.nnx = v.resX[l] + 1; L.nny = v.resY[l] + 1; L.nnz = v.resZ[l] + 1;
+    L.nN = v.nN[l];
+    return L;
+}
+
+namespace t3d {
+constexpr int KW_NZ = 16;   // node planes per Walsh-form tile of the single-problem solver
+dim3 kw_grid(const Grid& g);
+unsigned row_blocks(const LevelF& L);
+// z32 = M32(r) with rz and beta (mixed.hip); tmpReady: the finest restriction's
+// x-pass is already in s->tmp
+int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
+                 const int* done);
+// single-problem steps of the batched solver (mixed.hip)
+int mixed_true_residual(t3d_mixed* s, const double* b);
+int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it);
+int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double* stop);
+int mixed_fold(t3d_mixed* s, double* x);
+int mixed_resume(t3d_mixed* s, int itDone, int itNext, int flexible);
+}  // namespace t3d

@@ -149,6 +149,32 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form);
  * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials) */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 
+/* ---- batching (the chosen "batching" form; no reference counterpart: the
+ * reference runs its simulation_count problems one after another,
+ * cpp/src/driver.cpp:23-218).  Many problems of uneven sizes solved together
+ * by the mixed-precision MG-PCG: each kernel of an iteration is launched once
+ * for the whole batch (device block map -> problem, block), the iteration is
+ * replayed from hipGraphs, the Walsh-form tile depth and the row kernels'
+ * rows per block are autotuned on the batch.  All problems share one stream. */
+typedef struct t3d_batch t3d_batch;
+/* n cuboids (nely[i], nelx[i], nelz[i]); multigrid with the reference's
+ * defaults (non-dyadic, 5 levels, weight 0.6, NlimitDofs 1000) and Emin */
+int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, float Emin, t3d_batch** out);
+int t3d_batch_destroy(t3d_batch* b);
+/* the batch's problem / multigrid object i (owned by the batch) */
+int t3d_batch_problem(t3d_batch* b, int i, t3d_problem** p);
+int t3d_batch_mg(t3d_batch* b, int i, t3d_mg** g);
+/* E_dev[i]: compact element moduli of problem i (device); multigrid set-up */
+int t3d_batch_setup(t3d_batch* b, const double* const* E_dev);
+/* tiles: kwNZ in {8,16,32} node planes per Walsh-form tile, rowsPerBlock in
+ * {2,4,8}; (0, 0) = autotune on the next solve (the default) */
+int t3d_batch_set_tiles(t3d_batch* b, int kwNZ, int rowsPerBlock);
+int t3d_batch_info(const t3d_batch* b, int* kwNZ, int* rowsPerBlock);
+/* b_dev[i], x_dev[i]: FP64 full layout of problem i (x = initial guess and
+ * result); iters[i] as t3d_pcg_mixed; useGraph != 0 replays hipGraphs */
+int t3d_batch_solve(t3d_batch* b, const double* const* b_dev, double* const* x_dev, double tol, int maxIt,
+                    int trueEvery, int flexible, int useGraph, int* iters);
+
 /* ---- compliance ----
  * ComputeCompliance (cpp/include/fea.hpp:47, cpp/src/fea.cpp:587-677,851-864):
  * ce[e] = ue^T Ke ue (0 when E <= 1e-16), returns sum E*ce (canonical tree).
