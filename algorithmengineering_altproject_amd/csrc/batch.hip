@@ -233,6 +233,7 @@ struct t3d_batch {
     // graphs per parity
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     bool setupDone = false;
+    std::vector<char> active;   // problems whose blocks are in the maps
 };
 
 namespace t3d {
@@ -270,10 +271,14 @@ static int upload_map(const std::vector<BMap>& h, BMap** d, unsigned* n)
     return T3D_OK;
 }
 
-// descriptors and block maps for the current tile depth and rows per block
+// descriptors of every problem, and block maps of the active problems for the
+// current tile depth and rows per block
 static int batch_build(t3d_batch* B)
 {
+    // no launch in flight may still read the maps or descriptors replaced here
+    T3D_HIP(hipDeviceSynchronize());
     free_maps(B);
+    if (int(B->active.size()) != B->n) B->active.assign(B->n, 1);
     B->hP.assign(B->n, BProb{});
     std::vector<BMap> kw, rows, rx[8], yz[8], pro[8];
     std::vector<int> chol;
@@ -320,10 +325,11 @@ static int batch_build(t3d_batch* B)
         const unsigned kgz = unsigned((g.nnz + B->kwNZ - 1) / B->kwNZ);
         q.kwTiles = q.kgx * q.kgy * kgz;
         q.rowBlocks = unsigned((int64_t(g.nny) * g.nnz + B->rpb - 1) / B->rpb);
-        for (unsigned t = 0; t < q.kwTiles; ++t) kw.push_back({i, int(t)});
-        for (unsigned t = 0; t < q.rowBlocks; ++t) rows.push_back({i, int(t)});
         const int L = v.nlev - 1;
         B->maxLev = std::max(B->maxLev, L);
+        if (!B->active[i]) continue;
+        for (unsigned t = 0; t < q.kwTiles; ++t) kw.push_back({i, int(t)});
+        for (unsigned t = 0; t < q.rowBlocks; ++t) rows.push_back({i, int(t)});
         for (int l = 0; l < L; ++l) {
             const LevelF& Lf = q.lv[l].L;
             const LevelF& Lc = q.lv[l + 1].L;
@@ -565,6 +571,10 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
     hipStream_t st = B->stream;
     const int n = B->n;
     int rc;
+    if (std::count(B->active.begin(), B->active.end(), 1) != n) {
+        B->active.assign(n, 1);
+        if ((rc = batch_build(B))) return rc;
+    }
     // per-problem start: r = b - K64 x, |b|, V-cycle of the first iteration
     for (int i = 0; i < n; ++i) {
         t3d_problem* p = B->P[i];
@@ -632,8 +642,9 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         // first V-cycle's finest level with the chosen tiles
         batch_vcycle(B, flexible, 1, false);
     }
-    // graphs of one iteration per parity
-    if (useGraph) {
+    // graphs of one iteration per parity (re-captured after the maps change)
+    auto ensure_graphs = [&]() -> int {
+        if (!useGraph) return T3D_OK;
         for (int par = 0; par < 2; ++par) {
             if (B->graph[par]) continue;
             hipGraph_t gr;
@@ -643,7 +654,9 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
             T3D_HIP(hipGraphInstantiate(&B->graph[par], gr, nullptr, nullptr, 0));
             T3D_HIP(hipGraphDestroy(gr));
         }
-    }
+        return T3D_OK;
+    };
+    if ((rc = ensure_graphs())) return rc;
     std::vector<int> result(n, maxIt), skipped(n, 0), finished(n, 0);
     std::vector<int> hf(2 * n);
     const int CHUNK = 32;
@@ -696,6 +709,16 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         }
         for (int i = 0; i < n; ++i) allDone = allDone && finished[i];
         if (allDone) break;
+        // drop finished problems from the block maps once they hold a quarter
+        // of the operator's blocks (their blocks would only return early)
+        unsigned activeTiles = 0;
+        for (int i = 0; i < n; ++i)
+            if (!finished[i]) activeTiles += B->hP[i].kwTiles;
+        if (4 * activeTiles <= 3 * B->nKW) {
+            for (int i = 0; i < n; ++i) B->active[i] = finished[i] ? 0 : 1;
+            if ((rc = batch_build(B))) return rc;
+            if ((rc = ensure_graphs())) return rc;
+        }
         it0 = it1;
     }
     for (int i = 0; i < n; ++i) {

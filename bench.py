@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--design", choices=["uniform", "random-filtered"], default="uniform",
                     help="uniform rho = 0.12, or rho ~ U[1e-3, 1] (seeded) PDE-filtered with rmin = 2")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="batched workload instead: this many uneven cantilevers (seeded sizes, V0, voids)")
     ap.add_argument("--operator", choices=["walsh", "mfma3", "mfma6"], default="walsh",
                     help="FP32 operator of the mixed solve: Walsh form (FMA), MFMA bf16 x3 or MFMA bf16 x6")
     ap.add_argument("--kxu-reps", type=int, default=50)
@@ -117,6 +119,129 @@ def true_residual(t3d, p, db, dx):
     return float(np.linalg.norm(b - dy.download()) / np.linalg.norm(b))
 
 
+def batch_workload(rng, n):
+    """n cantilevers: nelx ~ U{48, 56, ..., 160}, nely = nelz = nelx/2 rounded to a
+    multiple of 8, V0 ~ U[0.08, 0.30]; every second one with 1-6 spherical voids
+    of radius 3-15 voxels at seeded centres inside the domain."""
+    probs = []
+    for i in range(n):
+        nelx = int(rng.choice(np.arange(48, 161, 8)))
+        ny = max(8, int(round(nelx / 2 / 8)) * 8)
+        v0 = float(rng.uniform(0.08, 0.30))
+        sph = []
+        if i % 2 == 1:
+            for _ in range(int(rng.integers(1, 7))):
+                r = float(rng.uniform(3, 15))
+                sph.append((float(rng.uniform(0, nelx)), float(rng.uniform(0, ny)), float(rng.uniform(0, ny)), r))
+        probs.append(((ny, nelx, ny), v0, sph))
+    return probs
+
+
+def batch_design(t3d, p, v0, spheres):
+    x = np.full(p.ne, np.float32(v0), np.float32)
+    if spheres:
+        f = p.model()["eleMapBack"].astype(np.int64)
+        ny, nx = p.resY, p.resX
+        ez = f // (ny * nx)
+        rem = f % (ny * nx)
+        ex, ey = rem // ny, ny - 1 - rem % ny
+        for cx, cy, cz, r in spheres:
+            x[(ex + 0.5 - cx) ** 2 + (ey + 0.5 - cy) ** 2 + (ez + 0.5 - cz) ** 2 <= r * r] = 0.0   # passive void
+    dx = t3d.DeviceArray.from_host(x)
+    dE = t3d.DeviceArray(p.ne)
+    t3d.simp(p, dx, 1.0, float(np.float32(1e-9)), dE)
+    return dE
+
+
+def main_batch(args, t3d):
+    rng = np.random.default_rng(args.seed)
+    probs = batch_workload(rng, args.batch)
+    dims = [d for d, _, _ in probs]
+    t0 = time.time()
+    B = t3d.Batch(dims, Emin=float(np.float32(1e-9)))
+    dEs = [batch_design(t3d, p, v, s) for p, (_, v, s) in zip(B.problems, probs)]
+    B.setup(dEs)
+    dbs, xs = [], []
+    for p in B.problems:
+        db = t3d.DeviceArray(p.nDOF)
+        p.load_vector(db)
+        dbs.append(db)
+        xs.append(t3d.DeviceArray(p.nDOF))
+    ne = sum(p.ne for p in B.problems)
+    nF = sum(p.nF for p in B.problems)
+    log("batch of %d problems, %d elements, %d free DOFs (set-up %.1fs)" % (len(probs), ne, nF, time.time() - t0))
+    for _ in range(max(1, args.warmup)):          # the first solve also autotunes the tiles
+        for x in xs:
+            x.zero()
+        B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every)
+    tiles = B.info()
+    t3d.lib().t3d_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for x in xs:
+            x.zero()
+        its = B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every)
+    t3d.lib().t3d_synchronize()
+    ms_batch = (time.perf_counter() - t0) / args.steps * 1e3
+    res = [true_residual(t3d, p, db, x) for p, db, x in zip(B.problems, dbs, xs)]
+    log("batched: %.1f ms for all %d solves (%.1f solves/s), tiles %s, iterations %d..%d, max true residual %.2e"
+        % (ms_batch, len(probs), len(probs) * 1e3 / ms_batch, tiles, min(its), max(its), max(res)))
+    # the same problems one after another with the single-problem solver
+    Ms = []
+    for i, p in enumerate(B.problems):
+        M = t3d.MG(p, Emin=float(np.float32(1e-9)))
+        M.setup(dEs[i])
+        mx = t3d.Mixed(p, M)
+        mx.setup()
+        Ms.append((M, mx))
+    for (M, mx), db, x in zip(Ms, dbs, xs):   # warm-up
+        x.zero()
+        mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every)
+    t3d.lib().t3d_synchronize()
+    t0 = time.perf_counter()
+    its_seq = []
+    for (M, mx), db, x in zip(Ms, dbs, xs):
+        x.zero()
+        its_seq.append(mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every))
+    t3d.lib().t3d_synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    log("sequential: %.1f ms for all solves (%.1f solves/s), speed-up %.2fx" % (ms_seq, len(probs) * 1e3 / ms_seq,
+                                                                             ms_seq / ms_batch))
+    line = {
+        "metric": METRIC,
+        "value": len(probs) * 1e3 / ms_batch,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (%d cantilevers, seed %d: nelx~U{48..160}, nely=nelz=nelx/2, V0~U[0.08,0.30], half with "
+                "1-6 spherical voids r~U[3,15])" % (len(probs), args.seed),
+        "config": {
+            "workload": "batched mixed-precision MG-PCG: %d uneven problems solved together to true residual < %.0e "
+                        "(one launch per kernel for the batch, hipGraph-replayed iterations, autotuned tiles); a "
+                        "step solves all %d" % (len(probs), args.tol, len(probs)),
+            "problems": len(probs),
+            "elements": ne,
+            "free_dofs": nF,
+            "tiles_kw_nz_rows_per_block": list(tiles),
+            "iterations": [int(v) for v in its],
+            "sequential_iterations": [int(v) for v in its_seq],
+            "max_true_residual": max(res),
+            "sequential_ms": ms_seq,
+            "sequential_solves_per_s": len(probs) * 1e3 / ms_seq,
+            "speedup_vs_sequential": ms_seq / ms_batch,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -126,6 +251,8 @@ def main():
     import algorithmengineering_altproject_amd as t3d
     name, cus, mem = t3d.device_info()
     log("device %s, %d CUs" % (name, cus))
+    if args.batch:
+        return main_batch(args, t3d)
     t0 = time.time()
     p = t3d.Problem(args.nely, args.nelx, args.nelz)
     log("problem %dx%dx%d: %d elements, %d free DOFs (setup %.1fs)" % (args.nely, args.nelx, args.nelz, p.ne, p.nF,
