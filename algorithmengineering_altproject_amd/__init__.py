@@ -83,6 +83,7 @@ def lib():
         "t3d_mixed_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
         "t3d_mixed_destroy": (C.c_int, [vp]),
         "t3d_mixed_setup": (C.c_int, [vp]),
+        "t3d_mixed_coarse_solve": (C.c_int, [vp, vp, vp]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
         "t3d_batch_create": (C.c_int, [C.c_int, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
         "t3d_batch_destroy": (C.c_int, [vp]),
@@ -416,6 +417,10 @@ class Mixed:
 
     def setup(self):
         check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
+
+    def coarse_solve(self, db, dx):
+        """Coarsest-level solve of the mixed V-cycle (inverse Cholesky factor), FP64 device arrays."""
+        check(lib().t3d_mixed_coarse_solve(self.h, db.ptr, dx.ptr), "t3d_mixed_coarse_solve")
 
     def set_operator(self, form):
         """Operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6."""

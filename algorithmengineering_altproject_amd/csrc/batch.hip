@@ -49,6 +49,9 @@ struct BProb {
     const double* LT;
     double* rc64;
     double* xc64;
+    const double* Li;   // inverse coarsest Cholesky factor and its transpose
+    const double* LiT;
+    double* yc64;
     const float* Ecell;
     const uint8_t* nodeFixed;
     float *pA, *pB, *Ap, *z, *r, *dx, *tmp;
@@ -146,15 +149,21 @@ __global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P
                      bx, cy, cz, 0, 0);
 }
 
-// coarsest Cholesky solve and its FP32 copy; one block per listed problem
-__global__ void __launch_bounds__(CHOL_MAX) kb_chol(const BProb* __restrict__ P, const int* __restrict__ list)
+// coarsest solve through the inverse Cholesky factor (coarse.hpp): G blocks
+// per listed problem; UPPER = false: yc64 = Li rc64, true: x = Li^T yc64
+// (masked, FP64 and FP32 copies)
+template <bool UPPER>
+__global__ void __launch_bounds__(64 * CO_WAVES) kb_coarse_mv(const BProb* __restrict__ P,
+                                                              const int* __restrict__ list, unsigned G)
 {
-    const BProb& q = P[list[blockIdx.x]];
+    const BProb& q = P[list[blockIdx.x / G]];
     if (*q.flags) return;
-    chol_solve_body(q.Nc, q.L, q.LT, q.rc64, q.lv[q.nlev - 1].mask, q.xc64);
-    __syncthreads();
-    float* xc = q.lv[q.nlev - 1].x;
-    for (int i = threadIdx.x; i < q.Nc; i += blockDim.x) xc[i] = float(q.xc64[i]);
+    const int gw = int(blockIdx.x % G) * CO_WAVES + int(threadIdx.x) / 64;
+    const int L = q.nlev - 1;
+    if (UPPER)
+        coarse_mv_body<true>(q.Nc, q.LiT, q.yc64, q.lv[L].mask, q.xc64, q.lv[L].x, gw, int(G) * CO_WAVES);
+    else
+        coarse_mv_body<false>(q.Nc, q.Li, q.rc64, nullptr, q.yc64, nullptr, gw, int(G) * CO_WAVES);
 }
 
 // coarse-level prolongation + smoothing into level l (l >= 1)
@@ -225,6 +234,7 @@ struct t3d_batch {
     unsigned nPro[8] = {};
     int* cholList = nullptr;
     int nChol = 0;
+    unsigned cholG = 1;   // workgroups per problem of the coarsest products
     int maxLev = 0;
     int* itDev = nullptr;
     // tiles
@@ -283,6 +293,7 @@ static int batch_build(t3d_batch* B)
     std::vector<BMap> kw, rows, rx[8], yz[8], pro[8];
     std::vector<int> chol;
     B->maxLev = 0;
+    B->cholG = 1;
     for (int i = 0; i < B->n; ++i) {
         t3d_problem* p = B->P[i];
         t3d_mixed* s = B->X[i];
@@ -307,6 +318,10 @@ static int batch_build(t3d_batch* B)
         q.LT = v.LT;
         q.rc64 = s->rc64;
         q.xc64 = s->xc64;
+        q.Li = s->Li;
+        q.LiT = s->LiT;
+        q.yc64 = s->yc64;
+        T3D_CHECK(v.Nc == s->NcInv, T3D_ERR_STATE, "t3d_batch: inverse coarsest factor not set up");
         q.Ecell = p->EcellF;
         q.nodeFixed = p->nodeFixed;
         q.pA = s->pA;
@@ -342,7 +357,10 @@ static int batch_build(t3d_batch* B)
             const unsigned nyz = unsigned((Lc.nnx + 63) / 64) * unsigned(Lc.nny) * unsigned(Lc.nnz);
             for (unsigned t = 0; t < nyz; ++t) yz[l].push_back({i, int(t)});
         }
-        if (v.Nc > 0) chol.push_back(i);
+        if (v.Nc > 0) {
+            chol.push_back(i);
+            B->cholG = std::max(B->cholG, coarse_blocks(v.Nc));
+        }
     }
     T3D_HIP(hipMemcpy(B->dP, B->hP.data(), sizeof(BProb) * B->n, hipMemcpyHostToDevice));
     int rc;
@@ -425,9 +443,13 @@ static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady)
             hipLaunchKernelGGL(kb_restrict_yz, dim3(B->nYZ[l]), dim3(64), 0, st, (const BProb*)B->dP,
                                (const BMap*)B->mapYZ[l], l);
     }
-    if (B->nChol)
-        hipLaunchKernelGGL(kb_chol, dim3(B->nChol), dim3(CHOL_MAX), 0, st, (const BProb*)B->dP,
-                           (const int*)B->cholList);
+    if (B->nChol) {
+        const dim3 g(unsigned(B->nChol) * B->cholG), blk(64 * CO_WAVES);
+        hipLaunchKernelGGL(kb_coarse_mv<false>, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList,
+                           B->cholG);
+        hipLaunchKernelGGL(kb_coarse_mv<true>, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList,
+                           B->cholG);
+    }
     for (int l = B->maxLev - 1; l >= 1; --l)
         if (B->nPro[l])
             hipLaunchKernelGGL(kb_prolong, dim3(B->nPro[l]), dim3(64), 0, st, (const BProb*)B->dP,

@@ -33,6 +33,7 @@
 
 #include "t3d_internal.hpp"
 
+#include "coarse.hpp"
 #include "mx_kernels.hpp"
 
 namespace t3d {
@@ -470,11 +471,44 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
     d2f_body(n, a, b, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
+// inverse Cholesky factor, 64 columns per workgroup (coarse.hpp)
+__global__ void __launch_bounds__(64) k_tri_inv(int N, const double* __restrict__ L, double* Li, double* LiT)
+{
+    tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
+}
+
+// one triangular product of the coarsest solve (coarse.hpp)
+template <bool UPPER>
+__global__ void __launch_bounds__(64 * CO_WAVES) k_coarse_mv(int N, const double* __restrict__ M,
+                                                             const double* __restrict__ v,
+                                                             const uint8_t* __restrict__ mask, double* __restrict__ out,
+                                                             float* __restrict__ out32, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    coarse_mv_body<UPPER>(N, M, v, mask, out, out32, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
+                          int(gridDim.x) * CO_WAVES);
+}
+
 }  // namespace t3d
 
 using namespace t3d;
 
 #include "mixed_internal.hpp"
+
+namespace t3d {
+// xc64 (and x32, if given) = Li^T Li rc64, masked
+static void launch_coarse(t3d_mixed* s, const uint8_t* mask, float* x32, const int* done)
+{
+    hipStream_t st = s->p->stream;
+    const int N = s->NcInv;
+    const dim3 g(coarse_blocks(N)), blk(64 * CO_WAVES);
+    hipLaunchKernelGGL(k_coarse_mv<false>, g, blk, 0, st, N, (const double*)s->Li, (const double*)s->rc64,
+                       (const uint8_t*)nullptr, s->yc64, (float*)nullptr, done);
+    hipLaunchKernelGGL(k_coarse_mv<true>, g, blk, 0, st, N, (const double*)s->LiT, (const double*)s->yc64, mask,
+                       s->xc64, x32, done);
+}
+
+}  // namespace t3d
 
 namespace t3d {
 
@@ -569,10 +603,8 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
         T3D_HIP(hipGetLastError());
     }
     if (chol) {
-        hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(1024), 0, st, v.Nc, v.L, v.LT, s->rc64, v.mask[L], s->xc64,
-                           done);
-        const int64_t nc = 3 * v.nN[L];
-        hipLaunchKernelGGL(k_d2f, dim3(unsigned((nc + 255) / 256)), dim3(256), 0, st, nc, s->xc64, s->x32[L]);
+        T3D_CHECK(s->NcInv == v.Nc, T3D_ERR_STATE, "mixed V-cycle: t3d_mixed_setup not called after t3d_mg_setup");
+        launch_coarse(s, v.mask[L], s->x32[L], done);
         T3D_HIP(hipGetLastError());
     }
     const float w = float(v.weight);
@@ -682,7 +714,8 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
+    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->Li, s->LiT, s->yc64,
+                 s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
     for (int l = 0; l < 8; ++l) {
@@ -841,12 +874,47 @@ int t3d_mixed_setup(t3d_mixed* s)
     MGView v;
     int rc = mg_view(s->M, v);
     if (rc) return rc;
+    hipStream_t st = s->p->stream;
     for (int l = 0; l < v.nlev; ++l) {
         const int64_t n = 3 * v.nN[l];
-        hipLaunchKernelGGL(k_dinv, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s->p->stream, n, v.D[l], s->D32[l]);
+        hipLaunchKernelGGL(k_dinv, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, n, v.D[l], s->D32[l]);
         T3D_HIP(hipGetLastError());
     }
-    T3D_HIP(hipStreamSynchronize(s->p->stream));
+    // inverse of the coarsest Cholesky factor (the coarsest size is fixed by
+    // the hierarchy; the factor exists only when the Galerkin matrix is SPD)
+    s->NcInv = 0;
+    if (v.Nc > 0) {
+        const size_t n2 = size_t(v.Nc) * size_t(v.Nc);
+        if (!s->Li) {
+            T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
+            T3D_HIP(hipMalloc(&s->LiT, sizeof(double) * n2));
+            T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
+        }
+        T3D_HIP(hipMemsetAsync(s->Li, 0, sizeof(double) * n2, st));
+        T3D_HIP(hipMemsetAsync(s->LiT, 0, sizeof(double) * n2, st));
+        hipLaunchKernelGGL(k_tri_inv, dim3(unsigned((v.Nc + 63) / 64)), dim3(64), 0, st, v.Nc, v.L, s->Li, s->LiT);
+        T3D_HIP(hipGetLastError());
+        s->NcInv = v.Nc;
+    }
+    T3D_HIP(hipStreamSynchronize(st));
+    return T3D_OK;
+}
+
+// x = the mixed solver's coarsest solve of b (device arrays of the coarsest
+// size); a test hook for the inverse-factor form
+int t3d_mixed_coarse_solve(t3d_mixed* s, const double* b, double* x)
+{
+    T3D_CHECK(s && b && x, T3D_ERR_ARG, "t3d_mixed_coarse_solve: null argument");
+    T3D_CHECK(s->NcInv > 0, T3D_ERR_STATE, "t3d_mixed_coarse_solve: no Cholesky factor at the coarsest level");
+    MGView v;
+    int rc = mg_view(s->M, v);
+    if (rc) return rc;
+    hipStream_t st = s->p->stream;
+    T3D_HIP(hipMemcpyAsync(s->rc64, b, sizeof(double) * s->NcInv, hipMemcpyDeviceToDevice, st));
+    launch_coarse(s, v.mask[v.nlev - 1], nullptr, nullptr);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipMemcpyAsync(x, s->xc64, sizeof(double) * s->NcInv, hipMemcpyDeviceToDevice, st));
+    T3D_HIP(hipStreamSynchronize(st));
     return T3D_OK;
 }
 

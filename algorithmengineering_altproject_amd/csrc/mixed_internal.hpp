@@ -1,6 +1,7 @@
 // mixed_internal.hpp -- the mixed-precision solver's state and the helpers
 // shared with the batched solver (batch.hip).
 #pragma once
+#include "coarse.hpp"
 #include "mx_kernels.hpp"
 
 // mixed-precision state attached to a multigrid object
@@ -18,6 +19,10 @@ struct t3d_mixed {
     float* tmp = nullptr;      // restriction pass-1 buffer
     double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
     double* xc64 = nullptr;
+    double* Li = nullptr;      // inverse Cholesky factor L^-1 (row-major, lower) and its transpose
+    double* LiT = nullptr;
+    double* yc64 = nullptr;    // Li * rc64
+    int NcInv = 0;             // size of Li (0: diagonal coarsest level)
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
     float* r = nullptr;        // recursive residual (FP32)
     float* dx = nullptr;       // FP32 increment of x since the last fold

@@ -124,11 +124,19 @@ int t3d_pcg_history(t3d_problem* p, int maxIt, int n, double* rn2_host);
  * counterpart: it replaces PCG_free + the V-cycle closure, cpp/src/solver.cpp:87-217,
  * cpp/src/multigrid/preconditioner.cpp:88-265, with FP32 K*u and V-cycle inside a
  * flexible FP64 outer loop; FP64 dots, alpha/beta, x, r, residual replacement
- * and coarsest Cholesky) ---- */
+ * and coarsest Cholesky, the last applied as x = Li^T (Li b) with the FP64
+ * inverse factor Li = L^-1 instead of the substitution of
+ * cpp/src/multigrid/coarsest.cpp:92-122) ---- */
 int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out);
 int t3d_mixed_destroy(t3d_mixed* s);
-/* refresh the FP32 copies of the multigrid diagonals after t3d_mg_setup */
+/* refresh the FP32 copies of the multigrid diagonals and the inverse coarsest
+ * Cholesky factor after t3d_mg_setup */
 int t3d_mixed_setup(t3d_mixed* s);
+/* x = the mixed solver's coarsest-level solve of b (FP64 device arrays of the
+ * coarsest size 3 nN; masked DOFs of x are 0); the counterpart of
+ * chol_solve_lower (cpp/src/multigrid/coarsest.cpp:92-122) through Li.
+ * T3D_ERR_STATE when the coarsest level has no Cholesky factor. */
+int t3d_mixed_coarse_solve(t3d_mixed* s, const double* b_dev, double* x_dev);
 /* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
  * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old.
  * Convergence is reported only when the true residual ||b - K x|| (FP64

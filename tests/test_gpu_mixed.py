@@ -204,3 +204,39 @@ def test_mixed_mfma_operator(t3d, dims, uniform, form):
         assert itm < 4000 and res < tol
         assert abs(Cm - Cw) <= bound * abs(Cw)
     mx.set_operator(0)
+
+
+@pytest.mark.parametrize("dims,uniform", [((24, 48, 24), 0.2), ((32, 72, 32), None), ((8, 16, 8), 0.12)])
+def test_coarse_solve_inverse_factor(t3d, dims, uniform):
+    """The mixed V-cycle's coarsest solve x = Li^T (Li b), Li = L^-1 computed
+    on the device, against float64 substitution with the same factor L
+    (scipy's triangular solves; the plain FP64 path's own substitution is the
+    reference's order, chol_solve_lower in coarsest.cpp:92-122): fixed DOFs
+    zero, free DOFs within 64 kappa(L L^T) 2^-53 relative; the residual of
+    L L^T x = b on the free DOFs within 64 sqrt(N) ||A|| ||x|| 2^-53."""
+    import scipy.linalg as sla
+    p, dE, M, mx, db = _setup(t3d, dims, uniform)
+    nc, ok = M.info()
+    assert ok and nc > 0
+    _, L = M.coarsest()
+    N = L.shape[0]
+    assert N == nc
+    fixed = M.mask(len(M.nN) - 1).astype(bool)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(N)
+    dx = t3d.DeviceArray(N)
+    mx.coarse_solve(t3d.DeviceArray.from_host(b), dx)
+    x = dx.download()
+    y = sla.solve_triangular(L, b, lower=True)
+    xr = sla.solve_triangular(L.T, y, lower=False)
+    A = L @ L.T
+    eps = 2.0 ** -53
+    kappa = np.linalg.cond(A)
+    assert np.all(x[fixed] == 0.0)
+    free = ~fixed
+    err = np.max(np.abs(x[free] - xr[free])) / np.max(np.abs(xr[free]))
+    assert err <= 64 * kappa * eps, (err, kappa)
+    xs = np.where(fixed, 0.0, x)
+    xrs = np.where(fixed, 0.0, xr)
+    res = np.abs(A @ xs - A @ xrs)[free].max()
+    assert res <= 64 * np.sqrt(N) * np.linalg.norm(A, 2) * np.max(np.abs(xs)) * eps, res
