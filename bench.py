@@ -54,6 +54,10 @@ def parse():
                     help="batched workload instead: this many uneven cantilevers (seeded sizes, V0, voids)")
     ap.add_argument("--operator", choices=["walsh", "mfma3", "mfma6"], default="walsh",
                     help="FP32 operator of the mixed solve: Walsh form (FMA), MFMA bf16 x3 or MFMA bf16 x6")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch mode: launch each iteration's kernels directly instead of replaying hipGraphs "
+                         "(same kernels; for kernel-trace runs)")
+    ap.add_argument("--no-seq", action="store_true", help="batch mode: skip the sequential comparison")
     ap.add_argument("--kxu-reps", type=int, default=50)
     ap.add_argument("--kxu-only", action="store_true", help="only the K*u timing blocks (for counter runs)")
     ap.add_argument("--copy-only", action="store_true", help="only the streaming-copy calibration kernel")
@@ -173,40 +177,42 @@ def main_batch(args, t3d):
     for _ in range(max(1, args.warmup)):          # the first solve also autotunes the tiles
         for x in xs:
             x.zero()
-        B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every)
+        B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
     tiles = B.info()
     t3d.lib().t3d_synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for x in xs:
             x.zero()
-        its = B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every)
+        its = B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
     t3d.lib().t3d_synchronize()
     ms_batch = (time.perf_counter() - t0) / args.steps * 1e3
     res = [true_residual(t3d, p, db, x) for p, db, x in zip(B.problems, dbs, xs)]
     log("batched: %.1f ms for all %d solves (%.1f solves/s), tiles %s, iterations %d..%d, max true residual %.2e"
         % (ms_batch, len(probs), len(probs) * 1e3 / ms_batch, tiles, min(its), max(its), max(res)))
     # the same problems one after another with the single-problem solver
-    Ms = []
-    for i, p in enumerate(B.problems):
-        M = t3d.MG(p, Emin=float(np.float32(1e-9)))
-        M.setup(dEs[i])
-        mx = t3d.Mixed(p, M)
-        mx.setup()
-        Ms.append((M, mx))
-    for (M, mx), db, x in zip(Ms, dbs, xs):   # warm-up
-        x.zero()
-        mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every)
-    t3d.lib().t3d_synchronize()
-    t0 = time.perf_counter()
-    its_seq = []
-    for (M, mx), db, x in zip(Ms, dbs, xs):
-        x.zero()
-        its_seq.append(mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every))
-    t3d.lib().t3d_synchronize()
-    ms_seq = (time.perf_counter() - t0) * 1e3
-    log("sequential: %.1f ms for all solves (%.1f solves/s), speed-up %.2fx" % (ms_seq, len(probs) * 1e3 / ms_seq,
-                                                                             ms_seq / ms_batch))
+    ms_seq, its_seq = None, []
+    if not args.no_seq:
+        Ms = []
+        for i, p in enumerate(B.problems):
+            M = t3d.MG(p, Emin=float(np.float32(1e-9)))
+            M.setup(dEs[i])
+            mx = t3d.Mixed(p, M)
+            mx.setup()
+            Ms.append((M, mx))
+        for (M, mx), db, x in zip(Ms, dbs, xs):   # warm-up
+            x.zero()
+            mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every)
+        t3d.lib().t3d_synchronize()
+        t0 = time.perf_counter()
+        its_seq = []
+        for (M, mx), db, x in zip(Ms, dbs, xs):
+            x.zero()
+            its_seq.append(mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every))
+        t3d.lib().t3d_synchronize()
+        ms_seq = (time.perf_counter() - t0) * 1e3
+        log("sequential: %.1f ms for all solves (%.1f solves/s), speed-up %.2fx" % (ms_seq, len(probs) * 1e3 / ms_seq,
+                                                                                 ms_seq / ms_batch))
     line = {
         "metric": METRIC,
         "value": len(probs) * 1e3 / ms_batch,
@@ -233,8 +239,8 @@ def main_batch(args, t3d):
             "sequential_iterations": [int(v) for v in its_seq],
             "max_true_residual": max(res),
             "sequential_ms": ms_seq,
-            "sequential_solves_per_s": len(probs) * 1e3 / ms_seq,
-            "speedup_vs_sequential": ms_seq / ms_batch,
+            "sequential_solves_per_s": len(probs) * 1e3 / ms_seq if ms_seq else None,
+            "speedup_vs_sequential": ms_seq / ms_batch if ms_seq else None,
         },
         "roofline": None,
         "cpu_baseline": None,
