@@ -70,7 +70,7 @@ struct BMap {
 
 // ---- batched kernels: one block of a map entry per block ----
 template <int NZ>
-__global__ void __launch_bounds__(256) kb_kxw32(const BProb* __restrict__ P, const BMap* __restrict__ map,
+__global__ void __launch_bounds__(256, 5) kb_kxw32(const BProb* __restrict__ P, const BMap* __restrict__ map,
                                                const float* __restrict__ KW, int parity)
 {
     const BMap m = map[blockIdx.x];

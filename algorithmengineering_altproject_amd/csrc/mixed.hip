@@ -389,14 +389,25 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
     }
 }
 
+// xcd != 0: XCD-grouped tile order (each XCD works through a contiguous range
+// of tiles, x fastest, so x- and y-neighbours that share halo nodes run on
+// one L2); the partial slot is the tile index in both orders
 template <int NZ, bool FUSED>
-__global__ void __launch_bounds__(256) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
-                                               const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
-                                               const float* __restrict__ z, float* __restrict__ pcur,
-                                               float* __restrict__ y, double* __restrict__ partial,
-                                               const double* __restrict__ S, const int* __restrict__ done)
+__global__ void __launch_bounds__(256, 5) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
+                                                  const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
+                                                  const float* __restrict__ z, float* __restrict__ pcur,
+                                                  float* __restrict__ y, double* __restrict__ partial,
+                                                  const double* __restrict__ S, const int* __restrict__ done, int xcd)
 {
-    kxw32_body<NZ, FUSED>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+    unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (xcd) {
+        const TileId t = xcd_tile();
+        bx = t.x;
+        by = t.y;
+        bz = t.z;
+    }
+    const unsigned slot = (bz * gridDim.y + by) * gridDim.x + bx;
+    kxw32_body<NZ, FUSED>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, bx, by, bz, slot, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
@@ -542,7 +553,7 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
     else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
-                           nullptr, y, nullptr, nullptr, nullptr);
+                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd);
     else
         hipLaunchKernelGGL((k_kx32<KX_NZ, false>), kx_grid(p->g, KX_NZ), dim3(256), 0, p->stream, p->g,
                            (const float*)s->Ke32, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
@@ -568,7 +579,8 @@ static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* A
     else
         hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev,
-                           (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+                           (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
+                           s->kwXcd);
 }
 
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
@@ -947,7 +959,12 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form)
 // buffer, y = K p, block partials of p.y), as launched once per PCG iteration
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
 {
-    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 4, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 6, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    const int order = s->kwXcd;
+    if (op >= 5) {   // 5: fused, 6: Walsh-form operator, in the other tile order
+        s->kwXcd = !order;
+        op = op == 5 ? 4 : 0;
+    }
     t3d_problem* p = s->p;
     hipEvent_t e0, e1;
     T3D_HIP(hipEventCreate(&e0));
@@ -965,6 +982,7 @@ int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, 
     float t = 0.f;
     T3D_HIP(hipEventElapsedTime(&t, e0, e1));
     *ms = t / float(reps);
+    s->kwXcd = order;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return T3D_OK;

@@ -13,6 +13,7 @@ struct t3d_mixed {
     unsigned short* KA2 = nullptr;  // element matrix as 2-piece bf16 MFMA A operands
     unsigned short* KA3 = nullptr;  // ... and as 3-piece
     int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6
+    int kwXcd = 0;             // tile order of the Walsh-form kernel: 0 hardware, 1 XCD-grouped
     float* D32[8] = {};        // FP32 inverse diagonals 1 / max(1e-30, D)
     float* r32[8] = {};
     float* x32[8] = {};

@@ -142,48 +142,56 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
     // Node planes are staged in LDS; rows of PX nodes are 3*PX contiguous
-    // floats in memory.  The plain operator fetches a plane into registers one
-    // step ahead (fetch) and writes it to LDS after the arithmetic (commit), so
-    // the loads are in flight meanwhile; the fused form (two inputs) stages in
-    // place (stage), which keeps its register count at 6 waves per SIMD.
-    constexpr bool AHEAD = !FUSED;
+    // floats in memory.  Each thread owns SPT slots of a staged plane, whose
+    // offsets within a node plane do not depend on z: they are computed once.
+    // A plane is fetched into registers one step ahead (fetch) and written to
+    // LDS after the arithmetic (commit), so its loads are in flight meanwhile;
+    // the fused form forms p = z + beta * p_prev at the commit and stores the
+    // owned nodes of p.
     constexpr int SPT = (3 * PN + CX * CY - 1) / (CX * CY);
+    uint32_t soff[SPT];        // offset of slot j within a node plane
+    int slds[SPT];             // LDS index c * PN + row * PX + lx of slot j (-1: none)
+    uint32_t sval = 0u, sown = 0u;   // bit j: slot j inside the grid / an owned node
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int f = tid + j * CX * CY;
+        const int row = f / (3 * PX);
+        const int k = f - row * (3 * PX);
+        const int lx = k / 3, c = k - 3 * lx;
+        const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+        const bool slot = f < 3 * PN;
+        const bool in = slot && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny;
+        slds[j] = slot ? c * PN + row * PX + lx : -1;
+        soff[j] = in ? uint32_t(3 * (Y * g.nnx + X) + c) : 0u;
+        if (in) sval |= 1u << j;
+        if (in && lx >= 1 && lx <= NX && row >= 1 && row <= NY) sown |= 1u << j;
+    }
+    const int64_t plane = int64_t(3) * g.nnx * g.nny;
     float fu[SPT], fz[SPT];
     auto fetch = [&](int Z) {
         const bool zin = Z >= 0 && Z < g.nnz;
+        const float* up = u + (zin ? Z : 0) * plane;
+        const float* zp = FUSED ? z + (zin ? Z : 0) * plane : z;
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
-            const int f = tid + j * CX * CY;
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-            fu[j] = 0.f;
-            fz[j] = 0.f;
-            if (f < 3 * PN && zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
-                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
-                fu[j] = u[gi];
-                if (FUSED) fz[j] = z[gi];
-            }
+            const bool ok = zin && ((sval >> j) & 1u);
+            fu[j] = ok ? up[soff[j]] : 0.f;
+            if (FUSED) fz[j] = ok ? zp[soff[j]] : 0.f;
         }
     };
     auto commit = [&](int Z, int q) {
         const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
+        float* pp = FUSED ? pcur + (own ? Z : 0) * plane : pcur;
+        float* sq = &sP[q][0][0];
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
-            const int f = tid + j * CX * CY;
-            if (f >= 3 * PN) continue;
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
+            if (slds[j] < 0) continue;
             float v = fu[j];
             if (FUSED) {
                 v = fmaf(beta, v, fz[j]);
-                const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-                if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY && X < g.nnx && Y < g.nny)
-                    pcur[3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c] = v;
+                if (own && ((sown >> j) & 1u)) pp[soff[j]] = v;
             }
-            sP[q][c][row * PX + lx] = v;
+            sq[slds[j]] = v;
         }
     };
     // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
@@ -198,27 +206,6 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
-    // direct staging (fused form): load, combine and store in one pass
-    auto stage = [&](int Z, int q) {
-        const bool zin = Z >= 0 && Z < g.nnz;
-        const bool own = Z >= z0 && Z < z0 + NZ;
-        for (int f = tid; f < 3 * PN; f += CX * CY) {
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-            float v = 0.f;
-            if (zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
-                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
-                v = u[gi];
-                if (FUSED) {
-                    v = fmaf(beta, v, z[gi]);
-                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) pcur[gi] = v;
-                }
-            }
-            sP[q][c][row * PX + lx] = v;
-        }
-    };
     auto modulus = [&](int cz) {
         float E = 0.f;
         if (colIn && cz >= 0 && cz < g.resZ) {
@@ -227,15 +214,10 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
         }
         return E;
     };
-    if (AHEAD) {
-        fetch(z0 - 1);
-        commit(z0 - 1, 0);
-        fetch(z0);
-        commit(z0, 1);
-    } else {
-        stage(z0 - 1, 0);
-        stage(z0, 1);
-    }
+    fetch(z0 - 1);
+    commit(z0 - 1, 0);
+    fetch(z0);
+    commit(z0, 1);
     float Enext = modulus(z0 - 1);
     __syncthreads();
     face(0, Fp);
@@ -244,7 +226,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     for (int s = 0; s < nsteps; ++s) {
         const int qn = (s + 1) % 3, qo = s % 3;
         const bool more = s + 1 < nsteps;
-        if (AHEAD && more) fetch(z0 + s + 1);   // plane of the next step, in flight during this one
+        if (more) fetch(z0 + s + 1);   // plane of the next step, in flight during this one
         const float E = Enext;         // voxel (cx, cy, cz), cz = z0 - 1 + s
         const int cz = z0 - 1 + s;
         if (more) Enext = modulus(cz + 1);
@@ -315,12 +297,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
             }
         }
         }
-        if (more) {
-            if (AHEAD)
-                commit(z0 + s + 1, (s + 2) % 3);
-            else
-                stage(z0 + s + 1, (s + 2) % 3);
-        }
+        if (more) commit(z0 + s + 1, (s + 2) % 3);
         __syncthreads();
     }
     if (FUSED) {

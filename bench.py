@@ -59,6 +59,8 @@ def parse():
                          "(same kernels; for kernel-trace runs)")
     ap.add_argument("--no-seq", action="store_true", help="batch mode: skip the sequential comparison")
     ap.add_argument("--kxu-reps", type=int, default=50)
+    ap.add_argument("--tile-order-compare", action="store_true",
+                    help="with --kxu-only: also time the Walsh-form kernels in the other workgroup tile order")
     ap.add_argument("--kxu-only", action="store_true", help="only the K*u timing blocks (for counter runs)")
     ap.add_argument("--copy-only", action="store_true", help="only the streaming-copy calibration kernel")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain solves")
@@ -319,6 +321,12 @@ def main():
     log("K*u FP32 fused (solver kernel, %s): %.4f ms/launch, %.0f GB/s algorithmic"
         % (args.operator, fused_ms, bfused / fused_ms / 1e6))
     if args.kxu_only:
+        if args.operator == "walsh" and args.tile_order_compare:   # the other tile order
+            mx.time_kxu(u32, y32, 3, 5)
+            alt_f = mx.time_kxu(u32, y32, args.kxu_reps, 5)
+            alt_p = mx.time_kxu(u32, y32, args.kxu_reps, 6)
+            log("Walsh form, other tile order: fused %.4f ms (%.0f GB/s), operator %.4f ms (%.0f GB/s)"
+                % (alt_f, bfused / alt_f / 1e6, alt_p, b32 / alt_p / 1e6))
         return
 
     db = t3d.DeviceArray(p.nDOF)

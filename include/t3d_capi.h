@@ -154,7 +154,9 @@ int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_d
 int t3d_mixed_set_operator(t3d_mixed* s, int form);
 /* average milliseconds per launch over reps launches, HIP events on the
  * problem's stream: op 0..3 the FP32 operator of form 0..3; op 4 the solver's
- * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials) */
+ * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials);
+ * op 5 / op 6: the Walsh-form fused kernel / operator in the other workgroup
+ * tile order (hardware order vs XCD-grouped) */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 
 /* ---- batching (the chosen "batching" form; no reference counterpart: the
