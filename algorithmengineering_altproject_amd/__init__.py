@@ -50,6 +50,7 @@ def lib():
         "t3d_memset": (C.c_int, [vp, C.c_int, C.c_size_t]),
         "t3d_synchronize": (C.c_int, []),
         "t3d_problem_create_cuboid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+        "t3d_problem_create_voxels": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),
         "t3d_problem_destroy": (C.c_int, [vp]),
         "t3d_problem_sizes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "t3d_problem_model": (C.c_int, [vp] + [vp] * 8),
@@ -86,6 +87,7 @@ def lib():
         "t3d_mixed_coarse_solve": (C.c_int, [vp, vp, vp]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
         "t3d_batch_create": (C.c_int, [C.c_int, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
+        "t3d_batch_create_voxels": (C.c_int, [C.c_int, vp, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
         "t3d_batch_destroy": (C.c_int, [vp]),
         "t3d_batch_problem": (C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p)]),
         "t3d_batch_mg": (C.c_int, [vp, C.c_int, C.POINTER(C.c_void_p)]),
@@ -162,9 +164,18 @@ class Problem:
     """Device problem: CreateVoxelFEAmodel_Cuboid + ApplyBoundaryConditions
     (cpp/src/fea.cpp:181-447) with device mirrors."""
 
-    def __init__(self, nely, nelx, nelz):
+    def __init__(self, nely, nelx, nelz, solid=None):
+        """solid: optional voxel mask, uint8 indexed [ez, ex, ey] (flat (ez*nelx + ex)*nely + ey),
+        nonzero = solid; void voxels carry no element.  None: the full cuboid."""
         p = C.c_void_p()
-        check(lib().t3d_problem_create_cuboid(nely, nelx, nelz, C.byref(p)), "t3d_problem_create_cuboid")
+        if solid is None:
+            check(lib().t3d_problem_create_cuboid(nely, nelx, nelz, C.byref(p)), "t3d_problem_create_cuboid")
+        else:
+            sm = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+            if sm.size != nely * nelx * nelz:
+                raise ValueError("solid mask needs nely*nelx*nelz entries")
+            check(lib().t3d_problem_create_voxels(nely, nelx, nelz, _ptr(sm), C.byref(p)),
+                  "t3d_problem_create_voxels")
         self.h = p
         s = (C.c_int64 * 10)()
         check(lib().t3d_problem_sizes(self.h, s), "t3d_problem_sizes")
@@ -447,13 +458,23 @@ class Batch:
     """Many problems solved together by the batched mixed-precision MG-PCG
     (t3d_batch_*): dims = [(nely, nelx, nelz), ...]."""
 
-    def __init__(self, dims, Emin=1e-9):
+    def __init__(self, dims, Emin=1e-9, solids=None):
+        """solids: optional list of voxel masks (as Problem's solid; None entries = cuboid)."""
         n = len(dims)
         ny = np.array([d[0] for d in dims], np.int32)
         nx = np.array([d[1] for d in dims], np.int32)
         nz = np.array([d[2] for d in dims], np.int32)
         h = C.c_void_p()
-        check(lib().t3d_batch_create(n, _ptr(ny), _ptr(nx), _ptr(nz), Emin, C.byref(h)), "t3d_batch_create")
+        if solids is None:
+            check(lib().t3d_batch_create(n, _ptr(ny), _ptr(nx), _ptr(nz), Emin, C.byref(h)), "t3d_batch_create")
+        else:
+            keep = [None if m is None else np.ascontiguousarray(m, dtype=np.uint8).reshape(-1) for m in solids]
+            for d, m in zip(dims, keep):
+                if m is not None and m.size != d[0] * d[1] * d[2]:
+                    raise ValueError("solid mask needs nely*nelx*nelz entries")
+            ptrs = C.cast((C.c_void_p * n)(*[None if m is None else m.ctypes.data for m in keep]), C.c_void_p)
+            check(lib().t3d_batch_create_voxels(n, _ptr(ny), _ptr(nx), _ptr(nz), ptrs, Emin, C.byref(h)),
+                  "t3d_batch_create_voxels")
         self.h = h
         self.n = n
         self.problems = []

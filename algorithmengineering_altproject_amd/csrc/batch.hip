@@ -494,6 +494,12 @@ int t3d_batch_destroy(t3d_batch* B)
 
 int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, float Emin, t3d_batch** out)
 {
+    return t3d_batch_create_voxels(n, nely, nelx, nelz, nullptr, Emin, out);
+}
+
+int t3d_batch_create_voxels(int n, const int* nely, const int* nelx, const int* nelz, const uint8_t* const* solid,
+                            float Emin, t3d_batch** out)
+{
     T3D_CHECK(n > 0 && n <= 4096 && nely && nelx && nelz && out, T3D_ERR_ARG, "t3d_batch_create: bad argument");
     *out = nullptr;
     t3d_batch* B = new t3d_batch();
@@ -509,7 +515,7 @@ int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, f
     }
     int rc = T3D_OK;
     for (int i = 0; i < n && !rc; ++i) {
-        rc = t3d_problem_create_cuboid(nely[i], nelx[i], nelz[i], &B->P[i]);
+        rc = t3d_problem_create_voxels(nely[i], nelx[i], nelz[i], solid ? solid[i] : nullptr, &B->P[i]);
         if (rc) break;
         // every problem of the batch works on the batch's stream
         B->ownStreams[i] = B->P[i]->stream;

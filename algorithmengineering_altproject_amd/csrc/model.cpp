@@ -9,10 +9,11 @@
 
 namespace t3d {
 
-void padding_dims(int nelx, int nely, int nelz, int& rx, int& ry, int& rz, int& L)
+void padding_dims(int nelx, int nely, int nelz, uint64_t numSolid, int& rx, int& ry, int& rz, int& L)
 {
-    // padding.hpp:93-123: L = number of (n+4)/8 reductions until n < 50000, at least 3
-    uint64_t n = uint64_t(nely) * uint64_t(nelx) * uint64_t(nelz);
+    // padding.hpp:23-53: L = number of (n+4)/8 reductions of the solid voxel
+    // count until n < 50000, at least 3
+    uint64_t n = numSolid;
     int l = 0;
     while (n >= 50000u) { ++l; n = (n + 4u) / 8u; }
     L = std::max(3, l);
@@ -90,13 +91,25 @@ static inline uint64_t interleave3(uint32_t v)
 
 void build_cuboid(HostModel& m, int nely, int nelx, int nelz)
 {
+    build_voxels(m, nely, nelx, nelz, nullptr);
+}
+
+void build_voxels(HostModel& m, int nely, int nelx, int nelz, const uint8_t* solid)
+{
+    auto isSolid = [&](int ex, int ey, int ez) {
+        return !solid || solid[(size_t(ez) * nelx + ex) * nely + ey] != 0;
+    };
+    uint64_t numSolid = 0;
+    for (int ez = 0; ez < nelz; ++ez)
+        for (int ex = 0; ex < nelx; ++ex)
+            for (int ey = 0; ey < nely; ++ey) numSolid += isSolid(ex, ey, ez) ? 1u : 0u;
     m.nelx = nelx; m.nely = nely; m.nelz = nelz;
-    padding_dims(nelx, nely, nelz, m.resX, m.resY, m.resZ, m.baseL);
+    padding_dims(nelx, nely, nelz, numSolid, m.resX, m.resY, m.resZ, m.baseL);
     const int nx = m.resX, ny = m.resY, nz = m.resZ;
     const int nnx = nx + 1, nny = ny + 1, nnz = nz + 1;
     m.numNodes = nnx * nny * nnz;
     m.numDOFs = 3 * m.numNodes;
-    const int ne = nelx * nely * nelz;
+    const int ne = int(numSolid);
     m.numElements = ne;
     // Morton-sorted solid voxels (fea.cpp:289-317); key = x<<2 | y<<1 | z bits
     struct KE { uint64_t key; int32_t full; };
@@ -105,6 +118,7 @@ void build_cuboid(HostModel& m, int nely, int nelx, int nelz)
     for (int ez = 0; ez < nelz; ++ez)
         for (int ex = 0; ex < nelx; ++ex)
             for (int ey = 0; ey < nely; ++ey) {
+                if (!isSolid(ex, ey, ez)) continue;   // void voxel: no element (fea.cpp:203-227)
                 const uint64_t key = (interleave3(ex) << 2) | (interleave3(ey) << 1) | interleave3(ez);
                 ord.push_back({key, ny * nx * ez + ny * ex + (ny - 1 - ey)});
             }

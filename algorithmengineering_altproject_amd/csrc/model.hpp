@@ -31,9 +31,16 @@ struct HostModel {
     std::vector<int32_t> eDofFreeMat;   // [24*ne]
 };
 
-void padding_dims(int nelx, int nely, int nelz, int& rx, int& ry, int& rz, int& L);
+void padding_dims(int nelx, int nely, int nelz, uint64_t numSolid, int& rx, int& ry, int& rz, int& L);
 std::array<double, 576> voxel_ke(double nu, double cellSize);
 void build_cuboid(HostModel& m, int nely, int nelx, int nelz);
+// general voxel domain: solid[(ez * nelx + ex) * nely + ey] != 0 marks a solid
+// voxel of the nely x nelx x nelz region (ey as in the reference's element
+// numbering, stored at full index ny*nx*ez + ny*ex + (ny-1-ey)); void voxels
+// carry no element, as the padding (fea.cpp:203-227), and the padding level
+// follows the solid voxel count (padding.hpp:42-53, TOP3D_XL.m:1763-1784).
+// solid == nullptr: the full cuboid.
+void build_voxels(HostModel& m, int nely, int nelx, int nelz, const uint8_t* solid);
 void apply_bcs(HostModel& m);
 
 // structured cell index used on the device: (cz*resY + ty)*resX + cx with

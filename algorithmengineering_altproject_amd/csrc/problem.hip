@@ -238,11 +238,21 @@ int t3d_problem_destroy(t3d_problem* p)
 
 int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out)
 {
-    T3D_CHECK(out && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_problem_create_cuboid: bad arguments");
+    return t3d_problem_create_voxels(nely, nelx, nelz, nullptr, out);
+}
+
+int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid, t3d_problem** out)
+{
+    T3D_CHECK(out && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_problem_create_voxels: bad arguments");
     *out = nullptr;
+    if (solid) {
+        size_t cnt = 0;
+        for (size_t i = 0; i < size_t(nely) * nelx * nelz; ++i) cnt += solid[i] != 0;
+        T3D_CHECK(cnt > 0, T3D_ERR_ARG, "t3d_problem_create_voxels: no solid voxel");
+    }
     t3d_problem* p = new t3d_problem();
     HostModel& m = p->host;
-    build_cuboid(m, nely, nelx, nelz);
+    build_voxels(m, nely, nelx, nelz, solid);
     apply_bcs(m);
     Grid& g = p->g;
     g.resX = m.resX; g.resY = m.resY; g.resZ = m.resZ;
@@ -251,7 +261,7 @@ int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out)
     g.ncell = int64_t(g.resX) * g.resY * g.resZ;
     if (3 * g.nN >= (int64_t(1) << 31)) {
         delete p;
-        set_error("t3d_problem_create_cuboid: more than 2^31 DOFs");
+        set_error("t3d_problem_create_voxels: more than 2^31 DOFs");
         return T3D_ERR_ARG;
     }
     p->ne = m.numElements;

@@ -56,6 +56,16 @@ int t3d_synchronize(void);
  * followed by ApplyBoundaryConditions (cpp/include/fea.hpp:39, cpp/src/fea.cpp:360-447):
  * builds the padded voxel model on the host and uploads its device mirror. */
 int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out);
+/* General voxel domain (the voxel-model generalisation of
+ * CreateVoxelFEAmodel_Cuboid, cpp/src/fea.cpp:181-358, and its padding rule,
+ * cpp/include/multigrid/padding.hpp:42-53 with the solid voxel count, as
+ * matlab/TOP3D_XL.m:1763-1784): solid[(ez*nelx + ex)*nely + ey] != 0 marks a
+ * solid voxel of the nely x nelx x nelz region (ey as in the reference's
+ * element numbering); void voxels carry no element, nodes touched by no
+ * element are fixed, and the reference's boundary conditions and load
+ * (fea.cpp:360-447) apply unchanged.  solid == NULL: the full cuboid.
+ * T3D_ERR_ARG if no voxel is solid. */
+int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid, t3d_problem** out);
 int t3d_problem_destroy(t3d_problem* p);
 /* sizes[0..9] = resX resY resZ numElements numNodes numDOFs numFree nnx nny nnz */
 int t3d_problem_sizes(const t3d_problem* p, int64_t* sizes);
@@ -170,6 +180,10 @@ typedef struct t3d_batch t3d_batch;
 /* n cuboids (nely[i], nelx[i], nelz[i]); multigrid with the reference's
  * defaults (non-dyadic, 5 levels, weight 0.6, NlimitDofs 1000) and Emin */
 int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, float Emin, t3d_batch** out);
+/* the same with voxel domains: solid[i] is problem i's mask as in
+ * t3d_problem_create_voxels (solid == NULL or solid[i] == NULL: full cuboid) */
+int t3d_batch_create_voxels(int n, const int* nely, const int* nelx, const int* nelz, const uint8_t* const* solid,
+                            float Emin, t3d_batch** out);
 int t3d_batch_destroy(t3d_batch* b);
 /* the batch's problem / multigrid object i (owned by the batch) */
 int t3d_batch_problem(t3d_batch* b, int i, t3d_problem** p);

@@ -97,6 +97,8 @@ def lib():
         L = C.CDLL(path)
         L.o_model_new.restype = C.POINTER(OModel)
         L.o_model_new.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.o_model_new_voxels.restype = C.POINTER(OModel)
+        L.o_model_new_voxels.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.o_model_free.argtypes = [C.POINTER(OModel)]
         L.o_canon_sum.restype = C.c_double
         L.o_canon_sum.argtypes = [f64p, C.c_size_t]
@@ -155,10 +157,17 @@ def _arr(ptr, n, dtype):
 class Model:
     """Oracle model (fea.cpp:181-447): arrays are copied out as numpy."""
 
-    def __init__(self, nely, nelx, nelz):
+    def __init__(self, nely, nelx, nelz, solid=None):
+        """solid: optional uint8 array indexed [ez, ex, ey] (flat (ez*nelx + ex)*nely + ey),
+        nonzero = solid voxel; None = the full cuboid."""
         L = lib()
         self.nely, self.nelx, self.nelz = nely, nelx, nelz
-        self.ptr = L.o_model_new(nely, nelx, nelz)
+        if solid is None:
+            self.ptr = L.o_model_new(nely, nelx, nelz)
+        else:
+            sm = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+            assert sm.size == nely * nelx * nelz
+            self.ptr = L.o_model_new_voxels(nely, nelx, nelz, sm.ctypes.data)
         m = self.ptr.contents
         self.resX, self.resY, self.resZ, self.baseL = m.resX, m.resY, m.resZ, m.baseL
         self.ne, self.nN, self.nDOF, self.nF = m.ne, m.nN, m.nDOF, m.nF

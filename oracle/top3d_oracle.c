@@ -106,12 +106,17 @@ static int pad_pow2(int n, int L)
     long step = 1L << L;
     return (int)(((long)n + step - 1) / step * step);
 }
-void o_padding(int nelx, int nely, int nelz, int* rx, int* ry, int* rz, int* L)
+/* padding of a region with nv solid voxels (compute_adjusted_dims,
+ * padding.hpp:42-53, counts the solid voxels) */
+static void padding_n(int nelx, int nely, int nelz, uint64_t nv, int* rx, int* ry, int* rz, int* L)
 {
-    uint64_t nv = (uint64_t)nely * (uint64_t)nelx * (uint64_t)nelz;
     int l = base_levels(nv);
     *rx = pad_pow2(nelx, l); *ry = pad_pow2(nely, l); *rz = pad_pow2(nelz, l);
     if (L) *L = l;
+}
+void o_padding(int nelx, int nely, int nelz, int* rx, int* ry, int* rz, int* L)
+{
+    padding_n(nelx, nely, nelz, (uint64_t)nely * (uint64_t)nelx * (uint64_t)nelz, rx, ry, rz, L);
 }
 
 /* ===================================================================== */
@@ -214,21 +219,31 @@ static const int A_DZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
 o_model* o_model_new(int nely, int nelx, int nelz)
 {
+    return o_model_new_voxels(nely, nelx, nelz, NULL);
+}
+
+/* solid[(ez*nelx + ex)*nely + ey] != 0: solid voxel of the original region
+ * (NULL: all); void voxels carry no element, like the padding
+ * (fea.cpp:203-227, generalised from the cuboid's all-solid region) */
+o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid)
+{
     o_model* m = (o_model*)calloc(1, sizeof(o_model));
     m->nelx = nelx; m->nely = nely; m->nelz = nelz;
-    o_padding(nelx, nely, nelz, &m->resX, &m->resY, &m->resZ, &m->baseL);
+    uint64_t nsolid = 0;
+    for (size_t i = 0; i < (size_t)nely * nelx * nelz; ++i) nsolid += (!solid || solid[i]) ? 1u : 0u;
+    padding_n(nelx, nely, nelz, nsolid, &m->resX, &m->resY, &m->resZ, &m->baseL);
     const int nx = m->resX, ny = m->resY, nz = m->resZ;
     m->nnx = nx + 1; m->nny = ny + 1; m->nnz = nz + 1;
     m->nN = m->nnx * m->nny * m->nnz;
     m->nDOF = 3 * m->nN;
-    /* solid voxels: the original region; padding is void (fea.cpp:203-227) */
-    int ne = nelx * nely * nelz;
+    int ne = (int)nsolid;
     m->ne = ne;
-    o_mk* el = XALLOC(o_mk, ne);
+    o_mk* el = XALLOC(o_mk, ne > 0 ? ne : 1);
     int k = 0;
     for (int ez = 0; ez < nelz; ++ez)
         for (int ex = 0; ex < nelx; ++ex)
             for (int ey = 0; ey < nely; ++ey) {
+                if (solid && !solid[((size_t)ez * nelx + ex) * nely + ey]) continue;
                 el[k].key = morton_key(ex, ey, ez);
                 el[k].full = ny * nx * ez + ny * ex + (ny - 1 - ey);
                 ++k;

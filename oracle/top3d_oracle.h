@@ -59,6 +59,8 @@ typedef struct o_model {
 
 /* Build the cuboid model of CreateVoxelFEAmodel_Cuboid + ApplyBoundaryConditions. */
 o_model* o_model_new(int nely, int nelx, int nelz);
+/* general voxel domain: solid[(ez*nelx + ex)*nely + ey] != 0 (NULL: cuboid) */
+o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid);
 void o_model_free(o_model* m);
 void o_voxel_ke(double nu, double cellSize, double Ke[576]);
 void o_padding(int nelx, int nely, int nelz, int* rx, int* ry, int* rz, int* L);
