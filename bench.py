@@ -58,6 +58,8 @@ def parse():
                     help="batch mode: launch each iteration's kernels directly instead of replaying hipGraphs "
                          "(same kernels; for kernel-trace runs)")
     ap.add_argument("--no-seq", action="store_true", help="batch mode: skip the sequential comparison")
+    ap.add_argument("--voids", choices=["removed", "density0"], default="removed",
+                    help="batch mode: spherical voids as absent voxels (default) or as elements of density 0")
     ap.add_argument("--kxu-reps", type=int, default=50)
     ap.add_argument("--tile-order-compare", action="store_true",
                     help="with --kxu-only: also time the Walsh-form kernels in the other workgroup tile order")
@@ -143,6 +145,18 @@ def batch_workload(rng, n):
     return probs
 
 
+def batch_mask(dims, spheres):
+    """solid[ez, ex, ey] of a cuboid with spherical voids (voxel centres inside a sphere are void)."""
+    nely, nelx, nelz = dims
+    if not spheres:
+        return None
+    ez, ex, ey = np.meshgrid(np.arange(nelz), np.arange(nelx), np.arange(nely), indexing="ij")
+    solid = np.ones((nelz, nelx, nely), np.uint8)
+    for cx, cy, cz, r in spheres:
+        solid[(ex + 0.5 - cx) ** 2 + (ey + 0.5 - cy) ** 2 + (ez + 0.5 - cz) ** 2 <= r * r] = 0
+    return solid
+
+
 def batch_design(t3d, p, v0, spheres):
     x = np.full(p.ne, np.float32(v0), np.float32)
     if spheres:
@@ -164,8 +178,14 @@ def main_batch(args, t3d):
     probs = batch_workload(rng, args.batch)
     dims = [d for d, _, _ in probs]
     t0 = time.time()
-    B = t3d.Batch(dims, Emin=float(np.float32(1e-9)))
-    dEs = [batch_design(t3d, p, v, s) for p, (_, v, s) in zip(B.problems, probs)]
+    if args.voids == "removed":   # void voxels carry no element (voxel-domain model)
+        solids = [batch_mask(d, s) for d, _, s in probs]
+        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)), solids=solids)
+        dEs = [batch_design(t3d, p, v, ()) for p, (_, v, s) in zip(B.problems, probs)]
+    else:                         # void voxels kept as elements of density 0 (E = Emin)
+        solids = [None] * len(probs)
+        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)))
+        dEs = [batch_design(t3d, p, v, s) for p, (_, v, s) in zip(B.problems, probs)]
     B.setup(dEs)
     dbs, xs = [], []
     for p in B.problems:
@@ -196,7 +216,7 @@ def main_batch(args, t3d):
     ms_seq, its_seq = None, []
     if not args.no_seq:
         Ms = []
-        for i, p in enumerate(B.problems):
+        for i, p in enumerate(B.problems):   # the batch's problems, each with its own multigrid and solver state
             M = t3d.MG(p, Emin=float(np.float32(1e-9)))
             M.setup(dEs[i])
             mx = t3d.Mixed(p, M)
@@ -228,7 +248,9 @@ def main_batch(args, t3d):
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (%d cantilevers, seed %d: nelx~U{48..160}, nely=nelz=nelx/2, V0~U[0.08,0.30], half with "
-                "1-6 spherical voids r~U[3,15])" % (len(probs), args.seed),
+                "1-6 spherical voids r~U[3,15], %s)" % (len(probs), args.seed,
+                                                        "voids removed from the voxel domain" if args.voids == "removed"
+                                                        else "voids as density-0 elements"),
         "config": {
             "workload": "batched mixed-precision MG-PCG: %d uneven problems solved together to true residual < %.0e "
                         "(one launch per kernel for the batch, hipGraph-replayed iterations, autotuned tiles); a "
