@@ -660,22 +660,56 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
         }
     }
     __syncthreads();
+    // Each thread owns the row positions k = k0 + tid + 256 u; their x-direction
+    // interpolation (coarse column, weight) and component do not depend on the
+    // row, so they are computed once per chunk of 256 U positions, and a row's
+    // loads are all issued before its arithmetic.  Within a chunk the partial
+    // sums run over rows, then positions, as one pass over the block's rows.
+    constexpr int U = 8;
     double srz = 0.0, szap = 0.0;
-    for (int rr = 0; rr < nr; ++rr) {
-        const int64_t nbase = (row0 + rr) * Lf.nnx;
-        for (int k = threadIdx.x; k < L; k += 256) {
+    for (int k0 = 0; k0 < L; k0 += 256 * U) {
+        int qo[U], cc[U], xs[U];
+        float w1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + int(threadIdx.x) + u * 256;
             const int xx = k / 3, c = k - 3 * xx;
             int cxx = xx / span, rx = xx - cxx * span;
             if (cxx >= Lc.resX) { cxx = Lc.resX - 1; rx = span; }
-            const float wx1 = float(rx) * inv;
-            const float v = (1.f - wx1) * sQ[rr * Lq + 3 * cxx + c] + wx1 * sQ[rr * Lq + 3 * cxx + 3 + c];
-            const int64_t i = 3 * nbase + k;
-            const float r = rf[i];
-            float out = 0.f;
-            if (!((fmask[nbase + xx] >> c) & 1u)) out = 2.f * (weight * r * Df[i]) + v;
-            xf[i] = out;
-            srz = fma(double(r), double(out), srz);
-            szap = fma(double(out), double(Ap[i]), szap);
+            w1[u] = float(rx) * inv;
+            qo[u] = 3 * cxx + c;
+            cc[u] = c;
+            xs[u] = xx;
+        }
+        for (int rr = 0; rr < nr; ++rr) {
+            const int64_t nbase = (row0 + rr) * Lf.nnx;
+            const float* q = sQ + rr * Lq;
+            float rv[U], dv[U], av[U];
+            unsigned mv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + int(threadIdx.x) + u * 256;
+                rv[u] = dv[u] = av[u] = 0.f;
+                mv[u] = 0xffu;
+                if (k < L) {
+                    const int64_t i = 3 * nbase + k;
+                    rv[u] = rf[i];
+                    dv[u] = Df[i];
+                    av[u] = Ap[i];
+                    mv[u] = fmask[nbase + xs[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = k0 + int(threadIdx.x) + u * 256;
+                if (k >= L) continue;
+                const float v = (1.f - w1[u]) * q[qo[u]] + w1[u] * q[qo[u] + 3];
+                float out = 0.f;
+                if (!((mv[u] >> cc[u]) & 1u)) out = 2.f * (weight * rv[u] * dv[u]) + v;
+                xf[3 * nbase + k] = out;
+                srz = fma(double(rv[u]), double(out), srz);
+                szap = fma(double(out), double(av[u]), szap);
+            }
         }
     }
     const double a = block_sum_256(srz, sh);
