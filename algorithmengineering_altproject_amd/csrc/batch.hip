@@ -70,7 +70,7 @@ struct BMap {
 
 // ---- batched kernels: one block of a map entry per block ----
 template <int NZ>
-__global__ void __launch_bounds__(256, 5) kb_kxw32(const BProb* __restrict__ P, const BMap* __restrict__ map,
+__global__ void __launch_bounds__(32 * KW_CY, KW_CY >= 16 ? 4 : 5) kb_kxw32(const BProb* __restrict__ P, const BMap* __restrict__ map,
                                                const float* __restrict__ KW, int parity)
 {
     const BMap m = map[blockIdx.x];
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256, 5) kb_kxw32(const BProb* __restrict__ P, 
     const unsigned tx = loc % q.kgx, ty = (loc / q.kgx) % q.kgy, tz = loc / (q.kgx * q.kgy);
     const float* pprev = parity ? q.pB : q.pA;
     float* pcur = parity ? q.pA : q.pB;
-    kxw32_body<NZ, true>(q.g, KW, q.Ecell, q.nodeFixed, pprev, q.z, pcur, q.Ap, q.part, q.S, q.flags, tx, ty, tz,
+    kxw32_body<NZ, true, KW_CY>(q.g, KW, q.Ecell, q.nodeFixed, pprev, q.z, pcur, q.Ap, q.part, q.S, q.flags, tx, ty, tz,
                          loc, 0);
 }
 
@@ -336,7 +336,7 @@ static int batch_build(t3d_batch* B)
         q.flags = s->iflags;
         const Grid& g = p->g;
         q.kgx = unsigned((g.nnx + wk::NX - 1) / wk::NX);
-        q.kgy = unsigned((g.nny + wk::NY - 1) / wk::NY);
+        q.kgy = unsigned((g.nny + KW_CY - 2) / (KW_CY - 1));
         const unsigned kgz = unsigned((g.nnz + B->kwNZ - 1) / B->kwNZ);
         q.kwTiles = q.kgx * q.kgy * kgz;
         q.rowBlocks = unsigned((int64_t(g.nny) * g.nnz + B->rpb - 1) / B->rpb);
@@ -382,7 +382,7 @@ static int batch_build(t3d_batch* B)
 
 static void launch_kw(t3d_batch* B, int parity)
 {
-    const dim3 grid(B->nKW), blk(256);
+    const dim3 grid(B->nKW), blk(32 * KW_CY);
     switch (B->kwNZ) {
     case 8: hipLaunchKernelGGL(kb_kxw32<8>, grid, blk, 0, B->stream, B->dP, B->mapKW, (const float*)B->X[0]->KW, parity); break;
     case 32: hipLaunchKernelGGL(kb_kxw32<32>, grid, blk, 0, B->stream, B->dP, B->mapKW, (const float*)B->X[0]->KW, parity); break;

@@ -392,8 +392,8 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
 // xcd != 0: XCD-grouped tile order (each XCD works through a contiguous range
 // of tiles, x fastest, so x- and y-neighbours that share halo nodes run on
 // one L2); the partial slot is the tile index in both orders
-template <int NZ, bool FUSED>
-__global__ void __launch_bounds__(256, 5) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
+template <int NZ, bool FUSED, int CYT>
+__global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
                                                   const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
                                                   const float* __restrict__ z, float* __restrict__ pcur,
                                                   float* __restrict__ y, double* __restrict__ partial,
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(256, 5) k_kxw32(Grid g, const float* __restric
         bz = t.z;
     }
     const unsigned slot = (bz * gridDim.y + by) * gridDim.x + bx;
-    kxw32_body<NZ, FUSED>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, bx, by, bz, slot, gridDim.x);
+    kxw32_body<NZ, FUSED, CYT>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, bx, by, bz, slot, gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
@@ -531,6 +531,12 @@ static dim3 kx_grid(const Grid& g, int nz)
                 unsigned((g.nnz + nz - 1) / nz));
 }
 
+dim3 kww_grid(const Grid& g)
+{
+    return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + KW_CY - 2) / (KW_CY - 1)),
+                unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
+}
+
 dim3 kw_grid(const Grid& g)
 {
     return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + wk::NY - 1) / wk::NY),
@@ -551,7 +557,7 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
                            (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
                            nullptr, nullptr, y, nullptr, nullptr, nullptr);
     else if (form == 0)
-        hipLaunchKernelGGL((k_kxw32<KW_NZ, false>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
                            nullptr, y, nullptr, nullptr, nullptr, s->kwXcd);
     else
@@ -577,7 +583,7 @@ static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* A
                            (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
                            pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
     else
-        hipLaunchKernelGGL((k_kxw32<KW_NZ, true>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, true, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev,
                            (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
                            s->kwXcd);
@@ -1010,7 +1016,8 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     double* S = s->sc;
     int* flags = s->iflags;
-    const dim3 kg = kw_grid(g);
+    const dim3 kg = s->form == 0 ? kww_grid(g) : kw_grid(g);   // tiles of the fused operator kernel
+    const unsigned nfused = kg.x * kg.y * kg.z;
     const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
     const unsigned xrb = row_blocks(L0);
     // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
@@ -1035,7 +1042,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         for (int it = it0; it < it1; ++it) {
             // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
             launch_fused(s, pPrevOf(it), pCurOf(it), s->Ap);
-            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, kg.x * kg.y * kg.z, S,
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, nfused, S,
                                (const int*)flags);
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
             // dx += alpha p ; r -= alpha Ap ; rn2 ; convergence test

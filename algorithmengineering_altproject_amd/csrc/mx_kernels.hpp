@@ -30,6 +30,26 @@ __device__ __forceinline__ double block_sum_256(double s, double* sh)
     return t;
 }
 
+// sum over an NT-thread block (NT a multiple of 64, at most 1024); valid in
+// thread 0; sh[NT / 64]
+template <int NT>
+__device__ __forceinline__ double block_sum_n(double s, double* sh)
+{
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        if (NT == 256) {
+            t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        } else {
+            for (int k = 0; k < NT / 64; ++k) t += sh[k];
+        }
+    }
+    return t;
+}
+
 // sum of partial[0..m) in a fixed order by a 1024-thread block (valid in thread 0)
 __device__ __forceinline__ double sum_partials_1024(const double* __restrict__ partial, unsigned m, double* sh)
 {
@@ -119,7 +139,8 @@ __device__ __forceinline__ void walsh2(float a00, float a10, float a01, float a1
     F[1][1] = d0 - d1;
 }
 
-template <int NZ, bool FUSED>
+// CYT: voxel rows of a tile (the workgroup is 32 x CYT threads)
+template <int NZ, bool FUSED, int CYT = wk::CY>
 __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
                                                const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
                                                const float* __restrict__ z, float* __restrict__ pcur,
@@ -127,11 +148,16 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
                                                const double* __restrict__ S, const int* __restrict__ done,
     unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_)
 {
-    using namespace wk;
+    using wk::IN;
+    using wk::NNZ;
+    using wk::OUT;
+    constexpr int CX = wk::CX, CY = CYT;
+    constexpr int NX = CX - 1, NY = CY - 1;
+    constexpr int PX = CX + 1, PY = CY + 1, PN = PX * PY;
     if (FUSED && *done) return;
     __shared__ float sP[3][3][PN];     // three staged node planes, [buffer][component][node]
     __shared__ float sC[12][CX * CY];  // face-corner forces [(2 oy + ox) * 3 + c][thread]
-    __shared__ double sh[4];
+    __shared__ double sh[CX * CY / 64];
     const int tid = threadIdx.x;
     const int tx = tid % CX, ty = tid / CX;
     // hardware tile order: measured faster here than the XCD-grouped order
@@ -301,7 +327,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
         __syncthreads();
     }
     if (FUSED) {
-        const double t = block_sum_256(dot, sh);
+        const double t = block_sum_n<CX * CY>(dot, sh);
         if (tid == 0) partial[slot_] = t;
     }
 }
