@@ -139,6 +139,24 @@ __device__ __forceinline__ void walsh2(float a00, float a10, float a01, float a1
     F[1][1] = d0 - d1;
 }
 
+// Raw buffer resource over `bytes` bytes at `base` (gfx9 resource word 3).
+// Loads at offsets beyond the range return 0 and stores there are dropped;
+// the Walsh-form operator addresses its grid edges this way instead of with
+// branches, and keeps 32-bit offsets instead of 64-bit addresses.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(bytes), 0x00020000);
+}
+constexpr uint32_t BUF_OOB = 0x80000000u;   // beyond every range used with buf_rsrc
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
 // CYT: voxel rows of a tile (the workgroup is 32 x CYT threads)
 template <int NZ, bool FUSED, int CYT = wk::CY>
 __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
@@ -154,10 +172,13 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     constexpr int CX = wk::CX, CY = CYT;
     constexpr int NX = CX - 1, NY = CY - 1;
     constexpr int PX = CX + 1, PY = CY + 1, PN = PX * PY;
+    constexpr int NT = CX * CY;
+    constexpr int SPT = (3 * PN + NT - 1) / NT;   // staging slots per thread
+    constexpr int SPN = 3 * PN + 64;              // staged plane + 64 write-only dummy slots
     if (FUSED && *done) return;
-    __shared__ float sP[3][3][PN];     // three staged node planes, [buffer][component][node]
-    __shared__ float sC[12][CX * CY];  // face-corner forces [(2 oy + ox) * 3 + c][thread]
-    __shared__ double sh[CX * CY / 64];
+    __shared__ float sP[3][SPN];       // three staged node planes, [buffer][component * PN + node]
+    __shared__ float sC[12][NT];       // face-corner forces [(2 oy + ox) * 3 + c][thread]
+    __shared__ double sh[NT / 64];
     const int tid = threadIdx.x;
     const int tx = tid % CX, ty = tid / CX;
     // hardware tile order: measured faster here than the XCD-grouped order
@@ -167,84 +188,98 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
     const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
+    const int X = x0 + tx, Y = y0 + ty;             // the node this thread gathers
+    const bool ownNode = tx < NX && ty < NY && X < g.nnx && Y < g.nny;
     // Node planes are staged in LDS; rows of PX nodes are 3*PX contiguous
     // floats in memory.  Each thread owns SPT slots of a staged plane, whose
-    // offsets within a node plane do not depend on z: they are computed once.
-    // A plane is fetched into registers one step ahead (fetch) and written to
-    // LDS after the arithmetic (commit), so its loads are in flight meanwhile;
-    // the fused form forms p = z + beta * p_prev at the commit and stores the
-    // owned nodes of p.
-    constexpr int SPT = (3 * PN + CX * CY - 1) / (CX * CY);
-    uint32_t soff[SPT];        // offset of slot j within a node plane
-    int slds[SPT];             // LDS index c * PN + row * PX + lx of slot j (-1: none)
-    uint32_t sval = 0u, sown = 0u;   // bit j: slot j inside the grid / an owned node
+    // byte offsets within a node plane do not depend on z: they are computed
+    // once (BUF_OOB outside the grid, so those slots load 0).  A plane is
+    // fetched into registers one step ahead (fetch) and written to LDS after
+    // the arithmetic (commit), so its loads are in flight meanwhile; the fused
+    // form forms p = z + beta * p_prev at the commit and stores the owned
+    // nodes of p.  The voxel modulus and the node's fixed-DOF bits are also
+    // loaded one step ahead, so no step waits on a load it has just issued.
+    uint32_t voff[SPT];        // byte offset of slot j within a node plane
+    uint32_t vown[SPT];        // the same for owned nodes (p store), BUF_OOB otherwise
+    int slds[SPT];             // LDS index c * PN + row * PX + lx of slot j (a dummy slot if none)
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
-        const int f = tid + j * CX * CY;
+        const int f = tid + j * NT;
         const int row = f / (3 * PX);
         const int k = f - row * (3 * PX);
         const int lx = k / 3, c = k - 3 * lx;
-        const int X = x0 - 1 + lx, Y = y0 - 1 + row;
+        const int Xs = x0 - 1 + lx, Ys = y0 - 1 + row;
         const bool slot = f < 3 * PN;
-        const bool in = slot && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny;
-        slds[j] = slot ? c * PN + row * PX + lx : -1;
-        soff[j] = in ? uint32_t(3 * (Y * g.nnx + X) + c) : 0u;
-        if (in) sval |= 1u << j;
-        if (in && lx >= 1 && lx <= NX && row >= 1 && row <= NY) sown |= 1u << j;
+        const bool in = slot && Xs >= 0 && Xs < g.nnx && Ys >= 0 && Ys < g.nny;
+        slds[j] = slot ? c * PN + row * PX + lx : 3 * PN + (tid & 63);
+        voff[j] = in ? uint32_t(4 * (3 * (Ys * g.nnx + Xs) + c)) : BUF_OOB;
+        vown[j] = (in && lx >= 1 && lx <= NX && row >= 1 && row <= NY) ? voff[j] : BUF_OOB;
     }
     const int64_t plane = int64_t(3) * g.nnx * g.nny;
+    const uint32_t planeB = uint32_t(4 * plane);
+    const int64_t layer = int64_t(g.resX) * g.resY;
+    const uint32_t eoff = colIn ? uint32_t(4 * (cy * g.resX + cx)) : BUF_OOB;
+    const uint32_t noff = ownNode ? uint32_t(Y * g.nnx + X) : BUF_OOB;
+    auto plane_rsrc = [&](const float* base, int Z) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        return buf_rsrc(base + (zin ? Z * plane : 0), zin ? planeB : 0u);
+    };
     float fu[SPT], fz[SPT];
     auto fetch = [&](int Z) {
-        const bool zin = Z >= 0 && Z < g.nnz;
-        const float* up = u + (zin ? Z : 0) * plane;
-        const float* zp = FUSED ? z + (zin ? Z : 0) * plane : z;
+        const __amdgpu_buffer_rsrc_t ru = plane_rsrc(u, Z);
 #pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const bool ok = zin && ((sval >> j) & 1u);
-            fu[j] = ok ? up[soff[j]] : 0.f;
-            if (FUSED) fz[j] = ok ? zp[soff[j]] : 0.f;
+        for (int j = 0; j < SPT; ++j) fu[j] = buf_ld(ru, voff[j]);
+        if (FUSED) {
+            const __amdgpu_buffer_rsrc_t rz = plane_rsrc(z, Z);
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) fz[j] = buf_ld(rz, voff[j]);
         }
     };
     auto commit = [&](int Z, int q) {
         const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
-        float* pp = FUSED ? pcur + (own ? Z : 0) * plane : pcur;
-        float* sq = &sP[q][0][0];
+        const __amdgpu_buffer_rsrc_t rp = buf_rsrc(FUSED && own ? pcur + Z * plane : pcur, FUSED && own ? planeB : 0u);
+        float* sq = &sP[q][0];
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
-            if (slds[j] < 0) continue;
             float v = fu[j];
             if (FUSED) {
                 v = fmaf(beta, v, fz[j]);
-                if (own && ((sown >> j) & 1u)) pp[soff[j]] = v;
+                buf_st(v, rp, vown[j]);
             }
             sq[slds[j]] = v;
         }
+    };
+    // voxel modulus of layer cz (0 outside the grid), unclamped
+    auto modulus = [&](int cz) {
+        const bool in = cz >= 0 && cz < g.resZ;
+        return buf_ld(buf_rsrc(Ecell + (in ? cz * layer : 0), in ? uint32_t(4 * layer) : 0u), eoff);
+    };
+    // fixed-DOF bits of this thread's node in plane Z
+    auto fixed_bits = [&](int Z) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        const int64_t pn = int64_t(g.nnx) * g.nny;
+        return unsigned(__builtin_amdgcn_raw_buffer_load_b8(
+            buf_rsrc(nodeFixed + (zin ? Z * pn : 0), zin ? uint32_t(pn) : 0u), noff, 0, 0));
     };
     // 2D Walsh transform of this voxel column's face in buffer q: F[c][my][mx]
     auto face = [&](int q, float F[3][2][2]) {
         const int b = ty * PX + tx;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            walsh2(sP[q][c][b], sP[q][c][b + 1], sP[q][c][b + PX], sP[q][c][b + PX + 1], F[c]);
+            walsh2(sP[q][c * PN + b], sP[q][c * PN + b + 1], sP[q][c * PN + b + PX], sP[q][c * PN + b + PX + 1],
+                   F[c]);
     };
     float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.f;
-    auto modulus = [&](int cz) {
-        float E = 0.f;
-        if (colIn && cz >= 0 && cz < g.resZ) {
-            E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
-            if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
-        }
-        return E;
-    };
     fetch(z0 - 1);
     commit(z0 - 1, 0);
     fetch(z0);
     commit(z0, 1);
     float Enext = modulus(z0 - 1);
+    unsigned fxNext = 0u;
     __syncthreads();
     face(0, Fp);
     double dot = 0.0;
@@ -252,10 +287,15 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     for (int s = 0; s < nsteps; ++s) {
         const int qn = (s + 1) % 3, qo = s % 3;
         const bool more = s + 1 < nsteps;
-        if (more) fetch(z0 + s + 1);   // plane of the next step, in flight during this one
-        const float E = Enext;         // voxel (cx, cy, cz), cz = z0 - 1 + s
         const int cz = z0 - 1 + s;
-        if (more) Enext = modulus(cz + 1);
+        // voxel (cx, cy, cz) and the fixed bits of node (X, Y, cz), loaded last step
+        const float E = Enext <= 1.0e-16f ? 0.f : Enext;   // fea.cpp:743-744
+        const unsigned fx = fxNext;
+        if (more) {   // loads of the next step, in flight during this one
+            fetch(z0 + s + 1);
+            Enext = modulus(cz + 1);
+            fxNext = fixed_bits(cz + 1);
+        }
         face(qn, Fn);
         float uh[24], vh[24];
 #pragma unroll
@@ -288,40 +328,34 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
 #pragma unroll
             for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
         if (s > 0) {   // plane z0 - 1 belongs to the block below
-        // plane Z = cz: corner forces of this voxel column, then the node gather
-        const int Z = cz;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float C4[2][2];
-            walsh2(A[c][0][0], A[c][0][1], A[c][1][0], A[c][1][1], C4);
-            // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
-            sC[0 * 3 + c][tid] = C4[0][0];
-            sC[1 * 3 + c][tid] = C4[0][1];
-            sC[2 * 3 + c][tid] = C4[1][0];
-            sC[3 * 3 + c][tid] = C4[1][1];
-        }
-        __syncthreads();
-        const int X = x0 + tx, Y = y0 + ty;
-        if (tx < NX && ty < NY && X < g.nnx && Y < g.nny) {
-            // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)
-            float yv[3];
+            // plane Z = cz: corner forces of this voxel column, then the node gather
+            const int Z = cz;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float c00 = sC[3 * 3 + c][ty * CX + tx];              // column (X-1, Y-1), corner (1,1)
-                const float c10 = sC[2 * 3 + c][ty * CX + tx + 1];          // column (X,   Y-1), corner (0,1)
-                const float c01 = sC[1 * 3 + c][(ty + 1) * CX + tx];        // column (X-1, Y),   corner (1,0)
-                const float c11 = sC[0 * 3 + c][(ty + 1) * CX + tx + 1];    // column (X,   Y),   corner (0,0)
-                yv[c] = (c00 + c10) + (c01 + c11);
+                float C4[2][2];
+                walsh2(A[c][0][0], A[c][0][1], A[c][1][0], A[c][1][1], C4);
+                // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
+                sC[0 * 3 + c][tid] = C4[0][0];
+                sC[1 * 3 + c][tid] = C4[0][1];
+                sC[2 * 3 + c][tid] = C4[1][0];
+                sC[3 * 3 + c][tid] = C4[1][1];
             }
-            const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-            const unsigned fx = nodeFixed[n];
+            __syncthreads();
+            if (ownNode) {
+                // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)
+                const __amdgpu_buffer_rsrc_t ry = plane_rsrc(y, Z);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float v = ((fx >> c) & 1u) ? 0.f : yv[c];
-                y[3 * n + c] = v;
-                if (FUSED) dot = fma(double(sP[qo][c][(ty + 1) * PX + tx + 1]), double(v), dot);
+                for (int c = 0; c < 3; ++c) {
+                    const float c00 = sC[3 * 3 + c][ty * CX + tx];              // column (X-1, Y-1), corner (1,1)
+                    const float c10 = sC[2 * 3 + c][ty * CX + tx + 1];          // column (X,   Y-1), corner (0,1)
+                    const float c01 = sC[1 * 3 + c][(ty + 1) * CX + tx];        // column (X-1, Y),   corner (1,0)
+                    const float c11 = sC[0 * 3 + c][(ty + 1) * CX + tx + 1];    // column (X,   Y),   corner (0,0)
+                    const float yv = (c00 + c10) + (c01 + c11);
+                    const float v = ((fx >> c) & 1u) ? 0.f : yv;
+                    buf_st(v, ry, 4 * (3 * noff + c));
+                    if (FUSED) dot = fma(double(sP[qo][c * PN + (ty + 1) * PX + tx + 1]), double(v), dot);
+                }
             }
-        }
         }
         if (more) commit(z0 + s + 1, (s + 2) % 3);
         __syncthreads();
