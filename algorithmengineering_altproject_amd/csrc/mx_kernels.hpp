@@ -722,13 +722,16 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
     __syncthreads();
     // Each thread owns the row positions k = k0 + tid + 256 u; their x-direction
     // interpolation (coarse column, weight) and component do not depend on the
-    // row, so they are computed once per chunk of 256 U positions, and a row's
-    // loads are all issued before its arithmetic.  Within a chunk the partial
-    // sums run over rows, then positions, as one pass over the block's rows.
+    // row, so they are computed once per chunk of 256 U positions.  A row's
+    // loads are buffer loads over exactly that row (positions k >= L load 0
+    // and their stores are dropped), so they carry no branch and are all
+    // issued before the row's arithmetic.  Within a chunk the partial sums run
+    // over rows, then positions, as one pass over the block's rows.
     constexpr int U = 8;
     double srz = 0.0, szap = 0.0;
     for (int k0 = 0; k0 < L; k0 += 256 * U) {
-        int qo[U], cc[U], xs[U];
+        int qo[U], cc[U];
+        uint32_t xs[U];
         float w1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -739,34 +742,33 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
             w1[u] = float(rx) * inv;
             qo[u] = 3 * cxx + c;
             cc[u] = c;
-            xs[u] = xx;
+            xs[u] = uint32_t(xx);
         }
         for (int rr = 0; rr < nr; ++rr) {
             const int64_t nbase = (row0 + rr) * Lf.nnx;
             const float* q = sQ + rr * Lq;
+            const uint32_t rowB = uint32_t(4 * L);
+            const __amdgpu_buffer_rsrc_t rR = buf_rsrc(rf + 3 * nbase, rowB), rD = buf_rsrc(Df + 3 * nbase, rowB),
+                                         rA = buf_rsrc(Ap + 3 * nbase, rowB), rX = buf_rsrc(xf + 3 * nbase, rowB),
+                                         rM = buf_rsrc(fmask + nbase, uint32_t(Lf.nnx));
             float rv[U], dv[U], av[U];
             unsigned mv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int k = k0 + int(threadIdx.x) + u * 256;
-                rv[u] = dv[u] = av[u] = 0.f;
-                mv[u] = 0xffu;
-                if (k < L) {
-                    const int64_t i = 3 * nbase + k;
-                    rv[u] = rf[i];
-                    dv[u] = Df[i];
-                    av[u] = Ap[i];
-                    mv[u] = fmask[nbase + xs[u]];
-                }
+                const uint32_t off = 4u * uint32_t(k0 + int(threadIdx.x) + u * 256);
+                rv[u] = buf_ld(rR, off);
+                dv[u] = buf_ld(rD, off);
+                av[u] = buf_ld(rA, off);
+                mv[u] = __builtin_amdgcn_raw_buffer_load_b8(rM, xs[u], 0, 0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int k = k0 + int(threadIdx.x) + u * 256;
-                if (k >= L) continue;
+                const uint32_t off = 4u * uint32_t(k0 + int(threadIdx.x) + u * 256);
                 const float v = (1.f - w1[u]) * q[qo[u]] + w1[u] * q[qo[u] + 3];
                 float out = 0.f;
                 if (!((mv[u] >> cc[u]) & 1u)) out = 2.f * (weight * rv[u] * dv[u]) + v;
-                xf[3 * nbase + k] = out;
+                buf_st(out, rX, off);
+                // beyond the row rv = av = 0: the partials gain exact zeros
                 srz = fma(double(rv[u]), double(out), srz);
                 szap = fma(double(out), double(av[u]), szap);
             }
