@@ -72,10 +72,12 @@ __device__ __forceinline__ void coarse_mv_body(int N, const double* __restrict__
         const int k0 = UPPER ? i : 0;
         const int k1 = UPPER ? N : i + 1;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; k0 + u * 64 < k1; u += 4) {
 #pragma unroll
-        for (int u = 0; u < CHOL_MAX / 64; ++u) {
-            const int k = k0 + u * 64 + lane;
-            if (k < k1) acc[u & 3] = fma(Mi[k], v[k], acc[u & 3]);
+            for (int w = 0; w < 4; ++w) {
+                const int k = k0 + (u + w) * 64 + lane;
+                if (k < k1) acc[w] = fma(Mi[k], v[k], acc[w]);
+            }
         }
         double a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 #pragma unroll

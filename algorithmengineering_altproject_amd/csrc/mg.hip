@@ -23,6 +23,7 @@
 #include "mg_host.hpp"
 #include "t3d_internal.hpp"
 #include "chol.hpp"
+#include "coarse.hpp"
 
 namespace t3d {
 
@@ -145,10 +146,11 @@ __device__ __forceinline__ double hat_w(int i, int span, double inv)
     return 1.0 - double(d < 0 ? -d : d) * inv;
 }
 
+// norm = 0 (Galerkin form): the plain transpose of the prolongation, sum w r
 __global__ void __launch_bounds__(256) k_restrict(LevelDev Lf, LevelDev Lc, int span,
                                                   const double* __restrict__ rf, const uint8_t* __restrict__ cmask,
                                                   double* __restrict__ rc, const double* __restrict__ Dc,
-                                                  double* __restrict__ xc, const int* __restrict__ done)
+                                                  double* __restrict__ xc, const int* __restrict__ done, int norm)
 {
     if (done && *done) return;
     const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -181,7 +183,9 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDev Lf, LevelDev Lc, int 
         }
     }
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
-    if (wsum > 1e-12) {
+    if (!norm) {
+        r0 = v0; r1 = v1; r2 = v2;
+    } else if (wsum > 1e-12) {
         const double invW = 1.0 / wsum;
         r0 = v0 * invW; r1 = v1 * invW; r2 = v2 * invW;
     }
@@ -270,6 +274,206 @@ __global__ void __launch_bounds__(CHOL_MAX) k_chol_solve(int N, const double* __
     chol_solve_body(N, L, LT, b, mask, x);
 }
 
+// ---------------------------------------------------------------------------
+// Galerkin hierarchy (TOP3D_XL.m:945-1090, Solving_AssembleFEAstencil).  The
+// level-(l+1) operator is P^T A_l P with P the trilinear prolongation of the
+// hierarchy; as P is local to a coarse cell, it is formed cell by cell:
+//   K_c = sum over the cells f covered by coarse cell c of T_f^T K_f T_f,
+// T_f (24 x 24) interpolating c's corner DOFs to f's corner DOFs.  The finest
+// operator is A_0 = M K M + (I - M): a cell with fixed DOFs drops their rows
+// and columns and carries 1/nAdj on their diagonal, nAdj the solid cells at
+// the node (Solving_ApplyBConEleStiffMat_B, TOP3D_XL.m:1450-1463).  Coarse
+// DOFs with a zero diagonal (covering only void) are masked.
+// Element matrices are row-major 24 x 24 in local DOF order 3a + c.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void corner_offsets(int a, int& dx, int& dy, int& dz)
+{
+    const int q = a & 3;
+    dx = (q == 1 || q == 2) ? 1 : 0;
+    dy = (q <= 1) ? 1 : 0;
+    dz = a >> 2;
+}
+
+// solid cells among the (up to) eight around node (ix, iy, iz) of the finest grid
+__device__ __forceinline__ int solid_cells_at(const Grid& g, const int32_t* __restrict__ cellToCompact, int ix, int iy,
+                                              int iz)
+{
+    int n = 0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = ix - dx, cy = iy - dy, cz = iz - dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= g.resX || cy >= g.resY || cz >= g.resZ) continue;
+                if (cellToCompact[(int64_t(cz) * g.resY + cy) * g.resX + cx] >= 0) ++n;
+            }
+    return n;
+}
+
+// acc[q] += (T_f^T A T_f)[t + 256 q] for the matrix A staged in sA (row-major
+// 24 x 24); W: child weights W[a * 8 + A]; sH: LDS scratch
+__device__ __forceinline__ void gal_accumulate(const double* sA, double* sH, const double* __restrict__ W, double acc[3])
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = t + 256 * q;
+        if (e >= 576) break;
+        const int ii = e / 24, J = e - 24 * (e / 24), AJ = J / 3, cJ = J - 3 * AJ;
+        double h = 0.0;
+        for (int b = 0; b < 8; ++b) h += sA[ii * 24 + 3 * b + cJ] * W[b * 8 + AJ];
+        sH[e] = h;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = t + 256 * q;
+        if (e >= 576) break;
+        const int I = e / 24, J = e - 24 * (e / 24), AI = I / 3, cI = I - 3 * AI;
+        double v = 0.0;
+        for (int a = 0; a < 8; ++a) v += W[a * 8 + AI] * sH[(3 * a + cI) * 24 + J];
+        acc[q] += v;
+    }
+    __syncthreads();
+}
+
+// level-1 element matrices from the finest cells; one workgroup per coarse
+// cell.  Cells without fixed DOFs add E * G_f (G_f = T_f^T Ke T_f, precomputed).
+__global__ void __launch_bounds__(256) k_gal_fine(Grid g, LevelDev Lc, int s, const double* __restrict__ G,
+                                                  const double* __restrict__ Wch, const double* __restrict__ Ke,
+                                                  const double* __restrict__ Ecell,
+                                                  const uint8_t* __restrict__ nodeFixed,
+                                                  const int32_t* __restrict__ cellToCompact, double* __restrict__ Kel)
+{
+    const int64_t ce = blockIdx.x;
+    const int cx = int(ce % Lc.resX), cy = int((ce / Lc.resX) % Lc.resY), cz = int(ce / (int64_t(Lc.resX) * Lc.resY));
+    __shared__ double sA[576], sH[576];
+    const int t = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < s; ++k)
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i < s; ++i) {
+                const int f = (k * s + j) * s + i;
+                const int fx = cx * s + i, fy = cy * s + j, fz = cz * s + k;
+                const int64_t fc = (int64_t(fz) * g.resY + fy) * g.resX + fx;
+                if (cellToCompact[fc] < 0) continue;   // void cell: no element (uniform in the block)
+                double E = Ecell[fc];
+                if (E <= 1.0e-16) E = 0.0;             // fea.cpp:743-744
+                unsigned fixedAny = 0;
+                for (int a = 0; a < 8; ++a) {
+                    int dx, dy, dz;
+                    corner_offsets(a, dx, dy, dz);
+                    fixedAny |= nodeFixed[(int64_t(fz + dz) * g.nny + (fy + dy)) * g.nnx + (fx + dx)];
+                }
+                if (!fixedAny) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int e = t + 256 * q;
+                        if (e < 576) acc[q] += E * G[size_t(f) * 576 + e];
+                    }
+                    continue;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int e = t + 256 * q;
+                    if (e >= 576) break;
+                    const int ii = e / 24, jj = e - 24 * (e / 24);
+                    int dx, dy, dz;
+                    corner_offsets(ii / 3, dx, dy, dz);
+                    const int nx = fx + dx, ny = fy + dy, nz = fz + dz;
+                    const bool fi = (nodeFixed[(int64_t(nz) * g.nny + ny) * g.nnx + nx] >> (ii % 3)) & 1u;
+                    int ex, ey, ez;
+                    corner_offsets(jj / 3, ex, ey, ez);
+                    const bool fj =
+                        (nodeFixed[(int64_t(fz + ez) * g.nny + (fy + ey)) * g.nnx + (fx + ex)] >> (jj % 3)) & 1u;
+                    double v = (fi || fj) ? 0.0 : E * Ke[e];
+                    if (ii == jj && fi) v = 1.0 / double(solid_cells_at(g, cellToCompact, nx, ny, nz));
+                    sA[e] = v;
+                }
+                __syncthreads();
+                gal_accumulate(sA, sH, Wch + size_t(f) * 64, acc);
+            }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = t + 256 * q;
+        if (e < 576) Kel[size_t(ce) * 576 + e] = acc[q];
+    }
+}
+
+// level-(l+1) element matrices from the level-l ones (l >= 1)
+__global__ void __launch_bounds__(256) k_gal_coarse(LevelDev Lf, LevelDev Lc, int s, const double* __restrict__ Wch,
+                                                    const double* __restrict__ Kf, double* __restrict__ Kc)
+{
+    const int64_t ce = blockIdx.x;
+    const int cx = int(ce % Lc.resX), cy = int((ce / Lc.resX) % Lc.resY), cz = int(ce / (int64_t(Lc.resX) * Lc.resY));
+    __shared__ double sA[576], sH[576];
+    const int t = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < s; ++k)
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i < s; ++i) {
+                const int f = (k * s + j) * s + i;
+                const int64_t fc = (int64_t(cz * s + k) * Lf.resY + (cy * s + j)) * Lf.resX + (cx * s + i);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int e = t + 256 * q;
+                    if (e < 576) sA[e] = Kf[size_t(fc) * 576 + e];
+                }
+                __syncthreads();
+                gal_accumulate(sA, sH, Wch + size_t(f) * 64, acc);
+            }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int e = t + 256 * q;
+        if (e < 576) Kc[size_t(ce) * 576 + e] = acc[q];
+    }
+}
+
+// diagonal of a Galerkin level from its element matrices (cells in a fixed
+// order); DOFs with a zero diagonal are masked and get D = 1
+__global__ void __launch_bounds__(256) k_gal_diag(LevelDev L, const double* __restrict__ Kel, double* __restrict__ D,
+                                                  uint8_t* __restrict__ mask)
+{
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= L.nN) return;
+    const int ix = int(n % L.nnx), iy = int((n / L.nnx) % L.nny), iz = int(n / (int64_t(L.nnx) * L.nny));
+    double d[3] = {0.0, 0.0, 0.0};
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = ix - dx, cy = iy - dy, cz = iz - dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= L.resX || cy >= L.resY || cz >= L.resZ) continue;
+                const int a = corner_index(dx, dy, dz);
+                const double* K = Kel + size_t((int64_t(cz) * L.resY + cy) * L.resX + cx) * 576;
+                for (int c = 0; c < 3; ++c) d[c] += K[(3 * a + c) * 25];
+            }
+    unsigned bits = 0;
+    for (int c = 0; c < 3; ++c) {
+        if (!(d[c] > 0.0)) {
+            bits |= 1u << c;
+            d[c] = 1.0;
+        }
+        D[3 * n + c] = d[c];
+    }
+    mask[n] = uint8_t(bits);
+}
+
+// coarsest solve of the Galerkin form through the inverse factor (coarse.hpp)
+__global__ void __launch_bounds__(64) k_mg_tri_inv(int N, const double* __restrict__ L, double* Li, double* LiT)
+{
+    tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
+}
+
+template <bool UPPER>
+__global__ void __launch_bounds__(64 * CO_WAVES) k_mg_coarse_mv(int N, const double* __restrict__ M,
+                                                                const double* __restrict__ v,
+                                                                const uint8_t* __restrict__ mask,
+                                                                double* __restrict__ out, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    coarse_mv_body<UPPER>(N, M, v, mask, out, nullptr, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
+                          int(gridDim.x) * CO_WAVES);
+}
+
 }  // namespace t3d
 
 using namespace t3d;
@@ -290,6 +494,14 @@ struct t3d_mg {
     int cholOk = 1;
     bool setupDone = false;
     std::vector<double> Kc;   // last assembled coarsest matrix (host, for inspection)
+    // Galerkin form (mode 1)
+    int mode = 0;             // 0: the reference's V-cycle; 1: Galerkin hierarchy
+    double* Kel[8] = {};      // element matrices of levels >= 1, [cells][576]
+    double* Wch[8] = {};      // child interpolation weights of levels >= 1, [span^3][64]
+    double* Gch = nullptr;    // T_f^T Ke T_f per finest cell of a level-1 cell, [span^3][576]
+    double* Li = nullptr;     // inverse coarsest factor and its transpose (FP64 V-cycle)
+    double* LiT = nullptr;
+    double* yc = nullptr;
 };
 
 namespace t3d {
@@ -312,10 +524,16 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
         const bool last = (l + 1 == L);
         hipLaunchKernelGGL(k_restrict, dim3(unsigned((g->ld[l + 1].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
                            g->ld[l + 1], g->H.lv[l + 1].span, rf, g->mask[l + 1], g->r[l + 1],
-                           (last && !chol) ? g->D[L] : nullptr, g->x[L], done);
+                           (last && !chol) ? g->D[L] : nullptr, g->x[L], done, g->mode == 0 ? 1 : 0);
         T3D_HIP(hipGetLastError());
     }
-    if (chol) {
+    if (g->mode == 1) {   // x = Li^T (Li r), masked
+        hipLaunchKernelGGL(k_mg_coarse_mv<false>, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
+                           (const double*)g->Li, (const double*)g->r[L], (const uint8_t*)nullptr, g->yc, done);
+        hipLaunchKernelGGL(k_mg_coarse_mv<true>, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
+                           (const double*)g->LiT, (const double*)g->yc, (const uint8_t*)g->mask[L], g->x[L], done);
+        T3D_HIP(hipGetLastError());
+    } else if (chol) {
         hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(CHOL_MAX), 0, st, g->Nc, g->L, g->LT, g->r[L], g->mask[L],
                            g->x[L], done);
         T3D_HIP(hipGetLastError());
@@ -345,6 +563,7 @@ struct MGView {
     const double* LT;
     int Nc;
     double weight;
+    int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
 };
 int mg_view(const t3d_mg* g, MGView& v)
 {
@@ -361,8 +580,160 @@ int mg_view(const t3d_mg* g, MGView& v)
     v.LT = g->LT;
     v.Nc = g->Nc;
     v.weight = g->H.weight;
+    v.norm = g->mode == 0 ? 1 : 0;
     return T3D_OK;
 }
+}  // namespace t3d
+
+namespace t3d {
+
+// child interpolation weights of a span-s coarse cell:
+// W[f * 64 + a * 8 + A] = trilinear weight of coarse corner A at corner a of
+// the covered cell f = (k s + j) s + i (TOP3D_XL.m:1881-1980 tabulates the same
+// weights per embedded node)
+static std::vector<double> child_weights(int s)
+{
+    auto off = [](int a, int& dx, int& dy, int& dz) {
+        const int q = a & 3;
+        dx = (q == 1 || q == 2) ? 1 : 0;
+        dy = (q <= 1) ? 1 : 0;
+        dz = a >> 2;
+    };
+    std::vector<double> W(size_t(s) * s * s * 64);
+    for (int k = 0; k < s; ++k)
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i < s; ++i) {
+                const int f = (k * s + j) * s + i;
+                for (int a = 0; a < 8; ++a) {
+                    int dx, dy, dz;
+                    off(a, dx, dy, dz);
+                    const double px = double(i + dx) / s, py = double(j + dy) / s, pz = double(k + dz) / s;
+                    for (int A = 0; A < 8; ++A) {
+                        int ex, ey, ez;
+                        off(A, ex, ey, ez);
+                        W[size_t(f) * 64 + a * 8 + A] = (ex ? px : 1.0 - px) * (ey ? py : 1.0 - py) * (ez ? pz : 1.0 - pz);
+                    }
+                }
+            }
+    return W;
+}
+
+// G_f = T_f^T Ke T_f for each covered cell f
+static std::vector<double> child_products(const std::vector<double>& W, const double* Ke, int nch)
+{
+    std::vector<double> G(size_t(nch) * 576, 0.0);
+    for (int f = 0; f < nch; ++f) {
+        const double* w = &W[size_t(f) * 64];
+        std::vector<double> H(576, 0.0);   // H = Ke T_f
+        for (int ii = 0; ii < 24; ++ii)
+            for (int J = 0; J < 24; ++J) {
+                double h = 0.0;
+                for (int b = 0; b < 8; ++b) h += Ke[ii * 24 + 3 * b + J % 3] * w[b * 8 + J / 3];
+                H[ii * 24 + J] = h;
+            }
+        for (int I = 0; I < 24; ++I)
+            for (int J = 0; J < 24; ++J) {
+                double v = 0.0;
+                for (int a = 0; a < 8; ++a) v += w[a * 8 + I / 3] * H[(3 * a + I % 3) * 24 + J];
+                G[size_t(f) * 576 + I * 24 + J] = v;
+            }
+    }
+    return G;
+}
+
+// Galerkin set-up: element matrices of every coarse level, their diagonals and
+// masks, the coarsest matrix (assembled here in cell order), its Cholesky
+// factor and inverse factor
+static int mg_galerkin_setup(t3d_mg* g)
+{
+    t3d_problem* p = g->p;
+    hipStream_t st = p->stream;
+    const int B = 256;
+    const int L = g->nlev - 1;
+    T3D_CHECK(L >= 1, T3D_ERR_STATE, "Galerkin multigrid needs at least two levels");
+    for (int l = 1; l <= L; ++l) {
+        const int sp = g->H.lv[l].span;
+        const size_t nch = size_t(sp) * sp * sp;
+        const int64_t ncell = int64_t(g->H.lv[l].resX) * g->H.lv[l].resY * g->H.lv[l].resZ;
+        if (!g->Kel[l]) {
+            const std::vector<double> W = child_weights(sp);
+            T3D_HIP(hipMalloc(&g->Kel[l], sizeof(double) * 576 * size_t(ncell)));
+            T3D_HIP(hipMalloc(&g->Wch[l], sizeof(double) * W.size()));
+            T3D_HIP(hipMemcpy(g->Wch[l], W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice));
+            if (l == 1) {
+                const std::vector<double> G = child_products(W, p->host.Ke.data(), int(nch));
+                T3D_HIP(hipMalloc(&g->Gch, sizeof(double) * G.size()));
+                T3D_HIP(hipMemcpy(g->Gch, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice));
+            }
+        }
+        if (l == 1)
+            hipLaunchKernelGGL(k_gal_fine, dim3(unsigned(ncell)), dim3(256), 0, st, p->g, g->ld[1], sp,
+                               (const double*)g->Gch, (const double*)g->Wch[1], (const double*)p->Ke,
+                               (const double*)p->Ecell, (const uint8_t*)p->nodeFixed,
+                               (const int32_t*)p->cellToCompact, g->Kel[1]);
+        else
+            hipLaunchKernelGGL(k_gal_coarse, dim3(unsigned(ncell)), dim3(256), 0, st, g->ld[l - 1], g->ld[l], sp,
+                               (const double*)g->Wch[l], (const double*)g->Kel[l - 1], g->Kel[l]);
+        T3D_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gal_diag, dim3(unsigned((g->ld[l].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
+                           (const double*)g->Kel[l], g->D[l], g->mask[l]);
+        T3D_HIP(hipGetLastError());
+    }
+    // coarsest matrix, assembled on the host from the coarsest element matrices
+    const MGLevelDims& c = g->H.lv[L];
+    const int64_t ncell = int64_t(c.resX) * c.resY * c.resZ;
+    const int nnx = c.resX + 1, nny = c.resY + 1;
+    const int N = int(3 * c.nN);
+    std::vector<double> Ke(576 * size_t(ncell));
+    std::vector<uint8_t> mk(c.nN);
+    T3D_HIP(hipMemcpyAsync(Ke.data(), g->Kel[L], sizeof(double) * Ke.size(), hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipMemcpyAsync(mk.data(), g->mask[L], c.nN, hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipStreamSynchronize(st));
+    std::vector<double> A(size_t(N) * N, 0.0);
+    for (int64_t ce = 0; ce < ncell; ++ce) {
+        const int cx = int(ce % c.resX), cy = int((ce / c.resX) % c.resY), cz = int(ce / (int64_t(c.resX) * c.resY));
+        int node[8];
+        for (int a = 0; a < 8; ++a) {
+            const int q = a & 3;
+            const int dx = (q == 1 || q == 2) ? 1 : 0, dy = (q <= 1) ? 1 : 0, dz = a >> 2;
+            node[a] = ((cz + dz) * nny + (cy + dy)) * nnx + (cx + dx);
+        }
+        const double* K = &Ke[size_t(ce) * 576];
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j)
+                A[size_t(3 * node[i / 3] + i % 3) * N + 3 * node[j / 3] + j % 3] += K[i * 24 + j];
+    }
+    for (int i = 0; i < N; ++i)
+        if ((mk[i / 3] >> (i % 3)) & 1u) A[size_t(i) * N + i] = 1.0;   // a DOF covering only void
+    g->Kc = A;
+    if (!chol_factor(A, N)) {
+        g->cholOk = 0;
+        set_error("Galerkin multigrid: the coarsest matrix is not positive definite");
+        return T3D_ERR_STATE;
+    }
+    const size_t n2 = size_t(N) * N;
+    std::vector<double> AT(n2);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) AT[size_t(i) * N + j] = A[size_t(j) * N + i];
+    if (!g->L) {
+        T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->LT, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->Li, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->LiT, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->yc, sizeof(double) * size_t(N)));
+    }
+    T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
+    T3D_HIP(hipMemcpyAsync(g->LT, AT.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
+    T3D_HIP(hipMemsetAsync(g->Li, 0, sizeof(double) * n2, st));
+    T3D_HIP(hipMemsetAsync(g->LiT, 0, sizeof(double) * n2, st));
+    hipLaunchKernelGGL(k_mg_tri_inv, dim3(unsigned((N + 63) / 64)), dim3(64), 0, st, N, (const double*)g->L, g->Li,
+                       g->LiT);
+    T3D_HIP(hipGetLastError());
+    T3D_HIP(hipStreamSynchronize(st));   // A and AT are host buffers of this scope
+    g->Nc = N;
+    return T3D_OK;
+}
+
 }  // namespace t3d
 
 extern "C" {
@@ -377,6 +748,14 @@ int t3d_mg_destroy(t3d_mg* g)
         if (g->x[l]) (void)hipFree(g->x[l]);
         if (g->W[l]) (void)hipFree(g->W[l]);
     }
+    for (int l = 0; l < 8; ++l) {
+        if (g->Kel[l]) (void)hipFree(g->Kel[l]);
+        if (g->Wch[l]) (void)hipFree(g->Wch[l]);
+    }
+    if (g->Gch) (void)hipFree(g->Gch);
+    if (g->Li) (void)hipFree(g->Li);
+    if (g->LiT) (void)hipFree(g->LiT);
+    if (g->yc) (void)hipFree(g->yc);
     if (g->L) (void)hipFree(g->L);
     if (g->LT) (void)hipFree(g->LT);
     delete g;
@@ -441,7 +820,40 @@ int t3d_mg_info(const t3d_mg* g, int* nlev, int* dims, int* Nc, int* cholOk)
 int t3d_mg_mask(const t3d_mg* g, int level, uint8_t* out)
 {
     T3D_CHECK(g && out && level >= 0 && level < g->nlev, T3D_ERR_ARG, "t3d_mg_mask: bad argument");
+    if (g->mode == 1 && level > 0) {   // set up with the element matrices
+        T3D_CHECK(g->setupDone, T3D_ERR_STATE, "t3d_mg_mask: t3d_mg_setup not called");
+        const int64_t nN = g->H.lv[level].nN;
+        std::vector<uint8_t> bits(nN);
+        T3D_HIP(hipDeviceSynchronize());
+        T3D_HIP(hipMemcpy(bits.data(), g->mask[level], nN, hipMemcpyDeviceToHost));
+        for (int64_t n = 0; n < nN; ++n)
+            for (int c = 0; c < 3; ++c) out[3 * n + c] = (bits[n] >> c) & 1u;
+        return T3D_OK;
+    }
     std::copy(g->H.masks[level].begin(), g->H.masks[level].end(), out);
+    return T3D_OK;
+}
+
+// multigrid form (before t3d_mg_setup): 0 the reference's V-cycle, 1 the
+// Galerkin hierarchy of TOP3D_XL.m ('Adapted' V-cycle, TOP3D_XL.m:772-807)
+int t3d_mg_set_mode(t3d_mg* g, int mode)
+{
+    T3D_CHECK(g && (mode == 0 || mode == 1), T3D_ERR_ARG, "t3d_mg_set_mode: mode must be 0 or 1");
+    T3D_CHECK(mode == 0 || g->nlev > 1, T3D_ERR_ARG, "t3d_mg_set_mode: the Galerkin form needs two levels");
+    if (mode == g->mode) return T3D_OK;
+    if (mode == 0) {   // restore the reference's restricted masks
+        for (int l = 1; l < g->nlev; ++l) {
+            const int64_t nN = g->H.lv[l].nN;
+            std::vector<uint8_t> bits(nN);
+            for (int64_t n = 0; n < nN; ++n)
+                bits[n] = uint8_t(g->H.masks[l][3 * n] | (g->H.masks[l][3 * n + 1] << 1) |
+                                  (g->H.masks[l][3 * n + 2] << 2));
+            T3D_HIP(hipMemcpy(g->mask[l], bits.data(), nN, hipMemcpyHostToDevice));
+        }
+        T3D_HIP(hipDeviceSynchronize());
+    }
+    g->mode = mode;
+    g->setupDone = false;
     return T3D_OK;
 }
 
@@ -476,6 +888,15 @@ int t3d_mg_setup(t3d_mg* g, const double* E_dev)
     hipLaunchKernelGGL(k_diag_finest, dim3(unsigned((p->g.nN + B - 1) / B)), dim3(B), 0, st, p->g, p->Ke, p->Ecell,
                        g->mask[0], g->D[0]);
     T3D_HIP(hipGetLastError());
+    if (g->mode == 1) {
+        g->setupDone = false;
+        g->Nc = 0;
+        g->cholOk = 1;
+        rc = mg_galerkin_setup(g);
+        if (rc) return rc;
+        g->setupDone = true;
+        return T3D_OK;
+    }
     for (int l = 1; l < g->nlev; ++l) {
         hipLaunchKernelGGL(k_diag_coarsen, dim3(unsigned((g->ld[l].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l - 1],
                            g->ld[l], g->H.lv[l].span, g->W[l], g->D[l - 1], g->mask[l - 1], g->mask[l], g->D[l]);

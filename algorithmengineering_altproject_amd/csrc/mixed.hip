@@ -440,9 +440,11 @@ __global__ void __launch_bounds__(64) k_restrict_x(LevelF Lf, int cnx, int span,
 __global__ void __launch_bounds__(64) k_restrict_yz(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
                                                     const float* __restrict__ Dc, float* __restrict__ xc,
-                                                    double* __restrict__ rcd, const int* __restrict__ done)
+                                                    double* __restrict__ rcd, const int* __restrict__ done,
+                                                    int norm)
 {
-    restrict_yz_body(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+    restrict_yz_body(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(),
+                     gridDim.x, norm);
 }
 
 __global__ void __launch_bounds__(64) k_prolong_smooth32(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
@@ -617,7 +619,7 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
         }
         const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
         hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
-                           s->r32[l + 1], Dc, s->x32[L], rcd, done);
+                           s->r32[l + 1], Dc, s->x32[L], rcd, done, v.norm);
         T3D_HIP(hipGetLastError());
     }
     if (chol) {

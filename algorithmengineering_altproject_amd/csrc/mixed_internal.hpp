@@ -47,6 +47,7 @@ struct MGView {
     const double* LT;
     int Nc;
     double weight;
+    int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
 };
 int mg_view(const t3d_mg* g, MGView& v);
 __global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,

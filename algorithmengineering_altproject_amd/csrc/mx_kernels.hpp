@@ -459,7 +459,7 @@ __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span,
                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
                                                     const float* __restrict__ Dc, float* __restrict__ xc,
                                                     double* __restrict__ rcd, const int* __restrict__ done,
-    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_)
+    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_, int norm = 1)
 {
     if (done && *done) return;
     const int cx = bx_ * 64 + threadIdx.x;
@@ -490,7 +490,11 @@ __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span,
     }
     const float ws = wxs * wyz;
     float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    if (ws > 1e-12f) {
+    if (!norm) {   // Galerkin form: the plain transpose of the prolongation
+        r0 = v0;
+        r1 = v1;
+        r2 = v2;
+    } else if (ws > 1e-12f) {
         const float iw = 1.0f / ws;
         r0 = v0 * iw;
         r1 = v1 * iw;

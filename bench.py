@@ -43,7 +43,13 @@ def parse():
     ap.add_argument("--nely", type=int, default=192)
     ap.add_argument("--nelx", type=int, default=384)
     ap.add_argument("--nelz", type=int, default=192)
-    ap.add_argument("--tol", type=float, default=float(np.float32(1e-3)))
+    ap.add_argument("--mg", choices=["galerkin", "reference"], default="galerkin",
+                    help="multigrid of the timed mixed solves: the Galerkin hierarchy of the MATLAB original "
+                         "(default) or the C++ port's additive V-cycle")
+    ap.add_argument("--tol", type=float, default=None,
+                    help="tolerance of the timed solves (default 1e-6 with --mg galerkin, else --ref-tol)")
+    ap.add_argument("--ref-tol", type=float, default=float(np.float32(1e-3)),
+                    help="the reference's cgTol, used for the solves with the C++ port's V-cycle")
     ap.add_argument("--tight-tol", type=float, default=1e-5)
     ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--true-every", type=int, default=50, help="residual replacement period of the mixed solver")
@@ -174,6 +180,7 @@ def batch_design(t3d, p, v0, spheres):
 
 
 def main_batch(args, t3d):
+    btol = args.tol if args.tol is not None else args.ref_tol
     rng = np.random.default_rng(args.seed)
     probs = batch_workload(rng, args.batch)
     dims = [d for d, _, _ in probs]
@@ -199,14 +206,14 @@ def main_batch(args, t3d):
     for _ in range(max(1, args.warmup)):          # the first solve also autotunes the tiles
         for x in xs:
             x.zero()
-        B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
+        B.solve(dbs, xs, btol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
     tiles = B.info()
     t3d.lib().t3d_synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for x in xs:
             x.zero()
-        its = B.solve(dbs, xs, args.tol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
+        its = B.solve(dbs, xs, btol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
     t3d.lib().t3d_synchronize()
     ms_batch = (time.perf_counter() - t0) / args.steps * 1e3
     res = [true_residual(t3d, p, db, x) for p, db, x in zip(B.problems, dbs, xs)]
@@ -224,13 +231,13 @@ def main_batch(args, t3d):
             Ms.append((M, mx))
         for (M, mx), db, x in zip(Ms, dbs, xs):   # warm-up
             x.zero()
-            mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every)
+            mx.pcg(db, x, btol, args.maxit, trueEvery=args.true_every)
         t3d.lib().t3d_synchronize()
         t0 = time.perf_counter()
         its_seq = []
         for (M, mx), db, x in zip(Ms, dbs, xs):
             x.zero()
-            its_seq.append(mx.pcg(db, x, args.tol, args.maxit, trueEvery=args.true_every))
+            its_seq.append(mx.pcg(db, x, btol, args.maxit, trueEvery=args.true_every))
         t3d.lib().t3d_synchronize()
         ms_seq = (time.perf_counter() - t0) * 1e3
         log("sequential: %.1f ms for all solves (%.1f solves/s), speed-up %.2fx" % (ms_seq, len(probs) * 1e3 / ms_seq,
@@ -254,7 +261,7 @@ def main_batch(args, t3d):
         "config": {
             "workload": "batched mixed-precision MG-PCG: %d uneven problems solved together to true residual < %.0e "
                         "(one launch per kernel for the batch, hipGraph-replayed iterations, autotuned tiles); a "
-                        "step solves all %d" % (len(probs), args.tol, len(probs)),
+                        "step solves all %d" % (len(probs), btol, len(probs)),
             "problems": len(probs),
             "elements": ne,
             "free_dofs": nF,
@@ -272,6 +279,30 @@ def main_batch(args, t3d):
     print(json.dumps(line), flush=True)
 
 
+def timed_solves(t3d, mx, db, dsol, tol, maxit, true_every, warmup, steps):
+    """warmup + steps mixed solves from x0 = 0; (iterations, ms per solve)"""
+    it = 0
+    for _ in range(warmup):
+        dsol.zero()
+        it = mx.pcg(db, dsol, tol, maxit, trueEvery=true_every)
+    t3d.lib().t3d_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dsol.zero()
+        it = mx.pcg(db, dsol, tol, maxit, trueEvery=true_every)
+    t3d.lib().t3d_synchronize()
+    return it, (time.perf_counter() - t0) / steps * 1e3
+
+
+def fp64_solve(t3d, p, M, db, dsol, tol, maxit):
+    dsol.zero()
+    t3d.lib().t3d_synchronize()
+    t0 = time.perf_counter()
+    it = t3d.pcg(p, M, db, dsol, tol, maxit)
+    t3d.lib().t3d_synchronize()
+    return it, (time.perf_counter() - t0) * 1e3
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -283,6 +314,8 @@ def main():
     log("device %s, %d CUs" % (name, cus))
     if args.batch:
         return main_batch(args, t3d)
+    galerkin = args.mg == "galerkin"
+    tol = args.tol if args.tol is not None else (1e-6 if galerkin else args.ref_tol)
     t0 = time.time()
     p = t3d.Problem(args.nely, args.nelx, args.nelz)
     log("problem %dx%dx%d: %d elements, %d free DOFs (setup %.1fs)" % (args.nely, args.nelx, args.nelz, p.ne, p.nF,
@@ -307,13 +340,24 @@ def main():
         return
 
     t0 = time.time()
-    M = t3d.MG(p, Emin=float(np.float32(1e-9)))
+    M = t3d.MG(p, Emin=float(np.float32(1e-9)))   # the C++ port's V-cycle (bit-exact FP64 path)
     M.setup(dE)
     mx = t3d.Mixed(p, M)
     mx.setup()
     form = {"walsh": 0, "mfma3": 2, "mfma6": 3}[args.operator]
     mx.set_operator(form)
     log("MG: levels %s, coarsest Cholesky N=%d (setup %.2fs)" % (M.dims, M.info()[0], time.time() - t0))
+    Mg = mxg = None
+    gal_setup_s = None
+    if galerkin:
+        t0 = time.time()
+        Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
+        Mg.setup(dE)
+        mxg = t3d.Mixed(p, Mg)
+        mxg.setup()
+        mxg.set_operator(form)
+        gal_setup_s = time.time() - t0
+        log("Galerkin MG: coarsest Cholesky N=%d (set-up %.2fs)" % (Mg.info()[0], gal_setup_s))
 
     # ---- K*u blocks: HIP events around back-to-back launches on the problem stream
     uh = np.random.default_rng(0).uniform(-1, 1, p.nF)
@@ -355,49 +399,64 @@ def main():
     p.load_vector(db)
     dsol = t3d.DeviceArray(p.nDOF)
     dce = t3d.DeviceArray(p.ne)
-    # ---- mixed-precision solves (the timed steps)
-    iters = []
-    for _ in range(args.warmup):
-        dsol.zero()
-        iters.append(mx.pcg(db, dsol, args.tol, args.maxit, trueEvery=args.true_every))
-    t3d.lib().t3d_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dsol.zero()
-        iters.append(mx.pcg(db, dsol, args.tol, args.maxit, trueEvery=args.true_every))
-    t3d.lib().t3d_synchronize()
-    dt = time.perf_counter() - t0
-    it_m = iters[-1]
-    ms_step = dt / args.steps * 1e3
-    Cm = t3d.compliance(p, dE, dsol, dce)
+    comp = lambda: t3d.compliance(p, dE, dsol, dce)
+    # ---- the timed steps: mixed-precision solves with the selected multigrid
+    head_mx = mxg if galerkin else mx
+    it_m, ms_step = timed_solves(t3d, head_mx, db, dsol, tol, args.maxit, args.true_every, args.warmup, args.steps)
+    Cm = comp()
     res_m = true_residual(t3d, p, db, dsol)
-    log("mixed PCG: %d iterations/solve%s, %.1f ms/solve, %.3f ms/iteration, compliance %.9e, true residual %.3e"
-        % (it_m, " (capped)" if it_m >= args.maxit else "", ms_step, ms_step / it_m, Cm, res_m))
+    log("mixed PCG (%s MG): %d iterations/solve%s, %.1f ms/solve, %.3f ms/iteration, compliance %.9e, "
+        "true residual %.3e" % (args.mg, it_m, " (capped)" if it_m >= args.maxit else "", ms_step, ms_step / it_m,
+                                Cm, res_m))
+    # converged compliance of this load: FP64 PCG with the Galerkin V-cycle to 1e-10
+    C_star = None
+    if galerkin and not args.no_fp64:
+        it_s, _ = fp64_solve(t3d, p, Mg, db, dsol, 1e-10, 2000)
+        C_star = comp()
+        log("converged compliance (FP64, Galerkin MG, tol 1e-10, %d iterations): %.12e" % (it_s, C_star))
+    rel = (lambda C: abs(C - C_star) / abs(C_star)) if C_star else (lambda C: None)
+
+    fp64_same = None
+    if galerkin and not args.no_fp64:   # the same preconditioner in FP64: the precision comparison
+        it64g, ms64g = fp64_solve(t3d, p, Mg, db, dsol, tol, args.maxit)
+        C64g = comp()
+        res64g = true_residual(t3d, p, db, dsol)
+        fp64_same = {"solves_per_s": 1e3 / ms64g, "ms_per_solve": ms64g, "pcg_iters": it64g,
+                     "ms_per_iteration": ms64g / it64g, "compliance": C64g, "true_residual": res64g,
+                     "compliance_rel_err": rel(C64g), "speedup_mixed_vs_fp64": ms64g / ms_step,
+                     "compliance_rel_diff_mixed": abs(Cm - C64g) / abs(C64g)}
+        log("FP64 PCG (Galerkin MG): %d iterations, %.1f ms/solve, compliance %.9e, true residual %.3e; "
+            "mixed speed-up %.2fx" % (it64g, ms64g, C64g, res64g, ms64g / ms_step))
 
     fp64 = None
-    if not args.no_fp64:
-        dsol.zero()
-        t3d.lib().t3d_synchronize()
-        t0 = time.perf_counter()
-        it64 = t3d.pcg(p, M, db, dsol, args.tol, args.maxit)
-        t3d.lib().t3d_synchronize()
-        ms64 = (time.perf_counter() - t0) * 1e3
-        C64 = t3d.compliance(p, dE, dsol, dce)
+    if not args.no_fp64:   # the FP64 plain path: the reference's algorithm, bit-exact, at the reference's tol
+        it64, ms64 = fp64_solve(t3d, p, M, db, dsol, args.ref_tol, args.maxit)
+        C64 = comp()
         res64 = true_residual(t3d, p, db, dsol)
-        log("FP64 plain PCG: %d iterations%s, %.1f ms/solve, %.3f ms/iteration, compliance %.9e, true residual %.3e"
-            % (it64, " (capped)" if it64 >= args.maxit else "", ms64, ms64 / it64, C64, res64))
-        fp64 = {"solves_per_s": 1e3 / ms64, "ms_per_solve": ms64, "pcg_iters": it64,
+        log("FP64 plain PCG (reference V-cycle, tol %.0e): %d iterations%s, %.1f ms/solve, %.3f ms/iteration, "
+            "compliance %.9e, true residual %.3e" % (args.ref_tol, it64, " (capped)" if it64 >= args.maxit else "",
+                                                     ms64, ms64 / it64, C64, res64))
+        fp64 = {"tol": args.ref_tol, "solves_per_s": 1e3 / ms64, "ms_per_solve": ms64, "pcg_iters": it64,
                 "capped": it64 >= args.maxit, "ms_per_iteration": ms64 / it64, "compliance": C64,
-                "true_residual": res64, "speedup_mixed_vs_fp64": ms64 / ms_step,
-                "compliance_rel_diff": abs(Cm - C64) / abs(C64)}
+                "true_residual": res64, "compliance_rel_err": rel(C64), "speedup_mixed_vs_fp64": ms64 / ms_step,
+                "compliance_rel_diff_mixed": abs(Cm - C64) / abs(C64)}
+    mixed_ref = None
+    if galerkin:   # the mixed solver with the reference's V-cycle at the reference's tol
+        itr, msr = timed_solves(t3d, mx, db, dsol, args.ref_tol, args.maxit, args.true_every, 0, 1)
+        Cr = comp()
+        mixed_ref = {"tol": args.ref_tol, "solves_per_s": 1e3 / msr, "ms_per_solve": msr, "pcg_iters": itr,
+                     "ms_per_iteration": msr / itr, "compliance": Cr, "compliance_rel_err": rel(Cr),
+                     "true_residual": true_residual(t3d, p, db, dsol)}
+        log("mixed PCG (reference V-cycle, tol %.0e): %d iterations, %.1f ms/solve, %.3f ms/iteration"
+            % (args.ref_tol, itr, msr, msr / itr))
     tight = None
-    if not args.no_tight and not args.no_fp64:
+    if not galerkin and not args.no_tight and not args.no_fp64:
         dsol.zero()
         itm_t = mx.pcg(db, dsol, args.tight_tol, args.maxit, trueEvery=args.true_every)
-        Cm_t = t3d.compliance(p, dE, dsol, dce)
+        Cm_t = comp()
         dsol.zero()
         it64_t = t3d.pcg(p, M, db, dsol, args.tight_tol, args.maxit)
-        C64_t = t3d.compliance(p, dE, dsol, dce)
+        C64_t = comp()
         res64_t = true_residual(t3d, p, db, dsol)
         tight = {"tol": args.tight_tol, "mixed_iters": itm_t, "fp64_iters": it64_t, "mixed_compliance": Cm_t,
                  "fp64_compliance": C64_t, "fp64_true_residual": res64_t,
@@ -420,6 +479,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    opname = {0: "Walsh-form", 2: "MFMA bf16 x3", 3: "MFMA bf16 x6"}[form]
+    mgname = ("Galerkin multigrid of the MATLAB original ('Adapted' V-cycle, P^T restriction, coarsest Cholesky) "
+              "on the C++ port's levels" if galerkin else "the C++ port's additive V-cycle")
     line = {
         "metric": METRIC,
         "value": 1e3 / ms_step,
@@ -437,19 +499,24 @@ def main():
                       "cantilever BCs/load)" % args.seed),
         "config": {
             "workload": "MG-PCG solve from x0=0 to true residual < %.0e, cuboid %dx%dx%d (nely x nelx x nelz), "
-                        "%d free DOFs; mixed precision: FP32 %s K*u and FP32 V-cycle, FP64 dots/alpha/"
-                        "beta/true residual, flexible beta, residual replacement every %d iterations"
-                        % (args.tol, args.nely, args.nelx, args.nelz, p.nF,
-                           {0: "Walsh-form", 2: "MFMA bf16 x3", 3: "MFMA bf16 x6"}[form], args.true_every),
-            "tol": args.tol,
+                        "%d free DOFs; mixed precision: FP32 %s K*u and FP32 V-cycle of the %s, FP64 dots/alpha/"
+                        "beta/true residual/coarsest solve, flexible beta, residual replacement every %d iterations"
+                        % (tol, args.nely, args.nelx, args.nelz, p.nF, opname, mgname, args.true_every),
+            "multigrid": args.mg,
+            "tol": tol,
             "max_iters": args.maxit,
             "pcg_iters": it_m,
             "capped": it_m >= args.maxit,
             "ms_per_iteration": ms_step / it_m,
             "dof_per_s": p.nF * it_m / (ms_step / 1e3),
             "compliance": Cm,
+            "compliance_converged": C_star,
+            "compliance_rel_err": rel(Cm),
             "true_residual": res_m,
+            "galerkin_setup_s": gal_setup_s,
+            "fp64_same_multigrid": fp64_same,
             "fp64_plain": fp64,
+            "mixed_reference_vcycle": mixed_ref,
             "tight_tolerance_check": tight,
             "kxu_fp64_ms": kxu64_ms,
             "kxu_fp64_gbps": b64 / kxu64_ms / 1e6,

@@ -119,6 +119,16 @@ int t3d_mg_info(const t3d_mg* g, int* nlev, int* dims, int* Nc, int* cholOk);
 int t3d_mg_mask(const t3d_mg* g, int level, uint8_t* out_host);
 int t3d_mg_diag(const t3d_mg* g, int level, double* out_host);
 int t3d_mg_coarsest(const t3d_mg* g, double* Kc_host, double* L_host);
+/* Multigrid form, chosen before t3d_mg_setup.  0: the reference's V-cycle
+ * above (default).  1: the Galerkin hierarchy of the MATLAB original
+ * (matlab/TOP3D_XL.m:945-1090, Solving_AssembleFEAstencil) on the same levels
+ * and transfers: level operators A_{l+1} = P^T A_l P from per-cell element
+ * matrices (the finest A_0 = M K M + (I - M)), Jacobi diagonals of those
+ * operators, restriction by P^T (TOP3D_XL.m:809-823) and the coarsest level
+ * solved by Cholesky at any size, in the 'Adapted' V-cycle (TOP3D_XL.m:772-807).
+ * Not bit-comparable with the reference (no counterpart in the C++ port);
+ * t3d_pcg and t3d_pcg_mixed use it through the same object. */
+int t3d_mg_set_mode(t3d_mg* g, int mode);
 
 /* ---- PCG ----
  * PCG_free (cpp/include/solver.hpp:22-25, cpp/src/solver.cpp:87-217) on the full

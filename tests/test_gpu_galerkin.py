@@ -1,0 +1,123 @@
+"""Galerkin form of the multigrid preconditioner on the device
+(t3d_mg_set_mode(g, 1), the hierarchy of matlab/TOP3D_XL.m:772-1090 on the C++
+port's levels) against the scipy restatement in oracle/galerkin.py, which forms
+every level operator as the global product P^T A P.
+
+The device builds the same operators cell by cell in a different summation
+order, so the comparisons are to FP64 round-off: diagonals and the coarsest
+matrix within 1e-12 of their largest entry, the V-cycle within 1e-11.  PCG
+with this preconditioner (FP64 plain loop and the mixed-precision loop) is
+checked by its true residual and by the compliance of its solution."""
+import numpy as np
+import pytest
+
+import oracle as O
+from galerkin import GalerkinMG
+from test_voxels import spheres_mask
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((8, 16, 8), None), ((16, 32, 16), None), ((12, 20, 8), None),
+         ((16, 32, 16), [(10.0, 8.0, 8.0, 4.5), (22.0, 4.0, 10.0, 3.0)])]
+
+
+def _both(t3d, dims, spheres):
+    solid = spheres_mask(*dims, spheres) if spheres else None
+    return t3d.Problem(*dims, solid=solid), O.Model(*dims, solid=solid)
+
+
+def _E(m, seed):
+    E = np.random.default_rng(seed).uniform(1e-3, 1.0, m.ne)
+    E[::13] = 1e-17
+    return E
+
+
+def _setup(t3d, dims, spheres, seed):
+    p, m = _both(t3d, dims, spheres)
+    E = _E(m, seed)
+    g = O.MG(m, Emin=1e-9)
+    Go = GalerkinMG(m, g, E)
+    G = t3d.MG(p, Emin=1e-9, galerkin=True)
+    G.setup(t3d.DeviceArray.from_host(E))
+    return p, m, E, Go, G
+
+
+@pytest.mark.parametrize("dims,spheres", CASES)
+def test_diagonals_masks_coarsest(t3d, dims, spheres):
+    p, m, E, Go, G = _setup(t3d, dims, spheres, 1)
+    assert G.nlev == len(Go.A)
+    for l in range(G.nlev):
+        d = G.diag(l)
+        assert np.abs(d - Go.D[l]).max() <= 1e-12 * np.abs(Go.D[l]).max(), l
+        assert np.array_equal(G.mask(l).astype(bool), Go.mask[l]), l
+    nc, ok = G.info()
+    assert nc == Go.Ac.shape[0] and ok == 1
+    Kc, L = G.coarsest()
+    assert np.abs(Kc - Go.Ac).max() <= 1e-12 * np.abs(Go.Ac).max()
+    assert np.abs(L @ L.T - Go.Ac).max() <= 1e-11 * np.abs(Go.Ac).max()
+
+
+@pytest.mark.parametrize("dims,spheres", CASES)
+def test_vcycle(t3d, dims, spheres):
+    p, m, E, Go, G = _setup(t3d, dims, spheres, 2)
+    rng = np.random.default_rng(5)
+    r = m.free_to_full(rng.uniform(-1, 1, m.nF))
+    z_o = Go.apply(r)
+    dz = t3d.DeviceArray(m.nDOF)
+    G.apply(t3d.DeviceArray.from_host(r), dz)
+    z_g = dz.download()
+    assert np.abs(z_g - z_o).max() <= 1e-11 * np.abs(z_o).max()
+
+
+def test_mode_switch_restores_reference(t3d):
+    """Back in mode 0 the object reproduces the reference's V-cycle bit-exactly."""
+    dims = (8, 16, 8)
+    p, m = _both(t3d, dims, None)
+    E = _E(m, 3)
+    G = t3d.MG(p, Emin=1e-9, galerkin=True)
+    dE = t3d.DeviceArray.from_host(E)
+    G.setup(dE)
+    t3d.check(t3d.lib().t3d_mg_set_mode(G.h, 0), "t3d_mg_set_mode")
+    G.setup(dE)
+    g = O.MG(m, Emin=1e-9)
+    g.setup(E)
+    for l in range(g.nlev):
+        assert np.array_equal(G.mask(l), g.masks[l]) and np.array_equal(G.diag(l), g.diag[l]), l
+    r = m.free_to_full(np.random.default_rng(6).uniform(-1, 1, m.nF))
+    dz = t3d.DeviceArray(m.nDOF)
+    G.apply(t3d.DeviceArray.from_host(r), dz)
+    assert np.array_equal(dz.download(), g.apply(r))
+
+
+@pytest.mark.parametrize("dims,spheres", [((16, 32, 16), None), ((30, 60, 30), None),
+                                          ((16, 32, 16), [(10.0, 8.0, 8.0, 4.5), (22.0, 4.0, 10.0, 3.0)])])
+def test_pcg_fp64_and_mixed(t3d, dims, spheres):
+    """FP64 PCG_free with the Galerkin V-cycle follows the oracle's textbook PCG
+    (same iteration count within one, solutions within 1e-9); the mixed solver
+    with the same object converges on its FP64 true residual and its compliance
+    agrees with the FP64 one within 1e-6 at tol 1e-8."""
+    p, m, E, Go, G = _setup(t3d, dims, spheres, 4)
+    tol = 1e-8
+    it_o, x_o = Go.pcg(m.b, tol, 500)
+    db = t3d.DeviceArray(m.nDOF)
+    p.load_vector(db)
+    dx = t3d.DeviceArray(m.nDOF)
+    dx.zero()
+    it_g = t3d.pcg(p, G, db, dx, tol, 500)
+    x_g = dx.download()
+    assert abs(it_g - it_o) <= 1 and it_g < 500, (it_g, it_o)
+    assert np.linalg.norm(x_g - x_o) <= 1e-9 * np.linalg.norm(x_o) + np.linalg.norm(x_o) * tol * 10
+    dE = t3d.DeviceArray.from_host(E)
+    dce = t3d.DeviceArray(p.ne)
+    C64 = t3d.compliance(p, dE, dx, dce)
+    mx = t3d.Mixed(p, G)
+    mx.setup()
+    dx.zero()
+    it_m = mx.pcg(db, dx, tol, 500)
+    assert it_m < 500 and it_m <= 1.5 * it_g + 5, (it_m, it_g)
+    dy = t3d.DeviceArray(m.nDOF)
+    p.kxu_f64(dx, dy)
+    res = np.linalg.norm(m.b - dy.download()) / np.linalg.norm(m.b)
+    assert res < tol
+    Cm = t3d.compliance(p, dE, dx, dce)
+    assert abs(Cm - C64) <= 1e-6 * abs(C64)
