@@ -464,8 +464,9 @@ class Batch:
     """Many problems solved together by the batched mixed-precision MG-PCG
     (t3d_batch_*): dims = [(nely, nelx, nelz), ...]."""
 
-    def __init__(self, dims, Emin=1e-9, solids=None):
-        """solids: optional list of voxel masks (as Problem's solid; None entries = cuboid)."""
+    def __init__(self, dims, Emin=1e-9, solids=None, galerkin=False):
+        """solids: optional list of voxel masks (as Problem's solid; None entries = cuboid).
+        galerkin=True: every problem's multigrid in the Galerkin form (MG(galerkin=True))."""
         n = len(dims)
         ny = np.array([d[0] for d in dims], np.int32)
         nx = np.array([d[1] for d in dims], np.int32)
@@ -488,6 +489,11 @@ class Batch:
             ph = C.c_void_p()
             check(lib().t3d_batch_problem(self.h, i, C.byref(ph)), "t3d_batch_problem")
             self.problems.append(Problem.from_handle(ph))
+            if galerkin:
+                gh = C.c_void_p()
+                check(lib().t3d_batch_mg(self.h, i, C.byref(gh)), "t3d_batch_mg")
+                check(lib().t3d_mg_set_mode(gh, 1), "t3d_mg_set_mode")
+        self.galerkin = bool(galerkin)
 
     def __del__(self):
         try:

@@ -44,6 +44,7 @@ struct BProb {
     int nlev;
     BLevel lv[8];
     float weight;
+    int norm;   // restriction: 1 the reference's normalised form, 0 P^T (Galerkin multigrid)
     int Nc;
     const double* L;
     const double* LT;
@@ -146,7 +147,7 @@ __global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P
     const bool chol = q.Nc > 0;
     restrict_yz_body(q.lv[l].L, Lc, q.lv[l + 1].span, q.tmp, q.lv[l + 1].mask, q.lv[l + 1].r,
                      (last && !chol) ? q.lv[L].Dinv : nullptr, q.lv[L].x, (last && chol) ? q.rc64 : nullptr, q.flags,
-                     bx, cy, cz, 0, 0);
+                     bx, cy, cz, 0, 0, q.norm);
 }
 
 // coarsest solve through the inverse Cholesky factor (coarse.hpp): G blocks
@@ -313,6 +314,7 @@ static int batch_build(t3d_batch* B)
             q.lv[l].x = s->x32[l];
         }
         q.weight = float(v.weight);
+        q.norm = v.norm;
         q.Nc = v.Nc;
         q.L = v.L;
         q.LT = v.LT;

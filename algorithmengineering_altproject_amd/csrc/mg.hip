@@ -457,6 +457,111 @@ __global__ void __launch_bounds__(256) k_gal_diag(LevelDev L, const double* __re
     mask[n] = uint8_t(bits);
 }
 
+// ---------------------------------------------------------------------------
+// Dense right-looking blocked Cholesky A = L L^T in FP64 on the device (the
+// Galerkin form's coarsest matrix, row-major N x N, lower triangle used and
+// overwritten by L).  Per panel of CB columns at k0: the diagonal block is
+// factored by one workgroup in LDS, the rows below are solved against it, and
+// the trailing lower triangle is updated by 64 x 64 tiles.
+// ---------------------------------------------------------------------------
+constexpr int CB = 32;
+
+__global__ void __launch_bounds__(256) k_chol_diag(int N, double* __restrict__ A, int k0, int* __restrict__ fail)
+{
+    __shared__ double a[CB][CB + 1];
+    const int nb = min(CB, N - k0);
+    for (int e = threadIdx.x; e < CB * CB; e += 256) {
+        const int i = e / CB, j = e % CB;
+        a[i][j] = (i < nb && j < nb && j <= i) ? A[size_t(k0 + i) * N + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const double d = a[j][j];
+        __syncthreads();
+        if (!(d > 0.0)) {
+            if (threadIdx.x == 0) *fail = 1;
+            return;   // uniform: every thread read the same d
+        }
+        const double s = sqrt(d);
+        if (threadIdx.x < CB) {
+            const int i = threadIdx.x;
+            if (i == j) a[j][j] = s;
+            else if (i > j && i < nb) a[i][j] /= s;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < CB * CB; e += 256) {
+            const int i = e / CB, l = e % CB;
+            if (i > j && l > j && l <= i && i < nb) a[i][l] -= a[i][j] * a[l][j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < CB * CB; e += 256) {
+        const int i = e / CB, j = e % CB;
+        if (i < nb && j < nb && j <= i) A[size_t(k0 + i) * N + k0 + j] = a[i][j];
+    }
+}
+
+// rows i >= k0 + CB of the panel: L[i][k0:k0+CB] = A[i][k0:k0+CB] L_kk^-T
+__global__ void __launch_bounds__(256) k_chol_panel(int N, double* __restrict__ A, int k0,
+                                                    const int* __restrict__ fail)
+{
+    if (*fail) return;
+    __shared__ double l[CB][CB + 1];
+    for (int e = threadIdx.x; e < CB * CB; e += 256) {
+        const int i = e / CB, j = e % CB;
+        l[i][j] = (j <= i) ? A[size_t(k0 + i) * N + k0 + j] : 0.0;
+    }
+    __syncthreads();
+    const int i = k0 + CB + int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (i >= N) return;
+    double x[CB];
+    double* row = A + size_t(i) * N + k0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) x[j] = row[j];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        double v = x[j];
+#pragma unroll
+        for (int t = 0; t < j; ++t) v -= x[t] * l[j][t];
+        x[j] = v / l[j][j];
+    }
+#pragma unroll
+    for (int j = 0; j < CB; ++j) row[j] = x[j];
+}
+
+// trailing update of the lower triangle: A[i][j] -= sum_t L[i][k0+t] L[j][k0+t]
+// for k0 + CB <= j <= i < N, 64 x 64 tiles, 4 x 4 outputs per thread
+__global__ void __launch_bounds__(256) k_chol_update(int N, double* __restrict__ A, int k0,
+                                                     const int* __restrict__ fail)
+{
+    if (*fail) return;
+    const int b0 = k0 + CB;
+    const int ti = int(blockIdx.y), tj = int(blockIdx.x);
+    if (tj > ti) return;   // upper tiles
+    const int i0 = b0 + 64 * ti, j0 = b0 + 64 * tj;
+    __shared__ double Pi[64][CB + 1], Pj[64][CB + 1];
+    for (int e = threadIdx.x; e < 64 * CB; e += 256) {
+        const int r = e / CB, t = e % CB;
+        Pi[r][t] = (i0 + r < N) ? A[size_t(i0 + r) * N + k0 + t] : 0.0;
+        Pj[r][t] = (j0 + r < N) ? A[size_t(j0 + r) * N + k0 + t] : 0.0;
+    }
+    __syncthreads();
+    const int ri = (int(threadIdx.x) / 16) * 4, rj = (int(threadIdx.x) % 16) * 4;
+    double acc[4][4] = {};
+    for (int t = 0; t < CB; ++t)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += Pi[ri + a][t] * Pj[rj + b][t];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ri + a, j = j0 + rj + b;
+            if (i < N && j <= i) A[size_t(i) * N + j] -= acc[a][b];
+        }
+}
+
 // coarsest solve of the Galerkin form through the inverse factor (coarse.hpp)
 __global__ void __launch_bounds__(64) k_mg_tri_inv(int N, const double* __restrict__ L, double* Li, double* LiT)
 {
@@ -564,6 +669,8 @@ struct MGView {
     int Nc;
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
+    const double* Li;    // inverse coarsest factor and transpose when the multigrid keeps them
+    const double* LiT;
 };
 int mg_view(const t3d_mg* g, MGView& v)
 {
@@ -581,6 +688,8 @@ int mg_view(const t3d_mg* g, MGView& v)
     v.Nc = g->Nc;
     v.weight = g->H.weight;
     v.norm = g->mode == 0 ? 1 : 0;
+    v.Li = g->mode == 1 ? g->Li : nullptr;
+    v.LiT = g->mode == 1 ? g->LiT : nullptr;
     return T3D_OK;
 }
 }  // namespace t3d
@@ -706,30 +815,44 @@ static int mg_galerkin_setup(t3d_mg* g)
     for (int i = 0; i < N; ++i)
         if ((mk[i / 3] >> (i % 3)) & 1u) A[size_t(i) * N + i] = 1.0;   // a DOF covering only void
     g->Kc = A;
-    if (!chol_factor(A, N)) {
+    // factor on the device (upper triangle zeroed first), then the inverse factor
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) A[size_t(i) * N + j] = 0.0;
+    const size_t n2 = size_t(N) * N;
+    if (!g->L) {
+        T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->Li, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->LiT, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->yc, sizeof(double) * (size_t(N) + 1)));
+    }
+    int* fail = reinterpret_cast<int*>(g->yc + N);
+    T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
+    T3D_HIP(hipMemsetAsync(fail, 0, sizeof(int), st));
+    for (int k0 = 0; k0 < N; k0 += CB) {
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, N, g->L, k0, fail);
+        const int rest = N - k0 - CB;
+        if (rest > 0) {
+            hipLaunchKernelGGL(k_chol_panel, dim3(unsigned((rest + 255) / 256)), dim3(256), 0, st, N, g->L, k0,
+                               (const int*)fail);
+            const unsigned tiles = unsigned((rest + 63) / 64);
+            hipLaunchKernelGGL(k_chol_update, dim3(tiles, tiles), dim3(256), 0, st, N, g->L, k0, (const int*)fail);
+        }
+    }
+    T3D_HIP(hipGetLastError());
+    int hfail = 0;
+    T3D_HIP(hipMemcpyAsync(&hfail, fail, sizeof(int), hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipStreamSynchronize(st));   // A is a host buffer of this scope
+    if (hfail) {
         g->cholOk = 0;
         set_error("Galerkin multigrid: the coarsest matrix is not positive definite");
         return T3D_ERR_STATE;
     }
-    const size_t n2 = size_t(N) * N;
-    std::vector<double> AT(n2);
-    for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) AT[size_t(i) * N + j] = A[size_t(j) * N + i];
-    if (!g->L) {
-        T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->LT, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->Li, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->LiT, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->yc, sizeof(double) * size_t(N)));
-    }
-    T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
-    T3D_HIP(hipMemcpyAsync(g->LT, AT.data(), sizeof(double) * n2, hipMemcpyHostToDevice, st));
     T3D_HIP(hipMemsetAsync(g->Li, 0, sizeof(double) * n2, st));
     T3D_HIP(hipMemsetAsync(g->LiT, 0, sizeof(double) * n2, st));
     hipLaunchKernelGGL(k_mg_tri_inv, dim3(unsigned((N + 63) / 64)), dim3(64), 0, st, N, (const double*)g->L, g->Li,
                        g->LiT);
     T3D_HIP(hipGetLastError());
-    T3D_HIP(hipStreamSynchronize(st));   // A and AT are host buffers of this scope
+    T3D_HIP(hipStreamSynchronize(st));
     g->Nc = N;
     return T3D_OK;
 }
@@ -919,10 +1042,9 @@ int t3d_mg_setup(t3d_mg* g, const double* E_dev)
             std::vector<double> AT(size_t(N) * N);
             for (int i = 0; i < N; ++i)
                 for (int j = 0; j < N; ++j) AT[size_t(i) * N + j] = A[size_t(j) * N + i];
-            if (!g->L) {
-                T3D_HIP(hipMalloc(&g->L, sizeof(double) * size_t(N) * N));
-                T3D_HIP(hipMalloc(&g->LT, sizeof(double) * size_t(N) * N));
-            }
+            // (the Galerkin form allocates L of the same size but no LT)
+            if (!g->L) T3D_HIP(hipMalloc(&g->L, sizeof(double) * size_t(N) * N));
+            if (!g->LT) T3D_HIP(hipMalloc(&g->LT, sizeof(double) * size_t(N) * N));
             T3D_HIP(hipMemcpyAsync(g->L, A.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
             T3D_HIP(hipMemcpyAsync(g->LT, AT.data(), sizeof(double) * size_t(N) * N, hipMemcpyHostToDevice, st));
             T3D_HIP(hipStreamSynchronize(st));   // A and AT are host buffers of this scope

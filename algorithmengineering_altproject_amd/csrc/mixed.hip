@@ -734,7 +734,8 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64, s->Li, s->LiT, s->yc64,
+    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64,
+                 s->ownLi ? s->Li : nullptr, s->ownLi ? s->LiT : nullptr, s->yc64,
                  s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
@@ -903,13 +904,27 @@ int t3d_mixed_setup(t3d_mixed* s)
     // inverse of the coarsest Cholesky factor (the coarsest size is fixed by
     // the hierarchy; the factor exists only when the Galerkin matrix is SPD)
     s->NcInv = 0;
-    if (v.Nc > 0) {
+    if (v.Nc > 0 && v.Li) {   // the multigrid object keeps the inverse factor
+        if (s->ownLi) {
+            if (s->Li) (void)hipFree(s->Li);
+            if (s->LiT) (void)hipFree(s->LiT);
+        }
+        s->Li = const_cast<double*>(v.Li);
+        s->LiT = const_cast<double*>(v.LiT);
+        s->ownLi = false;
+        if (!s->yc64) T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
+        s->NcInv = v.Nc;
+    } else if (v.Nc > 0) {
         const size_t n2 = size_t(v.Nc) * size_t(v.Nc);
+        if (!s->ownLi) {
+            s->Li = s->LiT = nullptr;
+            s->ownLi = true;
+        }
         if (!s->Li) {
             T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
             T3D_HIP(hipMalloc(&s->LiT, sizeof(double) * n2));
-            T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
         }
+        if (!s->yc64) T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
         T3D_HIP(hipMemsetAsync(s->Li, 0, sizeof(double) * n2, st));
         T3D_HIP(hipMemsetAsync(s->LiT, 0, sizeof(double) * n2, st));
         hipLaunchKernelGGL(k_tri_inv, dim3(unsigned((v.Nc + 63) / 64)), dim3(64), 0, st, v.Nc, v.L, s->Li, s->LiT);

@@ -52,7 +52,9 @@ def parse():
                     help="the reference's cgTol, used for the solves with the C++ port's V-cycle")
     ap.add_argument("--tight-tol", type=float, default=1e-5)
     ap.add_argument("--maxit", type=int, default=4000)
-    ap.add_argument("--true-every", type=int, default=50, help="residual replacement period of the mixed solver")
+    ap.add_argument("--true-every", type=int, default=None,
+                    help="residual replacement period of the mixed solver (default 25 with the Galerkin multigrid, "
+                         "whose residual falls 1e3 in about 25 iterations, else 50)")
     ap.add_argument("--design", choices=["uniform", "random-filtered"], default="uniform",
                     help="uniform rho = 0.12, or rho ~ U[1e-3, 1] (seeded) PDE-filtered with rmin = 2")
     ap.add_argument("--seed", type=int, default=3)
@@ -180,18 +182,20 @@ def batch_design(t3d, p, v0, spheres):
 
 
 def main_batch(args, t3d):
-    btol = args.tol if args.tol is not None else args.ref_tol
+    galerkin = args.mg == "galerkin"
+    btol = args.tol if args.tol is not None else (1e-6 if galerkin else args.ref_tol)
+    bte = args.true_every if args.true_every is not None else (25 if galerkin else 50)
     rng = np.random.default_rng(args.seed)
     probs = batch_workload(rng, args.batch)
     dims = [d for d, _, _ in probs]
     t0 = time.time()
     if args.voids == "removed":   # void voxels carry no element (voxel-domain model)
         solids = [batch_mask(d, s) for d, _, s in probs]
-        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)), solids=solids)
+        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)), solids=solids, galerkin=galerkin)
         dEs = [batch_design(t3d, p, v, ()) for p, (_, v, s) in zip(B.problems, probs)]
     else:                         # void voxels kept as elements of density 0 (E = Emin)
         solids = [None] * len(probs)
-        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)))
+        B = t3d.Batch(dims, Emin=float(np.float32(1e-9)), galerkin=galerkin)
         dEs = [batch_design(t3d, p, v, s) for p, (_, v, s) in zip(B.problems, probs)]
     B.setup(dEs)
     dbs, xs = [], []
@@ -206,14 +210,14 @@ def main_batch(args, t3d):
     for _ in range(max(1, args.warmup)):          # the first solve also autotunes the tiles
         for x in xs:
             x.zero()
-        B.solve(dbs, xs, btol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
+        B.solve(dbs, xs, btol, args.maxit, trueEvery=bte, useGraph=not args.no_graph)
     tiles = B.info()
     t3d.lib().t3d_synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         for x in xs:
             x.zero()
-        its = B.solve(dbs, xs, btol, args.maxit, trueEvery=args.true_every, useGraph=not args.no_graph)
+        its = B.solve(dbs, xs, btol, args.maxit, trueEvery=bte, useGraph=not args.no_graph)
     t3d.lib().t3d_synchronize()
     ms_batch = (time.perf_counter() - t0) / args.steps * 1e3
     res = [true_residual(t3d, p, db, x) for p, db, x in zip(B.problems, dbs, xs)]
@@ -224,20 +228,20 @@ def main_batch(args, t3d):
     if not args.no_seq:
         Ms = []
         for i, p in enumerate(B.problems):   # the batch's problems, each with its own multigrid and solver state
-            M = t3d.MG(p, Emin=float(np.float32(1e-9)))
+            M = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=galerkin)
             M.setup(dEs[i])
             mx = t3d.Mixed(p, M)
             mx.setup()
             Ms.append((M, mx))
         for (M, mx), db, x in zip(Ms, dbs, xs):   # warm-up
             x.zero()
-            mx.pcg(db, x, btol, args.maxit, trueEvery=args.true_every)
+            mx.pcg(db, x, btol, args.maxit, trueEvery=bte)
         t3d.lib().t3d_synchronize()
         t0 = time.perf_counter()
         its_seq = []
         for (M, mx), db, x in zip(Ms, dbs, xs):
             x.zero()
-            its_seq.append(mx.pcg(db, x, btol, args.maxit, trueEvery=args.true_every))
+            its_seq.append(mx.pcg(db, x, btol, args.maxit, trueEvery=bte))
         t3d.lib().t3d_synchronize()
         ms_seq = (time.perf_counter() - t0) * 1e3
         log("sequential: %.1f ms for all solves (%.1f solves/s), speed-up %.2fx" % (ms_seq, len(probs) * 1e3 / ms_seq,
@@ -259,9 +263,11 @@ def main_batch(args, t3d):
                                                         "voids removed from the voxel domain" if args.voids == "removed"
                                                         else "voids as density-0 elements"),
         "config": {
-            "workload": "batched mixed-precision MG-PCG: %d uneven problems solved together to true residual < %.0e "
-                        "(one launch per kernel for the batch, hipGraph-replayed iterations, autotuned tiles); a "
-                        "step solves all %d" % (len(probs), btol, len(probs)),
+            "workload": "batched mixed-precision MG-PCG (%s multigrid): %d uneven problems solved together to true "
+                        "residual < %.0e (one launch per kernel for the batch, hipGraph-replayed iterations, "
+                        "autotuned tiles); a step solves all %d"
+                        % ("Galerkin" if galerkin else "reference", len(probs), btol, len(probs)),
+            "multigrid": args.mg,
             "problems": len(probs),
             "elements": ne,
             "free_dofs": nF,
@@ -316,6 +322,7 @@ def main():
         return main_batch(args, t3d)
     galerkin = args.mg == "galerkin"
     tol = args.tol if args.tol is not None else (1e-6 if galerkin else args.ref_tol)
+    te = args.true_every if args.true_every is not None else (25 if galerkin else 50)
     t0 = time.time()
     p = t3d.Problem(args.nely, args.nelx, args.nelz)
     log("problem %dx%dx%d: %d elements, %d free DOFs (setup %.1fs)" % (args.nely, args.nelx, args.nelz, p.ne, p.nF,
@@ -402,7 +409,7 @@ def main():
     comp = lambda: t3d.compliance(p, dE, dsol, dce)
     # ---- the timed steps: mixed-precision solves with the selected multigrid
     head_mx = mxg if galerkin else mx
-    it_m, ms_step = timed_solves(t3d, head_mx, db, dsol, tol, args.maxit, args.true_every, args.warmup, args.steps)
+    it_m, ms_step = timed_solves(t3d, head_mx, db, dsol, tol, args.maxit, te, args.warmup, args.steps)
     Cm = comp()
     res_m = true_residual(t3d, p, db, dsol)
     log("mixed PCG (%s MG): %d iterations/solve%s, %.1f ms/solve, %.3f ms/iteration, compliance %.9e, "
@@ -442,7 +449,7 @@ def main():
                 "compliance_rel_diff_mixed": abs(Cm - C64) / abs(C64)}
     mixed_ref = None
     if galerkin:   # the mixed solver with the reference's V-cycle at the reference's tol
-        itr, msr = timed_solves(t3d, mx, db, dsol, args.ref_tol, args.maxit, args.true_every, 0, 1)
+        itr, msr = timed_solves(t3d, mx, db, dsol, args.ref_tol, args.maxit, 50, 0, 1)
         Cr = comp()
         mixed_ref = {"tol": args.ref_tol, "solves_per_s": 1e3 / msr, "ms_per_solve": msr, "pcg_iters": itr,
                      "ms_per_iteration": msr / itr, "compliance": Cr, "compliance_rel_err": rel(Cr),
@@ -452,7 +459,7 @@ def main():
     tight = None
     if not galerkin and not args.no_tight and not args.no_fp64:
         dsol.zero()
-        itm_t = mx.pcg(db, dsol, args.tight_tol, args.maxit, trueEvery=args.true_every)
+        itm_t = mx.pcg(db, dsol, args.tight_tol, args.maxit, trueEvery=te)
         Cm_t = comp()
         dsol.zero()
         it64_t = t3d.pcg(p, M, db, dsol, args.tight_tol, args.maxit)
@@ -501,7 +508,7 @@ def main():
             "workload": "MG-PCG solve from x0=0 to true residual < %.0e, cuboid %dx%dx%d (nely x nelx x nelz), "
                         "%d free DOFs; mixed precision: FP32 %s K*u and FP32 V-cycle of the %s, FP64 dots/alpha/"
                         "beta/true residual/coarsest solve, flexible beta, residual replacement every %d iterations"
-                        % (tol, args.nely, args.nelx, args.nelz, p.nF, opname, mgname, args.true_every),
+                        % (tol, args.nely, args.nelx, args.nelz, p.nF, opname, mgname, te),
             "multigrid": args.mg,
             "tol": tol,
             "max_iters": args.maxit,

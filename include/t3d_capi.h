@@ -195,7 +195,9 @@ int t3d_batch_create(int n, const int* nely, const int* nelx, const int* nelz, f
 int t3d_batch_create_voxels(int n, const int* nely, const int* nelx, const int* nelz, const uint8_t* const* solid,
                             float Emin, t3d_batch** out);
 int t3d_batch_destroy(t3d_batch* b);
-/* the batch's problem / multigrid object i (owned by the batch) */
+/* the batch's problem / multigrid object i (owned by the batch); a problem's
+ * multigrid may be switched to the Galerkin form (t3d_mg_set_mode) before
+ * t3d_batch_setup */
 int t3d_batch_problem(t3d_batch* b, int i, t3d_problem** p);
 int t3d_batch_mg(t3d_batch* b, int i, t3d_mg** g);
 /* E_dev[i]: compact element moduli of problem i (device); multigrid set-up */

@@ -43,8 +43,8 @@ V0S = [0.12, 0.2, 0.3, 0.08, 0.15, 0.25]
 VOIDS = [(), ((24, 8, 8, 5),), (), ((12, 6, 8, 3), (6, 6, 8, 3)), (), ()]
 
 
-def build(t3d, dims, v0s, voids):
-    B = t3d.Batch(dims)
+def build(t3d, dims, v0s, voids, galerkin=False):
+    B = t3d.Batch(dims, galerkin=galerkin)
     dEs = [design(t3d, p, v, s) for p, v, s in zip(B.problems, v0s, voids)]
     B.setup(dEs)
     dbs = []
@@ -55,12 +55,12 @@ def build(t3d, dims, v0s, voids):
     return B, dEs, dbs
 
 
-def sequential(t3d, dims, v0s, voids, tol, maxIt):
+def sequential(t3d, dims, v0s, voids, tol, maxIt, galerkin=False, trueEvery=50):
     out = []
     for d, v, s in zip(dims, v0s, voids):
         p = t3d.Problem(*d)
         dE = design(t3d, p, v, s)
-        M = t3d.MG(p, Emin=1e-9)
+        M = t3d.MG(p, Emin=1e-9, galerkin=galerkin)
         M.setup(dE)
         mx = t3d.Mixed(p, M)
         mx.setup()
@@ -68,7 +68,7 @@ def sequential(t3d, dims, v0s, voids, tol, maxIt):
         p.load_vector(db)
         x = t3d.DeviceArray(p.nDOF)
         x.zero()
-        it = mx.pcg(db, x, tol, maxIt)
+        it = mx.pcg(db, x, tol, maxIt, trueEvery=trueEvery)
         out.append((it, x.download()))
     return out
 
@@ -120,3 +120,23 @@ def test_batch_autotuned(t3d):
     assert list(its2) == list(its)
     for x, f in zip(xs, first):
         assert np.array_equal(x.download(), f)
+
+
+def test_batch_galerkin_matches_sequential(t3d):
+    """Every problem with the Galerkin multigrid: the batch runs the single
+    solver's arithmetic (same iterations, bit-identical solutions) and reaches
+    1e-6 in tens of iterations."""
+    tol, maxIt, te = 1e-6, 500, 25
+    B, dEs, dbs = build(t3d, DIMS, V0S, VOIDS, galerkin=True)
+    B.set_tiles(16, 4)
+    xs = [t3d.DeviceArray(p.nDOF) for p in B.problems]
+    for x in xs:
+        x.zero()
+    its = B.solve(dbs, xs, tol, maxIt, trueEvery=te)
+    ref = sequential(t3d, DIMS, V0S, VOIDS, tol, maxIt, galerkin=True, trueEvery=te)
+    for i, (p, x, db) in enumerate(zip(B.problems, xs, dbs)):
+        it_ref, x_ref = ref[i]
+        print(DIMS[i], "batch", its[i], "sequential", it_ref)
+        assert its[i] == it_ref and its[i] < 150
+        assert np.array_equal(x.download(), x_ref)
+        assert true_rel_residual(t3d, p, db, x) < tol
