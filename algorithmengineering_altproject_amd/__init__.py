@@ -98,6 +98,8 @@ def lib():
         "t3d_batch_solve": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "t3d_mixed_set_operator": (C.c_int, [vp, C.c_int]),
         "t3d_mixed_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
+        "t3d_mixed_set_timing": (C.c_int, [vp, C.c_int]),
+        "t3d_mixed_get_timing": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "t3d_pcg_mixed": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
         "t3d_top3d_run": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(C.c_double),
@@ -452,6 +454,16 @@ class Mixed:
         ms = C.c_float()
         check(lib().t3d_mixed_time_op(self.h, form, du.ptr, dy.ptr, reps, C.byref(ms)), "t3d_mixed_time_op")
         return ms.value
+
+    def set_timing(self, on=True):
+        """HIP-event timing of the fused operator kernel inside pcg() (totals zeroed)."""
+        check(lib().t3d_mixed_set_timing(self.h, int(on)), "t3d_mixed_set_timing")
+
+    def timing(self):
+        """(milliseconds, launches) of the fused operator kernel since set_timing."""
+        ms, n = C.c_double(), C.c_longlong()
+        check(lib().t3d_mixed_get_timing(self.h, C.byref(ms), C.byref(n)), "t3d_mixed_get_timing")
+        return ms.value, n.value
 
     def pcg(self, db, dx, tol, maxIt, trueEvery=50, flexible=True):
         it = C.c_int()

@@ -739,6 +739,8 @@ int t3d_mixed_destroy(t3d_mixed* s)
                  s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
+    for (hipEvent_t e : s->tev)
+        if (e) (void)hipEventDestroy(e);
     for (int l = 0; l < 8; ++l) {
         if (s->D32[l]) (void)hipFree(s->D32[l]);
         if (s->r32[l]) (void)hipFree(s->r32[l]);
@@ -1011,6 +1013,27 @@ int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, 
     return T3D_OK;
 }
 
+// per-launch HIP-event timing of the solver's fused operator kernel inside
+// t3d_pcg_mixed: on != 0 creates the events and zeroes the totals
+int t3d_mixed_set_timing(t3d_mixed* s, int on)
+{
+    T3D_CHECK(s, T3D_ERR_ARG, "t3d_mixed_set_timing: null");
+    if (on && !s->tev[0])
+        for (hipEvent_t& e : s->tev) T3D_HIP(hipEventCreate(&e));
+    s->timing = on != 0;
+    s->timedMs = 0.0;
+    s->timedLaunches = 0;
+    return T3D_OK;
+}
+
+int t3d_mixed_get_timing(const t3d_mixed* s, double* ms_total, long long* launches)
+{
+    T3D_CHECK(s && ms_total && launches, T3D_ERR_ARG, "t3d_mixed_get_timing: null");
+    *ms_total = s->timedMs;
+    *launches = s->timedLaunches;
+    return T3D_OK;
+}
+
 // Mixed-precision MG-PCG on the full layout (b, x FP64, device).  The problem's
 // modulus and the multigrid object (after t3d_mg_setup + t3d_mixed_setup) are
 // used; trueEvery > 0 recomputes the residual in FP64 every trueEvery iterations.
@@ -1058,7 +1081,9 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         const int it1 = std::min(maxIt, it0 + CHUNK);
         for (int it = it0; it < it1; ++it) {
             // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
+            if (s->timing) T3D_HIP(hipEventRecord(s->tev[2 * (it - it0)], st));
             launch_fused(s, pPrevOf(it), pCurOf(it), s->Ap);
+            if (s->timing) T3D_HIP(hipEventRecord(s->tev[2 * (it - it0) + 1], st));
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, nfused, S,
                                (const int*)flags);
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
@@ -1083,6 +1108,15 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         }
         T3D_HIP(hipMemcpyAsync(s->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
+        if (s->timing) {   // launches of iterations that ran (after convergence the kernel returns at once)
+            const int last = s->hostFlags[0] ? std::min(it1, s->hostFlags[1]) : it1;
+            for (int it = it0; it < last; ++it) {
+                float ms = 0.f;
+                T3D_HIP(hipEventElapsedTime(&ms, s->tev[2 * (it - it0)], s->tev[2 * (it - it0) + 1]));
+                s->timedMs += ms;
+                ++s->timedLaunches;
+            }
+        }
         if (!s->hostFlags[0]) {
             it0 = it1;
             continue;

@@ -33,6 +33,11 @@ struct t3d_mixed {
     int* iflags = nullptr;     // [0] done [1] iterations
     int hostFlags[4];
     int nlev = 0;
+    // per-launch HIP-event timing of the fused operator kernel inside the solves
+    bool timing = false;
+    hipEvent_t tev[64] = {};
+    double timedMs = 0.0;
+    long long timedLaunches = 0;
 };
 
 // internal accessors of the multigrid object (mg.hip)

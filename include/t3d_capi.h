@@ -178,6 +178,12 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form);
  * op 5 / op 6: the Walsh-form fused kernel / operator in the other workgroup
  * tile order (hardware order vs XCD-grouped) */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
+/* HIP-event timing of the solver's fused operator kernel inside t3d_pcg_mixed
+ * (no counterpart in the reference; for the benchmark's roofline): on != 0
+ * creates the events and zeroes the totals; get returns the summed kernel
+ * milliseconds and the number of launches of iterations that ran. */
+int t3d_mixed_set_timing(t3d_mixed* s, int on);
+int t3d_mixed_get_timing(const t3d_mixed* s, double* ms_total, long long* launches);
 
 /* ---- batching (the chosen "batching" form; no reference counterpart: the
  * reference runs its simulation_count problems one after another,
