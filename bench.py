@@ -5,20 +5,22 @@ A step is one multigrid-preconditioned PCG solve (PCG_free, cpp/src/solver.cpp:8
 from x0 = 0 to ||b - K x|| < tol ||b|| on the 192x384x192 cuboid (14.2M voxels,
 42.9M free DOFs), with the reference's boundary conditions and load, element
 moduli from SIMP of the uniform initial design rho = 0.12 (Emin = 1e-9), and the
-multigrid preconditioner built for those moduli (built once, outside the timed
-region).  tol is the reference's default cgTol = 1e-3f; the iteration cap is
-raised from the reference's 800 to 4000 because neither solver reaches 1e-3
-within 800 iterations at this size (both need ~1340).
+multigrid preconditioner built for those moduli (outside the timed region).
 
-`value` is solves/s of the mixed-precision path (FP32 Walsh-form K*u and FP32
-V-cycle inside a flexible FP64 outer loop, true-residual stopping), inputs
-resident in HBM.  The FP64 plain path (bit-identical to the reference's
-operation order) is solved on the same problem in the same run; the line
-carries both solve rates, their ratio, both compliances, and a second
-comparison at tol = 1e-5 where both solves are converged to the 1e-6 level.
-`roofline` is the dominant kernel of the mixed solve (the fused FP32 K*u),
-timed with HIP events on its stream; `cpu_baseline` is the reference's own
-OpenMP code timed on this host.
+`value` is solves/s of the mixed-precision solver (FP32 Walsh-form K*u and FP32
+V-cycle inside a flexible FP64 outer loop, true-residual stopping) with the
+Galerkin multigrid of the MATLAB original (matlab/TOP3D_XL.m:772-1090) on the
+C++ port's levels, to tol = 1e-6, inputs resident in HBM.  The same run solves
+the problem with that multigrid in FP64, with the FP64 plain path (the C++
+port's V-cycle, bit-identical to the reference's arithmetic) and with the
+mixed solver on that V-cycle at the reference's cgTol = 1e-3, and reports every
+compliance against the converged compliance (FP64 to 1e-10).  `--mg reference`
+times the mixed solver on the C++ port's V-cycle instead (iteration cap raised
+from the reference's 800 to 4000: neither solver reaches 1e-3 within 800
+iterations at this size).  `roofline` is the dominant kernel of the mixed solve
+(the fused FP32 K*u), timed with HIP events around each of its launches inside
+the timed solves, on its stream; `cpu_baseline` is the reference's own OpenMP
+code timed on this host.  `--batch 64` runs the 64-problem uneven batch.
 """
 import argparse
 import json
