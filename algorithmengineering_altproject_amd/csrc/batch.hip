@@ -80,8 +80,8 @@ __global__ void __launch_bounds__(32 * KW_CY, KW_CY >= 16 ? 4 : 5) kb_kxw32(cons
     const unsigned tx = loc % q.kgx, ty = (loc / q.kgx) % q.kgy, tz = loc / (q.kgx * q.kgy);
     const float* pprev = parity ? q.pB : q.pA;
     float* pcur = parity ? q.pA : q.pB;
-    kxw32_body<NZ, true, KW_CY>(q.g, KW, q.Ecell, q.nodeFixed, pprev, q.z, pcur, q.Ap, q.part, q.S, q.flags, tx, ty, tz,
-                         loc, 0);
+    kxw32_body<NZ, true, KW_CY>(q.g, KW, q.Ecell, q.nodeFixed, pprev, q.z, pcur, q.Ap, q.part, q.S, q.flags, q.dx,
+                                parity, tx, ty, tz, loc, 0);
 }
 
 __global__ void __launch_bounds__(1024) kb_fin_alpha(const BProb* __restrict__ P)
@@ -106,8 +106,7 @@ __global__ void __launch_bounds__(256) kb_xr_rows(const BProb* __restrict__ P, c
 {
     const BMap m = map[blockIdx.x];
     const BProb& q = P[m.prob];
-    const float* pcur = parity ? q.pA : q.pB;
-    xr_rows_body<R>(q.g.nnx, q.g.nny * q.g.nnz, pcur, q.Ap, q.dx, q.r, q.part, q.S, q.lv[1].L.nnx, q.lv[1].span,
+    xr_rows_body<R>(q.g.nnx, q.g.nny * q.g.nnz, q.Ap, q.r, q.part, q.S, q.lv[1].L.nnx, q.lv[1].span,
                     withRestrict ? q.tmp : nullptr, q.flags, unsigned(m.local), 0, 0, unsigned(m.local), 0);
 }
 
@@ -204,6 +203,8 @@ __global__ void kb_init(const BProb* __restrict__ P, double tol)
         const double nb = sqrt(q.S[S_NORMB2]);
         q.S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
         q.S[S_BETA] = 0.0;
+        q.S[S_ALPHA_DX] = 0.0;
+        q.S[S_PLAST] = 0.0;
         q.flags[0] = 0;
         q.flags[1] = -1;
     }

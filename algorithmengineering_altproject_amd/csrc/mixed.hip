@@ -184,21 +184,33 @@ __global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const doubl
     const double t = block_sum_256(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
-// x += dx ; dx = 0
+// x += dx + alpha p (the increment of the last iteration, not yet applied by
+// an operator launch) ; dx = 0 ; followed by k_mx_applied
 __global__ void __launch_bounds__(256) k_mx_fold(int64_t n, double* __restrict__ x, float* __restrict__ dx,
-                                                 const int* __restrict__ done)
+                                                 const float* __restrict__ pA, const float* __restrict__ pB,
+                                                 const double* __restrict__ S, const int* __restrict__ done)
 {
     if (done && *done) return;
+    const float a = float(S[S_ALPHA_DX]);
+    const float* p = S[S_PLAST] != 0.0 ? pA : pB;
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
-        x[i] += double(dx[i]);
+        x[i] += double(fmaf(a, p[i], dx[i]));
         dx[i] = 0.f;
     }
+}
+// the pending increment is in x: the next operator launch adds nothing
+__global__ void k_mx_applied(double* __restrict__ S, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    S[S_ALPHA_DX] = 0.0;
 }
 __global__ void k_mx_init(double tol, double* __restrict__ S, int* __restrict__ flags)
 {
     const double nb = sqrt(S[S_NORMB2]);
     S[S_STOP] = (1.0e-30 < nb ? nb : 1.0e-30) * tol;
     S[S_BETA] = 0.0;
+    S[S_ALPHA_DX] = 0.0;
+    S[S_PLAST] = 0.0;
     flags[0] = 0;
     flags[1] = -1;
 }
@@ -254,7 +266,7 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
                                                const float* __restrict__ u, const float* __restrict__ z,
                                                float* __restrict__ pcur, float* __restrict__ y,
                                                double* __restrict__ partial, const double* __restrict__ S,
-                                               const int* __restrict__ done)
+                                               const int* __restrict__ done, float* __restrict__ dxv, int pid)
 {
     using namespace wk;
     if (FUSED && *done) return;
@@ -267,6 +279,9 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
     const int jx = lane & 31, kh = lane >> 5;
     const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
+    const float alphaDx = FUSED ? float(S[S_ALPHA_DX]) : 0.0f;
+    if (FUSED && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0)
+        const_cast<double*>(S)[S_PLAST] = double(pid);
     // A operands (element matrix pieces), [K step][piece][lane][8]
     bf16x8 A[2][PIECES];
 #pragma unroll
@@ -291,8 +306,12 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
                 const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
                 v = u[gi];
                 if (FUSED) {
+                    const float pp = v;
                     v = fmaf(beta, v, z[gi]);
-                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) pcur[gi] = v;
+                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) {
+                        pcur[gi] = v;
+                        dxv[gi] = fmaf(alphaDx, pp, dxv[gi]);
+                    }
                 }
             }
             sP[q][c][row * PX + lx] = v;
@@ -397,7 +416,8 @@ __global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, c
                                                   const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
                                                   const float* __restrict__ z, float* __restrict__ pcur,
                                                   float* __restrict__ y, double* __restrict__ partial,
-                                                  const double* __restrict__ S, const int* __restrict__ done, int xcd)
+                                                  const double* __restrict__ S, const int* __restrict__ done, int xcd,
+                                                  float* __restrict__ dxv, int pid)
 {
     unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (xcd) {
@@ -407,7 +427,8 @@ __global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, c
         bz = t.z;
     }
     const unsigned slot = (bz * gridDim.y + by) * gridDim.x + bx;
-    kxw32_body<NZ, FUSED, CYT>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, bx, by, bz, slot, gridDim.x);
+    kxw32_body<NZ, FUSED, CYT>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, dxv, pid, bx, by, bz, slot,
+                               gridDim.x);
 }
 
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
@@ -455,13 +476,12 @@ __global__ void __launch_bounds__(64) k_prolong_smooth32(LevelF Lf, LevelF Lc, i
     prolong_smooth32_body(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-__global__ void __launch_bounds__(256) k_xr_rows(int nnx, int NR, const float* __restrict__ p,
-                                                 const float* __restrict__ Ap, float* __restrict__ dx,
+__global__ void __launch_bounds__(256) k_xr_rows(int nnx, int NR, const float* __restrict__ Ap,
                                                  float* __restrict__ r, double* __restrict__ partial,
                                                  const double* __restrict__ S, int cnx, int span,
                                                  float* __restrict__ t, const int* __restrict__ done)
 {
-    xr_rows_body<RPB>(nnx, NR, p, Ap, dx, r, partial, S, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+    xr_rows_body<RPB>(nnx, NR, Ap, r, partial, S, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_rx_rows(int nnx, int NR, const float* __restrict__ r, int cnx, int span,
@@ -553,15 +573,15 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
     if (form == 2)
         hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
-                           nullptr, nullptr, y, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0);
     else if (form == 3)
         hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
-                           nullptr, nullptr, y, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0);
     else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
-                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd);
+                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0);
     else
         hipLaunchKernelGGL((k_kx32<KX_NZ, false>), kx_grid(p->g, KX_NZ), dim3(256), 0, p->stream, p->g,
                            (const float*)s->Ke32, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
@@ -572,23 +592,37 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 
 unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz + RPB - 1) / RPB); }
 
-// the solver's fused operator launch: p = z + beta p_prev, Ap = K p, p.Ap partials
-static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* Ap)
+// the solver's fused operator launch: p = z + beta p_prev, dx += alpha_prev
+// p_prev, Ap = K p, p.Ap partials; pid: 1 if pcur is s->pA
+static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* Ap, int pid)
 {
     t3d_problem* p = s->p;
     if (s->form == 2)
         hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
-                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
+                           s->dx, pid);
     else if (s->form == 3)
         hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
                            (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
-                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags);
+                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
+                           s->dx, pid);
     else
         hipLaunchKernelGGL((k_kxw32<KW_NZ, true, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev,
                            (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
-                           s->kwXcd);
+                           s->kwXcd, s->dx, pid);
+}
+
+// x += dx + pending alpha p ; dx = 0 (skipped for a finished problem when done != nullptr)
+static void launch_fold(t3d_mixed* s, double* x, const int* done)
+{
+    t3d_problem* p = s->p;
+    const int64_t n = 3 * p->g.nN;
+    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
+    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, p->stream, n, x, s->dx, (const float*)s->pA,
+                       (const float*)s->pB, (const double*)s->sc, done);
+    hipLaunchKernelGGL(k_mx_applied, dim3(1), dim3(1), 0, p->stream, s->sc, done);
 }
 
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
@@ -672,7 +706,7 @@ int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it)
     t3d_problem* p = s->p;
     const int64_t n = 3 * p->g.nN;
     const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
-    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, p->stream, n, x, s->dx, (const int*)s->iflags);
+    launch_fold(s, x, (const int*)s->iflags);
     int rc = launch_kxu_f64(p, x, p->wAp, s->iflags);
     if (rc) return rc;
     hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
@@ -703,7 +737,7 @@ int mixed_fold(t3d_mixed* s, double* x)
     t3d_problem* p = s->p;
     const int64_t n = 3 * p->g.nN;
     const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
-    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, p->stream, n, x, s->dx, nullptr);
+    launch_fold(s, x, nullptr);
     T3D_HIP(hipGetLastError());
     return T3D_OK;
 }
@@ -999,7 +1033,7 @@ int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, 
         if (op < 4)
             launch_kx32(s, op, u, y);
         else
-            launch_fused(s, u, s->pB, y);
+            launch_fused(s, u, s->pB, y, 0);
     }
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipEventRecord(e1, p->stream));
@@ -1082,21 +1116,21 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         for (int it = it0; it < it1; ++it) {
             // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
             if (s->timing) T3D_HIP(hipEventRecord(s->tev[2 * (it - it0)], st));
-            launch_fused(s, pPrevOf(it), pCurOf(it), s->Ap);
+            launch_fused(s, pPrevOf(it), pCurOf(it), s->Ap, it & 1);
             if (s->timing) T3D_HIP(hipEventRecord(s->tev[2 * (it - it0) + 1], st));
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, st, (const double*)s->part, nfused, S,
                                (const int*)flags);
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
             // dx += alpha p ; r -= alpha Ap ; rn2 ; convergence test
             hipLaunchKernelGGL(k_xr_rows, dim3(xrb), dim3(256), sizeof(float) * RPB * 3 * L0.nnx, st, L0.nnx,
-                               L0.nny * L0.nnz, (const float*)pCurOf(it), (const float*)s->Ap, s->dx, r, s->part,
+                               L0.nny * L0.nnz, (const float*)s->Ap, r, s->part,
                                (const double*)S, L1.nnx, v.span[1], replace ? nullptr : s->tmp, (const int*)flags);
             hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, xrb, S, flags, it,
                                replace ? 0 : 1, 1);
             if (replace) {
                 // x += dx, then residual replacement with the FP64 operator (the
                 // direction is kept)
-                hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, (const int*)flags);
+                launch_fold(s, x, (const int*)flags);
                 if ((rc = launch_kxu_f64(p, x, Ap64, flags))) return rc;
                 hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r,
                                    s->part, (const int*)flags);
@@ -1124,7 +1158,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         // the residual passed the test at iteration itDone: confirm with the
         // true residual b - K x (FP64); otherwise continue from there
         const int itDone = s->hostFlags[1] - 1;
-        hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, nullptr);
+        launch_fold(s, x, nullptr);
         if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
         hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
                            nullptr);
@@ -1142,7 +1176,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         if ((rc = mixed_vcycle(s, v, r, flexible, 0, false, flags))) return rc;
         it0 = itDone + 1;
     }
-    hipLaunchKernelGGL(k_mx_fold, dim3(fb), dim3(256), 0, st, n, x, s->dx, nullptr);
+    launch_fold(s, x, nullptr);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(st));
     if (iters) *iters = result;

@@ -12,7 +12,12 @@ namespace t3d {
 
 // scalar slots of the mixed solver
 enum {
-    S_NORMB2 = 0, S_STOP = 1, S_RZ = 2, S_ZAP = 4, S_PAP = 5, S_ALPHA = 6, S_BETA = 7, S_RN2 = 8, S_RZNEW = 9
+    S_NORMB2 = 0, S_STOP = 1, S_RZ = 2, S_ZAP = 4, S_PAP = 5, S_ALPHA = 6, S_BETA = 7, S_RN2 = 8, S_RZNEW = 9,
+    // the x increment of an iteration, dx += alpha p, is added by the next
+    // iteration's operator kernel (which reads that p as p_prev) or by the
+    // fold: S_ALPHA_DX is the alpha still to apply (0 after a fold), S_PLAST
+    // which buffer (1: pA, 0: pB) holds the last direction
+    S_ALPHA_DX = 10, S_PLAST = 11
 };
 
 // ---------------------------------------------------------------------------
@@ -164,6 +169,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
                                                const float* __restrict__ z, float* __restrict__ pcur,
                                                float* __restrict__ y, double* __restrict__ partial,
                                                const double* __restrict__ S, const int* __restrict__ done,
+                                               float* __restrict__ dxv, int pid,
     unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_)
 {
     using wk::IN;
@@ -186,6 +192,8 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     // planes, far apart in time)
     const int x0 = bx_ * NX, y0 = by_ * NY, z0 = bz_ * NZ;
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
+    const float alphaDx = FUSED ? float(S[S_ALPHA_DX]) : 0.0f;
+    if (FUSED && slot_ == 0 && tid == 0) const_cast<double*>(S)[S_PLAST] = double(pid);
     const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
     const int X = x0 + tx, Y = y0 + ty;             // the node this thread gathers
@@ -197,8 +205,10 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     // fetched into registers one step ahead (fetch) and written to LDS after
     // the arithmetic (commit), so its loads are in flight meanwhile; the fused
     // form forms p = z + beta * p_prev at the commit and stores the owned
-    // nodes of p.  The voxel modulus and the node's fixed-DOF bits are also
-    // loaded one step ahead, so no step waits on a load it has just issued.
+    // nodes of p, and adds the previous iteration's alpha p_prev to the x
+    // increment dx of the owned nodes.  The voxel modulus and the node's
+    // fixed-DOF bits are also loaded one step ahead, so no step waits on a
+    // load it has just issued.
     uint32_t voff[SPT];        // byte offset of slot j within a node plane
     uint32_t vown[SPT];        // the same for owned nodes (p store), BUF_OOB otherwise
     int slds[SPT];             // LDS index c * PN + row * PX + lx of slot j (a dummy slot if none)
@@ -224,7 +234,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
         const bool zin = Z >= 0 && Z < g.nnz;
         return buf_rsrc(base + (zin ? Z * plane : 0), zin ? planeB : 0u);
     };
-    float fu[SPT], fz[SPT];
+    float fu[SPT], fz[SPT], fd[SPT];
     auto fetch = [&](int Z) {
         const __amdgpu_buffer_rsrc_t ru = plane_rsrc(u, Z);
 #pragma unroll
@@ -233,16 +243,23 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
             const __amdgpu_buffer_rsrc_t rz = plane_rsrc(z, Z);
 #pragma unroll
             for (int j = 0; j < SPT; ++j) fz[j] = buf_ld(rz, voff[j]);
+            // dx of the owned nodes, in the planes this block owns
+            const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
+            const __amdgpu_buffer_rsrc_t rd = buf_rsrc(own ? dxv + Z * plane : dxv, own ? planeB : 0u);
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) fd[j] = buf_ld(rd, vown[j]);
         }
     };
     auto commit = [&](int Z, int q) {
         const bool own = Z >= z0 && Z < z0 + NZ && Z < g.nnz;
         const __amdgpu_buffer_rsrc_t rp = buf_rsrc(FUSED && own ? pcur + Z * plane : pcur, FUSED && own ? planeB : 0u);
+        const __amdgpu_buffer_rsrc_t rd = buf_rsrc(FUSED && own ? dxv + Z * plane : dxv, FUSED && own ? planeB : 0u);
         float* sq = &sP[q][0];
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             float v = fu[j];
             if (FUSED) {
+                buf_st(fmaf(alphaDx, v, fd[j]), rd, vown[j]);
                 v = fmaf(beta, v, fz[j]);
                 buf_st(v, rp, vown[j]);
             }
@@ -379,6 +396,7 @@ __device__ __forceinline__ void fin_alpha_body(const double* __restrict__ partia
     if (threadIdx.x == 0) {
         S[S_PAP] = pAp;
         S[S_ALPHA] = S[S_RZ] / (1.0e-30 < pAp ? pAp : 1.0e-30);
+        S[S_ALPHA_DX] = S[S_ALPHA];
     }
 }
 
@@ -611,11 +629,11 @@ __device__ __forceinline__ void restrict_x_rows(const float* sR, int L, int nnx,
         }
 }
 
-// dx += alpha p ; r -= alpha Ap (FP32 storage, FP64 alpha) ; partial r.r ;
+// r -= alpha Ap (FP32 storage, FP64 alpha) ; partial r.r (the x increment
+// dx += alpha p is applied by the next operator launch or the fold) ;
 // optionally (t != nullptr) the x-pass of the restriction of the new r
 template <int R>
-__device__ __forceinline__ void xr_rows_body(int nnx, int NR, const float* __restrict__ p,
-                                                 const float* __restrict__ Ap, float* __restrict__ dx,
+__device__ __forceinline__ void xr_rows_body(int nnx, int NR, const float* __restrict__ Ap,
                                                  float* __restrict__ r, double* __restrict__ partial,
                                                  const double* __restrict__ S, int cnx, int span,
                                                  float* __restrict__ t, const int* __restrict__ done,
@@ -628,7 +646,6 @@ __device__ __forceinline__ void xr_rows_body(int nnx, int NR, const float* __res
     const int64_t row0 = int64_t(bx_) * R;
     const int nr = min(R, int(NR - row0));
     const double alpha = S[S_ALPHA];
-    const float alphaf = float(alpha);
     double s = 0.0;
     // the block's rows are one contiguous range starting at an even index
     // (R is even), so float2 accesses are aligned
@@ -637,24 +654,18 @@ __device__ __forceinline__ void xr_rows_body(int nnx, int NR, const float* __res
     for (int e = 2 * threadIdx.x; e < ne; e += 512) {
         const int64_t i = base + e;
         if (e + 1 < ne) {
-            const float2 pp = *reinterpret_cast<const float2*>(p + i);
             const float2 aa = *reinterpret_cast<const float2*>(Ap + i);
-            float2 xx = *reinterpret_cast<const float2*>(dx + i);
             float2 rv = *reinterpret_cast<const float2*>(r + i);
-            xx.x = fmaf(alphaf, pp.x, xx.x);
-            xx.y = fmaf(alphaf, pp.y, xx.y);
             const double v0 = fma(-alpha, double(aa.x), double(rv.x));
             const double v1 = fma(-alpha, double(aa.y), double(rv.y));
             rv.x = float(v0);
             rv.y = float(v1);
-            *reinterpret_cast<float2*>(dx + i) = xx;
             *reinterpret_cast<float2*>(r + i) = rv;
             s = fma(v0, v0, s);
             s = fma(v1, v1, s);
             sR[e] = rv.x;
             sR[e + 1] = rv.y;
         } else {
-            dx[i] = fmaf(alphaf, p[i], dx[i]);
             const double v = fma(-alpha, double(Ap[i]), double(r[i]));
             r[i] = float(v);
             s = fma(v, v, s);

@@ -385,7 +385,7 @@ def main():
     fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 4)
     b64 = 8 * (2 * p.nF + p.ne)       # read u, write y (free DOFs), read E
     b32 = 4 * (2 * p.nF + p.ne)
-    bfused = 4 * (4 * p.nF + p.ne)    # read p_prev, z; write p, y; read E
+    bfused = 4 * (6 * p.nF + p.ne)    # read p_prev, z, dx; write p, y, dx; read E
     log("K*u FP64 (plain, bit-exact): %.4f ms/apply, %.0f GB/s algorithmic, %.2f TFLOP/s"
         % (kxu64_ms, b64 / kxu64_ms / 1e6, 1152 * p.ne / kxu64_ms / 1e9))
     log("K*u FP32 Walsh form: %.4f ms/apply, %.0f GB/s algorithmic" % (kxu32_ms, b32 / kxu32_ms / 1e6))
