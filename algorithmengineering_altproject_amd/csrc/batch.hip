@@ -50,9 +50,7 @@ struct BProb {
     const double* LT;
     double* rc64;
     double* xc64;
-    const double* Li;   // inverse coarsest Cholesky factor and its transpose
-    const double* LiT;
-    double* yc64;
+    const double* Ainv;   // coarsest inverse Li^T Li
     const float* Ecell;
     const uint8_t* nodeFixed;
     float *pA, *pB, *Ap, *z, *r, *dx, *tmp;
@@ -149,10 +147,8 @@ __global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P
                      bx, cy, cz, 0, 0, q.norm);
 }
 
-// coarsest solve through the inverse Cholesky factor (coarse.hpp): G blocks
-// per listed problem; UPPER = false: yc64 = Li rc64, true: x = Li^T yc64
-// (masked, FP64 and FP32 copies)
-template <bool UPPER>
+// coarsest solve x = Ainv rc64 (coarse.hpp): G blocks per listed problem,
+// masked, FP64 and FP32 copies
 __global__ void __launch_bounds__(64 * CO_WAVES) kb_coarse_mv(const BProb* __restrict__ P,
                                                               const int* __restrict__ list, unsigned G)
 {
@@ -160,10 +156,7 @@ __global__ void __launch_bounds__(64 * CO_WAVES) kb_coarse_mv(const BProb* __res
     if (*q.flags) return;
     const int gw = int(blockIdx.x % G) * CO_WAVES + int(threadIdx.x) / 64;
     const int L = q.nlev - 1;
-    if (UPPER)
-        coarse_mv_body<true>(q.Nc, q.LiT, q.yc64, q.lv[L].mask, q.xc64, q.lv[L].x, gw, int(G) * CO_WAVES);
-    else
-        coarse_mv_body<false>(q.Nc, q.Li, q.rc64, nullptr, q.yc64, nullptr, gw, int(G) * CO_WAVES);
+    coarse_mv_body<2>(q.Nc, q.Ainv, q.rc64, q.lv[L].mask, q.xc64, q.lv[L].x, gw, int(G) * CO_WAVES);
 }
 
 // coarse-level prolongation + smoothing into level l (l >= 1)
@@ -321,9 +314,7 @@ static int batch_build(t3d_batch* B)
         q.LT = v.LT;
         q.rc64 = s->rc64;
         q.xc64 = s->xc64;
-        q.Li = s->Li;
-        q.LiT = s->LiT;
-        q.yc64 = s->yc64;
+        q.Ainv = s->Ainv;
         T3D_CHECK(v.Nc == s->NcInv, T3D_ERR_STATE, "t3d_batch: inverse coarsest factor not set up");
         q.Ecell = p->EcellF;
         q.nodeFixed = p->nodeFixed;
@@ -448,10 +439,7 @@ static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady)
     }
     if (B->nChol) {
         const dim3 g(unsigned(B->nChol) * B->cholG), blk(64 * CO_WAVES);
-        hipLaunchKernelGGL(kb_coarse_mv<false>, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList,
-                           B->cholG);
-        hipLaunchKernelGGL(kb_coarse_mv<true>, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList,
-                           B->cholG);
+        hipLaunchKernelGGL(kb_coarse_mv, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList, B->cholG);
     }
     for (int l = B->maxLev - 1; l >= 1; --l)
         if (B->nPro[l])

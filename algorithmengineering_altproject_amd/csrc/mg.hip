@@ -568,15 +568,19 @@ __global__ void __launch_bounds__(64) k_mg_tri_inv(int N, const double* __restri
     tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
 }
 
-template <bool UPPER>
+__global__ void __launch_bounds__(256) k_mg_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
+{
+    gram_body(N, Li, Ainv, int(blockIdx.y), int(blockIdx.x));
+}
+
 __global__ void __launch_bounds__(64 * CO_WAVES) k_mg_coarse_mv(int N, const double* __restrict__ M,
                                                                 const double* __restrict__ v,
                                                                 const uint8_t* __restrict__ mask,
                                                                 double* __restrict__ out, const int* __restrict__ done)
 {
     if (done && *done) return;
-    coarse_mv_body<UPPER>(N, M, v, mask, out, nullptr, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
-                          int(gridDim.x) * CO_WAVES);
+    coarse_mv_body<2>(N, M, v, mask, out, nullptr, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
+                      int(gridDim.x) * CO_WAVES);
 }
 
 }  // namespace t3d
@@ -604,9 +608,9 @@ struct t3d_mg {
     double* Kel[8] = {};      // element matrices of levels >= 1, [cells][576]
     double* Wch[8] = {};      // child interpolation weights of levels >= 1, [span^3][64]
     double* Gch = nullptr;    // T_f^T Ke T_f per finest cell of a level-1 cell, [span^3][576]
-    double* Li = nullptr;     // inverse coarsest factor and its transpose (FP64 V-cycle)
-    double* LiT = nullptr;
-    double* yc = nullptr;
+    double* Li = nullptr;     // inverse coarsest factor L^-1
+    double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles
+    double* yc = nullptr;     // device flag of the factorisation (after N doubles)
 };
 
 namespace t3d {
@@ -632,11 +636,9 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
                            (last && !chol) ? g->D[L] : nullptr, g->x[L], done, g->mode == 0 ? 1 : 0);
         T3D_HIP(hipGetLastError());
     }
-    if (g->mode == 1) {   // x = Li^T (Li r), masked
-        hipLaunchKernelGGL(k_mg_coarse_mv<false>, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
-                           (const double*)g->Li, (const double*)g->r[L], (const uint8_t*)nullptr, g->yc, done);
-        hipLaunchKernelGGL(k_mg_coarse_mv<true>, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
-                           (const double*)g->LiT, (const double*)g->yc, (const uint8_t*)g->mask[L], g->x[L], done);
+    if (g->mode == 1) {   // x = Ainv r = Li^T Li r, masked
+        hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
+                           (const double*)g->Ainv, (const double*)g->r[L], (const uint8_t*)g->mask[L], g->x[L], done);
         T3D_HIP(hipGetLastError());
     } else if (chol) {
         hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(CHOL_MAX), 0, st, g->Nc, g->L, g->LT, g->r[L], g->mask[L],
@@ -669,8 +671,7 @@ struct MGView {
     int Nc;
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
-    const double* Li;    // inverse coarsest factor and transpose when the multigrid keeps them
-    const double* LiT;
+    const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
 };
 int mg_view(const t3d_mg* g, MGView& v)
 {
@@ -688,8 +689,7 @@ int mg_view(const t3d_mg* g, MGView& v)
     v.Nc = g->Nc;
     v.weight = g->H.weight;
     v.norm = g->mode == 0 ? 1 : 0;
-    v.Li = g->mode == 1 ? g->Li : nullptr;
-    v.LiT = g->mode == 1 ? g->LiT : nullptr;
+    v.Ainv = g->mode == 1 ? g->Ainv : nullptr;
     return T3D_OK;
 }
 }  // namespace t3d
@@ -819,10 +819,10 @@ static int mg_galerkin_setup(t3d_mg* g)
     for (int i = 0; i < N; ++i)
         for (int j = i + 1; j < N; ++j) A[size_t(i) * N + j] = 0.0;
     const size_t n2 = size_t(N) * N;
-    if (!g->L) {
-        T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
+    if (!g->L) T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
+    if (!g->Li) {
         T3D_HIP(hipMalloc(&g->Li, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->LiT, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->Ainv, sizeof(double) * n2));
         T3D_HIP(hipMalloc(&g->yc, sizeof(double) * (size_t(N) + 1)));
     }
     int* fail = reinterpret_cast<int*>(g->yc + N);
@@ -848,9 +848,10 @@ static int mg_galerkin_setup(t3d_mg* g)
         return T3D_ERR_STATE;
     }
     T3D_HIP(hipMemsetAsync(g->Li, 0, sizeof(double) * n2, st));
-    T3D_HIP(hipMemsetAsync(g->LiT, 0, sizeof(double) * n2, st));
     hipLaunchKernelGGL(k_mg_tri_inv, dim3(unsigned((N + 63) / 64)), dim3(64), 0, st, N, (const double*)g->L, g->Li,
-                       g->LiT);
+                       nullptr);
+    const unsigned nt = unsigned((N + 63) / 64);
+    hipLaunchKernelGGL(k_mg_gram, dim3(nt, nt), dim3(256), 0, st, N, (const double*)g->Li, g->Ainv);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(st));
     g->Nc = N;
@@ -877,7 +878,7 @@ int t3d_mg_destroy(t3d_mg* g)
     }
     if (g->Gch) (void)hipFree(g->Gch);
     if (g->Li) (void)hipFree(g->Li);
-    if (g->LiT) (void)hipFree(g->LiT);
+    if (g->Ainv) (void)hipFree(g->Ainv);
     if (g->yc) (void)hipFree(g->yc);
     if (g->L) (void)hipFree(g->L);
     if (g->LT) (void)hipFree(g->LT);

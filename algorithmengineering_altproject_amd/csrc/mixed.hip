@@ -510,16 +510,22 @@ __global__ void __launch_bounds__(64) k_tri_inv(int N, const double* __restrict_
     tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
 }
 
-// one triangular product of the coarsest solve (coarse.hpp)
-template <bool UPPER>
+// Ainv = Li^T Li (coarse.hpp)
+__global__ void __launch_bounds__(256) k_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
+{
+    gram_body(N, Li, Ainv, int(blockIdx.y), int(blockIdx.x));
+}
+
+// the coarsest solve's product (coarse.hpp)
+template <int TRI>
 __global__ void __launch_bounds__(64 * CO_WAVES) k_coarse_mv(int N, const double* __restrict__ M,
                                                              const double* __restrict__ v,
                                                              const uint8_t* __restrict__ mask, double* __restrict__ out,
                                                              float* __restrict__ out32, const int* __restrict__ done)
 {
     if (done && *done) return;
-    coarse_mv_body<UPPER>(N, M, v, mask, out, out32, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
-                          int(gridDim.x) * CO_WAVES);
+    coarse_mv_body<TRI>(N, M, v, mask, out, out32, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
+                        int(gridDim.x) * CO_WAVES);
 }
 
 }  // namespace t3d
@@ -529,15 +535,13 @@ using namespace t3d;
 #include "mixed_internal.hpp"
 
 namespace t3d {
-// xc64 (and x32, if given) = Li^T Li rc64, masked
+// xc64 (and x32, if given) = Ainv rc64 = Li^T Li rc64, masked
 static void launch_coarse(t3d_mixed* s, const uint8_t* mask, float* x32, const int* done)
 {
     hipStream_t st = s->p->stream;
     const int N = s->NcInv;
     const dim3 g(coarse_blocks(N)), blk(64 * CO_WAVES);
-    hipLaunchKernelGGL(k_coarse_mv<false>, g, blk, 0, st, N, (const double*)s->Li, (const double*)s->rc64,
-                       (const uint8_t*)nullptr, s->yc64, (float*)nullptr, done);
-    hipLaunchKernelGGL(k_coarse_mv<true>, g, blk, 0, st, N, (const double*)s->LiT, (const double*)s->yc64, mask,
+    hipLaunchKernelGGL(k_coarse_mv<2>, g, blk, 0, st, N, (const double*)s->Ainv, (const double*)s->rc64, mask,
                        s->xc64, x32, done);
 }
 
@@ -769,7 +773,7 @@ int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
     void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64,
-                 s->ownLi ? s->Li : nullptr, s->ownLi ? s->LiT : nullptr, s->yc64,
+                 s->Li, s->ownAinv ? s->Ainv : nullptr, s->yc64,
                  s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
@@ -940,30 +944,26 @@ int t3d_mixed_setup(t3d_mixed* s)
     // inverse of the coarsest Cholesky factor (the coarsest size is fixed by
     // the hierarchy; the factor exists only when the Galerkin matrix is SPD)
     s->NcInv = 0;
-    if (v.Nc > 0 && v.Li) {   // the multigrid object keeps the inverse factor
-        if (s->ownLi) {
-            if (s->Li) (void)hipFree(s->Li);
-            if (s->LiT) (void)hipFree(s->LiT);
-        }
-        s->Li = const_cast<double*>(v.Li);
-        s->LiT = const_cast<double*>(v.LiT);
-        s->ownLi = false;
-        if (!s->yc64) T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
+    if (v.Nc > 0 && v.Ainv) {   // the multigrid object keeps the coarsest inverse
+        if (s->ownAinv && s->Ainv) (void)hipFree(s->Ainv);
+        if (s->Li) (void)hipFree(s->Li);
+        s->Li = nullptr;
+        s->Ainv = const_cast<double*>(v.Ainv);
+        s->ownAinv = false;
         s->NcInv = v.Nc;
     } else if (v.Nc > 0) {
         const size_t n2 = size_t(v.Nc) * size_t(v.Nc);
-        if (!s->ownLi) {
-            s->Li = s->LiT = nullptr;
-            s->ownLi = true;
+        if (!s->ownAinv) {
+            s->Ainv = nullptr;
+            s->ownAinv = true;
         }
-        if (!s->Li) {
-            T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
-            T3D_HIP(hipMalloc(&s->LiT, sizeof(double) * n2));
-        }
-        if (!s->yc64) T3D_HIP(hipMalloc(&s->yc64, sizeof(double) * size_t(v.Nc)));
+        if (!s->Li) T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
+        if (!s->Ainv) T3D_HIP(hipMalloc(&s->Ainv, sizeof(double) * n2));
         T3D_HIP(hipMemsetAsync(s->Li, 0, sizeof(double) * n2, st));
-        T3D_HIP(hipMemsetAsync(s->LiT, 0, sizeof(double) * n2, st));
-        hipLaunchKernelGGL(k_tri_inv, dim3(unsigned((v.Nc + 63) / 64)), dim3(64), 0, st, v.Nc, v.L, s->Li, s->LiT);
+        hipLaunchKernelGGL(k_tri_inv, dim3(unsigned((v.Nc + 63) / 64)), dim3(64), 0, st, v.Nc, v.L, s->Li,
+                           (double*)nullptr);
+        const unsigned nt = unsigned((v.Nc + 63) / 64);
+        hipLaunchKernelGGL(k_gram, dim3(nt, nt), dim3(256), 0, st, v.Nc, (const double*)s->Li, s->Ainv);
         T3D_HIP(hipGetLastError());
         s->NcInv = v.Nc;
     }

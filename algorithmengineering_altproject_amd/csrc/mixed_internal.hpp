@@ -20,9 +20,9 @@ struct t3d_mixed {
     float* tmp = nullptr;      // restriction pass-1 buffer
     double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
     double* xc64 = nullptr;
-    double* Li = nullptr;      // inverse Cholesky factor L^-1 (row-major, lower) and its transpose
-    double* LiT = nullptr;
-    bool ownLi = true;         // false: Li/LiT are the multigrid object's (Galerkin form)
+    double* Li = nullptr;      // inverse Cholesky factor L^-1 (row-major, lower; set-up only)
+    double* Ainv = nullptr;    // Li^T Li, the coarsest inverse applied per V-cycle
+    bool ownAinv = true;       // false: Ainv is the multigrid object's (Galerkin form)
     double* yc64 = nullptr;    // Li * rc64
     int NcInv = 0;             // size of Li (0: diagonal coarsest level)
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
@@ -54,8 +54,7 @@ struct MGView {
     int Nc;
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
-    const double* Li;    // inverse coarsest factor and transpose when the multigrid keeps them
-    const double* LiT;
+    const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
 };
 int mg_view(const t3d_mg* g, MGView& v);
 __global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,
