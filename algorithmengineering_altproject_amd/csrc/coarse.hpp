@@ -21,7 +21,7 @@
 namespace t3d {
 
 constexpr int CO_WAVES = 4;   // wavefronts per workgroup of the products
-constexpr int CO_ROWS = 2;    // rows per wavefront (grid size = ceil(N / (CO_WAVES * CO_ROWS)))
+constexpr int CO_ROWS = 1;    // rows per wavefront (grid size = ceil(N / (CO_WAVES * CO_ROWS)))
 
 inline unsigned coarse_blocks(int N)
 {
@@ -105,15 +105,16 @@ __device__ __forceinline__ void coarse_mv_body(int N, const double* __restrict__
         const double* Mi = M + size_t(i) * N;
         const int k0 = TRI == 1 ? i : 0;
         const int k1 = TRI == 0 ? i + 1 : N;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int u = 0; k0 + u * 64 < k1; u += 4) {
+        // eight independent partial sums, so a wavefront has eight row loads in flight
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; k0 + u * 64 < k1; u += 8) {
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
+            for (int w = 0; w < 8; ++w) {
                 const int k = k0 + (u + w) * 64 + lane;
                 if (k < k1) acc[w] = fma(Mi[k], v[k], acc[w]);
             }
         }
-        double a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
         if (lane == 0) {

@@ -490,13 +490,15 @@ __global__ void __launch_bounds__(256) k_rx_rows(int nnx, int NR, const float* _
     rx_rows_body<RPB>(nnx, NR, r, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
+template <int U>
 __global__ void __launch_bounds__(256) k_pro_rows(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                   const float* __restrict__ rf, const float* __restrict__ Df,
                                                   const uint8_t* __restrict__ fmask, float weight,
                                                   float* __restrict__ xf, const float* __restrict__ Ap,
                                                   double* __restrict__ partial, const int* __restrict__ done)
 {
-    pro_rows_body<RPB>(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, Ap, partial, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+    pro_rows_body<RPB, U>(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, Ap, partial, done, blockIdx.x, blockIdx.y,
+                          blockIdx.z, linear_block(), gridDim.x);
 }
 
 __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -675,7 +677,13 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
     }
     const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
     const unsigned nb = row_blocks(L0);
-    hipLaunchKernelGGL(k_pro_rows, dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0, L1, v.span[1],
+    // positions per pass: 5 chunks cover rows of up to 426 nodes in one pass
+    if (3 * L0.nnx <= 256 * 5)
+        hipLaunchKernelGGL(k_pro_rows<5>, dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0, L1, v.span[1],
+                       (const float*)s->x32[1], r, (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap,
+                       s->part, done);
+    else
+        hipLaunchKernelGGL(k_pro_rows<8>, dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0, L1, v.span[1],
                        (const float*)s->x32[1], r, (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap,
                        s->part, done);
     hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, nb, s->sc, flexible, first,

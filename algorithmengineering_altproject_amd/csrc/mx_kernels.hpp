@@ -472,14 +472,18 @@ __device__ __forceinline__ void restrict_x_body(LevelF Lf, int cnx, int span, co
 
 // restriction pass 2 (y, z), normalisation by the in-range weight sum, coarse
 // mask; at the coarsest level optionally the diagonal solve xc = rc / D or an
-// FP64 copy of rc for the Cholesky solve (transfers.cpp:449-533)
-__device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
+// FP64 copy of rc for the Cholesky solve (transfers.cpp:449-533).  SP > 0: the
+// span as a compile-time constant, taps unrolled with buffer loads (a tap
+// outside the fine grid loads 0 with weight 0, which leaves every sum exactly
+// as the skipping loop of SP = 0 leaves it), so all of a thread's loads are in
+// flight together.
+template <int SP>
+__device__ __forceinline__ void restrict_yz_t(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
                                                     const float* __restrict__ Dc, float* __restrict__ xc,
-                                                    double* __restrict__ rcd, const int* __restrict__ done,
-    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_, int norm = 1)
+                                                    double* __restrict__ rcd, int norm, unsigned bx_, unsigned by_,
+                                                    unsigned bz_)
 {
-    if (done && *done) return;
     const int cx = bx_ * 64 + threadIdx.x;
     if (cx >= Lc.nnx) return;
     const int cy = by_, cz = bz_;
@@ -491,19 +495,50 @@ __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span,
         if (fx >= 0 && fx < Lf.nnx) wxs += 1.0f - float(d < 0 ? -d : d) * inv;
     }
     float v0 = 0.f, v1 = 0.f, v2 = 0.f, wyz = 0.f;
-    for (int dz = -(span - 1); dz <= span - 1; ++dz) {
-        const int fz = cz * span + dz;
-        if (fz < 0 || fz >= Lf.nnz) continue;
-        const float wz = 1.0f - float(dz < 0 ? -dz : dz) * inv;
-        for (int dy = -(span - 1); dy <= span - 1; ++dy) {
-            const int fy = cy * span + dy;
-            if (fy < 0 || fy >= Lf.nny) continue;
-            const float w = wz * (1.0f - float(dy < 0 ? -dy : dy) * inv);
-            const int64_t i = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
-            v0 = fmaf(w, t[i], v0);
-            v1 = fmaf(w, t[i + 1], v1);
-            v2 = fmaf(w, t[i + 2], v2);
-            wyz += w;
+    constexpr int NTAP = SP > 0 ? 2 * SP - 1 : 1;
+    if constexpr (SP > 0) {
+        const __amdgpu_buffer_rsrc_t rt =
+            buf_rsrc(t, uint32_t(4 * 3 * int64_t(Lf.nnz) * Lf.nny * cnx));
+#pragma unroll
+        for (int dz = -(SP - 1); dz <= SP - 1; ++dz) {
+            const int fz = cz * SP + dz;
+            const bool zin = fz >= 0 && fz < Lf.nnz;
+            const float wz = 1.0f - float(dz < 0 ? -dz : dz) / float(SP);
+            float a0[NTAP], a1[NTAP], a2[NTAP], ww[NTAP];
+#pragma unroll
+            for (int dy = -(SP - 1); dy <= SP - 1; ++dy) {
+                const int fy = cy * SP + dy;
+                const bool in = zin && fy >= 0 && fy < Lf.nny;
+                const uint32_t off = in ? uint32_t(4 * 3 * ((fz * Lf.nny + fy) * cnx + cx)) : BUF_OOB;
+                const int q = dy + SP - 1;
+                a0[q] = buf_ld(rt, off);
+                a1[q] = buf_ld(rt, off + 4);
+                a2[q] = buf_ld(rt, off + 8);
+                ww[q] = in ? wz * (1.0f - float(dy < 0 ? -dy : dy) / float(SP)) : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < NTAP; ++q) {
+                v0 = fmaf(ww[q], a0[q], v0);
+                v1 = fmaf(ww[q], a1[q], v1);
+                v2 = fmaf(ww[q], a2[q], v2);
+                wyz += ww[q];
+            }
+        }
+    } else {
+        for (int dz = -(span - 1); dz <= span - 1; ++dz) {
+            const int fz = cz * span + dz;
+            if (fz < 0 || fz >= Lf.nnz) continue;
+            const float wz = 1.0f - float(dz < 0 ? -dz : dz) * inv;
+            for (int dy = -(span - 1); dy <= span - 1; ++dy) {
+                const int fy = cy * span + dy;
+                if (fy < 0 || fy >= Lf.nny) continue;
+                const float w = wz * (1.0f - float(dy < 0 ? -dy : dy) * inv);
+                const int64_t i = 3 * ((int64_t(fz) * Lf.nny + fy) * cnx + cx);
+                v0 = fmaf(w, t[i], v0);
+                v1 = fmaf(w, t[i + 1], v1);
+                v2 = fmaf(w, t[i + 2], v2);
+                wyz += w;
+            }
         }
     }
     const float ws = wxs * wyz;
@@ -536,6 +571,21 @@ __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span,
         xc[3 * n + 1] = (m & 2u) ? 0.f : r1 * Dc[3 * n + 1];
         xc[3 * n + 2] = (m & 4u) ? 0.f : r2 * Dc[3 * n + 2];
     }
+}
+
+__device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
+                                                    const uint8_t* __restrict__ cmask, float* __restrict__ rc,
+                                                    const float* __restrict__ Dc, float* __restrict__ xc,
+                                                    double* __restrict__ rcd, const int* __restrict__ done,
+    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_, int norm = 1)
+{
+    if (done && *done) return;
+    if (span == 4)
+        restrict_yz_t<4>(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, norm, bx_, by_, bz_);
+    else if (span == 2)
+        restrict_yz_t<2>(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, norm, bx_, by_, bz_);
+    else
+        restrict_yz_t<0>(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, norm, bx_, by_, bz_);
 }
 
 // trilinear prolongation of xc at fine node (fx, fy, fz), clamped at the
@@ -701,7 +751,7 @@ __device__ __forceinline__ void rx_rows_body(int nnx, int NR, const float* __res
 // finest level: z = mask ? 0 : 2t + P(x_c), t = w r/D,
 // with the partials of r.z and z.Ap (two per block).  The y/z part of the
 // trilinear prolongation is evaluated once per row and coarse x in LDS.
-template <int R>
+template <int R, int U = 8>
 __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                   const float* __restrict__ rf, const float* __restrict__ Df,
                                                   const uint8_t* __restrict__ fmask, float weight,
@@ -741,8 +791,9 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
     // loads are buffer loads over exactly that row (positions k >= L load 0
     // and their stores are dropped), so they carry no branch and are all
     // issued before the row's arithmetic.  Within a chunk the partial sums run
-    // over rows, then positions, as one pass over the block's rows.
-    constexpr int U = 8;
+    // over rows, then positions, as one pass over the block's rows.  U: chunks
+    // of 256 positions per pass (a row of 3 nnx positions takes ceil(3 nnx /
+    // (256 U)) passes; the chunk positions beyond the row load nothing).
     double srz = 0.0, szap = 0.0;
     for (int k0 = 0; k0 < L; k0 += 256 * U) {
         int qo[U], cc[U];
