@@ -380,7 +380,7 @@ class Params(C.Structure):
         ("E0", C.c_float), ("Emin", C.c_float), ("nu", C.c_float), ("penal", C.c_float), ("cgTol", C.c_float),
         ("cgMaxIt", C.c_int), ("V0", C.c_float), ("nLoop", C.c_int), ("rMin", C.c_float),
         ("mgMaxLevels", C.c_int), ("mgWeight", C.c_float), ("nonDyadic", C.c_int), ("NlimitDofs", C.c_int),
-        ("rebuildEvery", C.c_int), ("precision", C.c_int),
+        ("rebuildEvery", C.c_int), ("precision", C.c_int), ("mgMode", C.c_int),
     ]
 
 

@@ -57,6 +57,138 @@ __device__ __forceinline__ void tri_inv_body(int N, const double* __restrict__ L
     }
 }
 
+// broadcast of lane l's double to the wavefront (l uniform)
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b & 0xffffffffll), l));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), l));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Columns [j0, j0 + NC) of Li = L^-1 (L row-major N x N, lower; its upper part
+// is never read) by column-oriented forward substitution in 32-row blocks, one
+// 256-thread workgroup per column group.  Thread t owns rows t, t + 256, ...,
+// t + 256 (R - 1) and keeps their entries of the NC columns in registers
+// (x = e_j at the start).  For each block from the one holding j0:
+//   * the wavefront owning the block's 32 rows (lanes l0 .. l0 + 31) solves the
+//     diagonal block: y_k = x_k / L[k][k] broadcast by readlane, then
+//     x_i -= L[i][k] y_k for the block's later rows, in k order;
+//   * the block's y go to LDS (double-buffered: one barrier per block);
+//   * every thread subtracts L[i][k] y_k, k over the block in order, from its
+//     rows below the block, reading its own 32-entry segment of row i.
+// Rows above j0 stay zero, so all N entries of each column are written (the
+// zero upper part of Li included).  Requires N <= 256 R.  The sums run in a
+// different order from tri_inv_body; both are FP64 forward substitutions.
+template <int R, int NC>
+__device__ __forceinline__ void tri_inv_blk_body(int N, const double* __restrict__ L, double* __restrict__ Li,
+                                                 int j0)
+{
+    __shared__ double ys[2][32][NC];
+    const int t = int(threadIdx.x), lane = t & 63, wave = t >> 6;
+    double x[R][NC];
+#pragma unroll
+    for (int m = 0; m < R; ++m)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) x[m][c] = (t + 256 * m == j0 + c) ? 1.0 : 0.0;
+    int buf = 0;
+    for (int k0 = j0 & ~31; k0 < N; k0 += 32, buf ^= 1) {
+        const int mk = k0 >> 8, wk = (k0 & 255) >> 6, l0 = k0 & 63;
+        if (wave == wk) {
+            const int r = lane - l0, i = k0 + r;
+            const bool blk = r >= 0 && r < 32;
+            const bool own = blk && i < N;
+            double xv[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) xv[c] = 0.0;
+#pragma unroll
+            for (int m = 0; m < R; ++m)
+                if (m == mk)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) xv[c] = x[m][c];
+            const double* Lr = L + size_t(own ? i : 0) * N + k0;
+            double lr[32];
+#pragma unroll
+            for (int s = 0; s < 32; ++s) lr[s] = (own && s < r) ? Lr[s] : 0.0;   // k0 + s < i < N
+            const double dinv = own ? 1.0 / Lr[r] : 0.0;
+#pragma unroll
+            for (int s = 0; s < 32; ++s) {
+                if (k0 + s < N) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const double y = readlane_f64(xv[c] * dinv, l0 + s);
+                        if (r == s) xv[c] = y;
+                        else if (r > s) xv[c] = fma(-lr[s], y, xv[c]);
+                    }
+                }
+            }
+            if (blk) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) ys[buf][r][c] = own ? xv[c] : 0.0;
+#pragma unroll
+                for (int m = 0; m < R; ++m)
+                    if (m == mk)
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) x[m][c] = xv[c];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            if (256 * m + 255 < k0 + 32) continue;   // every row of this slot is above the block's end
+            const int i = t + 256 * m;
+            if (i >= k0 + 32 && i < N) {
+                const double* Lr = L + size_t(i) * N + k0;
+                double a[32];
+#pragma unroll
+                for (int s = 0; s < 32; ++s) a[s] = Lr[s];
+#pragma unroll
+                for (int s = 0; s < 32; ++s)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) x[m][c] = fma(-a[s], ys[buf][s][c], x[m][c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int i = t + 256 * m;
+        if (i < N)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (j0 + c < N) Li[size_t(i) * N + j0 + c] = x[m][c];
+    }
+}
+
+template <int R, int NC>
+static __global__ void __launch_bounds__(256) k_tri_inv_blk(int N, const double* __restrict__ L,
+                                                            double* __restrict__ Li)
+{
+    tri_inv_blk_body<R, NC>(N, L, Li, int(blockIdx.x) * NC);
+}
+
+static __global__ void __launch_bounds__(64) k_tri_inv_cols(int N, const double* __restrict__ L, double* Li)
+{
+    tri_inv_body(N, L, Li, nullptr, int(blockIdx.x) * 64, int(threadIdx.x));
+}
+
+// Li = L^-1 on stream st: the blocked form up to N = 8192, else one thread per
+// column (Li cleared first, as that form writes only the lower part)
+static inline hipError_t launch_tri_inv(hipStream_t st, int N, const double* L, double* Li)
+{
+    if (N <= 2048)
+        hipLaunchKernelGGL((k_tri_inv_blk<8, 4>), dim3(unsigned((N + 3) / 4)), dim3(256), 0, st, N, L, Li);
+    else if (N <= 4096)
+        hipLaunchKernelGGL((k_tri_inv_blk<16, 4>), dim3(unsigned((N + 3) / 4)), dim3(256), 0, st, N, L, Li);
+    else if (N <= 8192)
+        hipLaunchKernelGGL((k_tri_inv_blk<32, 2>), dim3(unsigned((N + 1) / 2)), dim3(256), 0, st, N, L, Li);
+    else {
+        const hipError_t e = hipMemsetAsync(Li, 0, sizeof(double) * size_t(N) * size_t(N), st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_tri_inv_cols, dim3(unsigned((N + 63) / 64)), dim3(64), 0, st, N, L, Li);
+    }
+    return hipGetLastError();
+}
+
 // Ainv = Li^T Li for the lower-triangular Li (row-major N x N, zero above the
 // diagonal): the coarsest solve becomes one dense product.  Output tile
 // (ti, tj) of 64 x 64 per 256-thread workgroup, 4 x 4 entries per thread,

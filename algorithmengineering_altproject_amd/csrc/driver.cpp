@@ -39,7 +39,7 @@ void t3d_params_default(t3d_params* P)
     P->E0 = 1.0f; P->Emin = 1.0e-6f; P->nu = 0.3f; P->penal = 3.0f; P->cgTol = 1.0e-3f;
     P->cgMaxIt = 800; P->V0 = 0.12f; P->nLoop = 30; P->rMin = std::sqrt(3.0f);
     P->mgMaxLevels = 5; P->mgWeight = 0.6f; P->nonDyadic = 1; P->NlimitDofs = 1000;
-    P->rebuildEvery = 5; P->precision = 0;
+    P->rebuildEvery = 5; P->precision = 0; P->mgMode = 0;
 }
 
 // from filter.hip
@@ -53,6 +53,8 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     T3D_CHECK(P->penal == 3.0f, T3D_ERR_ARG, "t3d_top3d_run: only penal = 3 is supported (powf(rho,3) rounding)");
     T3D_CHECK(P->precision == 0 || P->precision == 1, T3D_ERR_ARG,
               "t3d_top3d_run: precision must be 0 (FP64 plain path) or 1 (mixed-precision solve)");
+    T3D_CHECK(P->mgMode == 0 || P->mgMode == 1, T3D_ERR_ARG,
+              "t3d_top3d_run: mgMode must be 0 (the C++ port's V-cycle) or 1 (Galerkin multigrid)");
     if (loopsRun) *loopsRun = 0;
     t3d_problem* p = nullptr;
     t3d_mg* M = nullptr;
@@ -75,6 +77,10 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     } guard{p, M, F, MX};
     if ((rc = t3d_filter_create(p, P->rMin, &F))) return rc;
     if ((rc = t3d_mg_create(p, P->nonDyadic, P->mgMaxLevels, P->mgWeight, P->Emin, P->NlimitDofs, &M))) return rc;
+    if (P->mgMode == 1 && (rc = t3d_mg_set_mode(M, 1))) return rc;
+    // residual replacement period of the mixed solver: the Galerkin multigrid's
+    // residual falls ~1e3 in 25 iterations, where the FP32 recursion has drifted
+    const int trueEvery = P->mgMode == 1 ? 25 : 50;
     // precision 1: the solves run in the mixed-precision PCG (FP32 operator and
     // V-cycle, flexible FP64 outer loop, true-residual stopping); everything
     // else is the plain path
@@ -87,7 +93,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         return r;
     };
     auto solve = [&](double* ud, int* its) {
-        return mixed ? t3d_pcg_mixed(MX, b_for_solve, ud, double(P->cgTol), P->cgMaxIt, 50, 1, its)
+        return mixed ? t3d_pcg_mixed(MX, b_for_solve, ud, double(P->cgTol), P->cgMaxIt, trueEvery, 1, its)
                      : t3d_pcg(p, M, b_for_solve, ud, double(P->cgTol), P->cgMaxIt, its);
     };
     const int ne = p->ne;
@@ -139,7 +145,9 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         ++loop;
         if ((rc = launch_simp(p, x, P->E0, P->Emin, E))) return rc;
         const double tS = now_s();
-        if (loop == 1 || (loop % P->rebuildEvery) == 0) {
+        // the Galerkin hierarchy encodes the moduli, so it is reassembled every loop, as the
+        // MATLAB original does (matlab/TOP3D_XL.m:122); the C++ port's V-cycle every rebuildEvery-th
+        if (P->mgMode == 1 || loop == 1 || (loop % P->rebuildEvery) == 0) {
             if ((rc = setup(E))) return rc;
         } else {
             if ((rc = t3d_set_modulus(p, E))) return rc;

@@ -563,11 +563,6 @@ __global__ void __launch_bounds__(256) k_chol_update(int N, double* __restrict__
 }
 
 // coarsest solve of the Galerkin form through the inverse factor (coarse.hpp)
-__global__ void __launch_bounds__(64) k_mg_tri_inv(int N, const double* __restrict__ L, double* Li, double* LiT)
-{
-    tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
-}
-
 __global__ void __launch_bounds__(256) k_mg_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
 {
     gram_body(N, Li, Ainv, int(blockIdx.y), int(blockIdx.x));
@@ -847,9 +842,7 @@ static int mg_galerkin_setup(t3d_mg* g)
         set_error("Galerkin multigrid: the coarsest matrix is not positive definite");
         return T3D_ERR_STATE;
     }
-    T3D_HIP(hipMemsetAsync(g->Li, 0, sizeof(double) * n2, st));
-    hipLaunchKernelGGL(k_mg_tri_inv, dim3(unsigned((N + 63) / 64)), dim3(64), 0, st, N, (const double*)g->L, g->Li,
-                       nullptr);
+    T3D_HIP(launch_tri_inv(st, N, g->L, g->Li));
     const unsigned nt = unsigned((N + 63) / 64);
     hipLaunchKernelGGL(k_mg_gram, dim3(nt, nt), dim3(256), 0, st, N, (const double*)g->Li, g->Ainv);
     T3D_HIP(hipGetLastError());

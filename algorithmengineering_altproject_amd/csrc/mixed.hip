@@ -506,12 +506,6 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
     d2f_body(n, a, b, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// inverse Cholesky factor, 64 columns per workgroup (coarse.hpp)
-__global__ void __launch_bounds__(64) k_tri_inv(int N, const double* __restrict__ L, double* Li, double* LiT)
-{
-    tri_inv_body(N, L, Li, LiT, int(blockIdx.x) * 64, int(threadIdx.x));
-}
-
 // Ainv = Li^T Li (coarse.hpp)
 __global__ void __launch_bounds__(256) k_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
 {
@@ -967,9 +961,7 @@ int t3d_mixed_setup(t3d_mixed* s)
         }
         if (!s->Li) T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
         if (!s->Ainv) T3D_HIP(hipMalloc(&s->Ainv, sizeof(double) * n2));
-        T3D_HIP(hipMemsetAsync(s->Li, 0, sizeof(double) * n2, st));
-        hipLaunchKernelGGL(k_tri_inv, dim3(unsigned((v.Nc + 63) / 64)), dim3(64), 0, st, v.Nc, v.L, s->Li,
-                           (double*)nullptr);
+        T3D_HIP(launch_tri_inv(st, v.Nc, v.L, s->Li));
         const unsigned nt = unsigned((v.Nc + 63) / 64);
         hipLaunchKernelGGL(k_gram, dim3(nt, nt), dim3(256), 0, st, v.Nc, (const double*)s->Li, s->Ainv);
         T3D_HIP(hipGetLastError());

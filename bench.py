@@ -357,7 +357,7 @@ def main():
     mx.set_operator(form)
     log("MG: levels %s, coarsest Cholesky N=%d (setup %.2fs)" % (M.dims, M.info()[0], time.time() - t0))
     Mg = mxg = None
-    gal_setup_s = None
+    gal_setup_s = gal_resetup_s = None
     if galerkin:
         t0 = time.time()
         Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
@@ -366,7 +366,13 @@ def main():
         mxg.setup()
         mxg.set_operator(form)
         gal_setup_s = time.time() - t0
-        log("Galerkin MG: coarsest Cholesky N=%d (set-up %.2fs)" % (Mg.info()[0], gal_setup_s))
+        # a repeated set-up (buffers allocated): what an optimisation loop pays per design
+        t0 = time.time()
+        Mg.setup(dE)
+        mxg.setup()
+        gal_resetup_s = time.time() - t0
+        log("Galerkin MG: coarsest Cholesky N=%d (first set-up %.2fs, repeated %.3fs)"
+            % (Mg.info()[0], gal_setup_s, gal_resetup_s))
 
     # ---- K*u blocks: HIP events around back-to-back launches on the problem stream
     uh = np.random.default_rng(0).uniform(-1, 1, p.nF)
@@ -383,6 +389,11 @@ def main():
     kxum6_ms = mx.time_kxu(u32, y32, args.kxu_reps, 3)
     mx.time_kxu(u32, y32, 3, 4)
     fused_ms = mx.time_kxu(u32, y32, args.kxu_reps, 4)
+    # streaming-copy rate of this GPU (FP64 vector copy, 2 x 8 B per DOF): boxes differ by up to ~20 %
+    p.time_op(1, du, dy, 3)
+    copy_ms = p.time_op(1, du, dy, args.kxu_reps)
+    copy_gbps = 2 * 8 * p.nDOF / copy_ms / 1e6
+    log("stream copy on this GPU: %.0f GB/s" % copy_gbps)
     b64 = 8 * (2 * p.nF + p.ne)       # read u, write y (free DOFs), read E
     b32 = 4 * (2 * p.nF + p.ne)
     bfused = 4 * (6 * p.nF + p.ne)    # read p_prev, z, dx; write p, y, dx; read E
@@ -532,6 +543,7 @@ def main():
             "compliance_rel_err": rel(Cm),
             "true_residual": res_m,
             "galerkin_setup_s": gal_setup_s,
+            "galerkin_resetup_s": gal_resetup_s,
             "fp64_same_multigrid": fp64_same,
             "fp64_plain": fp64,
             "mixed_reference_vcycle": mixed_ref,
@@ -559,6 +571,8 @@ def main():
             "launches_timed": k_n,
             "timing": "HIP events around every launch of the kernel in the timed solves, on the solver's stream",
             "ms_per_launch_back_to_back": fused_ms,
+            "stream_copy_gbps_this_gpu": copy_gbps,
+            "frac_of_stream_copy": bfused / fused_solve_ms / 1e6 / copy_gbps,
             "algorithmic_bytes": bfused,
         },
         "cpu_baseline": cpu,

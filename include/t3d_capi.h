@@ -260,6 +260,9 @@ typedef struct t3d_params {
     int NlimitDofs;
     int rebuildEvery;
     int precision;   /* 0 = FP64 plain path, 1 = mixed-precision solves (t3d_pcg_mixed) */
+    int mgMode;      /* 0 = the C++ port's V-cycle, 1 = Galerkin multigrid (t3d_mg_set_mode;
+                        no counterpart in the C++ port), reassembled every loop as in
+                        matlab/TOP3D_XL.m:122 (rebuildEvery applies to mode 0 only) */
 } t3d_params;
 typedef struct t3d_loop_record {
     int loop, pcgIters, filterIters, bisectSteps;
