@@ -104,6 +104,10 @@ def lib():
         "t3d_fp_probe": (C.c_int, [C.c_int, C.c_int64, vp, vp, vp]),
         "t3d_top3d_run": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "t3d_summary_format": (C.c_int, [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "t3d_stl_size": (C.c_int, [vp, vp, C.c_float, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "t3d_stl_image": (C.c_int, [vp, vp, C.c_float, vp, C.c_uint64]),
+        "t3d_export_surface_stl": (C.c_int, [vp, vp, C.c_char_p, C.c_float]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -216,6 +220,13 @@ class Problem:
         check(lib().t3d_problem_model(self.h, *[_ptr(out[k]) for k in
                                                 ("eleMapBack", "eNodMat", "color", "Ke", "F", "isFree",
                                                  "freeDofIndex", "eDofFree")]), "t3d_problem_model")
+        return out
+
+    def ele_map_back(self):
+        """eleMapBack[ne] alone (the full index of each compact element, fea.cpp:206-215)."""
+        out = np.zeros(self.ne, np.int32)
+        check(lib().t3d_problem_model(self.h, _ptr(out), None, None, None, None, None, None, None),
+              "t3d_problem_model")
         return out
 
     def free_to_full(self, dfree, dfull):
@@ -381,7 +392,29 @@ class Params(C.Structure):
         ("cgMaxIt", C.c_int), ("V0", C.c_float), ("nLoop", C.c_int), ("rMin", C.c_float),
         ("mgMaxLevels", C.c_int), ("mgWeight", C.c_float), ("nonDyadic", C.c_int), ("NlimitDofs", C.c_int),
         ("rebuildEvery", C.c_int), ("precision", C.c_int), ("mgMode", C.c_int),
+        ("stlPath", C.c_char_p), ("stlIso", C.c_float), ("summaryPath", C.c_char_p),
     ]
+
+
+class Summary(C.Structure):
+    """t3d_summary: the values of the reference's run summary (driver.cpp:225-258)."""
+    _fields_ = [
+        ("origX", C.c_int), ("origY", C.c_int), ("origZ", C.c_int),
+        ("resX", C.c_int), ("resY", C.c_int), ("resZ", C.c_int),
+        ("V0", C.c_float), ("iterations", C.c_int), ("cgMaxIt", C.c_int), ("Csolid", C.c_double),
+        ("objFirst", C.c_float), ("objLast", C.c_float),
+        ("sumCG", C.c_float), ("sumOpt", C.c_float), ("sumFilter", C.c_float), ("sumIter", C.c_float),
+        ("threads", C.c_int),
+    ]
+
+
+def summary_format(**kw):
+    """The reference's summary text (driver.cpp:225-258) for the given t3d_summary fields."""
+    S = Summary(**kw)
+    buf = C.create_string_buffer(4096)
+    n = C.c_size_t()
+    check(lib().t3d_summary_format(C.byref(S), buf, 4096, C.byref(n)), "t3d_summary_format")
+    return buf.raw[:n.value].decode()
 
 
 class LoopRecord(C.Structure):
@@ -415,6 +448,29 @@ def top3d_run(nely, nelx, nelz, **kw):
                               C.byref(nl)), "t3d_top3d_run")
     recs = [{f: getattr(r, f) for f, _ in LoopRecord._fields_} for r in rec[:nl.value]]
     return recs, x, Cs.value, si.value
+
+
+def stl_size(prob, xPhys, iso=0.5):
+    """(bytes, triangles) of the voxel-surface STL of the device density xPhys
+    (compact order, float32) at threshold iso (io.cpp:80-87)."""
+    nb, nt = C.c_uint64(), C.c_uint64()
+    check(lib().t3d_stl_size(prob.h, xPhys.ptr, float(iso), C.byref(nb), C.byref(nt)), "t3d_stl_size")
+    return nb.value, nt.value
+
+
+def stl_image(prob, xPhys, iso=0.5):
+    """The binary STL file of export_surface_stl (io.cpp:80-87), built on the
+    device, as bytes."""
+    nbytes, _ = stl_size(prob, xPhys, iso)
+    img = DeviceArray((nbytes + 3) // 4, np.uint32)
+    check(lib().t3d_stl_image(prob.h, xPhys.ptr, float(iso), img.ptr, nbytes), "t3d_stl_image")
+    return img.download().view(np.uint8)[:nbytes].tobytes()
+
+
+def export_surface_stl(prob, xPhys, path, iso=0.5):
+    """export_surface_stl (io.cpp:80-87): the device-built image written to path."""
+    check(lib().t3d_export_surface_stl(prob.h, xPhys.ptr, os.fsencode(path), float(iso)),
+          "t3d_export_surface_stl")
 
 
 class Mixed:

@@ -10,6 +10,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "t3d_internal.hpp"
@@ -40,6 +41,46 @@ void t3d_params_default(t3d_params* P)
     P->cgMaxIt = 800; P->V0 = 0.12f; P->nLoop = 30; P->rMin = std::sqrt(3.0f);
     P->mgMaxLevels = 5; P->mgWeight = 0.6f; P->nonDyadic = 1; P->NlimitDofs = 1000;
     P->rebuildEvery = 5; P->precision = 0; P->mgMode = 0;
+    P->stlPath = nullptr; P->stlIso = 0.3f; P->summaryPath = nullptr;
+}
+
+// the reference's summary text (driver.cpp:225-258); iostream's scientific and
+// fixed formats with the precisions it sets, as printf conversions
+int t3d_summary_format(const t3d_summary* S, char* buf, size_t cap, size_t* len)
+{
+    T3D_CHECK(S && buf, T3D_ERR_ARG, "t3d_summary_format: null argument");
+    const float avgPerIter = S->iterations > 0 ? S->sumIter / float(S->iterations) : 0.0f;
+    const float pctCG = S->sumIter > 0 ? 100.0f * S->sumCG / S->sumIter : 0.0f;
+    const float pctOpt = S->sumIter > 0 ? 100.0f * S->sumOpt / S->sumIter : 0.0f;
+    const float pctFilt = S->sumIter > 0 ? 100.0f * S->sumFilter / S->sumIter : 0.0f;
+    const int n = std::snprintf(
+        buf, cap,
+        "PARAMETERS\n"
+        "original dims: x=%d, y=%d, z=%d\n"
+        "padded dims: x=%d, y=%d, z=%d\n"
+        "v0: %g\n\n"
+        "SOLVING\n"
+        "iterations: %d\n"
+        "cg iterations: %d\n\n"
+        "RESULTS\n"
+        "compliance solid: %.6e\n"
+        "objective first: %.6e\n"
+        "objective last: %.6e\n\n"
+        "TIME\n"
+        "total solver time: %.6e\n"
+        "time per iter: %.4f\n"
+        "percentage time spent on cg: %.2f%%\n"
+        "percentage time spent on optim: %.2f%%\n"
+        "percentage time spent on filtering: %.2f%%\n\n"
+        "THREADS\n"
+        "%d\n\n"
+        "NOTES\n",
+        S->origX, S->origY, S->origZ, S->resX, S->resY, S->resZ, double(S->V0), S->iterations, S->cgMaxIt,
+        S->Csolid, double(S->objFirst), double(S->objLast), double(S->sumCG), double(avgPerIter), double(pctCG),
+        double(pctOpt), double(pctFilt), S->threads);
+    T3D_CHECK(n >= 0 && size_t(n) < cap, T3D_ERR_ARG, "t3d_summary_format: buffer too small");
+    if (len) *len = size_t(n);
+    return T3D_OK;
 }
 
 // from filter.hip
@@ -195,6 +236,31 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         }
     }
     T3D_HIP(hipDeviceSynchronize());
+    // the final design's surface (driver.cpp:272-275; xPhys = x in the C++ port, driver.cpp:186)
+    if (P->stlPath && (rc = t3d_export_surface_stl(p, x, P->stlPath, P->stlIso))) return rc;
+    if (P->summaryPath) {   // driver.cpp:225-270
+        t3d_summary S{};
+        S.origX = nelx; S.origY = nely; S.origZ = nelz;
+        S.resX = p->g.resX; S.resY = p->g.resY; S.resZ = p->g.resZ;
+        S.V0 = P->V0; S.iterations = loop; S.cgMaxIt = P->cgMaxIt; S.Csolid = Csolid;
+        S.threads = 1;
+        for (int k = 0; k < loop && k < recCap && rec; ++k) {
+            if (k == 0) S.objFirst = float(rec[k].Cdisp);
+            S.objLast = float(rec[k].Cdisp);
+            S.sumCG += float(rec[k].tSolve);
+            S.sumOpt += float(rec[k].tOpti);
+            S.sumFilter += float(rec[k].tFilter);
+            S.sumIter += float(rec[k].tTotal);
+        }
+        char text[2048];
+        size_t n = 0;
+        if ((rc = t3d_summary_format(&S, text, sizeof(text), &n))) return rc;
+        FILE* fp = std::fopen(P->summaryPath, "w");
+        T3D_CHECK(fp, T3D_ERR_ARG, std::string("t3d_top3d_run: cannot open ") + P->summaryPath);
+        const size_t wr = std::fwrite(text, 1, n, fp);
+        const int cl = std::fclose(fp);
+        T3D_CHECK(wr == n && cl == 0, T3D_ERR_ARG, std::string("t3d_top3d_run: write failed: ") + P->summaryPath);
+    }
     if (x_out) T3D_HIP(hipMemcpy(x_out, x, sizeof(float) * ne, hipMemcpyDeviceToHost));
     if (loopsRun) *loopsRun = loop;
     return T3D_OK;

@@ -94,6 +94,9 @@ struct t3d_problem {
     int pcgScCap = 0;
     int* pcgFlags = nullptr;            // [0] done, [1] iterations
     int* hostFlags = nullptr;           // pinned mirror
+    // voxel-surface STL scratch (stl.hip), allocated on first use
+    uint8_t* stlIns = nullptr;          // [ncell] inside flags in the volume's order
+    uint64_t* stlOff = nullptr;         // [blocks + 1] face offsets per block of 256 voxels
 };
 
 namespace t3d {

@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--no-seq", action="store_true", help="batch mode: skip the sequential comparison")
     ap.add_argument("--voids", choices=["removed", "density0"], default="removed",
                     help="batch mode: spherical voids as absent voxels (default) or as elements of density 0")
+    ap.add_argument("--stl", action="store_true",
+                    help="the voxel-surface STL image of a lattice design on the same grid instead of the solves")
     ap.add_argument("--kxu-reps", type=int, default=50)
     ap.add_argument("--tile-order-compare", action="store_true",
                     help="with --kxu-only: also time the Walsh-form kernels in the other workgroup tile order")
@@ -181,6 +183,75 @@ def batch_design(t3d, p, v0, spheres):
     dE = t3d.DeviceArray(p.ne)
     t3d.simp(p, dx, 1.0, float(np.float32(1e-9)), dE)
     return dE
+
+
+def main_stl(args, t3d):
+    """export_surface_stl (io.cpp:80-87) of a gyroid lattice design (period 32
+    voxels, density 0.5 + 0.35 g) at the driver's threshold 0.3 on the
+    nely x nelx x nelz cuboid: device images per second with the density resident
+    in HBM; the reference's extract_faces + write_stl_binary on the host as the
+    CPU baseline (writing to /tmp)."""
+    t0 = time.time()
+    p = t3d.Problem(args.nely, args.nelx, args.nelz)
+    emb = p.ele_map_back().astype(np.int64)
+    ny, nx, nz = p.resY, p.resX, p.resZ
+    ez, rem = emb // (ny * nx), emb % (ny * nx)
+    ex, ty = rem // ny, rem % ny
+    k = 2 * np.pi / 32
+    g = np.sin(k * ex) * np.cos(k * ty) + np.sin(k * ty) * np.cos(k * ez) + np.sin(k * ez) * np.cos(k * ex)
+    x = np.clip(0.5 + 0.35 * g, 0.0, 1.0).astype(np.float32)
+    del g, ez, rem, ex, ty
+    iso = float(np.float32(0.3))
+    dx = t3d.DeviceArray.from_host(x)
+    nbytes, ntri = t3d.stl_size(p, dx, iso)
+    img = t3d.DeviceArray((nbytes + 3) // 4, np.uint32)
+    log("STL workload %dx%dx%d: %d voxels, %d triangles, %.1f MB image (set-up %.1fs)"
+        % (args.nely, args.nelx, args.nelz, nx * ny * nz, ntri, nbytes / 1e6, time.time() - t0))
+    L = t3d.lib()
+    for _ in range(args.warmup):
+        t3d.check(L.t3d_stl_image(p.h, dx.ptr, iso, img.ptr, nbytes), "t3d_stl_image")
+    t3d.check(L.t3d_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t3d.check(L.t3d_stl_image(p.h, dx.ptr, iso, img.ptr, nbytes), "t3d_stl_image")
+    t3d.check(L.t3d_synchronize(), "sync")
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ncell = nx * ny * nz
+    # algorithmic bytes of one image: density and element-to-cell map (8 per element),
+    # the inside bytes written twice (fill, mark) and read twice (count, emit), the image
+    alg = 8 * p.ne + 4 * ncell + nbytes
+    log("STL image on the device: %.3f ms/image, %.0f GB/s algorithmic" % (ms, alg / ms / 1e6))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+        if O.have_ref(fast=True):
+            vol = np.zeros(ncell, np.float32)
+            vol[emb] = x
+            path = "/tmp/t3d_bench_ref.stl"
+            t1 = time.perf_counter()
+            rc = O.ref(fast=True).ref_stl_volume(ny, nx, nz, vol, iso, path.encode())
+            tc = time.perf_counter() - t1
+            same = rc == 0 and os.path.getsize(path) == nbytes
+            os.remove(path)
+            cpu = {"value": 1.0 / tc, "unit": "images/s", "cores": 1, "kind": "reference",
+                   "sample": "one image of the same volume: the reference's extract_faces + write_stl_binary "
+                             "(voxel_surface.hpp, export_stl.hpp; -O3 -ffast-math build), file written to /tmp; "
+                             "image size %s the device's" % ("equal to" if same else "DIFFERENT from")}
+            log("reference on the host: %.2f s/image" % tc)
+    line = {"metric": "voxel-surface STL images/s (export_surface_stl) at 14.2M voxels",
+            "value": 1e3 / ms, "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic (gyroid lattice density, period 32 voxels, threshold 0.3)",
+            "config": {"workload": "binary STL image of the voxel surface, cuboid %dx%dx%d (nely x nelx x nelz), "
+                                   "%d triangles" % (args.nely, args.nelx, args.nelz, ntri),
+                       "image_bytes": nbytes, "triangles": ntri, "voxels": ncell},
+            "roofline": {"bound": "hbm", "achieved": alg / ms / 1e6, "peak": 8000.0, "unit": "GB/s",
+                         "frac": alg / ms / 1e6 / 8000.0, "traffic": None, "algorithmic_bytes": alg,
+                         "timing": "wall clock around whole t3d_stl_image calls (both passes, the count's "
+                                   "device-to-host copy and the synchronisations included)"},
+            "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
 
 
 def main_batch(args, t3d):
@@ -322,6 +393,8 @@ def main():
     log("device %s, %d CUs" % (name, cus))
     if args.batch:
         return main_batch(args, t3d)
+    if args.stl:
+        return main_stl(args, t3d)
     galerkin = args.mg == "galerkin"
     tol = args.tol if args.tol is not None else (1e-6 if galerkin else args.ref_tol)
     te = args.true_every if args.true_every is not None else (25 if galerkin else 50)

@@ -263,6 +263,11 @@ typedef struct t3d_params {
     int mgMode;      /* 0 = the C++ port's V-cycle, 1 = Galerkin multigrid (t3d_mg_set_mode;
                         no counterpart in the C++ port), reassembled every loop as in
                         matlab/TOP3D_XL.m:122 (rebuildEvery applies to mode 0 only) */
+    const char* stlPath; /* when set, the final design's voxel surface at stlIso is written
+                            there (t3d_export_surface_stl), as driver.cpp:272-275 does */
+    float stlIso;    /* default 0.3 (driver.cpp:274) */
+    const char* summaryPath; /* when set, the run's summary (t3d_summary_format) is written
+                                there, as driver.cpp:264-270 writes it to out/comments/ */
 } t3d_params;
 typedef struct t3d_loop_record {
     int loop, pcgIters, filterIters, bisectSteps;
@@ -275,6 +280,43 @@ void t3d_params_default(t3d_params* p);
  * receives one record per loop, x_out_host[ne] (optional) the final design. */
 int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_record* rec, int recCap,
                   float* x_out_host, double* Csolid, int* solidIters, int* loopsRun);
+
+/* ---- run summary (driver.cpp:225-258): the text the reference prints after a
+ * run and stores as markdown.  Values as the reference computes them: objFirst
+ * and objLast are the first and last loop's Cdisp as float, the sums are the
+ * per-loop solve, optimisation, filtering and total times in seconds, threads
+ * the host threads of the run. */
+typedef struct t3d_summary {
+    int origX, origY, origZ;     /* nelx, nely, nelz */
+    int resX, resY, resZ;        /* padded dims */
+    float V0;
+    int iterations;              /* loops run */
+    int cgMaxIt;
+    double Csolid;
+    float objFirst, objLast;
+    float sumCG, sumOpt, sumFilter, sumIter;
+    int threads;
+} t3d_summary;
+/* the text into buf[cap] (NUL-terminated), *len = its length; T3D_ERR_ARG if it does not fit */
+int t3d_summary_format(const t3d_summary* s, char* buf, size_t cap, size_t* len);
+
+/* ---- voxel-surface STL of a design (io.cpp:80-87 export_surface_stl with
+ * geometry_out/voxel_surface.hpp:12-93 extract_faces and
+ * geometry_out/export_stl.hpp:14-33 write_stl_binary), built on the device.
+ * xPhys_dev[ne] is the density in compact element order; a voxel of the padded
+ * resY x resX x resZ volume is inside when its density is >= iso (cells without
+ * an element hold 0), and each inside voxel's face towards an outside voxel or
+ * the volume's boundary becomes two triangles, in the reference's order.  The
+ * image is the reference's file byte for byte: 80-byte header, uint32 triangle
+ * count, 50 bytes per triangle.  T3D_ERR_ARG beyond 2^32 - 1 triangles. */
+/* image size: *nbytes = 84 + 50 * (*ntri) (either pointer may be null) */
+int t3d_stl_size(t3d_problem* p, const float* xPhys_dev, float iso, uint64_t* nbytes, uint64_t* ntri);
+/* the image into image_dev (4-byte aligned device memory of nbytes, which must
+ * equal the size t3d_stl_size returns for the same arguments) */
+int t3d_stl_image(t3d_problem* p, const float* xPhys_dev, float iso, void* image_dev, uint64_t nbytes);
+/* export_surface_stl (io.cpp:80-87): the image built on the device, copied to
+ * the host and written to path */
+int t3d_export_surface_stl(t3d_problem* p, const float* xPhys_dev, const char* path, float iso);
 
 /* ---- diagnostics (no reference counterpart): device floating-point semantics
  * the plain path relies on.  op 0: out = sqrtf(a) (float), 1: out = a / b

@@ -143,6 +143,8 @@ def lib():
         L.o_oc_update.restype = C.c_int
         L.o_oc_update.argtypes = [C.c_int, f32p, f32p, C.c_float, C.c_float, f32p, C.POINTER(C.c_float)]
         L.o_params_default.argtypes = [C.POINTER(OParams)]
+        L.o_stl_image.restype = C.c_uint64
+        L.o_stl_image.argtypes = [C.c_int, C.c_int, C.c_int, f32p, C.c_float, C.c_void_p, C.c_uint64]
         L.o_top3d_run.restype = C.c_int
         L.o_top3d_run.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(OParams), C.POINTER(OLoopRec), f32p,
                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -331,6 +333,25 @@ def run(nely, nelx, nelz, **kw):
     return recs, x, Cs.value, si.value
 
 
+def stl_volume(model, xPhys):
+    """The padded volume of export_surface_stl (io.cpp:81-83): vol[eleMapBack[e]] =
+    xPhys[e], 0 on cells without an element; vol[y + ny x + ny nx z]."""
+    vol = np.zeros(model.resX * model.resY * model.resZ, np.float32)
+    vol[model.eleMapBack] = np.asarray(xPhys, np.float32)
+    return vol
+
+
+def stl_image(ny, nx, nz, vol, iso):
+    """Bytes of the binary STL of extract_faces + write_stl_binary
+    (voxel_surface.hpp:12-93, export_stl.hpp:14-33) for vol[ny*nx*nz]."""
+    vol = np.ascontiguousarray(vol, np.float32).reshape(-1)
+    assert vol.size == ny * nx * nz
+    n = lib().o_stl_image(ny, nx, nz, vol, float(iso), None, 0)
+    out = np.zeros(n, np.uint8)
+    lib().o_stl_image(ny, nx, nz, vol, float(iso), out.ctypes.data, n)
+    return out.tobytes()
+
+
 # ---------------------------------------------------------------------------
 # reference build (oracle/_ref), present only where it was built
 # ---------------------------------------------------------------------------
@@ -377,6 +398,10 @@ def ref(fast=False):
     R.ref_time_kxu.argtypes = ii + [f64p, f64p, C.c_int]
     R.ref_time_pcg.restype = C.c_double
     R.ref_time_pcg.argtypes = ii + [f64p, C.c_float, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    R.ref_export_stl.restype = C.c_int
+    R.ref_export_stl.argtypes = ii + [f32p, C.c_char_p, C.c_float]
+    R.ref_stl_volume.restype = C.c_int
+    R.ref_stl_volume.argtypes = ii + [f32p, C.c_float, C.c_char_p]
     _ref[key] = R
     return R
 

@@ -11,6 +11,9 @@
 #include "fea.hpp"
 #include "solver.hpp"
 #include "filter.hpp"
+#include "io.hpp"
+#include "geometry_out/export_stl.hpp"
+#include "geometry_out/voxel_surface.hpp"
 #include "multigrid/multigrid.hpp"
 #include "multigrid/detail/diagonal.hpp"
 #include "multigrid/detail/coarsest.hpp"
@@ -389,6 +392,25 @@ double ref_time_pcg(int nely, int nelx, int nelz, const double* E, float Emin, d
     if (iters) *iters = it;
     s.pb.params.youngsModulusMin = 1.0e-6f;
     return ms;
+}
+
+// export_surface_stl (io.cpp:80-87) of the cuboid model: xPhys[ne] in compact order
+int ref_export_stl(int nely, int nelx, int nelz, const float* xPhys, const char* path, float iso)
+{
+    RefState& s = state(nely, nelx, nelz);
+    std::vector<float> x(xPhys, xPhys + s.pb.mesh.numElements);
+    export_surface_stl(s.pb, x, path, iso);
+    return 0;
+}
+
+// extract_faces (voxel_surface.hpp:12-93) + write_stl_binary (export_stl.hpp:14-33) of vol[ny*nx*nz]
+int ref_stl_volume(int ny, int nx, int nz, const float* vol, float iso, const char* path)
+{
+    std::vector<float> v(vol, vol + (size_t)ny * nx * nz);
+    std::vector<std::array<float, 3>> verts;
+    std::vector<std::array<uint32_t, 3>> faces;
+    voxsurf::extract_faces(v, ny, nx, nz, iso, verts, faces);
+    return ioexp::write_stl_binary(path, verts, faces) ? 0 : 1;
 }
 
 }  // extern "C"

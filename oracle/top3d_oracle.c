@@ -1335,3 +1335,73 @@ int o_top3d_run(int nely, int nelx, int nelz, const o_params* P,
     o_mg_free(g); o_filter_free(flt); o_model_free(m);
     return loop;
 }
+
+/* ===================================================================== */
+/* voxel-surface STL (io.cpp:80-87 export_surface_stl,                    */
+/* geometry_out/voxel_surface.hpp:12-93 extract_faces,                    */
+/* geometry_out/export_stl.hpp:14-33 write_stl_binary)                    */
+/* ===================================================================== */
+/* Face d of voxel (y, x, z), d = 0..5 for -X, +X, -Y, +Y, -Z, +Z
+ * (voxel_surface.hpp:52-88): present when the voxel is inside and its
+ * neighbour across the face is outside or beyond the volume. */
+static int stl_face_present(int ny, int nx, int nz, const float* vol, float iso, int y, int x, int z, int d)
+{
+    const size_t pl = (size_t)ny * nx;
+    const size_t v = (size_t)y + (size_t)ny * x + pl * z;
+    if (!(vol[v] >= iso)) return 0;
+    const int axis = d >> 1, up = d & 1;
+    const int c = axis == 0 ? x : axis == 1 ? y : z;
+    const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
+    const size_t step = axis == 0 ? (size_t)ny : axis == 1 ? 1 : pl;
+    if (up ? c == n - 1 : c == 0) return 1;
+    const size_t w = up ? v + step : v - step;
+    return !(vol[w] >= iso);
+}
+
+static uint8_t* stl_put_tri(uint8_t* p, const float* a, const float* b, const float* c)
+{
+    memset(p, 0, 12);                  /* normal (0, 0, 0) */
+    memcpy(p + 12, a, 12);             /* little-endian host: the bytes of the floats */
+    memcpy(p + 24, b, 12);
+    memcpy(p + 36, c, 12);
+    p[48] = 0; p[49] = 0;              /* attribute */
+    return p + 50;
+}
+
+uint64_t o_stl_image(int ny, int nx, int nz, const float* vol, float iso, uint8_t* out, uint64_t cap)
+{
+    uint64_t faces = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y)
+                for (int d = 0; d < 6; ++d) faces += (uint64_t)stl_face_present(ny, nx, nz, vol, iso, y, x, z, d);
+    const uint64_t bytes = 84 + 100 * faces;
+    if (!out || cap < bytes) return bytes;
+    memset(out, 0, 80);
+    memcpy(out, "Generated by TOP3D_XL", 21);
+    const uint32_t ntri = (uint32_t)(2 * faces);
+    out[80] = (uint8_t)(ntri & 0xffu); out[81] = (uint8_t)((ntri >> 8) & 0xffu);
+    out[82] = (uint8_t)((ntri >> 16) & 0xffu); out[83] = (uint8_t)(ntri >> 24);
+    uint8_t* p = out + 84;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y)
+                for (int d = 0; d < 6; ++d) {
+                    if (!stl_face_present(ny, nx, nz, vol, iso, y, x, z, d)) continue;
+                    /* the face's corner with the smallest coordinates, vertex order (x, y, z)
+                     * (voxel_surface.hpp:19-45), and its two in-plane unit steps a, b:
+                     * x-faces (y, z), y-faces (x, z), z-faces (x, y) */
+                    const int axis = d >> 1, up = d & 1;
+                    const float o[3] = {(float)(x + (axis == 0 && up)), (float)(y + (axis == 1 && up)),
+                                        (float)(z + (axis == 2 && up))};
+                    const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
+                    float c0[3], c1[3], c2[3], c3[3];
+                    for (int k = 0; k < 3; ++k) c0[k] = c1[k] = c2[k] = c3[k] = o[k];
+                    c1[ia] += 1.0f;
+                    c2[ia] += 1.0f; c2[ib] += 1.0f;
+                    c3[ib] += 1.0f;
+                    p = stl_put_tri(p, c0, c1, c2);
+                    p = stl_put_tri(p, c0, c2, c3);
+                }
+    return bytes;
+}

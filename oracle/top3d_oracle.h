@@ -169,6 +169,12 @@ typedef struct o_loop_rec {
 int o_top3d_run(int nely, int nelx, int nelz, const o_params* p,
                 o_loop_rec* rec, float* x_out, double* Csolid, int* solidIters);
 
+/* ---- voxel-surface STL (io.cpp:80-87, geometry_out/voxel_surface.hpp:12-93,
+ * geometry_out/export_stl.hpp:14-33) ----
+ * vol[y + ny*x + ny*nx*z]; returns the image size 84 + 100 * faces; the image
+ * is written only when out != NULL and cap >= that size. */
+uint64_t o_stl_image(int ny, int nx, int nz, const float* vol, float iso, uint8_t* out, uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,7 +121,9 @@ def test_cli_known_answers(tmp_path):
     the solid compliance and the loop-1 objective (printed to 9 significant
     digits) agree with the oracle to print precision; later objectives are
     chaotic in the reference itself, so they are compared against the spread of
-    the reference's own runs with 1, 2 and 4 OpenMP threads."""
+    the reference's own runs with 1, 2 and 4 OpenMP threads, widened to the
+    1e-5 relative spread that the reference shows across thread counts and
+    repeated runs by loop 4 (three runs can happen to agree more closely)."""
     import re
     import subprocess
     cli = os.path.join(O.HERE, "_ref", "top3d_xl_cli")
@@ -142,5 +144,5 @@ def test_cli_known_answers(tmp_path):
     assert abs(recs[0]["Cdisp"] - objs[0]) <= 1e-8 * abs(objs[0])
     for k in range(1, len(objs)):
         vals = [r[1][k] for r in runs]
-        spread = max(vals) - min(vals)
-        assert min(vals) - 2 * spread <= recs[k]["Cdisp"] <= max(vals) + 2 * spread, (k, recs[k]["Cdisp"], vals)
+        spread = max(2 * (max(vals) - min(vals)), 1e-5 * abs(vals[0]))
+        assert min(vals) - spread <= recs[k]["Cdisp"] <= max(vals) + spread, (k, recs[k]["Cdisp"], vals)
