@@ -336,15 +336,91 @@ __device__ __forceinline__ void gal_accumulate(const double* sA, double* sH, con
     __syncthreads();
 }
 
-// level-1 element matrices from the finest cells; one workgroup per coarse
-// cell.  Cells without fixed DOFs add E * G_f (G_f = T_f^T Ke T_f, precomputed).
-__global__ void __launch_bounds__(256) k_gal_fine(Grid g, LevelDev Lc, int s, const double* __restrict__ G,
-                                                  const double* __restrict__ Wch, const double* __restrict__ Ke,
-                                                  const double* __restrict__ Ecell,
-                                                  const uint8_t* __restrict__ nodeFixed,
-                                                  const int32_t* __restrict__ cellToCompact, double* __restrict__ Kel)
+// Level 1 as a product.  A level-1 cell's matrix is sum_f E_f G_f over its
+// s^3 finest cells f, G_f = T_f^T Ke T_f shared by all cells; with the
+// effective moduli Eeff[cell][f] (E_f, or 0 for void cells, moduli <= 1e-16
+// and cells with fixed DOFs) the whole level is Kel = Eeff G, an
+// [ncell x s^3] by [s^3 x 576] FP64 product; k_gal_fix then adds the cells
+// with fixed DOFs (a static list): for those finest cells it forms
+// T_f^T (E M Ke M + D_fix) T_f through LDS.
+__global__ void k_gal_eff(Grid g, LevelDev Lc, int s, const double* __restrict__ Ecell,
+                          const uint8_t* __restrict__ nodeFixed, const int32_t* __restrict__ cellToCompact,
+                          double* __restrict__ Eeff)
 {
-    const int64_t ce = blockIdx.x;
+    const int nch = s * s * s;
+    const int64_t n = int64_t(Lc.resX) * Lc.resY * Lc.resZ * nch;
+    for (int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < n;
+         idx += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t ce = idx / nch;
+        const int f = int(idx - ce * nch);
+        const int cx = int(ce % Lc.resX), cy = int((ce / Lc.resX) % Lc.resY),
+                  cz = int(ce / (int64_t(Lc.resX) * Lc.resY));
+        const int i = f % s, j = (f / s) % s, k = f / (s * s);
+        const int fx = cx * s + i, fy = cy * s + j, fz = cz * s + k;
+        const int64_t fc = (int64_t(fz) * g.resY + fy) * g.resX + fx;
+        double E = 0.0;
+        if (cellToCompact[fc] >= 0) {
+            unsigned fixedAny = 0;
+            for (int a = 0; a < 8; ++a) {
+                int dx, dy, dz;
+                corner_offsets(a, dx, dy, dz);
+                fixedAny |= nodeFixed[(int64_t(fz + dz) * g.nny + (fy + dy)) * g.nnx + (fx + dx)];
+            }
+            const double e = Ecell[fc];
+            if (!fixedAny) E = (e <= 1.0e-16) ? 0.0 : e;   // fea.cpp:743-744
+        }
+        Eeff[idx] = E;
+    }
+}
+
+// Kel[ncell][576] = Eeff[ncell][nch] G[nch][576]: 64 cells (blockIdx.x) x 64
+// entries (blockIdx.y) per 256-thread workgroup, 4 x 4 per thread, nch in LDS
+// chunks of 16
+__global__ void __launch_bounds__(256) k_gal_gemm(int64_t ncell, int nch, const double* __restrict__ Eeff,
+                                                  const double* __restrict__ G, double* __restrict__ Kel)
+{
+    __shared__ double Sa[16][65], Sb[16][65];
+    const int64_t c0 = int64_t(blockIdx.x) * 64;
+    const int e0 = int(blockIdx.y) * 64;
+    const int t = int(threadIdx.x);
+    const int ri = (t / 16) * 4, rj = (t % 16) * 4;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < nch; k0 += 16) {
+        for (int q = t; q < 16 * 64; q += 256) {
+            const int r = q / 16, kk = q % 16;        // Eeff rows: 16 consecutive children of a cell
+            const bool in = c0 + r < ncell && k0 + kk < nch;
+            Sa[kk][r] = in ? Eeff[(c0 + r) * nch + k0 + kk] : 0.0;
+            const int kb = q / 64, c = q % 64;        // G rows: 64 consecutive entries
+            Sb[kb][c] = (k0 + kb < nch && e0 + c < 576) ? G[size_t(k0 + kb) * 576 + e0 + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fma(Sa[kk][ri + a], Sb[kk][rj + b], acc[a][b]);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int64_t c = c0 + ri + a;
+        if (c >= ncell) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (e0 + rj + b < 576) Kel[c * 576 + e0 + rj + b] = acc[a][b];
+    }
+}
+
+// the level-1 cells with fixed DOFs (fixCells): add T_f^T (E M Ke M + D_fix) T_f
+// of their finest cells with fixed DOFs, the terms k_gal_eff left out
+__global__ void __launch_bounds__(256) k_gal_fix(Grid g, LevelDev Lc, int s, const int32_t* __restrict__ fixCells,
+                                                 const double* __restrict__ Wch, const double* __restrict__ Ke,
+                                                 const double* __restrict__ Ecell,
+                                                 const uint8_t* __restrict__ nodeFixed,
+                                                 const int32_t* __restrict__ cellToCompact, double* __restrict__ Kel)
+{
+    const int64_t ce = fixCells[blockIdx.x];
     const int cx = int(ce % Lc.resX), cy = int((ce / Lc.resX) % Lc.resY), cz = int(ce / (int64_t(Lc.resX) * Lc.resY));
     __shared__ double sA[576], sH[576];
     const int t = threadIdx.x;
@@ -355,23 +431,16 @@ __global__ void __launch_bounds__(256) k_gal_fine(Grid g, LevelDev Lc, int s, co
                 const int f = (k * s + j) * s + i;
                 const int fx = cx * s + i, fy = cy * s + j, fz = cz * s + k;
                 const int64_t fc = (int64_t(fz) * g.resY + fy) * g.resX + fx;
-                if (cellToCompact[fc] < 0) continue;   // void cell: no element (uniform in the block)
-                double E = Ecell[fc];
-                if (E <= 1.0e-16) E = 0.0;             // fea.cpp:743-744
+                if (cellToCompact[fc] < 0) continue;   // uniform in the block
                 unsigned fixedAny = 0;
                 for (int a = 0; a < 8; ++a) {
                     int dx, dy, dz;
                     corner_offsets(a, dx, dy, dz);
                     fixedAny |= nodeFixed[(int64_t(fz + dz) * g.nny + (fy + dy)) * g.nnx + (fx + dx)];
                 }
-                if (!fixedAny) {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const int e = t + 256 * q;
-                        if (e < 576) acc[q] += E * G[size_t(f) * 576 + e];
-                    }
-                    continue;
-                }
+                if (!fixedAny) continue;
+                double E = Ecell[fc];
+                if (E <= 1.0e-16) E = 0.0;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const int e = t + 256 * q;
@@ -395,7 +464,7 @@ __global__ void __launch_bounds__(256) k_gal_fine(Grid g, LevelDev Lc, int s, co
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int e = t + 256 * q;
-        if (e < 576) Kel[size_t(ce) * 576 + e] = acc[q];
+        if (e < 576) Kel[size_t(ce) * 576 + e] += acc[q];
     }
 }
 
@@ -603,6 +672,9 @@ struct t3d_mg {
     double* Kel[8] = {};      // element matrices of levels >= 1, [cells][576]
     double* Wch[8] = {};      // child interpolation weights of levels >= 1, [span^3][64]
     double* Gch = nullptr;    // T_f^T Ke T_f per finest cell of a level-1 cell, [span^3][576]
+    double* Eeff = nullptr;   // level-1 effective moduli, [cells][span^3]
+    int32_t* fixCells = nullptr;   // level-1 cells covering finest cells with fixed DOFs
+    int nFix = 0;
     double* Li = nullptr;     // inverse coarsest factor L^-1
     double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles
     double* yc = nullptr;     // device flag of the factorisation (after N doubles)
@@ -745,6 +817,33 @@ static std::vector<double> child_products(const std::vector<double>& W, const do
     return G;
 }
 
+// level-1 cells (structured index, increasing) that cover a solid finest cell
+// with a fixed DOF: the cells around each fixed node
+static std::vector<int32_t> fixed_level1_cells(const t3d_problem* p, const MGLevelDims& c)
+{
+    const HostModel& m = p->host;
+    const Grid& g = p->g;
+    const int s = c.span;
+    std::vector<uint8_t> solid(size_t(g.ncell), 0), mark(size_t(c.resX) * c.resY * c.resZ, 0);
+    for (int e = 0; e < m.numElements; ++e) solid[size_t(cell_of_full(m.eleMapBack[e], m.resX, m.resY))] = 1;
+    for (int64_t n = 0; n < g.nN; ++n) {
+        if (m.isFreeDOF[3 * n] && m.isFreeDOF[3 * n + 1] && m.isFreeDOF[3 * n + 2]) continue;
+        const int ix = int(n % g.nnx), iy = int((n / g.nnx) % g.nny), iz = int(n / (int64_t(g.nnx) * g.nny));
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int fx = ix - dx, fy = iy - dy, fz = iz - dz;
+                    if (fx < 0 || fy < 0 || fz < 0 || fx >= g.resX || fy >= g.resY || fz >= g.resZ) continue;
+                    if (!solid[(size_t(fz) * g.resY + fy) * g.resX + fx]) continue;
+                    mark[(size_t(fz / s) * c.resY + fy / s) * c.resX + fx / s] = 1;
+                }
+    }
+    std::vector<int32_t> out;
+    for (size_t i = 0; i < mark.size(); ++i)
+        if (mark[i]) out.push_back(int32_t(i));
+    return out;
+}
+
 // Galerkin set-up: element matrices of every coarse level, their diagonals and
 // masks, the coarsest matrix (assembled here in cell order), its Cholesky
 // factor and inverse factor
@@ -768,14 +867,26 @@ static int mg_galerkin_setup(t3d_mg* g)
                 const std::vector<double> G = child_products(W, p->host.Ke.data(), int(nch));
                 T3D_HIP(hipMalloc(&g->Gch, sizeof(double) * G.size()));
                 T3D_HIP(hipMemcpy(g->Gch, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice));
+                T3D_HIP(hipMalloc(&g->Eeff, sizeof(double) * nch * size_t(ncell)));
+                const std::vector<int32_t> fix = fixed_level1_cells(p, g->H.lv[1]);
+                g->nFix = int(fix.size());
+                if (g->nFix) {
+                    T3D_HIP(hipMalloc(&g->fixCells, sizeof(int32_t) * fix.size()));
+                    T3D_HIP(hipMemcpy(g->fixCells, fix.data(), sizeof(int32_t) * fix.size(), hipMemcpyHostToDevice));
+                }
             }
         }
-        if (l == 1)
-            hipLaunchKernelGGL(k_gal_fine, dim3(unsigned(ncell)), dim3(256), 0, st, p->g, g->ld[1], sp,
-                               (const double*)g->Gch, (const double*)g->Wch[1], (const double*)p->Ke,
-                               (const double*)p->Ecell, (const uint8_t*)p->nodeFixed,
-                               (const int32_t*)p->cellToCompact, g->Kel[1]);
-        else
+        if (l == 1) {
+            hipLaunchKernelGGL(k_gal_eff, dim3(4096), dim3(256), 0, st, p->g, g->ld[1], sp, (const double*)p->Ecell,
+                               (const uint8_t*)p->nodeFixed, (const int32_t*)p->cellToCompact, g->Eeff);
+            hipLaunchKernelGGL(k_gal_gemm, dim3(unsigned((ncell + 63) / 64), 9), dim3(256), 0, st, ncell, int(nch),
+                               (const double*)g->Eeff, (const double*)g->Gch, g->Kel[1]);
+            if (g->nFix)
+                hipLaunchKernelGGL(k_gal_fix, dim3(unsigned(g->nFix)), dim3(256), 0, st, p->g, g->ld[1], sp,
+                                   (const int32_t*)g->fixCells, (const double*)g->Wch[1], (const double*)p->Ke,
+                                   (const double*)p->Ecell, (const uint8_t*)p->nodeFixed,
+                                   (const int32_t*)p->cellToCompact, g->Kel[1]);
+        } else
             hipLaunchKernelGGL(k_gal_coarse, dim3(unsigned(ncell)), dim3(256), 0, st, g->ld[l - 1], g->ld[l], sp,
                                (const double*)g->Wch[l], (const double*)g->Kel[l - 1], g->Kel[l]);
         T3D_HIP(hipGetLastError());
@@ -870,6 +981,8 @@ int t3d_mg_destroy(t3d_mg* g)
         if (g->Wch[l]) (void)hipFree(g->Wch[l]);
     }
     if (g->Gch) (void)hipFree(g->Gch);
+    if (g->Eeff) (void)hipFree(g->Eeff);
+    if (g->fixCells) (void)hipFree(g->fixCells);
     if (g->Li) (void)hipFree(g->Li);
     if (g->Ainv) (void)hipFree(g->Ainv);
     if (g->yc) (void)hipFree(g->yc);

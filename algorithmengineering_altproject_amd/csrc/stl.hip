@@ -17,13 +17,13 @@
 //     (x, y, z floats) and a zero uint16 attribute -- 50 bytes.
 //
 // Device form: inside bytes for the whole volume (one pass over the cells,
-// one over the elements), a count pass (faces per block of 256 voxels in the
-// volume's own order), one workgroup's exclusive scan of the block counts,
-// and an emit pass in which each thread writes its voxel's faces at its
-// offset.  A face is 100 bytes, so face k starts at byte 84 + 100 k, a
-// multiple of 4: each thread writes 25 aligned dwords per face, the second
-// triangle's vertex floats shifted by two bytes.  Faces of one block are
-// contiguous, so a wavefront's stores fill whole cache lines between them.
+// one over the elements), a count pass (faces per block of 2048 consecutive
+// voxels in the volume's own order, 8 per thread), one workgroup's exclusive
+// scan of the block counts, and an emit pass.  A face is 100 bytes, so face k
+// starts at byte 84 + 100 k, a multiple of 4: 25 dwords, the second
+// triangle's vertex floats shifted by two bytes.  The faces of a block are one
+// contiguous range of the image; the emit pass stages them in LDS and writes
+// the range with consecutive lanes on consecutive dwords.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -34,8 +34,11 @@
 
 namespace t3d {
 
-constexpr int STL_B = 256;     // voxels per block of the count and emit passes
-constexpr int STL_SCAN = 1024; // threads of the scan workgroup
+constexpr int STL_B = 256;                // threads of the count and emit workgroups
+constexpr int STL_VT = 8;                 // consecutive voxels per thread
+constexpr int STL_BV = STL_B * STL_VT;    // voxels per block
+constexpr int STL_SB = 384;               // faces staged in LDS per round of the emit pass
+constexpr int STL_SCAN = 1024;            // threads of the scan workgroup
 
 struct StlVol {
     int ny, nx, nz;
@@ -61,22 +64,46 @@ __global__ void k_stl_mark(int ne, const int32_t* __restrict__ compactToCell, in
     ins[v] = xPhys[e] >= iso ? 1 : 0;
 }
 
-// face bits of voxel v = (y, x, z): bit k for direction k of -X, +X, -Y, +Y, -Z, +Z
-// (cell counts fit in 32 bits: cells are int32-indexed throughout)
-__device__ __forceinline__ unsigned stl_faces(const uint8_t* __restrict__ ins, const StlVol& V, int64_t v64)
+// face bits of the STL_VT voxels from v0 (6 bits each, bit k of a voxel for
+// direction k of -X, +X, -Y, +Y, -Z, +Z); *cnt = their number of faces.  v0 is
+// a multiple of STL_VT, so the voxels' own inside bytes are one aligned 8-byte
+// load (the inside array is padded to whole blocks).  Cell counts fit in 32 bits.
+__device__ __forceinline__ uint64_t stl_masks(const uint8_t* __restrict__ ins, const StlVol& V, int64_t v0,
+                                              unsigned* cnt)
 {
-    if (v64 >= V.n || !ins[v64]) return 0u;
-    const int v = int(v64);
+    *cnt = 0u;
+    if (v0 >= V.n) return 0ull;
+    const uint64_t own = *reinterpret_cast<const uint64_t*>(ins + v0);
+    if (!own) return 0ull;
+    int v = int(v0);
     const int pl = V.ny * V.nx;
-    const int y = v % V.ny, x = (v / V.ny) % V.nx, z = v / pl;
-    unsigned m = 0u;
-    if (x == 0 || !ins[v - V.ny]) m |= 1u;
-    if (x == V.nx - 1 || !ins[v + V.ny]) m |= 2u;
-    if (y == 0 || !ins[v - 1]) m |= 4u;
-    if (y == V.ny - 1 || !ins[v + 1]) m |= 8u;
-    if (z == 0 || !ins[v - pl]) m |= 16u;
-    if (z == V.nz - 1 || !ins[v + pl]) m |= 32u;
-    return m;
+    int y = v % V.ny, x = (v / V.ny) % V.nx, z = v / pl;
+    uint64_t all = 0ull;
+    unsigned c = 0u;
+#pragma unroll
+    for (int i = 0; i < STL_VT; ++i) {
+        if (int64_t(v) < V.n && ((own >> (8 * i)) & 0xffull)) {
+            unsigned m = 0u;
+            if (x == 0 || !ins[v - V.ny]) m |= 1u;
+            if (x == V.nx - 1 || !ins[v + V.ny]) m |= 2u;
+            if (y == 0 || !ins[v - 1]) m |= 4u;
+            if (y == V.ny - 1 || !ins[v + 1]) m |= 8u;
+            if (z == 0 || !ins[v - pl]) m |= 16u;
+            if (z == V.nz - 1 || !ins[v + pl]) m |= 32u;
+            all |= uint64_t(m) << (6 * i);
+            c += unsigned(__popc(m));
+        }
+        ++v;
+        if (++y == V.ny) {
+            y = 0;
+            if (++x == V.nx) {
+                x = 0;
+                ++z;
+            }
+        }
+    }
+    *cnt = c;
+    return all;
 }
 
 // exclusive prefix of c over the workgroup (STL_B threads); *total = the sum
@@ -105,8 +132,9 @@ __device__ __forceinline__ unsigned block_excl_scan(unsigned c, unsigned* total)
 __global__ void __launch_bounds__(STL_B) k_stl_count(StlVol V, const uint8_t* __restrict__ ins,
                                                      uint64_t* __restrict__ blk)
 {
-    const int64_t v = int64_t(blockIdx.x) * STL_B + threadIdx.x;
-    const unsigned c = unsigned(__popc(stl_faces(ins, V, v)));
+    const int64_t v0 = int64_t(blockIdx.x) * STL_BV + int64_t(threadIdx.x) * STL_VT;
+    unsigned c;
+    (void)stl_masks(ins, V, v0, &c);
     unsigned total;
     (void)block_excl_scan(c, &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
@@ -170,25 +198,50 @@ __device__ __forceinline__ void stl_put_face(uint32_t* __restrict__ w, int kind,
     w[24] = s[8] >> 16;
 }
 
+// each thread's faces go to LDS in rounds of STL_SB faces of the block (a face
+// to its slot of 25 dwords), and the whole workgroup copies each round to the
+// image with consecutive lanes on consecutive dwords
 __global__ void __launch_bounds__(STL_B) k_stl_emit(StlVol V, const uint8_t* __restrict__ ins,
                                                     const uint64_t* __restrict__ blk, uint32_t* __restrict__ faces)
 {
-    const int64_t v = int64_t(blockIdx.x) * STL_B + threadIdx.x;
-    const unsigned m = stl_faces(ins, V, v);
-    unsigned total;
-    const unsigned before = block_excl_scan(unsigned(__popc(m)), &total);
-    if (!m) return;
-    const int vi = int(v), pl = V.ny * V.nx;
-    const int y = vi % V.ny, x = (vi / V.ny) % V.nx, z = vi / pl;
-    uint64_t k = blk[blockIdx.x] + before;
-    for (int kind = 0; kind < 6; ++kind) {
-        if (!((m >> kind) & 1u)) continue;
-        uint32_t w[25];
-        stl_put_face(w, kind, y, x, z);
-        uint32_t* dst = faces + 25 * k;
-#pragma unroll
-        for (int j = 0; j < 25; ++j) dst[j] = w[j];
-        ++k;
+    __shared__ uint32_t stage[STL_SB * 25];
+    const int64_t v0 = int64_t(blockIdx.x) * STL_BV + int64_t(threadIdx.x) * STL_VT;
+    unsigned cnt;
+    const uint64_t masks = stl_masks(ins, V, v0, &cnt);
+    unsigned F;
+    const unsigned f0 = block_excl_scan(cnt, &F);
+    if (F == 0u) return;   // uniform over the workgroup
+    const uint64_t gbase = blk[blockIdx.x];
+    int y0 = 0, x0 = 0, z0 = 0;
+    if (cnt) {
+        const int v = int(v0), pl = V.ny * V.nx;
+        y0 = v % V.ny; x0 = (v / V.ny) % V.nx; z0 = v / pl;
+    }
+    for (unsigned r0 = 0; r0 < F; r0 += STL_SB) {
+        if (cnt && f0 < r0 + STL_SB && f0 + cnt > r0) {
+            unsigned f = f0;
+            int y = y0, x = x0, z = z0;
+            for (int i = 0; i < STL_VT; ++i) {
+                const unsigned m = unsigned(masks >> (6 * i)) & 63u;
+                for (int kind = 0; kind < 6; ++kind) {
+                    if (!((m >> kind) & 1u)) continue;
+                    if (f >= r0 && f < r0 + STL_SB) stl_put_face(stage + 25 * (f - r0), kind, y, x, z);
+                    ++f;
+                }
+                if (++y == V.ny) {
+                    y = 0;
+                    if (++x == V.nx) {
+                        x = 0;
+                        ++z;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const unsigned nf = (F - r0 < unsigned(STL_SB)) ? F - r0 : unsigned(STL_SB);
+        uint32_t* dst = faces + 25 * (gbase + r0);
+        for (unsigned d = threadIdx.x; d < 25u * nf; d += STL_B) dst[d] = stage[d];
+        __syncthreads();
     }
 }
 
@@ -200,11 +253,12 @@ int stl_prepare(t3d_problem* p, const float* xPhys, float iso, StlVol& V, int64_
     hipStream_t st = p->stream;
     V.ny = p->g.resY; V.nx = p->g.resX; V.nz = p->g.resZ;
     V.n = p->g.ncell;
-    nb = (V.n + STL_B - 1) / STL_B;
-    if (!p->stlIns) T3D_HIP(hipMalloc(&p->stlIns, size_t(V.n)));
+    nb = (V.n + STL_BV - 1) / STL_BV;
+    const int64_t padded = nb * STL_BV;   // whole blocks: the 8-byte loads of the last one stay inside
+    if (!p->stlIns) T3D_HIP(hipMalloc(&p->stlIns, size_t(padded)));
     if (!p->stlOff) T3D_HIP(hipMalloc(&p->stlOff, sizeof(uint64_t) * size_t(nb + 1)));
     const uint8_t outsideVal = (0.0f >= iso) ? 1 : 0;   // cells without an element hold density 0
-    hipLaunchKernelGGL(k_stl_fill, dim3(4096), dim3(256), 0, st, V.n, outsideVal, p->stlIns);
+    hipLaunchKernelGGL(k_stl_fill, dim3(4096), dim3(256), 0, st, padded, outsideVal, p->stlIns);
     if (p->ne > 0)
         hipLaunchKernelGGL(k_stl_mark, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne,
                            (const int32_t*)p->compactToCell, p->g.resX, p->g.resY, xPhys, iso, p->stlIns);
