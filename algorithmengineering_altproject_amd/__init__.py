@@ -80,6 +80,7 @@ def lib():
         "t3d_filter_apply": (C.c_int, [vp, vp, vp, C.POINTER(C.c_int)]),
         "t3d_simp": (C.c_int, [vp, vp, C.c_float, C.c_float, vp]),
         "t3d_oc_update": (C.c_int, [vp, vp, vp, C.c_float, C.c_float, vp, vp]),
+        "t3d_oc_update_passive": (C.c_int, [vp, vp, vp, vp, C.c_float, C.c_float, vp, vp]),
         "t3d_seq_sum_f32": (C.c_int, [vp, vp, C.c_int64, C.POINTER(C.c_float)]),
         "t3d_params_default": (None, [vp]),
         "t3d_mixed_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
@@ -364,9 +365,12 @@ def simp(prob, dx, E0, Emin, dE):
     check(lib().t3d_simp(prob.h, dx.ptr, E0, Emin, dE.ptr), "t3d_simp")
 
 
-def oc_update(prob, dx, ddc, V0, move, dxnew):
+def oc_update(prob, dx, ddc, V0, move, dxnew, passive=None):
+    """OC update (driver.cpp:169-189); passive: optional device uint8 mask in compact
+    order, elements held at density 1 (matlab/TOP3D_XL.m:154)."""
     out = np.zeros(4, np.float32)
-    check(lib().t3d_oc_update(prob.h, dx.ptr, ddc.ptr, V0, move, dxnew.ptr, _ptr(out)), "t3d_oc_update")
+    check(lib().t3d_oc_update_passive(prob.h, dx.ptr, ddc.ptr, passive.ptr if passive is not None else None, V0,
+                                      move, dxnew.ptr, _ptr(out)), "t3d_oc_update_passive")
     return dict(change=np.float32(out[0]), sharpness=np.float32(out[1]), volFrac=np.float32(out[2]),
                 steps=int(out[3]))
 
@@ -393,6 +397,7 @@ class Params(C.Structure):
         ("mgMaxLevels", C.c_int), ("mgWeight", C.c_float), ("nonDyadic", C.c_int), ("NlimitDofs", C.c_int),
         ("rebuildEvery", C.c_int), ("precision", C.c_int), ("mgMode", C.c_int),
         ("stlPath", C.c_char_p), ("stlIso", C.c_float), ("summaryPath", C.c_char_p),
+        ("solid", C.c_void_p), ("passive", C.c_void_p),
     ]
 
 
@@ -434,13 +439,23 @@ def default_params(**kw):
     return p
 
 
-def top3d_run(nely, nelx, nelz, **kw):
+def top3d_run(nely, nelx, nelz, solid=None, passive=None, **kw):
     """TOP3D_XL_GLOBAL (cpp/src/driver.cpp:23-218) for one simulation, on the device.
-    Returns (records, x, Csolid, solidIters)."""
+    solid / passive: optional uint8 voxel masks indexed [ez, ex, ey] (a voxel domain;
+    voxels held at density 1).  Returns (records, x, Csolid, solidIters)."""
     P = default_params(**kw)
+    keep = []
+    for name, mask in (("solid", solid), ("passive", passive)):
+        if mask is not None:
+            a = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+            if a.size != nely * nelx * nelz:
+                raise ValueError("%s mask needs nely*nelx*nelz entries" % name)
+            keep.append(a)
+            setattr(P, name, a.ctypes.data)
     rec = (LoopRecord * max(1, P.nLoop))()
-    # the design has numElements = nelx*nely*nelz entries (padding is void)
-    x = np.zeros(nelx * nely * nelz, np.float32)
+    # the design has numElements entries: every voxel of the cuboid, or the solid ones
+    ne = nelx * nely * nelz if solid is None else int(np.count_nonzero(keep[0]))
+    x = np.zeros(ne, np.float32)
     Cs = C.c_double()
     si = C.c_int()
     nl = C.c_int()

@@ -42,6 +42,7 @@ void t3d_params_default(t3d_params* P)
     P->mgMaxLevels = 5; P->mgWeight = 0.6f; P->nonDyadic = 1; P->NlimitDofs = 1000;
     P->rebuildEvery = 5; P->precision = 0; P->mgMode = 0;
     P->stlPath = nullptr; P->stlIso = 0.3f; P->summaryPath = nullptr;
+    P->solid = nullptr; P->passive = nullptr;
 }
 
 // the reference's summary text (driver.cpp:225-258); iostream's scientific and
@@ -101,7 +102,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     t3d_mg* M = nullptr;
     t3d_filter* F = nullptr;
     t3d_mixed* MX = nullptr;
-    int rc = t3d_problem_create_cuboid(nely, nelx, nelz, &p);
+    int rc = t3d_problem_create_voxels(nely, nelx, nelz, P->solid, &p);
     if (rc) return rc;
     struct Guard {
         t3d_problem*& p;
@@ -151,6 +152,21 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     T3D_HIP(hipMalloc(&bu.p, sizeof(double) * nD));
     T3D_HIP(hipMalloc(&bb.p, sizeof(double) * nD));
     T3D_HIP(hipMalloc(&bsc.p, sizeof(double) * 16));
+    // passive voxels in compact element order (matlab/TOP3D_XL.m:85, 154)
+    DevBuf bpas;
+    if (P->passive) {
+        const HostModel& m = p->host;
+        std::vector<uint8_t> pc(size_t(ne), 0);
+        for (int e = 0; e < ne; ++e) {
+            const int full = m.eleMapBack[e];
+            const int ez = full / (m.resX * m.resY), rem = full % (m.resX * m.resY);
+            const int ex = rem / m.resY, ey = m.resY - 1 - rem % m.resY;
+            pc[size_t(e)] = P->passive[(size_t(ez) * nelx + ex) * nely + ey] ? 1 : 0;
+        }
+        T3D_HIP(hipMalloc(&bpas.p, size_t(ne)));
+        T3D_HIP(hipMemcpy(bpas.p, pc.data(), size_t(ne), hipMemcpyHostToDevice));
+    }
+    const uint8_t* passive = (const uint8_t*)bpas.p;
     float* x = (float*)bx.p;
     float* xn = (float*)bxn.p;
     double* E = (double*)bE.p;
@@ -206,7 +222,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         T3D_HIP(hipStreamSynchronize(st));
         const double tFilter = now_s() - tF;
         if ((rc = launch_dc_post(p, dcf, x, dc))) return rc;
-        if ((rc = launch_oc(p, x, dc, P->V0, 0.2f, xn, fsc))) return rc;
+        if ((rc = launch_oc(p, x, dc, P->V0, 0.2f, xn, fsc, passive))) return rc;
         double hs[3];
         float hf[5];
         T3D_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, st));

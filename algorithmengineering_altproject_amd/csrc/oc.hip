@@ -220,11 +220,16 @@ __device__ float block_seq_sum(int64_t n, const Gen& gen, SeqShared& sh)
     return res;
 }
 
+// passive (nullable): elements held at density 1 (matlab/TOP3D_XL.m:154)
 struct GenXnew {
     const float* x;
     const float* dc;
+    const uint8_t* passive;
     float lmid, move;
-    __device__ float operator()(int64_t i) const { return oc_xnew(x[i], dc[i], lmid, move); }
+    __device__ float operator()(int64_t i) const
+    {
+        return (passive && passive[i]) ? 1.0f : oc_xnew(x[i], dc[i], lmid, move);
+    }
 };
 struct GenX {
     const float* x;
@@ -238,7 +243,8 @@ struct GenX1mX {
 // OC bisection (driver.cpp:170-183) in one persistent workgroup: out[0] = last
 // lmid, out[1] = bisection steps
 __global__ void __launch_bounds__(SS_THREADS) k_oc_bisect(int ne, const float* __restrict__ x,
-                                                          const float* __restrict__ dc, float V0, float move,
+                                                          const float* __restrict__ dc,
+                                                          const uint8_t* __restrict__ passive, float V0, float move,
                                                           float* __restrict__ out)
 {
     __shared__ SeqShared sh;
@@ -246,7 +252,7 @@ __global__ void __launch_bounds__(SS_THREADS) k_oc_bisect(int ne, const float* _
     int steps = 0;
     while (double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6) {
         lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
-        GenXnew gx{x, dc, lmid, move};
+        GenXnew gx{x, dc, passive, lmid, move};
         const float vol = ((block_seq_sum(ne, gx, sh)) / (float(ne)));
         if (__fsub_rn(vol, V0) > 0) l1 = lmid;
         else l2 = lmid;
@@ -260,13 +266,13 @@ __global__ void __launch_bounds__(SS_THREADS) k_oc_bisect(int ne, const float* _
 
 // xnew for the final multiplier and change = max|xnew - x| (driver.cpp:184)
 __global__ void k_oc_apply(int ne, const float* __restrict__ x, const float* __restrict__ dc,
-                           const float* __restrict__ lm, float move, float* __restrict__ xnew,
-                           unsigned* __restrict__ changeBits)
+                           const uint8_t* __restrict__ passive, const float* __restrict__ lm, float move,
+                           float* __restrict__ xnew, unsigned* __restrict__ changeBits)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     float d = 0.0f;
     if (e < ne) {
-        const float v = oc_xnew(x[e], dc[e], lm[0], move);
+        const float v = (passive && passive[e]) ? 1.0f : oc_xnew(x[e], dc[e], lm[0], move);
         xnew[e] = v;
         d = fabsf(__fsub_rn(v, x[e]));
     }
@@ -446,14 +452,15 @@ int launch_dc_post(t3d_problem* p, const float* filt, const float* x, float* dc)
     return T3D_OK;
 }
 // scal: [0] lmid [1] steps [2] changeBits [3] sharpness [4] volFrac (float slots)
-int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal)
+int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal,
+              const uint8_t* passive)
 {
     hipStream_t st = p->stream;
-    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(SS_THREADS), 0, st, p->ne, x, dc, V0, move, scal);
+    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(SS_THREADS), 0, st, p->ne, x, dc, passive, V0, move, scal);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipMemsetAsync(scal + 2, 0, sizeof(float), st));
-    hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, scal, move,
-                       xnew, reinterpret_cast<unsigned*>(scal + 2));
+    hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, passive,
+                       scal, move, xnew, reinterpret_cast<unsigned*>(scal + 2));
     T3D_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_oc_stats, dim3(1), dim3(SS_THREADS), 0, st, p->ne, xnew, scal + 3);
     T3D_HIP(hipGetLastError());
@@ -500,9 +507,15 @@ int t3d_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E)
 // sharpness, out[2] volume fraction, out[3] bisection steps (host)
 int t3d_oc_update(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* out)
 {
+    return t3d_oc_update_passive(p, x, dc, nullptr, V0, move, xnew, out);
+}
+
+int t3d_oc_update_passive(t3d_problem* p, const float* x, const float* dc, const uint8_t* passive, float V0,
+                          float move, float* xnew, float* out)
+{
     T3D_CHECK(p && x && dc && xnew, T3D_ERR_ARG, "t3d_oc_update: null argument");
     float* sc = reinterpret_cast<float*>(p->scal + 16);
-    int rc = launch_oc(p, x, dc, V0, move, xnew, sc);
+    int rc = launch_oc(p, x, dc, V0, move, xnew, sc, passive);
     if (rc) return rc;
     float h[5];
     T3D_HIP(hipMemcpyAsync(h, sc, sizeof(float) * 5, hipMemcpyDeviceToHost, p->stream));

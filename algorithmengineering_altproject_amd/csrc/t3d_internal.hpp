@@ -116,5 +116,6 @@ int launch_fill_f32(t3d_problem* p, int64_t n, float v, float* a);
 int launch_sens(t3d_problem* p, const float* x, const double* ce, const double* Csolid, float E0, float Emin,
                 float penal, float* xdc);
 int launch_dc_post(t3d_problem* p, const float* filt, const float* x, float* dc);
-int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal);
+int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float move, float* xnew, float* scal,
+              const uint8_t* passive = nullptr);
 }  // namespace t3d

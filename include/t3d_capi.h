@@ -241,6 +241,12 @@ int t3d_simp(t3d_problem* p, const float* x_dev, float E0, float Emin, double* E
  * out_host[0] = change, [1] = sharpness, [2] = volume fraction, [3] = bisection steps */
 int t3d_oc_update(t3d_problem* p, const float* x_dev, const float* dc_dev, float V0, float move, float* xnew_dev,
                   float* out_host);
+/* The same with passive elements (passive_dev[ne], compact order, nonzero =
+ * passive) held at density 1 in every trial of the bisection and in xnew, as
+ * matlab/TOP3D_XL.m:154,169 (the C++ port declares passive layers,
+ * core.hpp:39-41, but never applies them).  passive_dev == NULL: t3d_oc_update. */
+int t3d_oc_update_passive(t3d_problem* p, const float* x_dev, const float* dc_dev, const uint8_t* passive_dev,
+                          float V0, float move, float* xnew_dev, float* out_host);
 /* std::accumulate<float> of v[0..n) reproduced exactly (driver.cpp:181,188,201) */
 int t3d_seq_sum_f32(t3d_problem* p, const float* v_dev, int64_t n, float* out_host);
 
@@ -268,6 +274,10 @@ typedef struct t3d_params {
     float stlIso;    /* default 0.3 (driver.cpp:274) */
     const char* summaryPath; /* when set, the run's summary (t3d_summary_format) is written
                                 there, as driver.cpp:264-270 writes it to out/comments/ */
+    const uint8_t* solid;    /* optional voxel domain, solid[(ez*nelx + ex)*nely + ey] != 0
+                                (t3d_problem_create_voxels); NULL = the cuboid */
+    const uint8_t* passive;  /* optional passive voxels, same indexing: held at density 1 by
+                                the OC update (t3d_oc_update_passive); NULL = none */
 } t3d_params;
 typedef struct t3d_loop_record {
     int loop, pcgIters, filterIters, bisectSteps;

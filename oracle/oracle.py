@@ -143,6 +143,12 @@ def lib():
         L.o_oc_update.restype = C.c_int
         L.o_oc_update.argtypes = [C.c_int, f32p, f32p, C.c_float, C.c_float, f32p, C.POINTER(C.c_float)]
         L.o_params_default.argtypes = [C.POINTER(OParams)]
+        L.o_oc_update_passive.restype = C.c_int
+        L.o_oc_update_passive.argtypes = [C.c_int, f32p, f32p, C.c_void_p, C.c_float, C.c_float, f32p,
+                                          C.POINTER(C.c_float)]
+        L.o_top3d_run_ex.restype = C.c_int
+        L.o_top3d_run_ex.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(OParams),
+                                     C.POINTER(OLoopRec), f32p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.o_stl_image.restype = C.c_uint64
         L.o_stl_image.argtypes = [C.c_int, C.c_int, C.c_int, f32p, C.c_float, C.c_void_p, C.c_uint64]
         L.o_top3d_run.restype = C.c_int
@@ -318,16 +324,21 @@ def default_params(**kw):
     return p
 
 
-def run(nely, nelx, nelz, **kw):
-    """Solid solve + OC loops (driver.cpp:23-218). Returns (records, x, Csolid, solidIters)."""
+def run(nely, nelx, nelz, solid=None, passive=None, **kw):
+    """Solid solve + OC loops (driver.cpp:23-218), on a voxel domain when solid is
+    given, with passive voxels held at 1 (matlab/TOP3D_XL.m:154) when passive is
+    given (uint8 masks indexed [ez, ex, ey]). Returns (records, x, Csolid, solidIters)."""
     p = default_params(**kw)
     rec = (OLoopRec * max(1, p.nLoop))()
-    m = Model(nely, nelx, nelz)
+    m = Model(nely, nelx, nelz, solid=solid)
     x = np.zeros(m.ne, np.float32)
     del m
     Cs = C.c_double()
     si = C.c_int()
-    n = lib().o_top3d_run(nely, nelx, nelz, C.byref(p), rec, x, C.byref(Cs), C.byref(si))
+    sm = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+    pm = None if passive is None else np.ascontiguousarray(passive, np.uint8).reshape(-1)
+    n = lib().o_top3d_run_ex(nely, nelx, nelz, None if sm is None else sm.ctypes.data,
+                             None if pm is None else pm.ctypes.data, C.byref(p), rec, x, C.byref(Cs), C.byref(si))
     recs = [dict(loop=r.loop, pcgIters=r.pcgIters, filterIters=r.filterIters, C=r.C, Cdisp=r.Cdisp,
                  volFrac=r.volFrac, sharpness=r.sharpness, change=r.change, fval=r.fval) for r in rec[:n]]
     return recs, x, Cs.value, si.value

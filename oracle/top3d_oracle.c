@@ -1244,11 +1244,25 @@ void o_sensitivity(int ne, const float* xPhys, const double* ce, double Csolid,
 int o_oc_update(int ne, const float* x, const float* dc, float V0, float move,
                 float* xnew, float* change)
 {
+    return o_oc_update_passive(ne, x, dc, NULL, V0, move, xnew, change);
+}
+
+/* passive[e] != 0: element e is held at density 1 inside the bisection, as
+ * matlab/TOP3D_XL.m:154 (xnew(passiveElements) = 1.0), so the volume of every
+ * trial counts it as 1; the C++ port declares but never uses passive layers
+ * (core.hpp:39-41).  passive == NULL: the C++ port's OC. */
+int o_oc_update_passive(int ne, const float* x, const float* dc, const uint8_t* passive, float V0, float move,
+                        float* xnew, float* change)
+{
     float l1 = 0.0f, l2 = 1e9f;
     int steps = 0;
     while ((double)((l2 - l1) / (l1 + l2)) > 1e-6) {
         float lmid = 0.5f * (l1 + l2);
         for (int e = 0; e < ne; ++e) {
+            if (passive && passive[e]) {
+                xnew[e] = 1.0f;
+                continue;
+            }
             float val = sqrtf(fmaxr(1e-30f, -dc[e] / lmid));
             float xe = fclampr(x[e] * val, x[e] - move, x[e] + move);
             xe = fclampr(xe, 0.0f, 1.0f);
@@ -1279,7 +1293,29 @@ void o_params_default(o_params* p)
 int o_top3d_run(int nely, int nelx, int nelz, const o_params* P,
                 o_loop_rec* rec, float* x_out, double* CsolidOut, int* solidIters)
 {
-    o_model* m = o_model_new(nely, nelx, nelz);
+    return o_top3d_run_ex(nely, nelx, nelz, NULL, NULL, P, rec, x_out, CsolidOut, solidIters);
+}
+
+/* passive mask of a voxel-order mask [(ez*nelx + ex)*nely + ey] in compact element order */
+static uint8_t* passive_compact(const o_model* m, const uint8_t* passive)
+{
+    if (!passive) return NULL;
+    uint8_t* pc = XALLOC(uint8_t, m->ne);
+    const int ny = m->resY, nx = m->resX;
+    for (int e = 0; e < m->ne; ++e) {
+        const int full = m->eleMapBack[e];
+        const int ez = full / (nx * ny), rem = full % (nx * ny);
+        const int ex = rem / ny, ey = ny - 1 - rem % ny;
+        pc[e] = passive[((size_t)ez * m->nelx + ex) * m->nely + ey] ? 1 : 0;
+    }
+    return pc;
+}
+
+int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, const o_params* P,
+                   o_loop_rec* rec, float* x_out, double* CsolidOut, int* solidIters)
+{
+    o_model* m = solid ? o_model_new_voxels(nely, nelx, nelz, solid) : o_model_new(nely, nelx, nelz);
+    uint8_t* pc = passive_compact(m, passive);
     o_filter* flt = o_filter_new(m, P->rMin);
     o_mg* g = o_mg_new(m, P->nonDyadic, P->mgMaxLevels, P->mgWeight, P->Emin, P->NlimitDofs);
     const int ne = m->ne;
@@ -1319,7 +1355,7 @@ int o_top3d_run(int nely, int nelx, int nelz, const o_params* P,
         for (int e = 0; e < ne; ++e) xdc[e] = x[e] * dc[e];
         o_filter_apply(m, flt, xdc, dcf);
         for (int e = 0; e < ne; ++e) dc[e] = dcf[e] / fmaxr(1.0e-3f, x[e]);
-        o_oc_update(ne, x, dc, P->V0, 0.2f, xnew, &change);
+        o_oc_update_passive(ne, x, dc, pc, P->V0, 0.2f, xnew, &change);
         memcpy(x, xnew, sizeof(float) * (size_t)ne);
         sharp = 4.0f * o_seq_sum_x1mx(x, ne) / (float)ne;
         float volFrac = o_seq_sum(x, ne) / (float)ne;
@@ -1331,7 +1367,7 @@ int o_top3d_run(int nely, int nelx, int nelz, const o_params* P,
         }
     }
     if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)ne);
-    free(items); free(b); free(x); free(xnew); free(dc); free(xdc); free(dcf); free(ce); free(E); free(u);
+    free(items); free(b); free(x); free(xnew); free(dc); free(xdc); free(dcf); free(ce); free(E); free(u); free(pc);
     o_mg_free(g); o_filter_free(flt); o_model_free(m);
     return loop;
 }

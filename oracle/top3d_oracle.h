@@ -146,6 +146,9 @@ void  o_sensitivity(int ne, const float* xPhys, const double* ce, double Csolid,
 /* OC bisection; returns number of bisection steps; writes xnew, change */
 int   o_oc_update(int ne, const float* x, const float* dc, float V0, float move,
                   float* xnew, float* change);
+/* the same with passive elements (passive[e] != 0) held at 1 (matlab/TOP3D_XL.m:154) */
+int   o_oc_update_passive(int ne, const float* x, const float* dc, const uint8_t* passive, float V0, float move,
+                          float* xnew, float* change);
 
 /* ---- full optimisation loop (driver.cpp:23-218) ---- */
 typedef struct o_params {
@@ -168,6 +171,10 @@ typedef struct o_loop_rec {
  * design; Csolid/solidIters receive the solid solve results. */
 int o_top3d_run(int nely, int nelx, int nelz, const o_params* p,
                 o_loop_rec* rec, float* x_out, double* Csolid, int* solidIters);
+/* the loop on a voxel domain (solid mask as o_model_new_voxels, NULL = cuboid)
+ * with passive elements (voxel-order mask, NULL = none) */
+int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, const o_params* p,
+                   o_loop_rec* rec, float* x_out, double* Csolid, int* solidIters);
 
 /* ---- voxel-surface STL (io.cpp:80-87, geometry_out/voxel_surface.hpp:12-93,
  * geometry_out/export_stl.hpp:14-33) ----

@@ -122,3 +122,44 @@ def test_top3d_loop_bitexact(t3d, dims, kw):
         for key in ("pcgIters", "filterIters", "C", "Cdisp", "volFrac", "sharpness", "change", "fval"):
             assert np.float64(a[key]) == np.float64(b[key]), (a["loop"], key, a[key], b[key])
     assert np.array_equal(x_g, x_o)
+
+
+def test_oc_passive_bitexact(t3d):
+    """OC with passive elements held at 1 (matlab/TOP3D_XL.m:154) against the oracle."""
+    import ctypes as C
+    dims = (30, 60, 30)
+    p = t3d.Problem(*dims)
+    m = O.Model(*dims)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0.01, 0.6, m.ne).astype(np.float32)
+    dc = -rng.uniform(0, 5, m.ne).astype(np.float32)
+    passive = (rng.random(m.ne) < 0.05).astype(np.uint8)
+    xnew_o = np.zeros(m.ne, np.float32)
+    ch = C.c_float()
+    steps_o = O.lib().o_oc_update_passive(m.ne, x, dc, passive.ctypes.data, 0.3, 0.2, xnew_o, C.byref(ch))
+    dxn = t3d.DeviceArray(m.ne, np.float32)
+    r = t3d.oc_update(p, t3d.DeviceArray.from_host(x), t3d.DeviceArray.from_host(dc), 0.3, 0.2, dxn,
+                      passive=t3d.DeviceArray.from_host(passive))
+    assert r["steps"] == steps_o
+    assert np.array_equal(dxn.download(), xnew_o)
+    assert r["change"].tobytes() == np.float32(ch.value).tobytes()
+
+
+def test_loop_voxel_domain_passive_bitexact(t3d):
+    """The loop on a voxel domain with voids and a passive layer at the loaded
+    end (t3d_params.solid / passive) against the oracle's loop, record by record."""
+    from test_voxels import spheres_mask
+    dims = (16, 32, 16)
+    solid = spheres_mask(*dims, [(10.0, 8.0, 8.0, 4.5), (22.0, 4.0, 10.0, 3.0)])
+    passive = np.zeros_like(solid)
+    passive[:, 31:, :] = 1
+    passive &= solid
+    kw = dict(nLoop=4, V0=0.2, cgMaxIt=200, Emin=1e-9, rMin=1.5)
+    rec_o, x_o, Cs_o, si_o = O.run(*dims, solid=solid, passive=passive, **kw)
+    rec_g, x_g, Cs_g, si_g = t3d.top3d_run(*dims, solid=solid, passive=passive, **kw)
+    assert Cs_g == Cs_o and si_g == si_o
+    assert len(rec_g) == len(rec_o) == 4
+    for a, b in zip(rec_o, rec_g):
+        for key in ("pcgIters", "filterIters", "C", "Cdisp", "volFrac", "sharpness", "change", "fval"):
+            assert np.float64(a[key]) == np.float64(b[key]), (a["loop"], key, a[key], b[key])
+    assert np.array_equal(x_g, x_o)

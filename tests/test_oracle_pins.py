@@ -11,6 +11,7 @@ index, a transposed operand, a dropped term) fails:
   * correctly rounded cube against exact rational arithmetic;
   * the canonical reduction against its written definition on small inputs.
 """
+import ctypes as C
 from fractions import Fraction as Fr
 
 import numpy as np
@@ -322,3 +323,62 @@ def test_sequential_float_sum():
     for x in v:
         s = np.float32(s + x)
     assert O.lib().o_seq_sum(v, v.size) == s
+
+
+def _oc(ne, x, dc, V0, passive=None):
+    xn = np.zeros(ne, np.float32)
+    ch = C.c_float()
+    pm = None if passive is None else np.ascontiguousarray(passive, np.uint8)
+    steps = O.lib().o_oc_update_passive(ne, x, dc, None if pm is None else pm.ctypes.data, V0, 0.2, xn,
+                                        C.byref(ch))
+    return xn, steps, ch.value
+
+
+def test_oc_passive_pins():
+    """Passive elements (matlab/TOP3D_XL.m:154,169) in the oracle's OC: no
+    passive element reproduces the C++ port's OC bit for bit; passive elements
+    end at 1; and since they count as 1 in every trial volume, the other
+    elements follow the plain OC of the remaining elements with the volume
+    target (V0 ne - |P|) / (ne - |P|), up to the bisection's 1e-6 tolerance
+    (a passive element left out of the volume, or counted at its old density,
+    moves them by far more)."""
+    rng = np.random.default_rng(11)
+    ne = 5000
+    x = rng.uniform(0.01, 0.6, ne).astype(np.float32)   # targets within the move limits
+    dc = -rng.uniform(0, 5, ne).astype(np.float32)
+    V0 = np.float32(0.3)
+    xa, sa, ca = _oc(ne, x, dc, V0)
+    xb = np.zeros(ne, np.float32)
+    chb = C.c_float()
+    sb = O.lib().o_oc_update(ne, x, dc, V0, 0.2, xb, C.byref(chb))
+    assert np.array_equal(xa, xb) and sa == sb and ca == chb.value
+    assert np.array_equal(_oc(ne, x, dc, V0, np.zeros(ne, np.uint8))[0], xb)
+    passive = (rng.random(ne) < 0.1).astype(np.uint8)
+    xp, _, _ = _oc(ne, x, dc, V0, passive)
+    assert np.all(xp[passive == 1] == 1.0)
+    act = passive == 0
+    nP = int(passive.sum())
+    V0r = np.float32((float(V0) * ne - nP) / (ne - nP))
+    xr, _, _ = _oc(int(act.sum()), np.ascontiguousarray(x[act]), np.ascontiguousarray(dc[act]), V0r)
+    assert np.abs(xp[act] - xr).max() <= 1e-5
+    wrong, _, _ = _oc(int(act.sum()), np.ascontiguousarray(x[act]), np.ascontiguousarray(dc[act]), V0)
+    assert np.abs(wrong - xr).max() > 1e-3
+    assert np.all(_oc(ne, x, dc, V0, np.ones(ne, np.uint8))[0] == 1.0)
+
+
+def test_loop_voxels_passive_runs():
+    """The oracle's loop on a voxel domain with passive voxels: the passive
+    voxels hold density 1 after every update and the volume fraction tracks V0."""
+    from test_voxels import spheres_mask
+    dims = (8, 16, 8)
+    solid = spheres_mask(*dims, [(8.0, 4.0, 4.0, 2.5)])
+    passive = np.zeros_like(solid)
+    passive[:, 15:, :] = 1          # the layer at the loaded end (ex = 15)
+    passive &= solid
+    recs, x, Cs, _ = O.run(*dims, solid=solid, passive=passive, nLoop=3, V0=0.3, rMin=1.5, Emin=1e-9)
+    m = O.Model(*dims, solid=solid)
+    ez, rem = m.eleMapBack // (m.resX * m.resY), m.eleMapBack % (m.resX * m.resY)
+    ex, ey = rem // m.resY, m.resY - 1 - rem % m.resY
+    pc = passive.reshape(-1)[(ez * dims[1] + ex) * dims[0] + ey].astype(bool)
+    assert pc.sum() > 0 and np.all(x[pc] == 1.0)
+    assert len(recs) == 3 and all(abs(r["volFrac"] - 0.3) < 2e-3 for r in recs)
