@@ -71,6 +71,7 @@ def lib():
         "t3d_mg_diag": (C.c_int, [vp, C.c_int, vp]),
         "t3d_mg_coarsest": (C.c_int, [vp, vp, vp]),
         "t3d_mg_set_mode": (C.c_int, [vp, C.c_int]),
+        "t3d_mg_set_vcycle": (C.c_int, [vp, C.c_int]),
         "t3d_pcg": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int)]),
         "t3d_pcg_history": (C.c_int, [vp, C.c_int, C.c_int, vp]),
         "t3d_compliance": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
@@ -267,7 +268,8 @@ class MG:
     """build_static_once + make_diagonal_preconditioner_from_static
     (cpp/src/multigrid/preconditioner.cpp:19-266) on the device."""
 
-    def __init__(self, prob, nonDyadic=1, maxLevels=5, weight=0.6, Emin=1e-6, NlimitDofs=1000, galerkin=False):
+    def __init__(self, prob, nonDyadic=1, maxLevels=5, weight=0.6, Emin=1e-6, NlimitDofs=1000, galerkin=False,
+                 standard=False):
         """galerkin=True: the Galerkin hierarchy of the MATLAB original
         (matlab/TOP3D_XL.m:772-1090) on the same levels, t3d_mg_set_mode(1)."""
         self.prob = prob
@@ -278,6 +280,8 @@ class MG:
         self.galerkin = bool(galerkin)
         if galerkin:
             check(lib().t3d_mg_set_mode(self.h, 1), "t3d_mg_set_mode")
+            if standard:
+                check(lib().t3d_mg_set_vcycle(self.h, 1), "t3d_mg_set_vcycle")
         nl = C.c_int()
         dims = (C.c_int * 32)()
         nc = C.c_int()

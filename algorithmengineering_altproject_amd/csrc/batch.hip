@@ -296,6 +296,7 @@ static int batch_build(t3d_batch* B)
         int rc = mg_view(B->M[i], v);
         if (rc) return rc;
         T3D_CHECK(v.nlev >= 2, T3D_ERR_STATE, "t3d_batch: every hierarchy needs at least two levels");
+        T3D_CHECK(v.vtype == 0, T3D_ERR_STATE, "t3d_batch: the 'Standard' V-cycle runs in the FP64 path only");
         BProb& q = B->hP[i];
         q.g = p->g;
         q.nlev = v.nlev;

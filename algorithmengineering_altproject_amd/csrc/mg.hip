@@ -336,6 +336,108 @@ __device__ __forceinline__ void gal_accumulate(const double* sA, double* sH, con
     __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// MATLAB's 'Standard' V-cycle (TOP3D_XL.m:783-803) in the Galerkin form:
+//   down:  x_l = w r_l / D_l;  r_{l+1} = P^T (r_l - A_l x_l)
+//   up:    x_l += P x_{l+1};   x_l += w (r_l - A_l x_l) / D_l
+// every result masked.  A_0 is the finest K*u (fixed DOFs 0); A_l (l >= 1)
+// is applied from the level's element matrices by k_gal_apply.
+// ---------------------------------------------------------------------------
+// op 0: x = w r / D;  op 1: x = r - y;  op 2: x += w (r - y) / D  (masked -> 0)
+__global__ void k_std_point(int op, int64_t nN, const double* __restrict__ r, const double* __restrict__ y,
+                            const double* __restrict__ D, const uint8_t* __restrict__ mask, double weight,
+                            double* __restrict__ x, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= nN) return;
+    const unsigned m = mask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t i = 3 * n + c;
+        double out = 0.0;
+        if (!((m >> c) & 1u)) {
+            if (op == 1) {
+                out = r[i] - y[i];
+            } else {
+                const double d = D[i];
+                const double dd = 1.0e-30 < d ? d : 1.0e-30;
+                out = (op == 0) ? weight * r[i] / dd : x[i] + weight * (r[i] - y[i]) / dd;
+            }
+        }
+        x[i] = out;
+    }
+}
+
+// xf += P xc on the fine level (trilinear, as k_prolong_smooth), masked -> 0
+__global__ void __launch_bounds__(256) k_prolong_add(LevelDev Lf, LevelDev Lc, int span,
+                                                     const double* __restrict__ xc,
+                                                     const uint8_t* __restrict__ fmask, double* __restrict__ xf,
+                                                     const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lf.nN) return;
+    const int fx = int(n % Lf.nnx), fy = int((n / Lf.nnx) % Lf.nny), fz = int(n / (int64_t(Lf.nnx) * Lf.nny));
+    int cx = fx / span, cy = fy / span, cz = fz / span;
+    int rx = fx % span, ry = fy % span, rz = fz % span;
+    if (cx >= Lc.resX) { cx = Lc.resX - 1; rx = span; }
+    if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = span; }
+    if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = span; }
+    const double inv = 1.0 / span;
+    const double wx1 = double(rx) * inv, wx0 = 1.0 - wx1;
+    const double wy1 = double(ry) * inv, wy0 = 1.0 - wy1;
+    const double wz1 = double(rz) * inv, wz0 = 1.0 - wz1;
+    const int64_t b00 = (int64_t(Lc.nnx) * Lc.nny * cz + int64_t(Lc.nnx) * cy + cx);
+    const int64_t b01 = b00 + 1, b10 = b00 + Lc.nnx, b11 = b10 + 1;
+    const int64_t s = int64_t(Lc.nnx) * Lc.nny;
+    const int64_t n00 = b00 + s, n01 = n00 + 1, n10 = n00 + Lc.nnx, n11 = n10 + 1;
+    const unsigned m = fmask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double v = xc[3 * b00 + c] * wx0 * wy0 * wz0 + xc[3 * b01 + c] * wx1 * wy0 * wz0 +
+                         xc[3 * b10 + c] * wx0 * wy1 * wz0 + xc[3 * b11 + c] * wx1 * wy1 * wz0 +
+                         xc[3 * n00 + c] * wx0 * wy0 * wz1 + xc[3 * n01 + c] * wx1 * wy0 * wz1 +
+                         xc[3 * n10 + c] * wx0 * wy1 * wz1 + xc[3 * n11 + c] * wx1 * wy1 * wz1;
+        xf[3 * n + c] = ((m >> c) & 1u) ? 0.0 : xf[3 * n + c] + v;
+    }
+}
+
+// y = A_l x from the level's element matrices (row-major 24 x 24 per cell, local
+// DOF 3a + c): one thread per node gathers the rows of its corner in each of
+// the (up to) eight cells around it
+__global__ void __launch_bounds__(256) k_gal_apply(LevelDev L, const double* __restrict__ Kel,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= L.nN) return;
+    const int ix = int(n % L.nnx), iy = int((n / L.nnx) % L.nny), iz = int(n / (int64_t(L.nnx) * L.nny));
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = ix - dx, cy = iy - dy, cz = iz - dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= L.resX || cy >= L.resY || cz >= L.resZ) continue;
+                // corner a of the node in the cell (corner_offsets(a) == (dx, dy, dz))
+                const int a = 4 * dz + (dy ? (dx ? 1 : 0) : (dx ? 2 : 3));
+                const double* K = Kel + ((int64_t(cz) * L.resY + cy) * L.resX + cx) * 576 + 3 * a * 24;
+                for (int b = 0; b < 8; ++b) {
+                    int ox, oy, oz;
+                    corner_offsets(b, ox, oy, oz);
+                    const int64_t nb = (int64_t(cz + oz) * L.nny + (cy + oy)) * L.nnx + (cx + ox);
+                    const double u0 = x[3 * nb], u1 = x[3 * nb + 1], u2 = x[3 * nb + 2];
+                    acc0 += K[3 * b] * u0 + K[3 * b + 1] * u1 + K[3 * b + 2] * u2;
+                    acc1 += K[24 + 3 * b] * u0 + K[24 + 3 * b + 1] * u1 + K[24 + 3 * b + 2] * u2;
+                    acc2 += K[48 + 3 * b] * u0 + K[48 + 3 * b + 1] * u1 + K[48 + 3 * b + 2] * u2;
+                }
+            }
+    y[3 * n] = acc0;
+    y[3 * n + 1] = acc1;
+    y[3 * n + 2] = acc2;
+}
+
 // Level 1 as a product.  A level-1 cell's matrix is sum_f E_f G_f over its
 // s^3 finest cells f, G_f = T_f^T Ke T_f shared by all cells; with the
 // effective moduli Eeff[cell][f] (E_f, or 0 for void cells, moduli <= 1e-16
@@ -675,12 +777,58 @@ struct t3d_mg {
     double* Eeff = nullptr;   // level-1 effective moduli, [cells][span^3]
     int32_t* fixCells = nullptr;   // level-1 cells covering finest cells with fixed DOFs
     int nFix = 0;
+    int vtype = 0;            // Galerkin V-cycle: 0 'Adapted', 1 'Standard' (FP64 path)
+    double* ys[8] = {};       // 'Standard': A_l x and residuals per level l < L
     double* Li = nullptr;     // inverse coarsest factor L^-1
     double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles
     double* yc = nullptr;     // device flag of the factorisation (after N doubles)
 };
 
 namespace t3d {
+
+// the 'Standard' V-cycle of the Galerkin form (TOP3D_XL.m:783-803); z holds x_0
+static int mg_apply_standard(t3d_mg* g, const double* r0, double* z, const int* done)
+{
+    t3d_problem* p = g->p;
+    hipStream_t st = p->stream;
+    const int L = g->nlev - 1;
+    const int B = 256;
+    auto grid = [&](int l) { return dim3(unsigned((g->ld[l].nN + B - 1) / B)); };
+    auto apply = [&](int l, const double* x, double* y) -> int {
+        if (l == 0) return launch_kxu_f64(p, x, y, done);
+        hipLaunchKernelGGL(k_gal_apply, grid(l), dim3(B), 0, st, g->ld[l], (const double*)g->Kel[l], x, y, done);
+        return hipGetLastError() == hipSuccess ? T3D_OK : T3D_ERR_HIP;
+    };
+    const double w = g->H.weight;
+    int rc;
+    for (int l = 0; l < L; ++l) {
+        const double* rl = (l == 0) ? r0 : g->r[l];
+        double* xl = (l == 0) ? z : g->x[l];
+        hipLaunchKernelGGL(k_std_point, grid(l), dim3(B), 0, st, 0, g->ld[l].nN, rl, (const double*)nullptr,
+                           (const double*)g->D[l], (const uint8_t*)g->mask[l], w, xl, done);
+        if ((rc = apply(l, xl, g->ys[l]))) return rc;
+        hipLaunchKernelGGL(k_std_point, grid(l), dim3(B), 0, st, 1, g->ld[l].nN, rl, (const double*)g->ys[l],
+                           (const double*)nullptr, (const uint8_t*)g->mask[l], w, g->ys[l], done);
+        hipLaunchKernelGGL(k_restrict, grid(l + 1), dim3(B), 0, st, g->ld[l], g->ld[l + 1], g->H.lv[l + 1].span,
+                           (const double*)g->ys[l], (const uint8_t*)g->mask[l + 1], g->r[l + 1],
+                           (const double*)nullptr, g->x[L], done, 0);
+        T3D_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
+                       (const double*)g->Ainv, (const double*)g->r[L], (const uint8_t*)g->mask[L], g->x[L], done);
+    T3D_HIP(hipGetLastError());
+    for (int l = L - 1; l >= 0; --l) {
+        const double* rl = (l == 0) ? r0 : g->r[l];
+        double* xl = (l == 0) ? z : g->x[l];
+        hipLaunchKernelGGL(k_prolong_add, grid(l), dim3(B), 0, st, g->ld[l], g->ld[l + 1], g->H.lv[l + 1].span,
+                           (const double*)g->x[l + 1], (const uint8_t*)g->mask[l], xl, done);
+        if ((rc = apply(l, xl, g->ys[l]))) return rc;
+        hipLaunchKernelGGL(k_std_point, grid(l), dim3(B), 0, st, 2, g->ld[l].nN, rl, (const double*)g->ys[l],
+                           (const double*)g->D[l], (const uint8_t*)g->mask[l], w, xl, done);
+        T3D_HIP(hipGetLastError());
+    }
+    return T3D_OK;
+}
 
 int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
 {
@@ -694,6 +842,7 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
         T3D_HIP(hipGetLastError());
         return T3D_OK;
     }
+    if (g->mode == 1 && g->vtype == 1) return mg_apply_standard(g, r0, z, done);
     const bool chol = g->Nc > 0;
     for (int l = 0; l < L; ++l) {
         const double* rf = (l == 0) ? r0 : g->r[l];
@@ -739,6 +888,7 @@ struct MGView {
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
     const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
+    int vtype;           // 0 'Adapted', 1 'Standard' (FP64 path only)
 };
 int mg_view(const t3d_mg* g, MGView& v)
 {
@@ -757,6 +907,7 @@ int mg_view(const t3d_mg* g, MGView& v)
     v.weight = g->H.weight;
     v.norm = g->mode == 0 ? 1 : 0;
     v.Ainv = g->mode == 1 ? g->Ainv : nullptr;
+    v.vtype = g->vtype;
     return T3D_OK;
 }
 }  // namespace t3d
@@ -982,6 +1133,8 @@ int t3d_mg_destroy(t3d_mg* g)
     }
     if (g->Gch) (void)hipFree(g->Gch);
     if (g->Eeff) (void)hipFree(g->Eeff);
+    for (int l = 0; l < 8; ++l)
+        if (g->ys[l]) (void)hipFree(g->ys[l]);
     if (g->fixCells) (void)hipFree(g->fixCells);
     if (g->Li) (void)hipFree(g->Li);
     if (g->Ainv) (void)hipFree(g->Ainv);
@@ -1083,7 +1236,20 @@ int t3d_mg_set_mode(t3d_mg* g, int mode)
         T3D_HIP(hipDeviceSynchronize());
     }
     g->mode = mode;
+    if (mode == 0) g->vtype = 0;
     g->setupDone = false;
+    return T3D_OK;
+}
+
+int t3d_mg_set_vcycle(t3d_mg* g, int type)
+{
+    T3D_CHECK(g && (type == 0 || type == 1), T3D_ERR_ARG, "t3d_mg_set_vcycle: type must be 0 or 1");
+    T3D_CHECK(type == 0 || g->mode == 1, T3D_ERR_STATE,
+              "t3d_mg_set_vcycle: the 'Standard' V-cycle needs the Galerkin form (t3d_mg_set_mode(g, 1))");
+    if (type == 1)
+        for (int l = 0; l + 1 < g->nlev; ++l)
+            if (!g->ys[l]) T3D_HIP(hipMalloc(&g->ys[l], sizeof(double) * 3 * size_t(g->H.lv[l].nN)));
+    g->vtype = type;
     return T3D_OK;
 }
 

@@ -937,6 +937,8 @@ int t3d_mixed_setup(t3d_mixed* s)
     MGView v;
     int rc = mg_view(s->M, v);
     if (rc) return rc;
+    T3D_CHECK(v.vtype == 0, T3D_ERR_STATE,
+              "t3d_mixed_setup: the 'Standard' V-cycle runs in the FP64 path only (t3d_pcg, t3d_mg_apply)");
     hipStream_t st = s->p->stream;
     for (int l = 0; l < v.nlev; ++l) {
         const int64_t n = 3 * v.nN[l];

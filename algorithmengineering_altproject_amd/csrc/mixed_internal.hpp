@@ -55,6 +55,7 @@ struct MGView {
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
     const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
+    int vtype;           // 0 'Adapted', 1 'Standard' (FP64 path only)
 };
 int mg_view(const t3d_mg* g, MGView& v);
 __global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,

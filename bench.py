@@ -88,6 +88,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+class stdout_to_stderr:
+    """Sends what native code writes to file descriptor 1 to stderr for the
+    duration (the reference's code logs to std::cout; stdout carries only the
+    result line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
 def cpu_baseline(args, E, iters):
     """The reference's own OpenMP code (oracle/_ref, -O3 -ffast-math -fopenmp) on this
     host: a bounded sample of `cpu_iters` MG-PCG iterations on the same problem,
@@ -230,7 +248,8 @@ def main_stl(args, t3d):
             vol[emb] = x
             path = "/tmp/t3d_bench_ref.stl"
             t1 = time.perf_counter()
-            rc = O.ref(fast=True).ref_stl_volume(ny, nx, nz, vol, iso, path.encode())
+            with stdout_to_stderr():
+                rc = O.ref(fast=True).ref_stl_volume(ny, nx, nz, vol, iso, path.encode())
             tc = time.perf_counter() - t1
             same = rc == 0 and os.path.getsize(path) == nbytes
             os.remove(path)
@@ -529,6 +548,16 @@ def main():
                      "compliance_rel_diff_mixed": abs(Cm - C64g) / abs(C64g)}
         log("FP64 PCG (Galerkin MG): %d iterations, %.1f ms/solve, compliance %.9e, true residual %.3e; "
             "mixed speed-up %.2fx" % (it64g, ms64g, C64g, res64g, ms64g / ms_step))
+        # the same hierarchy with MATLAB's 'Standard' V-cycle (FP64 path only)
+        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 1), "t3d_mg_set_vcycle")
+        it64s, ms64s = fp64_solve(t3d, p, Mg, db, dsol, tol, args.maxit)
+        C64s = comp()
+        res64s = true_residual(t3d, p, db, dsol)
+        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 0), "t3d_mg_set_vcycle")
+        fp64_same["standard_vcycle"] = {"ms_per_solve": ms64s, "pcg_iters": it64s, "ms_per_iteration": ms64s / it64s,
+                                        "compliance_rel_err": rel(C64s), "true_residual": res64s}
+        log("FP64 PCG (Galerkin MG, 'Standard' V-cycle): %d iterations, %.1f ms/solve, %.3f ms/iteration, "
+            "true residual %.3e" % (it64s, ms64s, ms64s / it64s, res64s))
 
     fp64 = None
     if not args.no_fp64:   # the FP64 plain path: the reference's algorithm, bit-exact, at the reference's tol
@@ -569,7 +598,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args, dE.download(), fp64["pcg_iters"] if fp64 else it_m)
+            with stdout_to_stderr():
+                cpu = cpu_baseline(args, dE.download(), fp64["pcg_iters"] if fp64 else it_m)
             if cpu:
                 log("CPU reference: %s" % cpu["sample"])
         except Exception as e:  # the baseline is reported, never required

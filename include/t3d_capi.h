@@ -129,6 +129,13 @@ int t3d_mg_coarsest(const t3d_mg* g, double* Kc_host, double* L_host);
  * Not bit-comparable with the reference (no counterpart in the C++ port);
  * t3d_pcg and t3d_pcg_mixed use it through the same object. */
 int t3d_mg_set_mode(t3d_mg* g, int mode);
+/* V-cycle of the Galerkin form (matlab/TOP3D_XL.m:772-807): 0 'Adapted' (default;
+ * restriction of the residual itself, one Jacobi correction), 1 'Standard'
+ * (TOP3D_XL.m:783-803: the residual recomputed with the level operator before
+ * restriction and after prolongation, A_l applied from the level's element
+ * matrices).  'Standard' runs in the FP64 path (t3d_pcg, t3d_mg_apply); the
+ * mixed and batched solvers refuse it (T3D_ERR_STATE).  Needs mode 1. */
+int t3d_mg_set_vcycle(t3d_mg* g, int type);
 
 /* ---- PCG ----
  * PCG_free (cpp/include/solver.hpp:22-25, cpp/src/solver.cpp:87-217) on the full

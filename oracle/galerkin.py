@@ -116,8 +116,12 @@ class GalerkinMG:
         self.Ac = Ac
         self.cf = sla.cho_factor(Ac, lower=True)
 
-    def apply(self, r):
-        """'Adapted' V-cycle (TOP3D_XL.m:772-807); the output is masked on every level."""
+    def apply(self, r, standard=False):
+        """'Adapted' V-cycle (TOP3D_XL.m:772-807); the output is masked on every level.
+        standard=True: the 'Standard' V-cycle (:783-803), the residual recomputed with
+        the level operator before restriction and after prolongation."""
+        if standard:
+            return self._apply_standard(r)
         L = len(self.A) - 1
         rs = [np.where(self.mask[0], 0.0, r)]
         for l in range(L):
@@ -128,13 +132,29 @@ class GalerkinMG:
             x = np.where(self.mask[l], 0.0, (t + self.P[l + 1] @ x) + t)
         return x
 
-    def pcg(self, b, tol, maxit):
+    def _apply_standard(self, r):
+        L = len(self.A) - 1
+        m = self.mask
+        rs = [np.where(m[0], 0.0, r)]
+        xs = []
+        for l in range(L):
+            x = np.where(m[l], 0.0, self.w * rs[l] / self.D[l])
+            d = np.where(m[l], 0.0, rs[l] - self.A[l] @ x)
+            xs.append(x)
+            rs.append(np.where(m[l + 1], 0.0, self.P[l + 1].T @ d))
+        x = np.where(m[L], 0.0, sla.cho_solve(self.cf, rs[L]))
+        for l in range(L - 1, -1, -1):
+            xl = np.where(m[l], 0.0, xs[l] + self.P[l + 1] @ x)
+            x = np.where(m[l], 0.0, xl + self.w * (rs[l] - self.A[l] @ xl) / self.D[l])
+        return x
+
+    def pcg(self, b, tol, maxit, standard=False):
         """Textbook PCG on A_0 (with unit rows on every fixed DOF) with this
         preconditioner: (iterations, x)."""
         A = self.A0
         x = np.zeros_like(b)
         r = b.copy()
-        z = self.apply(r)
+        z = self.apply(r, standard)
         p = z.copy()
         rz = r @ z
         nb = np.linalg.norm(b)
@@ -145,7 +165,7 @@ class GalerkinMG:
             r -= a * Ap
             if np.linalg.norm(r) < tol * nb:
                 return it + 1, x
-            z = self.apply(r)
+            z = self.apply(r, standard)
             rzn = r @ z
             p = z + rzn / rz * p
             rz = rzn
