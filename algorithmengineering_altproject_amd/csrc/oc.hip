@@ -13,6 +13,7 @@
 // and powf(rho,2) are evaluated correctly rounded.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -240,28 +241,219 @@ struct GenX1mX {
     __device__ float operator()(int64_t i) const { return __fmul_rn(x[i], __fsub_rn(1.0f, x[i])); }
 };
 
-// OC bisection (driver.cpp:170-183) in one persistent workgroup: out[0] = last
-// lmid, out[1] = bisection steps
-__global__ void __launch_bounds__(SS_THREADS) k_oc_bisect(int ne, const float* __restrict__ x,
-                                                          const float* __restrict__ dc,
-                                                          const uint8_t* __restrict__ passive, float V0, float move,
-                                                          float* __restrict__ out)
+// ---------------------------------------------------------------------------
+// The OC bisection with its exact sums spread over the GPU.  Per bisection
+// step: k_bis_start takes lmid and sums the first elements sequentially; then
+// rounds of k_ss_agg (every workgroup forms the transducer aggregate of one
+// chunk of OC densities in a window after the sum's position, under the
+// current binade) and k_ss_carry (one wavefront composes the window's
+// aggregates in chunk order, finds the first binade crossing, locates its
+// element and restarts after it in the new binade; without a crossing it moves
+// past the window and doubles it) until the sum is complete; k_bis_end takes
+// the bisection decision.  Every kernel returns at once when its flags say
+// there is nothing to do.  OC densities are >= 1e-3 (or 1 for passive
+// elements), so after the first 1024 elements the sum is >= 1 and crosses at
+// most log2(n) + 1 binades; with at most log2(n / 2^15) window doublings,
+// OC_ROUNDS rounds always complete it (k_bis_end flags an error otherwise).
+// ---------------------------------------------------------------------------
+constexpr int OC_CT = 256;                 // threads of an aggregate workgroup
+constexpr int OC_CPER = 8;                 // elements per thread
+constexpr int OC_CH = OC_CT * OC_CPER;     // elements per chunk
+constexpr int OC_ROUNDS = 64;
+constexpr int OC_W0 = 16;                 // first window, chunks
+constexpr int OC_AGG_GRID = 1024;         // workgroups of k_ss_agg (grid-stride over the window)
+
+struct OcBis {
+    float l1, l2, lmid;
+    int steps, done, err;
+    float s;          // running float sum
+    int ssDone;
+    long long i0;     // next element of the sum
+    long long W;      // window of the next round, chunks
+};
+
+__device__ __forceinline__ seqsum::Tr shfl_down_tr(const seqsum::Tr& t, int d)
 {
-    __shared__ SeqShared sh;
-    float l1 = 0.0f, l2 = 1e9f, lmid = 0.0f;
-    int steps = 0;
-    while (double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6) {
-        lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
-        GenXnew gx{x, dc, passive, lmid, move};
-        const float vol = ((block_seq_sum(ne, gx, sh)) / (float(ne)));
-        if (__fsub_rn(vol, V0) > 0) l1 = lmid;
-        else l2 = lmid;
-        ++steps;
+    seqsum::Tr o;
+    o.r0 = __shfl_down((long long)t.r0, d, 64);
+    o.r1 = __shfl_down((long long)t.r1, d, 64);
+    return o;
+}
+
+__global__ void k_bis_init(OcBis* __restrict__ B)
+{
+    B->l1 = 0.0f; B->l2 = 1e9f; B->lmid = 0.0f;
+    B->steps = 0; B->done = 0; B->err = 0; B->s = 0.0f; B->ssDone = 1; B->i0 = 0; B->W = OC_W0;
+}
+
+__global__ void k_bis_start(OcBis* __restrict__ B, int ne, const float* __restrict__ x, const float* __restrict__ dc,
+                            const uint8_t* __restrict__ passive, float move)
+{
+    if (B->done) return;
+    const float l1 = B->l1, l2 = B->l2;
+    if (!(double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6)) {   // driver.cpp:170
+        B->done = 1;
+        return;
     }
-    if (threadIdx.x == 0) {
-        out[0] = lmid;
-        out[1] = float(steps);
+    const float lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
+    B->lmid = lmid;
+    GenXnew gx{x, dc, passive, lmid, move};
+    float s = 0.0f;
+    int64_t i = 0;
+    for (; i < ne && (i < SS_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, gx(i));
+    B->s = s;
+    B->i0 = i;
+    B->W = OC_W0;
+    B->ssDone = i >= ne ? 1 : 0;
+}
+
+// aggT[c] = transducer of chunk c = [i0 + c OC_CH, ...) under the binade of s
+__global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, int ne, const float* __restrict__ x,
+                                                  const float* __restrict__ dc, const uint8_t* __restrict__ passive,
+                                                  float move, seqsum::Tr* __restrict__ aggT)
+{
+    using namespace seqsum;
+    if (B->done || B->ssDone) return;
+    const int64_t i0 = B->i0;
+    const int64_t nch = (ne - i0 + OC_CH - 1) / OC_CH;
+    const int64_t nw = nch < B->W ? nch : B->W;
+    int64_t m;
+    int ue;
+    decompose(B->s, m, ue);
+    GenXnew gx{x, dc, passive, B->lmid, move};
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ Tr wt[OC_CT / 64];
+    for (int64_t c = blockIdx.x; c < nw; c += gridDim.x) {   // uniform over the workgroup
+        const int64_t c0 = i0 + c * OC_CH;
+        Tr F = tr_identity();
+#pragma unroll
+        for (int k = 0; k < OC_CPER; ++k) {
+            const int64_t idx = c0 + int64_t(tid) * OC_CPER + k;
+            if (idx < ne) F = tr_compose(F, tr_elem(gx(idx), ue));
+        }
+        // ordered wave reduction: lane i takes lane i + d's aggregate after its own
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Tr o = shfl_down_tr(F, d);
+            if ((lane & (2 * d - 1)) == 0 && lane + d < 64) F = tr_compose(F, o);
+        }
+        if (lane == 0) wt[wave] = F;
+        __syncthreads();
+        if (tid == 0) {
+            Tr acc = tr_identity();
+            for (int w = 0; w < OC_CT / 64; ++w) acc = tr_compose(acc, wt[w]);
+            aggT[c] = acc;
+        }
+        __syncthreads();
     }
+}
+
+// one wavefront: compose the chunk aggregates in order from i0; at the first
+// binade crossing, find its element, add it with a real float add and restart
+// after it; otherwise the sum is complete
+__global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int ne, const float* __restrict__ x,
+                                                 const float* __restrict__ dc, const uint8_t* __restrict__ passive,
+                                                 float move, const seqsum::Tr* __restrict__ aggT)
+{
+    using namespace seqsum;
+    if (B->done || B->ssDone) return;
+    const int lane = threadIdx.x;
+    const int64_t i0 = B->i0;
+    const float s = B->s;
+    int64_t m;
+    int ue;
+    decompose(s, m, ue);
+    const int64_t nch = (ne - i0 + OC_CH - 1) / OC_CH;
+    const int64_t nw = nch < B->W ? nch : B->W;
+    int64_t cross = -1;   // chunk holding the crossing
+    for (int64_t b = 0; b < nw && cross < 0; b += 64) {
+        const int64_t c = b + lane;
+        Tr inc = (c < nw) ? aggT[c] : tr_identity();
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const Tr o = shfl_up_tr(inc, d);
+            if (lane >= d) inc = tr_compose(o, inc);
+        }
+        const bool crosses = c < nw && m + tr_apply(inc, m) >= M_TOP;
+        const unsigned long long ball = __ballot(crosses);
+        if (ball) {
+            const int first = __ffsll((long long)ball) - 1;
+            Tr ex = shfl_up_tr(inc, 1);
+            if (lane == 0) ex = tr_identity();
+            const int64_t r = tr_apply(ex, m);
+            m += __shfl(r, first, 64);          // m before the crossing chunk
+            cross = b + first;
+        } else {
+            const Tr tot{__shfl(inc.r0, 63, 64), __shfl(inc.r1, 63, 64)};
+            m += tr_apply(tot, m);
+        }
+    }
+    if (cross < 0) {   // no crossing in the window: move past it (same binade), double it
+        if (lane == 0) {
+            B->s = ldexpf(float(m), ue);
+            const int64_t next = i0 + nw * OC_CH;
+            B->i0 = next < ne ? next : ne;
+            B->ssDone = next >= ne ? 1 : 0;
+            B->W = 2 * B->W;
+        }
+        return;
+    }
+    // the crossing chunk: 64 lanes x 32 elements, a wave scan, then one lane walks its run
+    GenXnew gx{x, dc, passive, B->lmid, move};
+    const int64_t cs = i0 + cross * OC_CH;
+    constexpr int RUN = OC_CH / 64;
+    Tr T = tr_identity();
+    for (int k = 0; k < RUN; ++k) {
+        const int64_t idx = cs + int64_t(lane) * RUN + k;
+        if (idx < ne) T = tr_compose(T, tr_elem(gx(idx), ue));
+    }
+    Tr inc = T;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const Tr o = shfl_up_tr(inc, d);
+        if (lane >= d) inc = tr_compose(o, inc);
+    }
+    const bool crosses = m + tr_apply(inc, m) >= M_TOP;
+    const unsigned long long ball = __ballot(crosses);
+    const int first = __ffsll((long long)ball) - 1;   // the chunk crosses, so some lane does
+    Tr ex = shfl_up_tr(inc, 1);
+    if (lane == 0) ex = tr_identity();
+    if (lane == first) {
+        int64_t mm = m + tr_apply(ex, m);
+        for (int k = 0; k < RUN; ++k) {
+            const int64_t idx = cs + int64_t(lane) * RUN + k;
+            const float v = gx(idx);
+            const int64_t r = tr_apply(tr_elem(v, ue), mm);
+            if (mm + r >= M_TOP) {
+                B->s = __fadd_rn(ldexpf(float(mm), ue), v);
+                B->i0 = idx + 1;
+                B->ssDone = (idx + 1 >= ne) ? 1 : 0;
+                break;
+            }
+            mm += r;
+        }
+    }
+}
+
+__global__ void k_bis_end(OcBis* __restrict__ B, int ne, float V0)
+{
+    if (B->done) return;
+    if (!B->ssDone) {   // more binade crossings than the rounds allow: cannot happen for OC densities
+        B->err = 1;
+        B->done = 1;
+        return;
+    }
+    const float vol = ((B->s) / (float(ne)));
+    if (__fsub_rn(vol, V0) > 0) B->l1 = B->lmid;
+    else B->l2 = B->lmid;
+    ++B->steps;
+}
+
+// out[0] = last lmid, out[1] = bisection steps
+__global__ void k_bis_out(const OcBis* __restrict__ B, float* __restrict__ out)
+{
+    out[0] = B->lmid;
+    out[1] = float(B->steps);
 }
 
 // xnew for the final multiplier and change = max|xnew - x| (driver.cpp:184)
@@ -456,7 +648,39 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
               const uint8_t* passive)
 {
     hipStream_t st = p->stream;
-    hipLaunchKernelGGL(k_oc_bisect, dim3(1), dim3(SS_THREADS), 0, st, p->ne, x, dc, passive, V0, move, scal);
+    const int ne = p->ne;
+    const int64_t nchMax = (int64_t(ne) + OC_CH - 1) / OC_CH + 1;
+    if (!p->ocBis) T3D_HIP(hipMalloc(&p->ocBis, sizeof(OcBis)));
+    if (p->ocAggCap < nchMax) {
+        if (p->ocAgg) T3D_HIP(hipFree(p->ocAgg));
+        p->ocAgg = nullptr;
+        T3D_HIP(hipMalloc(&p->ocAgg, sizeof(seqsum::Tr) * size_t(nchMax)));
+        p->ocAggCap = nchMax;
+    }
+    OcBis* B = static_cast<OcBis*>(p->ocBis);
+    seqsum::Tr* agg = static_cast<seqsum::Tr*>(p->ocAgg);
+    const unsigned aggGrid = unsigned(std::min<int64_t>(nchMax, OC_AGG_GRID));
+    hipLaunchKernelGGL(k_bis_init, dim3(1), dim3(1), 0, st, B);
+    // bisection steps in blocks of 8, checked on the host between blocks; the
+    // float bisection from [0, 1e9] ends within ~180 steps even when l1 stays 0
+    int hostDone[2] = {0, 0};
+    for (int step = 0; step < 256 && !hostDone[0]; step += 8) {
+        for (int k = 0; k < 8; ++k) {
+            hipLaunchKernelGGL(k_bis_start, dim3(1), dim3(1), 0, st, B, ne, x, dc, passive, move);
+            for (int r = 0; r < OC_ROUNDS; ++r) {
+                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, ne, x, dc, passive,
+                                   move, agg);
+                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, ne, x, dc, passive, move,
+                                   (const seqsum::Tr*)agg);
+            }
+            hipLaunchKernelGGL(k_bis_end, dim3(1), dim3(1), 0, st, B, ne, V0);
+        }
+        T3D_HIP(hipGetLastError());
+        T3D_HIP(hipMemcpyAsync(hostDone, &B->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipStreamSynchronize(st));
+    }
+    T3D_CHECK(hostDone[0] && !hostDone[1], T3D_ERR_STATE, "OC bisection: exact volume sum did not complete");
+    hipLaunchKernelGGL(k_bis_out, dim3(1), dim3(1), 0, st, (const OcBis*)B, scal);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipMemsetAsync(scal + 2, 0, sizeof(float), st));
     hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, passive,

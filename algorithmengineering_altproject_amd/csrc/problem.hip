@@ -227,7 +227,7 @@ int t3d_problem_destroy(t3d_problem* p)
     if (!p) return T3D_OK;
     void* bufs[] = {p->Ke, p->nodeFixed, p->compactToCell, p->cellToCompact, p->freeDofIndex,
                     p->Ecell, p->EcellF, p->bFull, p->red, p->scal, p->wr, p->wz, p->wp, p->wAp, p->pcgSc,
-                    p->pcgFlags, p->stlIns, p->stlOff};
+                    p->pcgFlags, p->stlIns, p->stlOff, p->ocBis, p->ocAgg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->hostFlags) (void)hipHostFree(p->hostFlags);

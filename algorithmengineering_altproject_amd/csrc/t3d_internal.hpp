@@ -97,6 +97,10 @@ struct t3d_problem {
     // voxel-surface STL scratch (stl.hip), allocated on first use
     uint8_t* stlIns = nullptr;          // [ncell] inside flags in the volume's order
     uint64_t* stlOff = nullptr;         // [blocks + 1] face offsets per block of 256 voxels
+    // OC bisection state and chunk aggregates of its exact sums (oc.hip)
+    void* ocBis = nullptr;
+    void* ocAgg = nullptr;
+    int64_t ocAggCap = 0;
 };
 
 namespace t3d {
