@@ -260,6 +260,8 @@ constexpr int OC_CT = 256;                 // threads of an aggregate workgroup
 constexpr int OC_CPER = 8;                 // elements per thread
 constexpr int OC_CH = OC_CT * OC_CPER;     // elements per chunk
 constexpr int OC_ROUNDS = 64;
+constexpr int OC_GROUP = 8;               // rounds launched between host checks
+constexpr int OC_PREFIX = 64;             // elements summed one by one before the rounds (any count is exact)
 constexpr int OC_W0 = 16;                 // first window, chunks
 constexpr int OC_AGG_GRID = 1024;         // workgroups of k_ss_agg (grid-stride over the window)
 
@@ -300,7 +302,7 @@ __global__ void k_bis_start(OcBis* __restrict__ B, int ne, const float* __restri
     GenXnew gx{x, dc, passive, lmid, move};
     float s = 0.0f;
     int64_t i = 0;
-    for (; i < ne && (i < SS_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, gx(i));
+    for (; i < ne && (i < OC_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, gx(i));
     B->s = s;
     B->i0 = i;
     B->W = OC_W0;
@@ -663,18 +665,25 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
     hipLaunchKernelGGL(k_bis_init, dim3(1), dim3(1), 0, st, B);
     // bisection steps in blocks of 8, checked on the host between blocks; the
     // float bisection from [0, 1e9] ends within ~180 steps even when l1 stays 0
+    // one step at a time: rounds in groups of OC_GROUP, the sum's completion
+    // (ssDone) read back between groups; a finished bisection ends the loop
     int hostDone[2] = {0, 0};
-    for (int step = 0; step < 256 && !hostDone[0]; step += 8) {
-        for (int k = 0; k < 8; ++k) {
-            hipLaunchKernelGGL(k_bis_start, dim3(1), dim3(1), 0, st, B, ne, x, dc, passive, move);
-            for (int r = 0; r < OC_ROUNDS; ++r) {
+    int hostSS[4] = {0, 0, 0, 0};   // done, err, s, ssDone
+    for (int step = 0; step < 256 && !hostDone[0]; ++step) {
+        hipLaunchKernelGGL(k_bis_start, dim3(1), dim3(1), 0, st, B, ne, x, dc, passive, move);
+        for (int r = 0; r < OC_ROUNDS; r += OC_GROUP) {
+            for (int k = 0; k < OC_GROUP; ++k) {
                 hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, ne, x, dc, passive,
                                    move, agg);
                 hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, ne, x, dc, passive, move,
                                    (const seqsum::Tr*)agg);
             }
-            hipLaunchKernelGGL(k_bis_end, dim3(1), dim3(1), 0, st, B, ne, V0);
+            T3D_HIP(hipGetLastError());
+            T3D_HIP(hipMemcpyAsync(hostSS, &B->done, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+            T3D_HIP(hipStreamSynchronize(st));
+            if (hostSS[0] || hostSS[3]) break;   // bisection finished, or this step's sum complete
         }
+        hipLaunchKernelGGL(k_bis_end, dim3(1), dim3(1), 0, st, B, ne, V0);
         T3D_HIP(hipGetLastError());
         T3D_HIP(hipMemcpyAsync(hostDone, &B->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
