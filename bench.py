@@ -20,7 +20,12 @@ from the reference's 800 to 4000: neither solver reaches 1e-3 within 800
 iterations at this size).  `roofline` is the dominant kernel of the mixed solve
 (the fused FP32 K*u), timed with HIP events around each of its launches inside
 the timed solves, on its stream; `cpu_baseline` is the reference's own OpenMP
-code timed on this host.  `--batch 64` runs the 64-problem uneven batch.
+code timed on this host.  `--batch 64` runs the 64-problem uneven batch;
+`--loop N` times N loops of the device-resident optimisation loop (mixed
+Galerkin solves against the C++ port's algorithm on the device); `--stl`
+times the device-built voxel-surface STL image of a lattice design on the
+same grid against the reference's writer on one host core.  Each mode prints
+its own single result line.
 """
 import argparse
 import json
