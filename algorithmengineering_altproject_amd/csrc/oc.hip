@@ -236,10 +236,6 @@ struct GenX {
     const float* x;
     __device__ float operator()(int64_t i) const { return x[i]; }
 };
-struct GenX1mX {
-    const float* x;
-    __device__ float operator()(int64_t i) const { return __fmul_rn(x[i], __fsub_rn(1.0f, x[i])); }
-};
 
 // ---------------------------------------------------------------------------
 // The OC bisection with its exact sums spread over the GPU.  Per bisection
@@ -273,6 +269,15 @@ struct OcBis {
     long long i0;     // next element of the sum
     long long W;      // window of the next round, chunks
 };
+
+// the summed elements: 0 = OC densities at B->lmid, 1 = x, 2 = x (1 - x)
+__device__ __forceinline__ float oc_gen(int mode, int64_t i, const float* __restrict__ x, const float* __restrict__ dc,
+                                        const uint8_t* __restrict__ passive, float lmid, float move)
+{
+    if (mode == 1) return x[i];
+    if (mode == 2) return __fmul_rn(x[i], __fsub_rn(1.0f, x[i]));
+    return (passive && passive[i]) ? 1.0f : oc_xnew(x[i], dc[i], lmid, move);
+}
 
 __device__ __forceinline__ seqsum::Tr shfl_down_tr(const seqsum::Tr& t, int d)
 {
@@ -310,9 +315,10 @@ __global__ void k_bis_start(OcBis* __restrict__ B, int ne, const float* __restri
 }
 
 // aggT[c] = transducer of chunk c = [i0 + c OC_CH, ...) under the binade of s
-__global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, int ne, const float* __restrict__ x,
-                                                  const float* __restrict__ dc, const uint8_t* __restrict__ passive,
-                                                  float move, seqsum::Tr* __restrict__ aggT)
+__global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, int mode, int ne,
+                                                  const float* __restrict__ x, const float* __restrict__ dc,
+                                                  const uint8_t* __restrict__ passive, float move,
+                                                  seqsum::Tr* __restrict__ aggT)
 {
     using namespace seqsum;
     if (B->done || B->ssDone) return;
@@ -322,7 +328,7 @@ __global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, i
     int64_t m;
     int ue;
     decompose(B->s, m, ue);
-    GenXnew gx{x, dc, passive, B->lmid, move};
+    const float lmid = B->lmid;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     __shared__ Tr wt[OC_CT / 64];
     for (int64_t c = blockIdx.x; c < nw; c += gridDim.x) {   // uniform over the workgroup
@@ -331,7 +337,7 @@ __global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, i
 #pragma unroll
         for (int k = 0; k < OC_CPER; ++k) {
             const int64_t idx = c0 + int64_t(tid) * OC_CPER + k;
-            if (idx < ne) F = tr_compose(F, tr_elem(gx(idx), ue));
+            if (idx < ne) F = tr_compose(F, tr_elem(oc_gen(mode, idx, x, dc, passive, lmid, move), ue));
         }
         // ordered wave reduction: lane i takes lane i + d's aggregate after its own
 #pragma unroll
@@ -353,7 +359,7 @@ __global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, i
 // one wavefront: compose the chunk aggregates in order from i0; at the first
 // binade crossing, find its element, add it with a real float add and restart
 // after it; otherwise the sum is complete
-__global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int ne, const float* __restrict__ x,
+__global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int mode, int ne, const float* __restrict__ x,
                                                  const float* __restrict__ dc, const uint8_t* __restrict__ passive,
                                                  float move, const seqsum::Tr* __restrict__ aggT)
 {
@@ -401,13 +407,13 @@ __global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int ne, 
         return;
     }
     // the crossing chunk: 64 lanes x 32 elements, a wave scan, then one lane walks its run
-    GenXnew gx{x, dc, passive, B->lmid, move};
+    const float lmid = B->lmid;
     const int64_t cs = i0 + cross * OC_CH;
     constexpr int RUN = OC_CH / 64;
     Tr T = tr_identity();
     for (int k = 0; k < RUN; ++k) {
         const int64_t idx = cs + int64_t(lane) * RUN + k;
-        if (idx < ne) T = tr_compose(T, tr_elem(gx(idx), ue));
+        if (idx < ne) T = tr_compose(T, tr_elem(oc_gen(mode, idx, x, dc, passive, lmid, move), ue));
     }
     Tr inc = T;
 #pragma unroll
@@ -424,7 +430,7 @@ __global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int ne, 
         int64_t mm = m + tr_apply(ex, m);
         for (int k = 0; k < RUN; ++k) {
             const int64_t idx = cs + int64_t(lane) * RUN + k;
-            const float v = gx(idx);
+            const float v = oc_gen(mode, idx, x, dc, passive, lmid, move);
             const int64_t r = tr_apply(tr_elem(v, ue), mm);
             if (mm + r >= M_TOP) {
                 B->s = __fadd_rn(ldexpf(float(mm), ue), v);
@@ -435,6 +441,29 @@ __global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int ne, 
             mm += r;
         }
     }
+}
+
+// start of an exact sum of x (mode 1) or x (1 - x) (mode 2): the elements are
+// added one by one until the sum is a normal float, then the rounds take over
+__global__ void k_es_start(OcBis* __restrict__ B, int mode, int ne, const float* __restrict__ x)
+{
+    float s = 0.0f;
+    int64_t i = 0;
+    for (; i < ne && (i < OC_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, oc_gen(mode, i, x, nullptr, nullptr, 0.0f, 0.0f));
+    B->done = 0;
+    B->err = 0;
+    B->s = s;
+    B->i0 = i;
+    B->W = OC_W0;
+    B->ssDone = i >= ne ? 1 : 0;
+}
+
+// sharpness = 4 sum(x(1-x))/ne (mode 2, out[0]) and volume fraction sum(x)/ne
+// (mode 1, out[1]) of xnew (driver.cpp:188-201)
+__global__ void k_es_out(const OcBis* __restrict__ B, int mode, int ne, float* __restrict__ out)
+{
+    if (mode == 2) out[0] = ((__fmul_rn(4.0f, B->s)) / (float(ne)));
+    else out[1] = ((B->s) / (float(ne)));
 }
 
 __global__ void k_bis_end(OcBis* __restrict__ B, int ne, float V0)
@@ -474,21 +503,6 @@ __global__ void k_oc_apply(int ne, const float* __restrict__ x, const float* __r
     unsigned b = __float_as_uint(d);
     for (int w = 32; w >= 1; w >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, w, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(changeBits, b);
-}
-
-// sharpness = 4*sum(x(1-x))/ne and volume fraction sum(x)/ne (driver.cpp:188-201)
-__global__ void __launch_bounds__(SS_THREADS) k_oc_stats(int ne, const float* __restrict__ x,
-                                                         float* __restrict__ out)
-{
-    __shared__ SeqShared sh;
-    GenX1mX g1{x};
-    const float s1 = block_seq_sum(ne, g1, sh);
-    GenX g2{x};
-    const float s2 = block_seq_sum(ne, g2, sh);
-    if (threadIdx.x == 0) {
-        out[0] = ((__fmul_rn(4.0f, s1)) / (float(ne)));
-        out[1] = ((s2) / (float(ne)));
-    }
 }
 
 // generic sequential float sum of an array (for tests)
@@ -673,9 +687,9 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
         hipLaunchKernelGGL(k_bis_start, dim3(1), dim3(1), 0, st, B, ne, x, dc, passive, move);
         for (int r = 0; r < OC_ROUNDS; r += OC_GROUP) {
             for (int k = 0; k < OC_GROUP; ++k) {
-                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, ne, x, dc, passive,
-                                   move, agg);
-                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, ne, x, dc, passive, move,
+                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, 0, ne, x, dc,
+                                   passive, move, agg);
+                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, 0, ne, x, dc, passive, move,
                                    (const seqsum::Tr*)agg);
             }
             T3D_HIP(hipGetLastError());
@@ -695,8 +709,25 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
     hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, passive,
                        scal, move, xnew, reinterpret_cast<unsigned*>(scal + 2));
     T3D_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_oc_stats, dim3(1), dim3(SS_THREADS), 0, st, p->ne, xnew, scal + 3);
-    T3D_HIP(hipGetLastError());
+    // sharpness and volume fraction of xnew, exact, spread over the GPU like the bisection's sums
+    for (int mode = 2; mode >= 1; --mode) {
+        hipLaunchKernelGGL(k_es_start, dim3(1), dim3(1), 0, st, B, mode, ne, (const float*)xnew);
+        int hs[4] = {0, 0, 0, 0};
+        for (int r = 0; r < 32 * OC_ROUNDS && !hs[3]; r += OC_GROUP) {
+            for (int k = 0; k < OC_GROUP; ++k) {
+                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, mode, ne,
+                                   (const float*)xnew, (const float*)nullptr, (const uint8_t*)nullptr, 0.0f, agg);
+                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, mode, ne, (const float*)xnew,
+                                   (const float*)nullptr, (const uint8_t*)nullptr, 0.0f, (const seqsum::Tr*)agg);
+            }
+            T3D_HIP(hipGetLastError());
+            T3D_HIP(hipMemcpyAsync(hs, &B->done, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+            T3D_HIP(hipStreamSynchronize(st));
+        }
+        T3D_CHECK(hs[3], T3D_ERR_STATE, "OC statistics: exact sum did not complete");
+        hipLaunchKernelGGL(k_es_out, dim3(1), dim3(1), 0, st, (const OcBis*)B, mode, ne, scal + 3);
+        T3D_HIP(hipGetLastError());
+    }
     return T3D_OK;
 }
 
