@@ -432,6 +432,7 @@ class LoopRecord(C.Structure):
         ("C", C.c_double), ("Cdisp", C.c_double),
         ("volFrac", C.c_float), ("sharpness", C.c_float), ("change", C.c_float), ("fval", C.c_float),
         ("tSolve", C.c_double), ("tOpti", C.c_double), ("tFilter", C.c_double), ("tTotal", C.c_double),
+        ("xCheck", C.c_uint64),
     ]
 
 

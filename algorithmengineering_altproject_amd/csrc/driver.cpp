@@ -97,6 +97,8 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
               "t3d_top3d_run: precision must be 0 (FP64 plain path) or 1 (mixed-precision solve)");
     T3D_CHECK(P->mgMode == 0 || P->mgMode == 1, T3D_ERR_ARG,
               "t3d_top3d_run: mgMode must be 0 (the C++ port's V-cycle) or 1 (Galerkin multigrid)");
+    T3D_CHECK(P->rebuildEvery > 0, T3D_ERR_ARG, "t3d_top3d_run: rebuildEvery must be positive");
+    T3D_CHECK(P->nLoop >= 0 && P->cgMaxIt >= 0, T3D_ERR_ARG, "t3d_top3d_run: negative nLoop or cgMaxIt");
     if (loopsRun) *loopsRun = 0;
     t3d_problem* p = nullptr;
     t3d_mg* M = nullptr;
@@ -176,7 +178,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     float* dc = (float*)bdc.p;
     double* u = (double*)bu.p;
     double* b = (double*)bb.p;
-    double* sc = (double*)bsc.p;   // [0] Csolid [1] C [2] cObj ; float slots from [8]
+    double* sc = (double*)bsc.p;   // [0] Csolid [1] C [2] cObj [3] design checksum; float slots from [8]
     float* fsc = reinterpret_cast<float*>(sc + 8);
 
     T3D_HIP(hipMemcpyAsync(b, p->bFull, sizeof(double) * nD, hipMemcpyDeviceToDevice, st));
@@ -197,6 +199,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
 
     int loop = 0;
     float change = 1.0f, sharp = 1.0f;
+    float objFirst = 0.0f, objLast = 0.0f, sumCG = 0.0f, sumOpt = 0.0f, sumFilter = 0.0f, sumIter = 0.0f;
     while (loop < P->nLoop && double(change) > 1e-4 && double(sharp) > 0.01) {
         const double tIt = now_s();
         ++loop;
@@ -223,9 +226,10 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         const double tFilter = now_s() - tF;
         if ((rc = launch_dc_post(p, dcf, x, dc))) return rc;
         if ((rc = launch_oc(p, x, dc, P->V0, 0.2f, xn, fsc, passive))) return rc;
-        double hs[3];
+        if ((rc = x_check_dev(p, ne, xn, reinterpret_cast<unsigned long long*>(sc + 3)))) return rc;
+        double hs[4];
         float hf[5];
-        T3D_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * 3, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipMemcpyAsync(hs, sc, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipMemcpyAsync(hf, fsc, sizeof(float) * 5, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
         std::swap(x, xn);
@@ -233,6 +237,15 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         sharp = hf[3];
         const float volFrac = hf[4];
         const double tOpti = now_s() - tO;
+        const double Cdisp = (Csolid > 0) ? hs[2] * Csolid : hs[1];
+        // the summary's figures, accumulated as the reference does (driver.cpp:203-210)
+        if (loop == 1) objFirst = float(Cdisp);
+        objLast = float(Cdisp);
+        sumCG += float(tSolve);
+        sumOpt += float(tOpti);
+        sumFilter += float(tFilter);
+        const double tTotal = now_s() - tIt;
+        sumIter += float(tTotal);
         if (rec && loop - 1 < recCap) {
             t3d_loop_record& R = rec[loop - 1];
             R.loop = loop;
@@ -240,7 +253,7 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
             R.filterIters = t3d_filter_last_iters(F);
             R.bisectSteps = int(hf[1]);
             R.C = hs[1];
-            R.Cdisp = (Csolid > 0) ? hs[2] * Csolid : hs[1];
+            R.Cdisp = Cdisp;
             R.volFrac = volFrac;
             R.sharpness = sharp;
             R.change = change;
@@ -248,7 +261,8 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
             R.tSolve = tSolve;
             R.tOpti = tOpti;
             R.tFilter = tFilter;
-            R.tTotal = now_s() - tIt;
+            R.tTotal = tTotal;
+            std::memcpy(&R.xCheck, &hs[3], sizeof(uint64_t));
         }
     }
     T3D_HIP(hipDeviceSynchronize());
@@ -260,14 +274,8 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
         S.resX = p->g.resX; S.resY = p->g.resY; S.resZ = p->g.resZ;
         S.V0 = P->V0; S.iterations = loop; S.cgMaxIt = P->cgMaxIt; S.Csolid = Csolid;
         S.threads = 1;
-        for (int k = 0; k < loop && k < recCap && rec; ++k) {
-            if (k == 0) S.objFirst = float(rec[k].Cdisp);
-            S.objLast = float(rec[k].Cdisp);
-            S.sumCG += float(rec[k].tSolve);
-            S.sumOpt += float(rec[k].tOpti);
-            S.sumFilter += float(rec[k].tFilter);
-            S.sumIter += float(rec[k].tTotal);
-        }
+        S.objFirst = objFirst; S.objLast = objLast;
+        S.sumCG = sumCG; S.sumOpt = sumOpt; S.sumFilter = sumFilter; S.sumIter = sumIter;
         char text[2048];
         size_t n = 0;
         if ((rc = t3d_summary_format(&S, text, sizeof(text), &n))) return rc;

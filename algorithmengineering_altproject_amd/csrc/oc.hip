@@ -605,6 +605,41 @@ __global__ void __launch_bounds__(256) k_cobj_tiles(int ne, const double* __rest
     }
 }
 
+// o_x_check's position-keyed checksum of a float design (the oracle's
+// definition): sum of splitmix64((i << 32) | bits(x[i])) modulo 2^64.  Integer
+// addition is associative, so the block partials and the atomic combine give
+// the oracle's value in any order.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__global__ void k_zero_u64(unsigned long long* __restrict__ out) { *out = 0ull; }
+__global__ void __launch_bounds__(256) k_x_check(int64_t n, const float* __restrict__ x,
+                                                 unsigned long long* __restrict__ out)
+{
+    unsigned long long h = 0ull;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        h += splitmix64((static_cast<unsigned long long>(i) << 32) | __float_as_uint(x[i]));
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) h += __shfl_down(h, w, 64);
+    __shared__ unsigned long long sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, sh[0] + sh[1] + sh[2] + sh[3]);
+}
+
+int x_check_dev(t3d_problem* p, int64_t n, const float* x, unsigned long long* dst)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, p->stream, dst);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_x_check, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, p->stream, n, x, dst);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
 int compliance_dev(t3d_problem* p, const double* E, const double* U, double* ce, double* Cdst)
 {
     int rc = ensure_red(p, p->ne);

@@ -115,6 +115,8 @@ int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done);
 int compliance_dev(t3d_problem* p, const double* E, const double* U, double* ce, double* Cdst);
 int cobj_dev(t3d_problem* p, const double* E, const double* ce, const double* Csolid, double* dst);
 int launch_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E);
+// position-keyed checksum of x[0..n) to dst (device), as the oracle's o_x_check
+int x_check_dev(t3d_problem* p, int64_t n, const float* x, unsigned long long* dst);
 int launch_fill_f64(t3d_problem* p, int64_t n, double v, double* a);
 int launch_fill_f32(t3d_problem* p, int64_t n, float v, float* a);
 int launch_sens(t3d_problem* p, const float* x, const double* ce, const double* Csolid, float E0, float Emin,

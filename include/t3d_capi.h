@@ -291,6 +291,9 @@ typedef struct t3d_loop_record {
     double C, Cdisp;
     float volFrac, sharpness, change, fval;
     double tSolve, tOpti, tFilter, tTotal;   /* host wall-clock seconds */
+    uint64_t xCheck;   /* checksum of the design after the loop: sum over elements i
+                          (compact order) of splitmix64((i << 32) | bits(x[i])) mod 2^64
+                          (no reference counterpart; a per-loop density fingerprint) */
 } t3d_loop_record;
 void t3d_params_default(t3d_params* p);
 /* Runs the solid reference solve and up to P->nLoop OC loops; rec[recCap]

@@ -82,9 +82,10 @@ class OParams(C.Structure):
 
 class OLoopRec(C.Structure):
     _fields_ = [
-        ("loop", C.c_int), ("pcgIters", C.c_int), ("filterIters", C.c_int),
+        ("loop", C.c_int), ("pcgIters", C.c_int), ("filterIters", C.c_int), ("bisectSteps", C.c_int),
         ("C", C.c_double), ("Cdisp", C.c_double),
         ("volFrac", C.c_float), ("sharpness", C.c_float), ("change", C.c_float), ("fval", C.c_float),
+        ("xCheck", C.c_uint64),
     ]
 
 
@@ -138,6 +139,8 @@ def lib():
         L.o_seq_sum.argtypes = [f32p, C.c_int]
         L.o_seq_sum_x1mx.restype = C.c_float
         L.o_seq_sum_x1mx.argtypes = [f32p, C.c_int]
+        L.o_x_check.restype = C.c_uint64
+        L.o_x_check.argtypes = [f32p, C.c_int64]
         L.o_simp.argtypes = [C.c_int, f32p, C.c_float, C.c_float, f64p]
         L.o_sensitivity.argtypes = [C.c_int, f32p, f64p, C.c_double, C.c_float, C.c_float, C.c_float, f32p]
         L.o_oc_update.restype = C.c_int
@@ -339,8 +342,7 @@ def run(nely, nelx, nelz, solid=None, passive=None, **kw):
     pm = None if passive is None else np.ascontiguousarray(passive, np.uint8).reshape(-1)
     n = lib().o_top3d_run_ex(nely, nelx, nelz, None if sm is None else sm.ctypes.data,
                              None if pm is None else pm.ctypes.data, C.byref(p), rec, x, C.byref(Cs), C.byref(si))
-    recs = [dict(loop=r.loop, pcgIters=r.pcgIters, filterIters=r.filterIters, C=r.C, Cdisp=r.Cdisp,
-                 volFrac=r.volFrac, sharpness=r.sharpness, change=r.change, fval=r.fval) for r in rec[:n]]
+    recs = [{f: getattr(r, f) for f, _ in OLoopRec._fields_} for r in rec[:n]]
     return recs, x, Cs.value, si.value
 
 

@@ -1281,6 +1281,26 @@ int o_oc_update_passive(int ne, const float* x, const float* dc, const uint8_t* 
 /* ===================================================================== */
 /* driver (driver.cpp:23-218)                                             */
 /* ===================================================================== */
+static uint64_t splitmix64(uint64_t z)
+{
+    /* finaliser of Steele, Lea & Flood's SplitMix64 (OOPSLA 2014) */
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t o_x_check(const float* x, int64_t n)
+{
+    uint64_t h = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t b;
+        memcpy(&b, &x[i], 4);
+        h += splitmix64(((uint64_t)i << 32) | b);
+    }
+    return h;
+}
+
 void o_params_default(o_params* p)
 {
     /* core.hpp:26-42, driver.cpp:54, top3d_cli.cpp:33 */
@@ -1355,7 +1375,7 @@ int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uin
         for (int e = 0; e < ne; ++e) xdc[e] = x[e] * dc[e];
         o_filter_apply(m, flt, xdc, dcf);
         for (int e = 0; e < ne; ++e) dc[e] = dcf[e] / fmaxr(1.0e-3f, x[e]);
-        o_oc_update_passive(ne, x, dc, pc, P->V0, 0.2f, xnew, &change);
+        const int bsteps = o_oc_update_passive(ne, x, dc, pc, P->V0, 0.2f, xnew, &change);
         memcpy(x, xnew, sizeof(float) * (size_t)ne);
         sharp = 4.0f * o_seq_sum_x1mx(x, ne) / (float)ne;
         float volFrac = o_seq_sum(x, ne) / (float)ne;
@@ -1364,6 +1384,7 @@ int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uin
             r->loop = loop; r->pcgIters = its; r->filterIters = flt->lastIters;
             r->C = C; r->Cdisp = Cdisp; r->volFrac = volFrac; r->sharpness = sharp;
             r->change = change; r->fval = volFrac - P->V0;
+            r->bisectSteps = bsteps; r->xCheck = o_x_check(x, ne);
         }
     }
     if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)ne);

@@ -161,10 +161,17 @@ typedef struct o_params {
 void o_params_default(o_params* p);
 
 typedef struct o_loop_rec {
-    int loop, pcgIters, filterIters;
+    int loop, pcgIters, filterIters, bisectSteps;
     double C, Cdisp;
     float volFrac, sharpness, change, fval;
+    uint64_t xCheck;   /* o_x_check of the design after the loop */
 } o_loop_rec;
+
+/* Position-keyed checksum of a float design: sum over i of
+ * splitmix64(((uint64)i << 32) | bits(x[i])) modulo 2^64 (order-free, so a
+ * parallel device reduction gives the same value).  Not in the reference: a
+ * compact per-loop fingerprint of the density for the parity traces. */
+uint64_t o_x_check(const float* x, int64_t n);
 
 /* Runs the solid solve and up to p->nLoop OC loops.  rec has room for nLoop
  * records; returns the number of loops run; x_out [ne] receives the final
