@@ -75,6 +75,7 @@ def lib():
         "t3d_pcg": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int)]),
         "t3d_pcg_history": (C.c_int, [vp, C.c_int, C.c_int, vp]),
         "t3d_compliance": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
+        "t3d_compliance_noalloc": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "t3d_filter_create": (C.c_int, [vp, C.c_float, C.POINTER(vp)]),
         "t3d_filter_destroy": (C.c_int, [vp]),
         "t3d_filter_info": (C.c_int, [vp, vp, vp]),
@@ -362,6 +363,13 @@ class Filter:
 def compliance(prob, dE, dU, dce):
     c = C.c_double()
     check(lib().t3d_compliance(prob.h, dE.ptr, dU.ptr, dce.ptr, C.byref(c)), "t3d_compliance")
+    return c.value
+
+
+def compliance_noalloc(prob, dE, dU, dce):
+    """t3d_compliance_noalloc: the same result with no device allocation (static scratch)."""
+    c = C.c_double()
+    check(lib().t3d_compliance_noalloc(prob.h, dE.ptr, dU.ptr, dce.ptr, C.byref(c)), "t3d_compliance_noalloc")
     return c.value
 
 

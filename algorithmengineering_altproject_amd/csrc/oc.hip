@@ -17,6 +17,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "seqsum.hpp"
 #include "t3d_internal.hpp"
@@ -768,6 +769,48 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
 
 }  // namespace t3d
 
+// ---------------------------------------------------------------------------
+// Compliance without allocation: the canonical tree's tile partials and the
+// result live in one file-scope device array instead of the problem's lazily
+// allocated reduction buffer.  Each device holds its own copy of the array
+// (its address is looked up once per device with hipGetSymbolAddress and
+// kept); every call on a device shares it, so calls that overlap on one
+// device race.  Every slot a call reads is written by that call first, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t NOALLOC_TILES = (int64_t(T3D_NOALLOC_MAX_ELEMENTS) + t3d::CANON_TILE - 1) / t3d::CANON_TILE;
+constexpr int64_t NOALLOC_RESULT = 2 * NOALLOC_TILES;       // slot of the total
+constexpr int64_t NOALLOC_DOUBLES = NOALLOC_RESULT + 32;    // 1 MiB + 256 B
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace t3d {
+// device address of `scratch` on device dev (looked up with dev current, once)
+static int scratch_ptr(int dev, double** out)
+{
+    T3D_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, T3D_ERR_ARG, "scratch: device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        int cur = 0;
+        T3D_HIP(hipGetDevice(&cur));
+        if (cur != dev) T3D_HIP(hipSetDevice(dev));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) T3D_HIP(hipSetDevice(cur));
+        T3D_CHECK(e == hipSuccess && ptr, T3D_ERR_HIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return T3D_OK;
+}
+}  // namespace t3d
+
 using namespace t3d;
 
 extern "C" {
@@ -778,6 +821,24 @@ int t3d_compliance(t3d_problem* p, const double* E, const double* U, double* ce,
     int rc = compliance_dev(p, E, U, ce, p->scal);
     if (rc) return rc;
     if (C_host) T3D_HIP(hipMemcpyAsync(C_host, p->scal, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    T3D_HIP(hipStreamSynchronize(p->stream));
+    return T3D_OK;
+}
+
+int t3d_compliance_noalloc(t3d_problem* p, const double* E, const double* U, double* ce, double* C_host)
+{
+    T3D_CHECK(p && E && U && ce, T3D_ERR_ARG, "t3d_compliance_noalloc: null argument");
+    T3D_CHECK(p->ne <= T3D_NOALLOC_MAX_ELEMENTS, T3D_ERR_ARG,
+              "t3d_compliance_noalloc: more elements than T3D_NOALLOC_MAX_ELEMENTS");
+    double* buf = nullptr;
+    int rc = scratch_ptr(p->device, &buf);
+    if (rc) return rc;
+    const int64_t tiles = (p->ne + CANON_TILE - 1) / CANON_TILE;
+    hipLaunchKernelGGL(k_compliance, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, p->stream, p->g, p->ne,
+                       p->compactToCell, p->Ke, E, U, ce, buf);
+    T3D_HIP(hipGetLastError());
+    if ((rc = canon_finish_buf(p, buf, tiles, buf + NOALLOC_RESULT))) return rc;
+    if (C_host) T3D_HIP(hipMemcpyAsync(C_host, buf + NOALLOC_RESULT, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     T3D_HIP(hipStreamSynchronize(p->stream));
     return T3D_OK;
 }

@@ -93,8 +93,14 @@ int ensure_red(t3d_problem* p, int64_t items)
 // reduces p->red[0..ntiles) (first-level tile partials) to one value at dst
 int canon_finish(t3d_problem* p, int64_t ntiles, double* dst, const int* done)
 {
-    double* src = p->red;
-    double* alt = p->red + ntiles;
+    return canon_finish_buf(p, p->red, ntiles, dst, done);
+}
+
+// the same over buf[0..ntiles), with buf[ntiles..2 ntiles) as the second buffer
+int canon_finish_buf(t3d_problem* p, double* buf, int64_t ntiles, double* dst, const int* done)
+{
+    double* src = buf;
+    double* alt = buf + ntiles;
     int64_t m = ntiles;
     while (m > 1) {
         const int64_t m2 = (m + CANON_TILE - 1) / CANON_TILE;
@@ -267,6 +273,10 @@ int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid
     p->ne = m.numElements;
     p->nF = int64_t(m.freeDofIndex.size());
     auto fail = [&](int rc) { t3d_problem_destroy(p); return rc; };
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        return fail(T3D_ERR_HIP);
+    }
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         return fail(T3D_ERR_HIP);

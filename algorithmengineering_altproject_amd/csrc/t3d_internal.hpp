@@ -74,6 +74,7 @@ struct t3d_problem {
     t3d::Grid g;
     int ne = 0;
     int64_t nF = 0;
+    int device = 0;                     // the HIP device the problem was created on
     hipStream_t stream = nullptr;
     // device arrays
     double* Ke = nullptr;               // [576]
@@ -107,6 +108,8 @@ namespace t3d {
 int ensure_red(t3d_problem* p, int64_t items);
 // canonical sum of n items produced by kernel `first` into p->red; final value to dst (device)
 int canon_finish(t3d_problem* p, int64_t ntiles, double* dst, const int* done = nullptr);
+// the same over the tile partials buf[0..ntiles), buf[ntiles..2 ntiles) the second buffer
+int canon_finish_buf(t3d_problem* p, double* buf, int64_t ntiles, double* dst, const int* done = nullptr);
 int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst,
                   const int* done = nullptr);
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);

@@ -229,6 +229,18 @@ int t3d_batch_solve(t3d_batch* b, const double* const* b_dev, double* const* x_d
  * ce[e] = ue^T Ke ue (0 when E <= 1e-16), returns sum E*ce (canonical tree).
  * E: compact element moduli; U: full layout displacements; ce: compact [ne]. */
 int t3d_compliance(t3d_problem* p, const double* E_dev, const double* U_full_dev, double* ce_dev, double* C_host);
+/* t3d_compliance without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results, bit for bit; the reduction's partial sums live in a static device
+ * array of the library (about 1 MiB per device, held until the process ends)
+ * instead of the problem's lazily allocated buffer, and the call makes no
+ * hipMalloc/hipFree.  T3D_ERR_ARG above T3D_NOALLOC_MAX_ELEMENTS elements.
+ * All calls on one device share that array: calls that overlap in time on
+ * one device (other threads, other problems' streams) race. */
+#define T3D_NOALLOC_MAX_ELEMENTS 67108864
+int t3d_compliance_noalloc(t3d_problem* p, const double* E_dev, const double* U_full_dev, double* ce_dev,
+                           double* C_host);
 
 /* ---- PDE filter ----
  * SetupPDEFilter (cpp/include/filter.hpp:17, cpp/src/filter.cpp:9-75) */
