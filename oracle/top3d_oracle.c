@@ -8,6 +8,7 @@
  * a fixed order, so results do not depend on the thread count.
  */
 #include "top3d_oracle.h"
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -1359,6 +1360,8 @@ int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uin
     double Csolid = o_compliance(m, E, u, ce);
     if (CsolidOut) *CsolidOut = Csolid;
     if (solidIters) *solidIters = it0;
+    const int progress = getenv("O_LOOP_PROGRESS") != NULL;   /* one stderr line per solve */
+    if (progress) fprintf(stderr, "oracle loop: solid solve %d iterations\n", it0);
     int loop = 0;
     float change = 1.0f, sharp = 1.0f;
     double* items = XALLOC(double, ne);
@@ -1386,6 +1389,7 @@ int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uin
             r->change = change; r->fval = volFrac - P->V0;
             r->bisectSteps = bsteps; r->xCheck = o_x_check(x, ne);
         }
+        if (progress) fprintf(stderr, "oracle loop %d: %d PCG iterations, C %.17g\n", loop, its, C);
     }
     if (x_out) memcpy(x_out, x, sizeof(float) * (size_t)ne);
     free(items); free(b); free(x); free(xnew); free(dc); free(xdc); free(dcf); free(ce); free(E); free(u); free(pc);

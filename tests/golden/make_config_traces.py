@@ -9,14 +9,16 @@ same loops on the device and requires every record bit for bit.
                          most 800 iterations, the reference's multigrid rules
   trace_96x192x96.json   the FP64 drop-in workload: 96x192x96 (1.77M voxels,
                          5.42M free DOFs), same material, load and parameters,
-                         the first 5 OC loops (the oracle takes ~5 minutes a
-                         loop on 8 host cores)
+                         the first 5 OC loops (about 4 minutes in all on 16
+                         host cores with the oracle built with
+                         `make -C oracle OPT="-O3 -march=native"`, which gives
+                         the same bits as the default build)
 
 Each record holds the loop's PCG and filter iteration counts, bisection steps,
 compliance C and Cdisp (FP64, as float.hex), volume fraction, sharpness,
 change and fval (FP32 bit patterns) and o_x_check of the design after the loop.
 
-usage: python tests/golden/make_config_traces.py [cfg0|cfg1 ...]
+usage: [O_LOOP_PROGRESS=1] python tests/golden/make_config_traces.py [cfg0|cfg1 ...]
 """
 import json
 import os
