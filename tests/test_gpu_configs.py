@@ -47,3 +47,157 @@ def test_loop_trace_bitexact(t3d, trace):
     # the final design itself against the last record's checksum
     import oracle as O
     assert str(O.lib().o_x_check(np.ascontiguousarray(x, np.float32), x.size)) == T["records"][-1]["xCheck"]
+
+
+# ---------------------------------------------------------------------------
+# The chosen forms at their workloads' sizes
+# ---------------------------------------------------------------------------
+def _true_rel_residual(t3d, p, db, dx):
+    dy = t3d.DeviceArray(p.nDOF)
+    p.kxu_f64(dx, dy)
+    b = db.download()
+    return float(np.linalg.norm(b - dy.download()) / np.linalg.norm(b))
+
+
+def _simp(t3d, p, dx):
+    dE = t3d.DeviceArray(p.ne)
+    t3d.simp(p, dx, 1.0, float(np.float32(1e-9)), dE)
+    return dE
+
+
+def test_config2_mixed_vs_fp64_43M(t3d):
+    """192x384x192 (14.2M voxels, 42.9M free DOFs), uniform rho = 0.12: the
+    mixed-precision solve (FP32 Walsh-form K*u and V-cycle, flexible FP64 outer
+    PCG, Galerkin multigrid) reaches a true residual below 1e-6 and its
+    compliance is within 1e-6 relative of the FP64 solve with the same
+    multigrid; one optimisation loop with mixed solves against the same loop
+    with FP64 solves: compliance within 1e-6, density L-infinity reported and
+    bounded by 1e-3."""
+    dims = (192, 384, 192)
+    p = t3d.Problem(*dims)
+    dE = _simp(t3d, p, t3d.DeviceArray.from_host(np.full(p.ne, np.float32(0.12), np.float32)))
+    M = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
+    M.setup(dE)
+    mx = t3d.Mixed(p, M)
+    mx.setup()
+    db = t3d.DeviceArray(p.nDOF)
+    p.load_vector(db)
+    ce = t3d.DeviceArray(p.ne)
+    x = t3d.DeviceArray(p.nDOF)
+    x.zero()
+    it64 = t3d.pcg(p, M, db, x, 1e-6, 500)
+    C64 = t3d.compliance(p, dE, x, ce)
+    x.zero()
+    itm = mx.pcg(db, x, 1e-6, 500, trueEvery=25)
+    Cm = t3d.compliance(p, dE, x, ce)
+    res = _true_rel_residual(t3d, p, db, x)
+    print("config 2 solve: FP64 %d it, mixed %d it, true residual %.3e, C rel diff %.3e"
+          % (it64, itm, res, abs(Cm - C64) / C64))
+    assert it64 < 500 and itm < 500
+    assert res < 1e-6
+    assert abs(Cm - C64) <= 1e-6 * abs(C64)
+    del M, mx, x, db, ce, dE, p
+    kw = dict(nLoop=1, cgTol=1e-6, cgMaxIt=500, Emin=1e-9, mgMode=1)
+    rec64, x64, Cs64, _ = t3d.top3d_run(*dims, precision=0, **kw)
+    recm, xm, Csm, _ = t3d.top3d_run(*dims, precision=1, **kw)
+    linf = float(np.abs(xm.astype(np.float64) - x64.astype(np.float64)).max())
+    print("config 2 loop: Csolid rel diff %.3e, C rel diff %.3e, density L-inf %.3e, PCG %d / %d"
+          % (abs(Csm - Cs64) / Cs64, abs(recm[0]["C"] - rec64[0]["C"]) / rec64[0]["C"], linf,
+             rec64[0]["pcgIters"], recm[0]["pcgIters"]))
+    assert abs(Csm - Cs64) <= 1e-6 * Cs64
+    assert abs(recm[0]["C"] - rec64[0]["C"]) <= 1e-6 * rec64[0]["C"]
+    assert linf <= 1e-3
+
+
+def test_config3_mfma_operators_and_solve_101M(t3d):
+    """256x512x256 (33.5M voxels, 101M free DOFs), rho ~ U[1e-3, 1] (seed 3)
+    PDE-filtered with rMin 2: the MFMA operators against the FP64 operator
+    (bf16 x3: 2-piece products, |y - y64| <= 3e-5 max|y64|; bf16 x6: <= 5e-7),
+    and the mixed solve with the bf16 x6 operator and the Galerkin multigrid:
+    true residual below 1e-6, compliance within 1e-6 of the FP64 solve."""
+    dims = (256, 512, 256)
+    p = t3d.Problem(*dims)
+    raw = t3d.DeviceArray.from_host(np.random.default_rng(3).uniform(1e-3, 1.0, p.ne).astype(np.float32))
+    rho = t3d.DeviceArray(p.ne, np.float32)
+    t3d.Filter(p, 2.0).apply(raw, rho)
+    del raw
+    dE = _simp(t3d, p, rho)
+    p.set_modulus(dE)
+    M = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
+    M.setup(dE)
+    mx = t3d.Mixed(p, M)
+    mx.setup()
+    u = np.random.default_rng(0).uniform(-1, 1, p.nF)
+    du = t3d.DeviceArray(p.nDOF)
+    p.free_to_full(t3d.DeviceArray.from_host(u), du)
+    dy = t3d.DeviceArray(p.nDOF)
+    p.kxu_f64(du, dy)
+    y64 = dy.download()
+    del dy
+    ymax = np.abs(y64).max()
+    u32 = t3d.DeviceArray.from_host(du.download().astype(np.float32))
+    y32 = t3d.DeviceArray(p.nDOF, np.float32)
+    for form, bound in ((2, 3e-5), (3, 5e-7)):
+        mx.kxu_f32(u32, y32, form)
+        err = float(np.abs(y32.download().astype(np.float64) - y64).max() / ymax)
+        print("config 3 operator form %d: rel err %.3e" % (form, err))
+        assert err <= bound
+    del y64, u32, y32
+    db = t3d.DeviceArray(p.nDOF)
+    p.load_vector(db)
+    ce = t3d.DeviceArray(p.ne)
+    x = du
+    x.zero()
+    it64 = t3d.pcg(p, M, db, x, 1e-6, 500)
+    C64 = t3d.compliance(p, dE, x, ce)
+    mx.set_operator(3)
+    x.zero()
+    itm = mx.pcg(db, x, 1e-6, 500, trueEvery=25)
+    Cm = t3d.compliance(p, dE, x, ce)
+    res = _true_rel_residual(t3d, p, db, x)
+    print("config 3 solve: FP64 %d it, mixed bf16x6 %d it, true residual %.3e, C rel diff %.3e"
+          % (it64, itm, res, abs(Cm - C64) / C64))
+    assert it64 < 500 and itm < 500
+    assert res < 1e-6
+    assert abs(Cm - C64) <= 1e-6 * abs(C64)
+
+
+def test_config4_batch64_matches_single_solves(t3d):
+    """The seeded batch of 64 uneven cantilevers (nelx ~ U{48..160}, nely =
+    nelz = nelx/2, V0 ~ U[0.08, 0.30], every second one with 1-6 spherical
+    voids removed from the domain; seed 3), Galerkin multigrid, tol 1e-6: with
+    the single solver's tiles every problem's batched solve is bit-identical to
+    its own single solve, with the same iteration count and a true residual
+    below the tolerance."""
+    from bench import batch_design, batch_mask, batch_workload   # the benchmark's own generator
+    tol, maxIt, te = 1e-6, 500, 25
+    probs = batch_workload(np.random.default_rng(3), 64)
+    dims = [d for d, _, _ in probs]
+    solids = [batch_mask(d, s) for d, _, s in probs]
+    B = t3d.Batch(dims, Emin=float(np.float32(1e-9)), solids=solids, galerkin=True)
+    dEs = [batch_design(t3d, p, v, ()) for p, (_, v, _) in zip(B.problems, probs)]
+    B.setup(dEs)
+    B.set_tiles(16, 4)
+    dbs, xs = [], []
+    for p in B.problems:
+        db = t3d.DeviceArray(p.nDOF)
+        p.load_vector(db)
+        dbs.append(db)
+        x = t3d.DeviceArray(p.nDOF)
+        x.zero()
+        xs.append(x)
+    its = B.solve(dbs, xs, tol, maxIt, trueEvery=te)
+    for i, (p, dE, db, xb) in enumerate(zip(B.problems, dEs, dbs, xs)):
+        M = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
+        M.setup(dE)
+        mx = t3d.Mixed(p, M)
+        mx.setup()
+        x = t3d.DeviceArray(p.nDOF)
+        x.zero()
+        it = mx.pcg(db, x, tol, maxIt, trueEvery=te)
+        xb_h = xb.download()
+        assert it == its[i] and it < maxIt, (i, dims[i], it, its[i])
+        assert np.array_equal(x.download(), xb_h), (i, dims[i])
+        assert _true_rel_residual(t3d, p, db, xb) < tol, (i, dims[i])
+        del M, mx, x
+    print("config 4: 64 problems, iterations %d..%d" % (min(its), max(its)))
