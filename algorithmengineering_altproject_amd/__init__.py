@@ -75,6 +75,7 @@ def lib():
         "t3d_pcg": (C.c_int, [vp, vp, vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int)]),
         "t3d_pcg_history": (C.c_int, [vp, C.c_int, C.c_int, vp]),
         "t3d_compliance": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
+        "t3d_voxel_ke": (C.c_int, [C.c_double, C.c_double, vp]),
         "t3d_compliance_noalloc": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "t3d_filter_create": (C.c_int, [vp, C.c_float, C.POINTER(vp)]),
         "t3d_filter_destroy": (C.c_int, [vp]),
@@ -364,6 +365,13 @@ def compliance(prob, dE, dU, dce):
     c = C.c_double()
     check(lib().t3d_compliance(prob.h, dE.ptr, dU.ptr, dce.ptr, C.byref(c)), "t3d_compliance")
     return c.value
+
+
+def voxel_ke(nu=0.3, cellSize=1.0):
+    """ComputeVoxelKe (cpp/src/fea.cpp:117-179): the 24x24 element matrix (host)."""
+    Ke = np.zeros(576)
+    check(lib().t3d_voxel_ke(nu, cellSize, Ke.ctypes.data), "t3d_voxel_ke")
+    return Ke.reshape(24, 24)
 
 
 def compliance_noalloc(prob, dE, dU, dce):

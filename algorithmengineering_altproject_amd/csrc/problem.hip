@@ -242,6 +242,14 @@ int t3d_problem_destroy(t3d_problem* p)
     return T3D_OK;
 }
 
+int t3d_voxel_ke(double nu, double cellSize, double* Ke)
+{
+    T3D_CHECK(Ke, T3D_ERR_ARG, "t3d_voxel_ke: null output");
+    const std::array<double, 576> k = voxel_ke(nu, cellSize);
+    std::memcpy(Ke, k.data(), sizeof(double) * 576);
+    return T3D_OK;
+}
+
 int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out)
 {
     return t3d_problem_create_voxels(nely, nelx, nelz, nullptr, out);

@@ -56,6 +56,10 @@ int t3d_synchronize(void);
  * followed by ApplyBoundaryConditions (cpp/include/fea.hpp:39, cpp/src/fea.cpp:360-447):
  * builds the padded voxel model on the host and uploads its device mirror. */
 int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out);
+/* ComputeVoxelKe (cpp/include/fea.hpp:37, cpp/src/fea.cpp:117-179): the 24x24
+ * unit-modulus element matrix of a cubic voxel by 2x2x2 Gauss quadrature,
+ * row-major into Ke[576] (host memory), bit-identical to the reference's. */
+int t3d_voxel_ke(double nu, double cellSize, double* Ke);
 /* General voxel domain (the voxel-model generalisation of
  * CreateVoxelFEAmodel_Cuboid, cpp/src/fea.cpp:181-358, and its padding rule,
  * cpp/include/multigrid/padding.hpp:42-53 with the solid voxel count, as

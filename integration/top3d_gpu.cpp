@@ -1,0 +1,617 @@
+// top3d_gpu.cpp -- the C++ port's hot-path entry points on the MI355X library.
+//
+// A drop-in for five translation units of the reference's cpp/src:
+//   fea.cpp                    ComputeVoxelKe, CreateVoxelFEAmodel_Cuboid,
+//                              ApplyBoundaryConditions, K_times_u_finest (both
+//                              overloads), ComputeCompliance (cpp/include/fea.hpp:37-47)
+//   solver.cpp                 restrict_to_free, scatter_from_free,
+//                              make_jacobi_preconditioner, PCG_free (cpp/include/solver.hpp:8-27)
+//   filter.cpp                 SetupPDEFilter, ApplyPDEFilter (cpp/include/filter.hpp:17,28)
+//   multigrid/preconditioner.cpp  build_static_once,
+//                              make_diagonal_preconditioner_from_static
+//                              (cpp/include/multigrid/multigrid.hpp:64-68)
+//   driver.cpp                 TOP3D_XL_GLOBAL (cpp/include/driver.hpp:4)
+// with the reference's own declarations and return conventions, implemented
+// over the C ABI of include/t3d_capi.h (libt3d.so).  The rest of the C++ port
+// (core.cpp, io.cpp, multigrid/{hierarchy,masks,diagonal,transfers,coarsest}.cpp,
+// apps/top3d_cli.cpp) builds unchanged beside it.
+//
+// Build, from the reference's cpp/ directory (REPO = this repository):
+//   g++ -std=c++17 -O2 -fopenmp -Iinclude -I$REPO/include -c $REPO/integration/top3d_gpu.cpp
+//   g++ -std=c++17 -O2 -fopenmp -Iinclude apps/top3d_cli.cpp src/core.cpp src/io.cpp
+//       src/multigrid/{hierarchy,masks,diagonal,transfers,coarsest}.cpp top3d_gpu.o
+//       -L$REPO/algorithmengineering_altproject_amd -lt3d -Wl,-rpath,$REPO/algorithmengineering_altproject_amd
+//   (one command; oracle/Makefile's `shim` target builds it as oracle/_ref/top3d_xl_cli_gpu)
+//
+// Device state.  Each Problem gets one device context (model, boundary
+// conditions and load mirrored on the GPU), created by
+// CreateVoxelFEAmodel_Cuboid and keyed by the Problem's address; it holds the
+// live multigrid preconditioner (build_static_once /
+// make_diagonal_preconditioner_from_static) and the live PDE filter
+// (SetupPDEFilter), as the reference's driver keeps one of each per Problem.
+// A new preconditioner or filter replaces the device state that earlier
+// closures or PDEFilter copies of the same Problem use.  The host-vector
+// signatures stage their arguments through the device; TOP3D_XL_GLOBAL runs
+// the whole loop device-resident (t3d_top3d_run).
+//
+// Errors: a failing library call throws std::runtime_error with the
+// library's message (the reference has no error channel; its functions cannot
+// fail short of a crash).
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "core.hpp"
+#include "driver.hpp"
+#include "fea.hpp"
+#include "filter.hpp"
+#include "io.hpp"
+#include "multigrid/detail/hierarchy.hpp"
+#include "multigrid/detail/masks.hpp"
+#include "multigrid/multigrid.hpp"
+#include "solver.hpp"
+#include "t3d_capi.h"
+
+namespace top3d {
+namespace gpu {
+
+inline void ok(int rc, const char* what)
+{
+    if (rc != T3D_OK) throw std::runtime_error(std::string(what) + ": " + t3d_last_error());
+}
+
+// device buffer owned by a context
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        ok(t3d_malloc(reinterpret_cast<void**>(&p), sizeof(T) * (count ? count : 1)), "t3d_malloc");
+        n = count;
+    }
+    ~DBuf()
+    {
+        if (p) t3d_free(p);
+    }
+};
+
+struct MGConfig {
+    bool nonDyadic = true;
+    int maxLevels = 5;
+    float weight = 0.6f;
+    bool operator==(const MGConfig& o) const
+    {
+        return nonDyadic == o.nonDyadic && maxLevels == o.maxLevels && weight == o.weight;
+    }
+};
+
+struct Ctx {
+    t3d_problem* p = nullptr;
+    t3d_mg* mg = nullptr;          // live multigrid preconditioner
+    MGConfig mgCfg;
+    float mgEmin = 0.0f;
+    t3d_filter* filter = nullptr;  // live PDE filter
+    int64_t ne = 0, nN = 0, nF = 0;
+    DBuf<double> E, ce, full0, full1, free0;
+    DBuf<float> src, dst;
+    ~Ctx()
+    {
+        if (filter) t3d_filter_destroy(filter);
+        if (mg) t3d_mg_destroy(mg);
+        if (p) t3d_problem_destroy(p);
+    }
+    void upload_E(const std::vector<double>& eleE)
+    {
+        if (int64_t(eleE.size()) != ne) throw std::invalid_argument("element modulus vector has the wrong size");
+        ok(t3d_memcpy_h2d(E.p, eleE.data(), sizeof(double) * size_t(ne)), "t3d_memcpy_h2d");
+    }
+    // free vector (host) -> full layout on the device (fixed DOFs 0)
+    void free_to_full(const std::vector<double>& v, double* full)
+    {
+        if (int64_t(v.size()) != nF) throw std::invalid_argument("free-DOF vector has the wrong size");
+        ok(t3d_memcpy_h2d(free0.p, v.data(), sizeof(double) * size_t(nF)), "t3d_memcpy_h2d");
+        ok(t3d_memset(full, 0, sizeof(double) * size_t(3 * nN)), "t3d_memset");
+        ok(t3d_free_to_full(p, free0.p, full), "t3d_free_to_full");
+    }
+    void full_to_free(const double* full, std::vector<double>& v)
+    {
+        v.resize(size_t(nF));
+        ok(t3d_full_to_free(p, full, free0.p), "t3d_full_to_free");
+        ok(t3d_memcpy_d2h(v.data(), free0.p, sizeof(double) * size_t(nF)), "t3d_memcpy_d2h");
+    }
+    void soa_to_device(const DOFData& d, double* full)
+    {
+        if (int64_t(d.ux.size()) != nN || int64_t(d.uy.size()) != nN || int64_t(d.uz.size()) != nN)
+            throw std::invalid_argument("DOFData has the wrong size");
+        std::vector<double> aos(size_t(3 * nN));
+        for (int64_t n = 0; n < nN; ++n) {
+            aos[size_t(3 * n)] = d.ux[size_t(n)];
+            aos[size_t(3 * n + 1)] = d.uy[size_t(n)];
+            aos[size_t(3 * n + 2)] = d.uz[size_t(n)];
+        }
+        ok(t3d_memcpy_h2d(full, aos.data(), sizeof(double) * aos.size()), "t3d_memcpy_h2d");
+    }
+    void device_to_soa(const double* full, DOFData& d)
+    {
+        std::vector<double> aos(size_t(3 * nN));
+        ok(t3d_memcpy_d2h(aos.data(), full, sizeof(double) * aos.size()), "t3d_memcpy_d2h");
+        d.ux.resize(size_t(nN));
+        d.uy.resize(size_t(nN));
+        d.uz.resize(size_t(nN));
+        for (int64_t n = 0; n < nN; ++n) {
+            d.ux[size_t(n)] = aos[size_t(3 * n)];
+            d.uy[size_t(n)] = aos[size_t(3 * n + 1)];
+            d.uz[size_t(n)] = aos[size_t(3 * n + 2)];
+        }
+    }
+    void ensure_mg(const MGConfig& cfg, float Emin)
+    {
+        if (mg && mgCfg == cfg && mgEmin == Emin) return;
+        if (mg) t3d_mg_destroy(mg);
+        mg = nullptr;
+        // NlimitDofs = 1000: the coarsest-level Cholesky limit of preconditioner.cpp:24,53
+        ok(t3d_mg_create(p, cfg.nonDyadic ? 1 : 0, cfg.maxLevels, cfg.weight, Emin, 1000, &mg), "t3d_mg_create");
+        mgCfg = cfg;
+        mgEmin = Emin;
+    }
+};
+
+std::mutex g_mu;
+std::unordered_map<const Problem*, std::unique_ptr<Ctx>> g_ctx;
+
+std::unique_ptr<Ctx> make_ctx(int nely, int nelx, int nelz)
+{
+    std::unique_ptr<Ctx> c(new Ctx());
+    ok(t3d_problem_create_cuboid(nely, nelx, nelz, &c->p), "t3d_problem_create_cuboid");
+    int64_t s[10];
+    ok(t3d_problem_sizes(c->p, s), "t3d_problem_sizes");   // resX resY resZ ne nN nDOF nF nnx nny nnz
+    c->ne = s[3];
+    c->nN = s[4];
+    c->nF = s[6];
+    c->E.alloc(size_t(c->ne));
+    c->ce.alloc(size_t(c->ne));
+    c->full0.alloc(size_t(3 * c->nN));
+    c->full1.alloc(size_t(3 * c->nN));
+    c->free0.alloc(size_t(c->nF));
+    c->src.alloc(size_t(c->ne));
+    c->dst.alloc(size_t(c->ne));
+    return c;
+}
+
+// the device context of a Problem (built from its original dimensions when the
+// Problem did not come from CreateVoxelFEAmodel_Cuboid)
+Ctx& ctx(const Problem& pb)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_ctx.find(&pb);
+    if (it != g_ctx.end()) return *it->second;
+    auto c = make_ctx(pb.mesh.origResY, pb.mesh.origResX, pb.mesh.origResZ);
+    Ctx& ref = *c;
+    g_ctx[&pb] = std::move(c);
+    return ref;
+}
+
+// The device multigrid as a Preconditioner: callable on host free vectors like
+// the reference's closure, and recognised by PCG_free, which then runs the
+// whole solve on the device.
+struct DeviceMG {
+    Ctx* c;
+    void operator()(const std::vector<double>& r, std::vector<double>& z) const
+    {
+        c->free_to_full(r, c->full0.p);
+        ok(t3d_mg_apply(c->mg, c->full0.p, c->full1.p), "t3d_mg_apply");
+        c->full_to_free(c->full1.p, z);
+    }
+};
+
+}  // namespace gpu
+
+// frees the device context of a Problem (no counterpart in the reference, whose
+// Problem owns only host vectors)
+void release_gpu_context(const Problem& pb)
+{
+    std::lock_guard<std::mutex> lk(gpu::g_mu);
+    gpu::g_ctx.erase(&pb);
+}
+
+// ---------------------------------------------------------------------------
+// fea.hpp
+// ---------------------------------------------------------------------------
+std::array<double, 24 * 24> ComputeVoxelKe(double nu, double cellSize)
+{
+    std::array<double, 24 * 24> Ke{};
+    gpu::ok(t3d_voxel_ke(nu, cellSize, Ke.data()), "t3d_voxel_ke");
+    return Ke;
+}
+
+// The reference's model arrays, read back from the device problem (same
+// padding, Morton element order, colours, node numbering and Ke).
+void CreateVoxelFEAmodel_Cuboid(Problem& pb, int nely, int nelx, int nelz)
+{
+    std::unique_ptr<gpu::Ctx> c = gpu::make_ctx(nely, nelx, nelz);
+    int64_t s[10];
+    gpu::ok(t3d_problem_sizes(c->p, s), "t3d_problem_sizes");
+    CartesianMesh& m = pb.mesh;
+    m.eleSize = {1.0f, 1.0f, 1.0f};
+    m.origResX = nelx;
+    m.origResY = nely;
+    m.origResZ = nelz;
+    m.resX = int(s[0]);
+    m.resY = int(s[1]);
+    m.resZ = int(s[2]);
+    m.numElements = int(s[3]);
+    m.numNodes = int(s[4]);
+    m.numDOFs = int(s[5]);
+    const int ne = m.numElements, nN = m.numNodes;
+    m.eleMapBack.assign(size_t(ne), 0);
+    m.eNodMat.assign(size_t(8) * ne, 0);
+    m.coloring.elemColor.assign(size_t(ne), 0);
+    gpu::ok(t3d_problem_model(c->p, m.eleMapBack.data(), m.eNodMat.data(), m.coloring.elemColor.data(), m.Ke.data(),
+                              nullptr, nullptr, nullptr, nullptr),
+            "t3d_problem_model");
+    m.eleMapForward.assign(size_t(m.resX) * m.resY * m.resZ, -1);
+    for (int e = 0; e < ne; ++e) m.eleMapForward[size_t(m.eleMapBack[size_t(e)])] = e;
+    m.nodMapForward.resize(size_t(nN));
+    m.nodMapBack.resize(size_t(nN));
+    for (int n = 0; n < nN; ++n) m.nodMapForward[size_t(n)] = m.nodMapBack[size_t(n)] = n;   // full grid, lexicographic
+    m.eDofMat.resize(size_t(24) * ne);
+    for (size_t k = 0; k < size_t(8) * ne; ++k)
+        for (int d = 0; d < 3; ++d) m.eDofMat[3 * k + size_t(d)] = 3 * m.eNodMat[k] + d;
+    m.coloring.numColors = 8;
+    m.coloring.colorBuckets.assign(8, {});
+    for (int e = 0; e < ne; ++e) m.coloring.colorBuckets[size_t(m.coloring.elemColor[size_t(e)])].push_back(e);
+    // boundary: nodes shared by fewer than 8 elements, elements touching one
+    std::vector<uint8_t> deg(size_t(nN), 0);
+    for (int32_t n : m.eNodMat) ++deg[size_t(n)];
+    m.nodesOnBoundary.clear();
+    for (int n = 0; n < nN; ++n)
+        if (deg[size_t(n)] < 8) m.nodesOnBoundary.push_back(n);
+    m.elementsOnBoundary.clear();
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a)
+            if (deg[size_t(m.eNodMat[size_t(8 * e + a)])] < 8) {
+                m.elementsOnBoundary.push_back(e);
+                break;
+            }
+    pb.density.assign(size_t(ne), 1.0f);
+    pb.F.ux.assign(size_t(nN), 0.0);
+    pb.F.uy.assign(size_t(nN), 0.0);
+    pb.F.uz.assign(size_t(nN), 0.0);
+    pb.isFreeDOF.assign(size_t(m.numDOFs), 1);
+    pb.freeDofIndex.clear();
+    std::lock_guard<std::mutex> lk(gpu::g_mu);
+    gpu::g_ctx[&pb] = std::move(c);   // replaces a stale context at a reused address
+}
+
+// The device problem carries the reference's boundary conditions and load
+// (fixed x = 0 face, -Z load on the loaded slice, unused nodes fixed); this
+// copies them into the Problem.
+void ApplyBoundaryConditions(Problem& pb)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    const size_t nN = size_t(c.nN), ne = size_t(c.ne);
+    std::vector<double> F(3 * nN);
+    pb.isFreeDOF.assign(3 * nN, 0);
+    pb.freeDofIndex.assign(size_t(c.nF), 0);
+    pb.eDofFreeMat.assign(24 * ne, 0);
+    gpu::ok(t3d_problem_model(c.p, nullptr, nullptr, nullptr, nullptr, F.data(), pb.isFreeDOF.data(),
+                              pb.freeDofIndex.data(), pb.eDofFreeMat.data()),
+            "t3d_problem_model");
+    pb.F.ux.resize(nN);
+    pb.F.uy.resize(nN);
+    pb.F.uz.resize(nN);
+    for (size_t n = 0; n < nN; ++n) {
+        pb.F.ux[n] = F[3 * n];
+        pb.F.uy[n] = F[3 * n + 1];
+        pb.F.uz[n] = F[3 * n + 2];
+    }
+    pb.freeNodeIndex.resize(pb.freeDofIndex.size());
+    pb.freeCompIndex.resize(pb.freeDofIndex.size());
+    for (size_t i = 0; i < pb.freeDofIndex.size(); ++i) {
+        pb.freeNodeIndex[i] = pb.freeDofIndex[i] / 3;
+        pb.freeCompIndex[i] = pb.freeDofIndex[i] % 3;
+    }
+}
+
+void K_times_u_finest(const Problem& pb, const std::vector<double>& eleE, const std::vector<double>& uFree,
+                      std::vector<double>& yFree)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    if (int64_t(eleE.size()) != c.ne || int64_t(uFree.size()) != c.nF)
+        throw std::invalid_argument("K_times_u_finest: vector sizes do not match the problem");
+    yFree.resize(uFree.size());   // as fea.cpp:775-777
+    gpu::ok(t3d_kxu_free_host(c.p, eleE.data(), uFree.data(), yFree.data()), "t3d_kxu_free_host");
+}
+
+// the workspace overload: the device needs no host workspace (the reference's
+// KTimesUWorkspace fields are unused there too)
+void K_times_u_finest(const Problem& pb, const std::vector<double>& eleE, const std::vector<double>& uFree,
+                      std::vector<double>& yFree, KTimesUWorkspace&)
+{
+    K_times_u_finest(pb, eleE, uFree, yFree);
+}
+
+double ComputeCompliance(const Problem& pb, const std::vector<double>& eleE, const DOFData& U, std::vector<double>& ce)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    c.upload_E(eleE);
+    c.soa_to_device(U, c.full0.p);
+    double C = 0.0;
+    gpu::ok(t3d_compliance(c.p, c.E.p, c.full0.p, c.ce.p, &C), "t3d_compliance");
+    ce.resize(size_t(c.ne));
+    gpu::ok(t3d_memcpy_d2h(ce.data(), c.ce.p, sizeof(double) * ce.size()), "t3d_memcpy_d2h");
+    return C;
+}
+
+// ---------------------------------------------------------------------------
+// solver.hpp
+// ---------------------------------------------------------------------------
+void restrict_to_free(const Problem& pb, const DOFData& full, std::vector<double>& freev)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    c.soa_to_device(full, c.full0.p);
+    c.full_to_free(c.full0.p, freev);
+}
+
+// only the free DOFs of `full` are written, as solver.cpp:31-49
+void scatter_from_free(const Problem& pb, const std::vector<double>& freev, DOFData& full)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    if (int64_t(freev.size()) != c.nF) throw std::invalid_argument("scatter_from_free: free vector size");
+    c.soa_to_device(full, c.full0.p);
+    gpu::ok(t3d_memcpy_h2d(c.free0.p, freev.data(), sizeof(double) * freev.size()), "t3d_memcpy_h2d");
+    gpu::ok(t3d_free_to_full(c.p, c.free0.p, c.full0.p), "t3d_free_to_full");
+    c.device_to_soa(c.full0.p, full);
+}
+
+// Jacobi preconditioner on the free DOFs: z = r / diag(K_ff), diag summed over
+// the elements in ascending order with E <= 1e-16 skipped, entries not above
+// 1e-30 replaced by 1 (solver.cpp:51-75).  A host closure, as in the
+// reference; PCG_free applies it between device operator products.
+Preconditioner make_jacobi_preconditioner(const Problem& pb, const std::vector<double>& eleE)
+{
+    const CartesianMesh& m = pb.mesh;
+    std::vector<double> d(size_t(m.numDOFs), 0.0);
+    for (int e = 0; e < m.numElements; ++e) {
+        if (eleE[size_t(e)] <= 1.0e-16) continue;
+        for (int i = 0; i < 24; ++i) d[size_t(m.eDofMat[size_t(24 * e + i)])] += eleE[size_t(e)] * m.Ke[size_t(25 * i)];
+    }
+    std::vector<double> dFree(pb.freeDofIndex.size());
+    for (size_t i = 0; i < dFree.size(); ++i) {
+        const double v = d[size_t(pb.freeDofIndex[i])];
+        dFree[i] = v > 1.0e-30 ? v : 1.0;
+    }
+    return [dFree](const std::vector<double>& r, std::vector<double>& z) {
+        z.resize(r.size());
+        for (size_t i = 0; i < r.size(); ++i) z[i] = r[i] / dFree[i];
+    };
+}
+
+namespace {
+// PCG_free's iteration (solver.cpp:87-217) with a host-side preconditioner the
+// device does not know: the operator products run on the device, M and the
+// vector updates on the host.
+int pcg_host_precond(gpu::Ctx& c, const std::vector<double>& b, std::vector<double>& x, double tol, int maxIt,
+                     const Preconditioner& M, PCGFreeWorkspace& ws)
+{
+    const size_t n = b.size();
+    ws.r.resize(n);
+    ws.z.resize(n);
+    ws.p.resize(n);
+    ws.Ap.resize(n);
+    std::vector<double>&r = ws.r, &z = ws.z, &p = ws.p, &Ap = ws.Ap;
+    gpu::ok(t3d_set_modulus(c.p, c.E.p), "t3d_set_modulus");
+    auto apply_K = [&](const std::vector<double>& v, std::vector<double>& y) {
+        c.free_to_full(v, c.full0.p);
+        gpu::ok(t3d_kxu_f64(c.p, c.full0.p, c.full1.p), "t3d_kxu_f64");
+        c.full_to_free(c.full1.p, y);
+    };
+    auto dot = [](const std::vector<double>& a, const std::vector<double>& v) {
+        double s = 0.0;
+        for (size_t i = 0; i < a.size(); ++i) s += a[i] * v[i];
+        return s;
+    };
+    r = b;
+    if (x.empty()) {
+        x.assign(n, 0.0);
+    } else {
+        apply_K(x, Ap);
+        for (size_t i = 0; i < n; ++i) r[i] -= Ap[i];
+    }
+    M(r, z);
+    double rz = dot(r, z);
+    p = z;
+    const double stop = std::max(1.0e-30, std::sqrt(dot(b, b))) * tol;
+    for (int it = 0; it < maxIt; ++it) {
+        apply_K(p, Ap);
+        const double alpha = rz / std::max(1.0e-30, dot(p, Ap));
+        double rn2 = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+            rn2 += r[i] * r[i];
+        }
+        if (std::sqrt(rn2) < stop) return it + 1;
+        M(r, z);
+        const double rzNew = dot(r, z);
+        const double beta = rzNew / std::max(1.0e-30, rz);
+        rz = rzNew;
+        for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    return maxIt;
+}
+}  // namespace
+
+// With the device multigrid of this Problem (make_diagonal_preconditioner_from_static)
+// or with no preconditioner, the whole solve runs on the device (t3d_pcg: same
+// iteration, iteration count and stopping rule); any other M is applied on the
+// host between device operator products.  Returns the iterations, maxIt when
+// not converged, as solver.cpp.
+int PCG_free(const Problem& pb, const std::vector<double>& eleE, const std::vector<double>& bFree,
+             std::vector<double>& xFree, double tol, int maxIt, const Preconditioner& M, PCGFreeWorkspace& ws)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    c.upload_E(eleE);
+    const gpu::DeviceMG* dm = M ? M.target<gpu::DeviceMG>() : nullptr;
+    if (M && !(dm && dm->c == &c && c.mg)) return pcg_host_precond(c, bFree, xFree, tol, maxIt, M, ws);
+    gpu::ok(t3d_set_modulus(c.p, c.E.p), "t3d_set_modulus");
+    c.free_to_full(bFree, c.full1.p);
+    if (xFree.empty()) {
+        xFree.assign(bFree.size(), 0.0);
+        gpu::ok(t3d_memset(c.full0.p, 0, sizeof(double) * size_t(3 * c.nN)), "t3d_memset");
+    } else {
+        c.free_to_full(xFree, c.full0.p);
+    }
+    int it = 0;
+    gpu::ok(t3d_pcg(c.p, M ? c.mg : nullptr, c.full1.p, c.full0.p, tol, maxIt, &it), "t3d_pcg");
+    c.full_to_free(c.full0.p, xFree);
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid.hpp (build_static_once, make_diagonal_preconditioner_from_static;
+// BuildMGHierarchy stays the reference's hierarchy.cpp)
+// ---------------------------------------------------------------------------
+namespace mg {
+
+// The host hierarchy and masks by the reference's own code (hierarchy.cpp,
+// masks.cpp), which callers may inspect, and the device hierarchy with the
+// same levels and masks (t3d_mg_create, bit-identical).
+void build_static_once(const Problem& pb, const MGPrecondConfig& cfg, MGHierarchy& H,
+                       std::vector<std::vector<uint8_t>>& fixedMasks)
+{
+    H.levels.clear();
+    const int levels = ComputeAdaptiveMaxLevels(pb, cfg.nonDyadic, cfg.maxLevels, 1000);
+    BuildMGHierarchy(pb, cfg.nonDyadic, H, levels);
+    const MGLevel& Lc = H.levels.back();
+    std::cout << "[MG] levels=" << H.levels.size() << " coarsest=" << Lc.resX << "x" << Lc.resY << "x" << Lc.resZ
+              << " dofs=" << (3 * Lc.numNodes) << "\n";
+    MG_BuildFixedMasks(pb, H, fixedMasks);
+    gpu::ctx(pb).ensure_mg(gpu::MGConfig{cfg.nonDyadic, cfg.maxLevels, cfg.weight}, pb.params.youngsModulusMin);
+}
+
+// Diagonals and the coarsest Galerkin matrix with its Cholesky factor for the
+// moduli eleE, on the device (t3d_mg_setup); the returned closure is the
+// V-cycle of preconditioner.cpp:88-265 (t3d_mg_apply).
+Preconditioner make_diagonal_preconditioner_from_static(const Problem& pb, const MGHierarchy&,
+                                                        const std::vector<std::vector<uint8_t>>&,
+                                                        const std::vector<double>& eleE, const MGPrecondConfig& cfg)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    c.ensure_mg(gpu::MGConfig{cfg.nonDyadic, cfg.maxLevels, cfg.weight}, pb.params.youngsModulusMin);
+    c.upload_E(eleE);
+    gpu::ok(t3d_mg_setup(c.mg, c.E.p), "t3d_mg_setup");
+    return Preconditioner(gpu::DeviceMG{&c});
+}
+
+}  // namespace mg
+
+// ---------------------------------------------------------------------------
+// filter.hpp
+// ---------------------------------------------------------------------------
+// The Helmholtz kernel and node preconditioner are read back into the
+// PDEFilter; the warm-start state (lastXNode, lastRhsNode) lives on the device
+// with the Problem's filter, so the host copies stay at their initial zeros.
+PDEFilter SetupPDEFilter(const Problem& pb, float filterRadius)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    if (c.filter) t3d_filter_destroy(c.filter);
+    c.filter = nullptr;
+    gpu::ok(t3d_filter_create(c.p, filterRadius, &c.filter), "t3d_filter_create");
+    PDEFilter pf;
+    pf.diagPrecondNode.assign(size_t(c.nN), 0.0);
+    gpu::ok(t3d_filter_info(c.filter, pf.kernel.data(), pf.diagPrecondNode.data()), "t3d_filter_info");
+    pf.lastXNode.assign(size_t(c.nN), 0.0);
+    pf.lastRhsNode.assign(size_t(c.nN), 0.0);
+    return pf;
+}
+
+void ApplyPDEFilter(const Problem& pb, PDEFilter&, const std::vector<float>& srcEle, std::vector<float>& dstEle,
+                    PDEFilterWorkspace&)
+{
+    gpu::Ctx& c = gpu::ctx(pb);
+    if (!c.filter) throw std::logic_error("ApplyPDEFilter: SetupPDEFilter was not called for this Problem");
+    if (int64_t(srcEle.size()) != c.ne) throw std::invalid_argument("ApplyPDEFilter: element vector size");
+    gpu::ok(t3d_memcpy_h2d(c.src.p, srcEle.data(), sizeof(float) * srcEle.size()), "t3d_memcpy_h2d");
+    int its = 0;
+    gpu::ok(t3d_filter_apply(c.filter, c.src.p, c.dst.p, &its), "t3d_filter_apply");
+    dstEle.resize(size_t(c.ne));
+    gpu::ok(t3d_memcpy_d2h(dstEle.data(), c.dst.p, sizeof(float) * dstEle.size()), "t3d_memcpy_d2h");
+}
+
+// ---------------------------------------------------------------------------
+// driver.hpp
+// ---------------------------------------------------------------------------
+// simulation_count repeats of the optimisation loop (driver.cpp:23-278), each
+// device-resident (t3d_top3d_run) with the reference's parameters; the log
+// lines, the summary (out/comments/<tag>.md) and the STL (out/stl/<tag>.stl)
+// as the reference writes them, named by its own io.cpp helpers.
+void TOP3D_XL_GLOBAL(int nely, int nelx, int nelz, float V0, int nLoop, float rMin, int simulation_count)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int sim = 0; sim < simulation_count; ++sim) {
+        GlobalParams gp;
+        InitialSettings(gp);
+        std::printf("\n==========================Displaying Inputs==========================\n");
+        std::printf("Simulation: %d\n", sim);
+        std::printf("..............................................Volume Fraction: %6.4f\n", double(V0));
+        std::printf("..........................................Filter Radius: %6.4f Cells\n", double(rMin));
+        std::printf("................................................Cell Size: %10.4e\n", double(gp.cellSize));
+        std::printf(".................................................#CG Iterations: %4d\n", gp.cgMaxIt);
+        std::printf("...........................................Youngs Modulus: %10.4e\n", double(gp.youngsModulus));
+        std::printf("....................................Youngs Modulus (Min.): %10.4e\n",
+                    double(gp.youngsModulusMin));
+        std::printf("...........................................Poissons Ratio: %10.4e\n", double(gp.poissonRatio));
+        t3d_params P;
+        t3d_params_default(&P);
+        P.E0 = gp.youngsModulus;
+        P.Emin = gp.youngsModulusMin;
+        P.nu = gp.poissonRatio;
+        P.penal = gp.simpPenalty;
+        P.cgTol = gp.cgTol;
+        P.cgMaxIt = gp.cgMaxIt;
+        P.V0 = V0;
+        P.nLoop = nLoop;
+        P.rMin = rMin;
+        const std::string tag = generate_unique_tag("GLOBAL");
+        const std::string mdDir = out_comments_dir_for_cwd(), stlDir = out_stl_dir_for_cwd();
+        ensure_out_dir(mdDir);
+        ensure_out_dir(stlDir);
+        const std::string md = mdDir + tag + ".md", stl = stlDir + tag + ".stl";
+        P.summaryPath = md.c_str();
+        P.stlPath = stl.c_str();   // iso 0.3, as driver.cpp:274
+        std::vector<t3d_loop_record> rec(size_t(nLoop > 0 ? nLoop : 1));
+        double Csolid = 0.0;
+        int solidIters = 0, loops = 0;
+        gpu::ok(t3d_top3d_run(nely, nelx, nelz, &P, rec.data(), nLoop, nullptr, &Csolid, &solidIters, &loops),
+                "t3d_top3d_run");
+        for (int k = 0; k < loops; ++k) {   // the per-loop lines of driver.cpp:205-216
+            const t3d_loop_record& R = rec[size_t(k)];
+            std::printf(" It.:%4d Obj.:%16.8e Vol.:%6.4e Sharp.:%6.4e Cons.:%4.2e Ch.:%4.2e\n", R.loop, R.Cdisp,
+                        double(R.volFrac), double(R.sharpness), double(R.fval), double(R.change));
+            std::printf(" It.: %4d (Time)... Total per-It.: %8.2es; CG: %8.2es; Opti.: %8.2es; Filtering: %8.2es.\n",
+                        R.loop, R.tTotal, R.tSolve, R.tOpti, R.tFilter);
+        }
+        std::ifstream in(md);
+        std::stringstream text;
+        text << in.rdbuf();
+        std::printf("\n%s\n", text.str().c_str());
+        std::printf("STL file saved to: %s\n\nDone.\n", stl.c_str());
+    }
+    const float total = std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("\n............Total runtime time for %d simulations%.4es.\n", simulation_count, double(total));
+}
+
+}  // namespace top3d

@@ -40,3 +40,15 @@ def test_product_does_not_reference_oracle():
             if f.endswith((".hip", ".cpp", ".hpp", ".py", ".h")):
                 t = open(os.path.join(root, f), errors="replace").read()
                 assert "oracle" not in t.lower().replace("the oracle", ""), f
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libt3d.so not built")
+def test_voxel_ke_host_entry_matches_oracle(t3d):
+    """t3d_voxel_ke (ComputeVoxelKe, cpp/src/fea.cpp:117-179) is host code: it
+    runs without a GPU and equals the oracle's quadrature bit for bit."""
+    import numpy as np
+    import oracle as O
+    for nu, h in ((0.3, 1.0), (0.25, 2.0)):
+        Ke = np.zeros(576)
+        O.lib().o_voxel_ke(nu, h, Ke)
+        assert np.array_equal(t3d.voxel_ke(nu, h).reshape(-1), Ke)
