@@ -143,18 +143,13 @@ struct Ctx {
         }
         ok(t3d_memcpy_h2d(full, aos.data(), sizeof(double) * aos.size()), "t3d_memcpy_h2d");
     }
-    void device_to_soa(const double* full, DOFData& d)
+    // the free DOFs of a device full vector into d (its fixed DOFs untouched)
+    void free_dofs_to_soa(const double* full, const std::vector<int>& freeDofIndex, DOFData& d)
     {
         std::vector<double> aos(size_t(3 * nN));
         ok(t3d_memcpy_d2h(aos.data(), full, sizeof(double) * aos.size()), "t3d_memcpy_d2h");
-        d.ux.resize(size_t(nN));
-        d.uy.resize(size_t(nN));
-        d.uz.resize(size_t(nN));
-        for (int64_t n = 0; n < nN; ++n) {
-            d.ux[size_t(n)] = aos[size_t(3 * n)];
-            d.uy[size_t(n)] = aos[size_t(3 * n + 1)];
-            d.uz[size_t(n)] = aos[size_t(3 * n + 2)];
-        }
+        std::vector<double>* comp[3] = {&d.ux, &d.uy, &d.uz};
+        for (int gi : freeDofIndex) (*comp[gi % 3])[size_t(gi / 3)] = aos[size_t(gi)];
     }
     void ensure_mg(const MGConfig& cfg, float Emin)
     {
@@ -365,15 +360,15 @@ void restrict_to_free(const Problem& pb, const DOFData& full, std::vector<double
     c.full_to_free(c.full0.p, freev);
 }
 
-// only the free DOFs of `full` are written, as solver.cpp:31-49
+// only the free DOFs of `full` are written, as solver.cpp:31-49 (the device's
+// full layout holds 0 on fixed DOFs; the caller's fixed-DOF values stay)
 void scatter_from_free(const Problem& pb, const std::vector<double>& freev, DOFData& full)
 {
     gpu::Ctx& c = gpu::ctx(pb);
-    if (int64_t(freev.size()) != c.nF) throw std::invalid_argument("scatter_from_free: free vector size");
-    c.soa_to_device(full, c.full0.p);
-    gpu::ok(t3d_memcpy_h2d(c.free0.p, freev.data(), sizeof(double) * freev.size()), "t3d_memcpy_h2d");
-    gpu::ok(t3d_free_to_full(c.p, c.free0.p, c.full0.p), "t3d_free_to_full");
-    c.device_to_soa(c.full0.p, full);
+    if (int64_t(full.ux.size()) != c.nN || int64_t(full.uy.size()) != c.nN || int64_t(full.uz.size()) != c.nN)
+        throw std::invalid_argument("scatter_from_free: DOFData size");
+    c.free_to_full(freev, c.full0.p);
+    c.free_dofs_to_soa(c.full0.p, pb.freeDofIndex, full);
 }
 
 // Jacobi preconditioner on the free DOFs: z = r / diag(K_ff), diag summed over

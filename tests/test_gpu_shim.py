@@ -60,3 +60,53 @@ def test_reference_cli_on_the_shim(t3d, tmp_path):
             # (its OpenMP reductions and atomics make the loop chaotic in the last bits)
             assert math.isclose(float(a[1]), float(b[1]), rel_tol=1e-8 if k == 0 else 1e-5), (k, a, b)
             assert abs(float(a[2]) - float(b[2])) <= 1e-4 and abs(float(a[3]) - float(b[3])) <= 1e-4
+
+
+GPU_CHECK = os.path.join(REPO, "oracle", "_ref", "shim_check_gpu")
+REF_CHECK = os.path.join(REPO, "oracle", "_ref", "shim_check_ref")
+I32 = ("eleMapBack", "eNodMat", "eDofMat", "elemColor", "freeDofIndex", "eDofFreeMat", "iters")
+F32 = ("filter1", "filter2")
+# bit for bit: integer model data, Ke, the FP64 operator, the layout conversions
+# and the PDE filter (each bit-exact against the reference in its own tests)
+EXACT = I32 + F32 + ("Fz", "Ke", "Ku", "KKu", "scatter_uy", "restrict", "filter_kernel", "filter_diag", "x_zero")
+# round-off: ce by a sequential row dot on both sides but C by the canonical
+# tree, the V-cycle's coarsest Cholesky and PCG's dot products in another order
+CLOSE = {"ce": 1e-13, "C": 1e-13, "vcycle": 1e-12, "x_mg": 1e-9, "x_jacobi": 1e-9, "x_cg": 1e-10}
+
+
+def _load(d, name):
+    dt = np.int32 if name in I32 else (np.float32 if name in F32 else np.float64)
+    return np.fromfile(os.path.join(d, name + ".bin"), dtype=dt)
+
+
+@pytest.mark.skipif(not (os.path.exists(GPU_CHECK) and os.path.exists(REF_CHECK)),
+                    reason="shim checks not built (make -C oracle shim)")
+@pytest.mark.parametrize("dims", [(8, 16, 8), (12, 20, 8)])
+def test_every_entry_point_through_the_shim(tmp_path, dims):
+    """integration/shim_check.cpp through the reference's headers, once on the
+    shim (GPU) and once on the reference's own CPU sources: the model arrays,
+    Ke, K*u (both overloads), restrict_to_free / scatter_from_free and the PDE
+    filter (including its warm start) bit for bit; ComputeCompliance, the
+    V-cycle closure and PCG_free with the multigrid, Jacobi and no
+    preconditioner to round-off with identical iteration counts; a zero
+    right-hand side gives one iteration and x = 0 (solver.cpp:147-149)."""
+    outs = {}
+    for name, exe in (("gpu", GPU_CHECK), ("ref", REF_CHECK)):
+        d = str(tmp_path / name)
+        os.makedirs(d)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([exe] + [str(v) for v in dims], cwd=d, capture_output=True, text=True, timeout=300,
+                           env=env)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        outs[name] = d
+    for name in EXACT:
+        a, b = _load(outs["gpu"], name), _load(outs["ref"], name)
+        assert a.shape == b.shape and np.array_equal(a, b), name
+    for name, bound in CLOSE.items():
+        a, b = _load(outs["gpu"], name), _load(outs["ref"], name)
+        assert a.shape == b.shape
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        print(dims, name, "rel diff %.2e" % err)
+        assert err <= bound, (name, err)
+    its = _load(outs["gpu"], "iters")
+    assert its[3] == 1 and not np.any(_load(outs["gpu"], "x_zero"))
