@@ -1102,7 +1102,7 @@ static int mg_galerkin_setup(t3d_mg* g)
     if (hfail) {
         g->cholOk = 0;
         set_error("Galerkin multigrid: the coarsest matrix is not positive definite");
-        return T3D_ERR_STATE;
+        return T3D_ERR_NOT_SPD;
     }
     T3D_HIP(launch_tri_inv(st, N, g->L, g->Li));
     const unsigned nt = unsigned((N + 63) / 64);

@@ -33,7 +33,10 @@ extern "C" {
 #define T3D_ERR_HIP (-1)        /* a HIP runtime call failed */
 #define T3D_ERR_ARG (-2)        /* bad argument / size */
 #define T3D_ERR_STATE (-3)      /* object not set up for this call */
-#define T3D_ERR_NOT_SPD (-4)    /* coarsest matrix not SPD (diagonal fallback used) */
+#define T3D_ERR_NOT_SPD (-4)    /* the Galerkin form's coarsest matrix is not positive definite
+                                   (t3d_mg_set_mode 1 has no fallback); the reference form falls
+                                   back to the diagonal silently, as preconditioner.cpp:76-77,
+                                   and t3d_mg_info reports cholOk = 0 */
 
 typedef struct t3d_problem t3d_problem;
 typedef struct t3d_mg t3d_mg;
