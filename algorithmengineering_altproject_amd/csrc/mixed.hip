@@ -1107,6 +1107,10 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * n, st));
     T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * n, st));
     if ((rc = mixed_vcycle(s, v, r, flexible, 1, false, flags))) return rc;
+    // the initial residual norm, against which the solve checks for divergence
+    double rn2First = 0.0;
+    T3D_HIP(hipMemcpyAsync(&rn2First, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipStreamSynchronize(st));
     // iteration it reads p_prev and writes p_cur; the two buffers alternate
     auto pPrevOf = [&](int it) { return (it & 1) ? s->pB : s->pA; };
     auto pCurOf = [&](int it) { return (it & 1) ? s->pA : s->pB; };
@@ -1142,8 +1146,20 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             // z = M r ; beta ; rz
             if ((rc = mixed_vcycle(s, v, r, flexible, 0, !replace, flags))) return rc;
         }
+        double rn2Now = 0.0;
         T3D_HIP(hipMemcpyAsync(s->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+        T3D_HIP(hipMemcpyAsync(&rn2Now, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
+        // a non-finite residual, or one grown 1e6-fold from the start, cannot
+        // recover: the FP32 operator is too inexact for this problem
+        if (!std::isfinite(rn2Now) || rn2Now > 1.0e12 * rn2First) {
+            launch_fold(s, x, nullptr);
+            T3D_HIP(hipStreamSynchronize(st));
+            if (iters) *iters = it1;
+            set_error("t3d_pcg_mixed: the residual diverged (non-finite or grown 1e6-fold) by iteration " +
+                      std::to_string(it1) + "; the FP32 operator is too inexact for this problem");
+            return T3D_ERR_DIVERGED;
+        }
         if (s->timing) {   // launches of iterations that ran (after convergence the kernel returns at once)
             const int last = s->hostFlags[0] ? std::min(it1, s->hostFlags[1]) : it1;
             for (int it = it0; it < last; ++it) {

@@ -33,6 +33,7 @@ extern "C" {
 #define T3D_ERR_HIP (-1)        /* a HIP runtime call failed */
 #define T3D_ERR_ARG (-2)        /* bad argument / size */
 #define T3D_ERR_STATE (-3)      /* object not set up for this call */
+#define T3D_ERR_DIVERGED (-5)   /* t3d_pcg_mixed: the residual became non-finite or grew 1e6-fold */
 #define T3D_ERR_NOT_SPD (-4)    /* the Galerkin form's coarsest matrix is not positive definite
                                    (t3d_mg_set_mode 1 has no fallback); the reference form falls
                                    back to the diagonal silently, as preconditioner.cpp:76-77,
@@ -174,7 +175,9 @@ int t3d_mixed_coarse_solve(t3d_mixed* s, const double* b_dev, double* x_dev);
 /* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
  * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old.
  * Convergence is reported only when the true residual ||b - K x|| (FP64
- * operator) is below tol * max(1e-30, ||b||). */
+ * operator) is below tol * max(1e-30, ||b||).  T3D_ERR_DIVERGED (x holds the
+ * last iterate, *iters the iterations run) when the residual norm becomes
+ * non-finite or exceeds 1e6 times the initial one, checked every 32 iterations. */
 int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, double tol, int maxIt, int trueEvery,
                   int flexible, int* iters);
 /* y = K u in FP32 on the full layout (K_times_u_finest, cpp/src/fea.cpp:763-837,
@@ -184,7 +187,12 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, do
  * contraction on MFMA as split-bf16 x3 products (2-piece operands) accumulated
  * in FP32, form 3: the same with 3-piece operands and 6 products (bf16 x6) */
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_dev);
-/* operator of t3d_pcg_mixed: 0 Walsh form (default), 2 MFMA bf16 x3, 3 MFMA bf16 x6 */
+/* operator of t3d_pcg_mixed: 0 Walsh form (default), 2 MFMA bf16 x3, 3 MFMA bf16 x6.
+ * Form 2's products round the element matrix and the displacements to 16
+ * significant bits, so its error scales with the displacement itself, not
+ * with the strains that K u measures: on smooth fields (the low modes) it is
+ * O(1) relative at 96x192x96 and above, and t3d_pcg_mixed diverges there
+ * (T3D_ERR_DIVERGED); it converges only on small grids. */
 int t3d_mixed_set_operator(t3d_mixed* s, int form);
 /* average milliseconds per launch over reps launches, HIP events on the
  * problem's stream: op 0..3 the FP32 operator of form 0..3; op 4 the solver's

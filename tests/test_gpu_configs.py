@@ -160,6 +160,12 @@ def test_config3_mfma_operators_and_solve_101M(t3d):
     assert it64 < 500 and itm < 500
     assert res < 1e-6
     assert abs(Cm - C64) <= 1e-6 * abs(C64)
+    # bf16 x3: its error scales with the displacement, O(1) on the low modes at
+    # this size, and the solve reports the divergence instead of running on
+    mx.set_operator(2)
+    x.zero()
+    with pytest.raises(t3d.T3DError, match=r"\(-5\).*diverged"):
+        mx.pcg(db, x, 1e-6, 500, trueEvery=25)
 
 
 def test_config4_batch64_matches_single_solves(t3d):

@@ -79,9 +79,12 @@ def test_fp_semantics_correctly_rounded(t3d):
     assert np.array_equal(t3d.fp_probe(2, d), np.sqrt(d))
 
 
+@pytest.mark.parametrize("dims", [(30, 60, 30), (96, 192, 128)])
 @pytest.mark.parametrize("xmode", ["random", "uniform0.12"])
-def test_simp_and_oc_bitexact(t3d, xmode):
-    dims = (30, 60, 30)
+def test_simp_and_oc_bitexact(t3d, xmode, dims):
+    """54 000 elements (at most 27 chunks of 2048: one 64-lane batch of
+    chunk aggregates) and 2.36M elements (1152 chunks: grid-stride aggregation,
+    carries over more than 64 chunks, repeated window doublings)."""
     p = t3d.Problem(*dims)
     m = O.Model(*dims)
     rng = np.random.default_rng(4)
@@ -124,10 +127,11 @@ def test_top3d_loop_bitexact(t3d, dims, kw):
     assert np.array_equal(x_g, x_o)
 
 
-def test_oc_passive_bitexact(t3d):
-    """OC with passive elements held at 1 (matlab/TOP3D_XL.m:154) against the oracle."""
+@pytest.mark.parametrize("dims", [(30, 60, 30), (96, 192, 128)])
+def test_oc_passive_bitexact(t3d, dims):
+    """OC with passive elements held at 1 (matlab/TOP3D_XL.m:154) against the
+    oracle; 54 000 and 2.36M elements, as test_simp_and_oc_bitexact."""
     import ctypes as C
-    dims = (30, 60, 30)
     p = t3d.Problem(*dims)
     m = O.Model(*dims)
     rng = np.random.default_rng(12)
