@@ -87,7 +87,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain solves")
     ap.add_argument("--no-tight", action="store_true", help="skip the tight-tolerance comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=4, help="reference PCG iterations timed on the CPU")
+    ap.add_argument("--cpu-iters", type=int, default=20, help="reference PCG iterations timed on the CPU")
     return ap.parse_args()
 
 
@@ -113,38 +113,73 @@ class stdout_to_stderr:
         return False
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, E, iters):
-    """The reference's own OpenMP code (oracle/_ref, -O3 -ffast-math -fopenmp) on this
-    host: a bounded sample of `cpu_iters` MG-PCG iterations on the same problem,
-    scaled to solves/s with the iteration count of one FP64 solve (the reference
-    runs the same iterations as the FP64 plain path)."""
+    """The reference's own OpenMP code (oracle/_ref: its sources built as its job
+    builds them, -O3 -ffast-math -fopenmp) on this host's cores, threads bound
+    close to cores as its job script sets them (OMP_PROC_BIND=close,
+    OMP_PLACES=cores).  A bounded sample of the headline workload (192x384x192):
+    the median of 20 K_times_u_finest applies and 20 MG-PCG iterations, scaled to
+    solves/s with the iteration count of one FP64 solve (the reference runs the
+    FP64 plain path's iterations); and one whole reference solve of the
+    96x192x96 cantilever (uniform rho = 0.12, its tol 1e-3, its cap of 800)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     if not O.have_ref(fast=True):
         return None
+    cores = len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ["OMP_PROC_BIND"] = "close"   # read by libgomp when the reference library loads
+    os.environ["OMP_PLACES"] = "cores"
     R = O.ref(fast=True)
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     import ctypes as C
     t0 = time.time()
     sizes = np.zeros(7, np.int32)
     R.ref_sizes(args.nely, args.nelx, args.nelz, sizes)   # builds the reference model + MG statics
     t_setup = time.time() - t0
+    u = np.random.default_rng(1).uniform(-1, 1, int(sizes[5]))
+    Ec = np.ascontiguousarray(E)
+    kxu_ms = R.ref_time_kxu(args.nely, args.nelx, args.nelz, Ec, u, 20)
+    nF, ne = int(sizes[5]), int(sizes[3])
     it = C.c_int()
-    ms = R.ref_time_pcg(args.nely, args.nelx, args.nelz, np.ascontiguousarray(E), 1e-6, 0.0, args.cpu_iters,
+    ms = R.ref_time_pcg(args.nely, args.nelx, args.nelz, Ec, float(np.float32(1e-9)), 0.0, args.cpu_iters,
                         C.byref(it))
     per_it = ms / 1e3 / it.value
-    u = np.random.default_rng(1).uniform(-1, 1, int(sizes[5]))
-    kxu_ms = R.ref_time_kxu(args.nely, args.nelx, args.nelz, np.ascontiguousarray(E), u, 3)
+    # one whole solve of the 96x192x96 cantilever with the reference's tolerance and cap
+    d1 = (96, 192, 96)
+    s1 = np.zeros(7, np.int32)
+    R.ref_sizes(*d1, s1)
+    rho3 = np.float32(0.12) * np.float32(0.12) * np.float32(0.12)
+    emin = np.float32(1e-9)
+    E1 = np.full(int(s1[3]), float(emin + rho3 * (np.float32(1.0) - emin)))
+    it1 = C.c_int()
+    ms1 = R.ref_time_pcg(*d1, E1, float(emin), float(np.float32(1e-3)), 800, C.byref(it1))
     return {
         "value": 1.0 / (per_it * iters),
         "unit": "solves/s",
         "cores": cores,
         "kind": "reference",
+        "cpu": cpu_model(),
+        "omp": "OMP_NUM_THREADS=%d OMP_PROC_BIND=close OMP_PLACES=cores" % cores,
+        "kxu_ms_median_of_20": kxu_ms,
+        "kxu_gbps": 8.0 * (2 * nF + ne) / kxu_ms / 1e6,
+        "pcg_s_per_iteration": per_it,
+        "solve_96x192x96": {"iterations": it1.value, "seconds": ms1 / 1e3, "tol": 1e-3, "max_iters": 800},
         "sample": "reference PCG_free + MG V-cycle (oracle/_ref, OpenMP -O3 -ffast-math), %d iterations on the "
                   "same 192x384x192 problem = %.3f s/iteration, scaled by the %d iterations of one FP64 solve; "
-                  "reference K_times_u_finest %.1f ms/apply; model setup %.1f s not timed"
-                  % (it.value, per_it, iters, kxu_ms, t_setup),
+                  "reference K_times_u_finest median of 20 applies %.1f ms; one whole reference solve of "
+                  "96x192x96 (tol 1e-3, cap 800): %d iterations, %.1f s; model setup %.1f s not timed"
+                  % (it.value, per_it, iters, kxu_ms, it1.value, ms1 / 1e3, t_setup),
     }
 
 
@@ -640,6 +675,22 @@ def main():
         log("tol %.0e: mixed %d it C=%.9e, FP64 %d it C=%.9e (true residual %.2e), rel diff %.2e"
             % (args.tight_tol, itm_t, Cm_t, it64_t, C64_t, res64_t, tight["compliance_rel_diff"]))
 
+    small = None
+    if not args.no_fp64:   # the FP64 plain path on the 96x192x96 cantilever, as the CPU baseline's whole solve
+        p1 = t3d.Problem(96, 192, 96)
+        x1 = t3d.DeviceArray.from_host(np.full(p1.ne, np.float32(0.12), np.float32))
+        E1 = t3d.DeviceArray(p1.ne)
+        t3d.simp(p1, x1, 1.0, float(np.float32(1e-9)), E1)
+        M1 = t3d.MG(p1, Emin=float(np.float32(1e-9)))
+        M1.setup(E1)
+        b1 = t3d.DeviceArray(p1.nDOF)
+        p1.load_vector(b1)
+        s1 = t3d.DeviceArray(p1.nDOF)
+        fp64_solve(t3d, p1, M1, b1, s1, args.ref_tol, 800)   # warm-up
+        it1, ms1 = fp64_solve(t3d, p1, M1, b1, s1, args.ref_tol, 800)
+        small = {"iterations": it1, "ms_per_solve": ms1, "tol": args.ref_tol, "max_iters": 800}
+        log("FP64 plain PCG at 96x192x96 (tol %.0e, cap 800): %d iterations, %.1f ms/solve" % (args.ref_tol, it1, ms1))
+        del p1, x1, E1, M1, b1, s1
     cpu = None
     if not args.no_cpu:
         try:
@@ -694,6 +745,7 @@ def main():
             "galerkin_resetup_s": gal_resetup_s,
             "fp64_same_multigrid": fp64_same,
             "fp64_plain": fp64,
+            "fp64_plain_96x192x96": small,
             "mixed_reference_vcycle": mixed_ref,
             "tight_tolerance_check": tight,
             "kxu_fp64_ms": kxu64_ms,
