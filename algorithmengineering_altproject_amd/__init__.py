@@ -51,6 +51,11 @@ def lib():
         "t3d_synchronize": (C.c_int, []),
         "t3d_problem_create_cuboid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
         "t3d_problem_create_voxels": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.POINTER(vp)]),
+        "t3d_problem_create_bc": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp,
+                                            C.POINTER(vp)]),
+        "t3d_problem_create_model": (C.c_int, [vp, C.POINTER(vp)]),
+        "t3d_topvoxel_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "t3d_voxel_model_destroy": (C.c_int, [vp]),
         "t3d_problem_destroy": (C.c_int, [vp]),
         "t3d_problem_sizes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "t3d_problem_model": (C.c_int, [vp] + [vp] * 8),
@@ -175,11 +180,26 @@ class Problem:
     """Device problem: CreateVoxelFEAmodel_Cuboid + ApplyBoundaryConditions
     (cpp/src/fea.cpp:181-447) with device mirrors."""
 
-    def __init__(self, nely, nelx, nelz, solid=None):
+    def __init__(self, nely, nelx, nelz, solid=None, fix=None, loads=None, model=None):
         """solid: optional voxel mask, uint8 indexed [ez, ex, ey] (flat (ez*nelx + ex)*nely + ey),
-        nonzero = solid; void voxels carry no element.  None: the full cuboid."""
+        nonzero = solid; void voxels carry no element.  None: the full cuboid.
+        fix / loads: boundary conditions as lists instead of the cantilever's
+        (t3d_problem_create_bc): fix int32 [n, 4] = x y z mask, loads a pair
+        (nodes int32 [m, 3] = x y z, forces float64 [m, 3]); nodes on the original
+        box, y from its bottom.  model: a VoxelModel (domain, fixations, loads)."""
         p = C.c_void_p()
-        if solid is None:
+        if model is not None:
+            check(lib().t3d_problem_create_model(model.h, C.byref(p)), "t3d_problem_create_model")
+        elif fix is not None or loads is not None:
+            sm = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+            fx = np.ascontiguousarray(np.zeros((0, 4)) if fix is None else fix, np.int32).reshape(-1, 4)
+            ln, lf = (np.zeros((0, 3)), np.zeros((0, 3))) if loads is None else loads
+            ln = np.ascontiguousarray(ln, np.int32).reshape(-1, 3)
+            lf = np.ascontiguousarray(lf, np.float64).reshape(-1, 3)
+            check(lib().t3d_problem_create_bc(nely, nelx, nelz, None if sm is None else _ptr(sm), len(fx),
+                                              _ptr(fx), len(ln), _ptr(ln), _ptr(lf), C.byref(p)),
+                  "t3d_problem_create_bc")
+        elif solid is None:
             check(lib().t3d_problem_create_cuboid(nely, nelx, nelz, C.byref(p)), "t3d_problem_create_cuboid")
         else:
             sm = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
@@ -417,7 +437,7 @@ class Params(C.Structure):
         ("mgMaxLevels", C.c_int), ("mgWeight", C.c_float), ("nonDyadic", C.c_int), ("NlimitDofs", C.c_int),
         ("rebuildEvery", C.c_int), ("precision", C.c_int), ("mgMode", C.c_int),
         ("stlPath", C.c_char_p), ("stlIso", C.c_float), ("summaryPath", C.c_char_p),
-        ("solid", C.c_void_p), ("passive", C.c_void_p),
+        ("solid", C.c_void_p), ("passive", C.c_void_p), ("model", C.c_void_p),
     ]
 
 
@@ -460,11 +480,54 @@ def default_params(**kw):
     return p
 
 
-def top3d_run(nely, nelx, nelz, solid=None, passive=None, **kw):
+class VoxelModel:
+    """A TopVoxel file (matlab/TOP3D_XL.m:458-552) parsed by t3d_topvoxel_read:
+    nelx, nely, nelz, solid / passive masks and density (indexed [ez, ex, ey]),
+    fix (int32 [n, 4] = x y z mask), loadNode (int32 [m, 3]), loadF ([m, 3]),
+    extraLoadCases."""
+
+    class _S(C.Structure):
+        _fields_ = [("nelx", C.c_int), ("nely", C.c_int), ("nelz", C.c_int),
+                    ("solid", C.POINTER(C.c_uint8)), ("passive", C.POINTER(C.c_uint8)),
+                    ("density", C.POINTER(C.c_float)), ("nFix", C.c_int), ("fix", C.POINTER(C.c_int32)),
+                    ("nLoad", C.c_int), ("loadNode", C.POINTER(C.c_int32)), ("loadF", C.POINTER(C.c_double)),
+                    ("extraLoadCases", C.c_int)]
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        check(lib().t3d_topvoxel_read(os.fsencode(path), C.byref(h)), "t3d_topvoxel_read")
+        self.h = h
+        S = C.cast(h, C.POINTER(VoxelModel._S)).contents
+        self.nelx, self.nely, self.nelz = S.nelx, S.nely, S.nelz
+        nv = S.nelx * S.nely * S.nelz
+        arr = lambda ptr, n, dt: np.ctypeslib.as_array(ptr, (n,)).astype(dt).copy() if n else np.zeros(0, dt)
+        self.solid = arr(S.solid, nv, np.uint8)
+        self.passive = arr(S.passive, nv, np.uint8)
+        self.density = arr(S.density, nv, np.float32) if bool(S.density) else None
+        self.fix = arr(S.fix, 4 * S.nFix, np.int32).reshape(-1, 4)
+        self.loadNode = arr(S.loadNode, 3 * S.nLoad, np.int32).reshape(-1, 3)
+        self.loadF = arr(S.loadF, 3 * S.nLoad, np.float64).reshape(-1, 3)
+        self.extraLoadCases = S.extraLoadCases
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().t3d_voxel_model_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def top3d_run(nely, nelx, nelz, solid=None, passive=None, model=None, **kw):
     """TOP3D_XL_GLOBAL (cpp/src/driver.cpp:23-218) for one simulation, on the device.
     solid / passive: optional uint8 voxel masks indexed [ez, ex, ey] (a voxel domain;
-    voxels held at density 1).  Returns (records, x, Csolid, solidIters)."""
+    voxels held at density 1); model: a VoxelModel (its domain, boundary conditions,
+    load and passive elements).  Returns (records, x, Csolid, solidIters)."""
     P = default_params(**kw)
+    if model is not None:
+        P.model = model.h
+        if solid is None:
+            solid = model.solid
     keep = []
     for name, mask in (("solid", solid), ("passive", passive)):
         if mask is not None:

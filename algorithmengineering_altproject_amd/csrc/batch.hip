@@ -207,6 +207,16 @@ __global__ void kb_init(const BProb* __restrict__ P, double tol)
 
 using namespace t3d;
 
+// every problem's [done, iteration] flags into one array (one copy to the host per chunk)
+__global__ void kb_gather_flags(const BProb* __restrict__ P, int n, int* __restrict__ out)
+{
+    const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[2 * i] = P[i].flags[0];
+        out[2 * i + 1] = P[i].flags[1];
+    }
+}
+
 struct t3d_batch {
     int n = 0;
     hipStream_t stream = nullptr;
@@ -232,6 +242,8 @@ struct t3d_batch {
     unsigned cholG = 1;   // workgroups per problem of the coarsest products
     int maxLev = 0;
     int* itDev = nullptr;
+    int* flagsDev = nullptr;    // [2n] every problem's done flag and iteration, gathered for one copy
+    int* flagsHost = nullptr;   // pinned mirror of flagsDev
     // tiles
     int kwNZ = 16, rpb = 4;
     bool tuned = false, fixedTiles = false;
@@ -473,6 +485,8 @@ int t3d_batch_destroy(t3d_batch* B)
     free_maps(B);
     if (B->dP) (void)hipFree(B->dP);
     if (B->itDev) (void)hipFree(B->itDev);
+    if (B->flagsDev) (void)hipFree(B->flagsDev);
+    if (B->flagsHost) (void)hipHostFree(B->flagsHost);
     for (int i = 0; i < B->n; ++i) {
         if (B->P[i]) B->P[i]->stream = B->ownStreams[i];
         t3d_mixed_destroy(B->X[i]);
@@ -703,12 +717,20 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
                 hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->itDev);
             }
         }
+        if (!B->flagsDev) {
+            T3D_HIP(hipMalloc(&B->flagsDev, sizeof(int) * 2 * size_t(n)));
+            T3D_HIP(hipHostMalloc(&B->flagsHost, sizeof(int) * 2 * size_t(n)));
+        }
+        hipLaunchKernelGGL(kb_gather_flags, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, (const BProb*)B->dP,
+                           n, B->flagsDev);
         T3D_HIP(hipGetLastError());
+        T3D_HIP(hipMemcpyAsync(B->flagsHost, B->flagsDev, sizeof(int) * 2 * size_t(n), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
         bool any = false, allDone = true;
         for (int i = 0; i < n; ++i) {
             if (finished[i]) continue;
-            T3D_HIP(hipMemcpy(&hf[2 * i], B->X[i]->iflags, sizeof(int) * 2, hipMemcpyDeviceToHost));
+            hf[2 * i] = B->flagsHost[2 * i];
+            hf[2 * i + 1] = B->flagsHost[2 * i + 1];
             if (hf[2 * i]) any = true;
         }
         if (any) {

@@ -42,7 +42,7 @@ void t3d_params_default(t3d_params* P)
     P->mgMaxLevels = 5; P->mgWeight = 0.6f; P->nonDyadic = 1; P->NlimitDofs = 1000;
     P->rebuildEvery = 5; P->precision = 0; P->mgMode = 0;
     P->stlPath = nullptr; P->stlIso = 0.3f; P->summaryPath = nullptr;
-    P->solid = nullptr; P->passive = nullptr;
+    P->solid = nullptr; P->passive = nullptr; P->model = nullptr;
 }
 
 // the reference's summary text (driver.cpp:225-258); iostream's scientific and
@@ -99,12 +99,18 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
               "t3d_top3d_run: mgMode must be 0 (the C++ port's V-cycle) or 1 (Galerkin multigrid)");
     T3D_CHECK(P->rebuildEvery > 0, T3D_ERR_ARG, "t3d_top3d_run: rebuildEvery must be positive");
     T3D_CHECK(P->nLoop >= 0 && P->cgMaxIt >= 0, T3D_ERR_ARG, "t3d_top3d_run: negative nLoop or cgMaxIt");
+    const t3d_voxel_model* vm = P->model;
+    T3D_CHECK(!vm || (vm->nely == nely && vm->nelx == nelx && vm->nelz == nelz), T3D_ERR_ARG,
+              "t3d_top3d_run: nely/nelx/nelz differ from the voxel model's");
+    // a voxel model brings its domain, boundary conditions and load, and its
+    // passive elements unless P->passive is given
+    const uint8_t* passiveMask = P->passive ? P->passive : (vm ? vm->passive : nullptr);
     if (loopsRun) *loopsRun = 0;
     t3d_problem* p = nullptr;
     t3d_mg* M = nullptr;
     t3d_filter* F = nullptr;
     t3d_mixed* MX = nullptr;
-    int rc = t3d_problem_create_voxels(nely, nelx, nelz, P->solid, &p);
+    int rc = vm ? t3d_problem_create_model(vm, &p) : t3d_problem_create_voxels(nely, nelx, nelz, P->solid, &p);
     if (rc) return rc;
     struct Guard {
         t3d_problem*& p;
@@ -156,14 +162,14 @@ int t3d_top3d_run(int nely, int nelx, int nelz, const t3d_params* P, t3d_loop_re
     T3D_HIP(hipMalloc(&bsc.p, sizeof(double) * 16));
     // passive voxels in compact element order (matlab/TOP3D_XL.m:85, 154)
     DevBuf bpas;
-    if (P->passive) {
+    if (passiveMask) {
         const HostModel& m = p->host;
         std::vector<uint8_t> pc(size_t(ne), 0);
         for (int e = 0; e < ne; ++e) {
             const int full = m.eleMapBack[e];
             const int ez = full / (m.resX * m.resY), rem = full % (m.resX * m.resY);
             const int ex = rem / m.resY, ey = m.resY - 1 - rem % m.resY;
-            pc[size_t(e)] = P->passive[(size_t(ez) * nelx + ex) * nely + ey] ? 1 : 0;
+            pc[size_t(e)] = passiveMask[(size_t(ez) * nelx + ex) * nely + ey] ? 1 : 0;
         }
         T3D_HIP(hipMalloc(&bpas.p, size_t(ne)));
         T3D_HIP(hipMemcpy(bpas.p, pc.data(), size_t(ne), hipMemcpyHostToDevice));

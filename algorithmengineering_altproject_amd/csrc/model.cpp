@@ -172,7 +172,13 @@ void apply_bcs(HostModel& m)
         for (int iz = 0; iz <= izTop; ++iz)
             for (int iy = 0; iy < nny; ++iy) m.Fz[nnx * nny * iz + nnx * iy + ixLoad] += fz;
     }
-    // nodes touched by no element are fixed (fea.cpp:405-417)
+    finish_bcs(m);
+}
+
+// nodes touched by no element are fixed (fea.cpp:405-417); the free-DOF index
+// and the element-to-free-DOF map (fea.cpp:419-447)
+void finish_bcs(HostModel& m)
+{
     {
         std::vector<uint8_t> used(m.numNodes, 0);
         for (int32_t n : m.eNodMat) used[n] = 1;
@@ -188,6 +194,54 @@ void apply_bcs(HostModel& m)
         for (int a = 0; a < 8; ++a)
             for (int c = 0; c < 3; ++c)
                 m.eDofFreeMat[size_t(24) * e + 3 * a + c] = g2f[3 * m.eNodMat[size_t(8) * e + a] + c];
+}
+
+// Fixations and loads given as lists on the nodes of the original box, as a
+// TopVoxel file carries them (matlab/TOP3D_XL.m:470-552, applied as
+// FEA_ApplyBoundaryCondition, :1319-1360): a node listed twice keeps its first
+// entry (MATLAB's unique), a loaded node is not fixed (its setdiff), the load
+// is the node's force, the fixed DOFs are those whose flag is set, and nodes
+// touched by no element are fixed as in the cuboid (fea.cpp:405-417).
+int apply_bc_lists(HostModel& m, const BcLists& bc, std::string& err)
+{
+    const int nnx = m.resX + 1, nny = m.resY + 1;
+    auto node_of = [&](const int32_t* c, int& n) {
+        const int x = c[0], y = c[1], z = c[2];
+        if (x < 0 || x > m.nelx || y < 0 || y > m.nely || z < 0 || z > m.nelz) return false;
+        n = nnx * nny * z + nnx * (m.resY - y) + x;   // node y counts up from the box's bottom
+        return true;
+    };
+    std::fill(m.isFreeDOF.begin(), m.isFreeDOF.end(), 1);
+    std::fill(m.Fx.begin(), m.Fx.end(), 0.0);
+    std::fill(m.Fy.begin(), m.Fy.end(), 0.0);
+    std::fill(m.Fz.begin(), m.Fz.end(), 0.0);
+    std::vector<uint8_t> loaded(m.numNodes, 0);
+    for (int i = 0; i < bc.nLoad; ++i) {
+        int n = 0;
+        if (!node_of(bc.loadNode + 3 * i, n)) {
+            err = "load " + std::to_string(i) + ": node outside the box";
+            return -1;
+        }
+        if (loaded[n]) continue;
+        loaded[n] = 1;
+        m.Fx[n] = bc.loadF[3 * i];
+        m.Fy[n] = bc.loadF[3 * i + 1];
+        m.Fz[n] = bc.loadF[3 * i + 2];
+    }
+    std::vector<uint8_t> seen(m.numNodes, 0);
+    for (int i = 0; i < bc.nFix; ++i) {
+        int n = 0;
+        if (!node_of(bc.fix + 4 * i, n)) {
+            err = "fixation " + std::to_string(i) + ": node outside the box";
+            return -1;
+        }
+        if (seen[n] || loaded[n]) continue;
+        seen[n] = 1;
+        for (int c = 0; c < 3; ++c)
+            if ((bc.fix[4 * i + 3] >> c) & 1) m.isFreeDOF[3 * size_t(n) + c] = 0;
+    }
+    finish_bcs(m);
+    return 0;
 }
 
 }  // namespace t3d

@@ -10,6 +10,7 @@
 #pragma once
 #include <array>
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace t3d {
@@ -42,6 +43,22 @@ void build_cuboid(HostModel& m, int nely, int nelx, int nelz);
 // solid == nullptr: the full cuboid.
 void build_voxels(HostModel& m, int nely, int nelx, int nelz, const uint8_t* solid);
 void apply_bcs(HostModel& m);
+// the fixed-node, free-DOF and element-to-free-DOF maps after the BCs are set
+void finish_bcs(HostModel& m);
+
+// fixations and loads as lists on the nodes (x, y, z) of the original
+// nelx x nely x nelz box, y counted from the box's bottom (the element y of
+// the solid mask): fix[4 i] = x y z mask (bit c set: DOF c fixed),
+// loadNode[3 i] = x y z, loadF[3 i] = the node's force
+struct BcLists {
+    int nFix = 0;
+    const int32_t* fix = nullptr;
+    int nLoad = 0;
+    const int32_t* loadNode = nullptr;
+    const double* loadF = nullptr;
+};
+// 0 on success, -1 (err set) for a node outside the box
+int apply_bc_lists(HostModel& m, const BcLists& bc, std::string& err);
 
 // structured cell index used on the device: (cz*resY + ty)*resX + cx with
 // ty = resY-1-ey (node-y of the cell's lower face)

@@ -257,17 +257,57 @@ int t3d_problem_create_cuboid(int nely, int nelx, int nelz, t3d_problem** out)
 
 int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid, t3d_problem** out)
 {
-    T3D_CHECK(out && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_problem_create_voxels: bad arguments");
+    return create_problem(nely, nelx, nelz, solid, nullptr, out);
+}
+
+int t3d_problem_create_bc(int nely, int nelx, int nelz, const uint8_t* solid, int nFix, const int32_t* fix,
+                          int nLoad, const int32_t* loadNode, const double* loadF, t3d_problem** out)
+{
+    T3D_CHECK(nFix >= 0 && nLoad >= 0 && (nFix == 0 || fix) && (nLoad == 0 || (loadNode && loadF)), T3D_ERR_ARG,
+              "t3d_problem_create_bc: bad fixation or load list");
+    BcLists bc;
+    bc.nFix = nFix;
+    bc.fix = fix;
+    bc.nLoad = nLoad;
+    bc.loadNode = loadNode;
+    bc.loadF = loadF;
+    return create_problem(nely, nelx, nelz, solid, &bc, out);
+}
+
+int t3d_problem_create_model(const t3d_voxel_model* vm, t3d_problem** out)
+{
+    T3D_CHECK(vm && vm->solid, T3D_ERR_ARG, "t3d_problem_create_model: null model");
+    return t3d_problem_create_bc(vm->nely, vm->nelx, vm->nelz, vm->solid, vm->nFix, vm->fix, vm->nLoad,
+                                 vm->loadNode, vm->loadF, out);
+}
+
+}  // extern "C"
+
+namespace t3d {
+// the padded voxel model with the cuboid's boundary conditions (bc == nullptr)
+// or the given fixation and load lists, and its device mirror
+int create_problem(int nely, int nelx, int nelz, const uint8_t* solid, const BcLists* bc, t3d_problem** out)
+{
+    T3D_CHECK(out && nely > 0 && nelx > 0 && nelz > 0, T3D_ERR_ARG, "t3d_problem_create: bad arguments");
     *out = nullptr;
     if (solid) {
         size_t cnt = 0;
         for (size_t i = 0; i < size_t(nely) * nelx * nelz; ++i) cnt += solid[i] != 0;
-        T3D_CHECK(cnt > 0, T3D_ERR_ARG, "t3d_problem_create_voxels: no solid voxel");
+        T3D_CHECK(cnt > 0, T3D_ERR_ARG, "t3d_problem_create: no solid voxel");
     }
     t3d_problem* p = new t3d_problem();
     HostModel& m = p->host;
     build_voxels(m, nely, nelx, nelz, solid);
-    apply_bcs(m);
+    if (bc) {
+        std::string err;
+        if (apply_bc_lists(m, *bc, err)) {
+            delete p;
+            set_error("t3d_problem_create_bc: " + err);
+            return T3D_ERR_ARG;
+        }
+    } else {
+        apply_bcs(m);
+    }
     Grid& g = p->g;
     g.resX = m.resX; g.resY = m.resY; g.resZ = m.resZ;
     g.nnx = m.resX + 1; g.nny = m.resY + 1; g.nnz = m.resZ + 1;
@@ -332,6 +372,9 @@ int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid
     *out = p;
     return T3D_OK;
 }
+}  // namespace t3d
+
+extern "C" {
 
 int t3d_problem_sizes(const t3d_problem* p, int64_t* s)
 {

@@ -106,6 +106,8 @@ struct t3d_problem {
 
 namespace t3d {
 int ensure_red(t3d_problem* p, int64_t items);
+// a problem with the cuboid's boundary conditions (bc == nullptr) or the given lists
+int create_problem(int nely, int nelx, int nelz, const uint8_t* solid, const BcLists* bc, t3d_problem** out);
 // canonical sum of n items produced by kernel `first` into p->red; final value to dst (device)
 int canon_finish(t3d_problem* p, int64_t ntiles, double* dst, const int* done = nullptr);
 // the same over the tile partials buf[0..ntiles), buf[ntiles..2 ntiles) the second buffer

@@ -74,6 +74,39 @@ int t3d_voxel_ke(double nu, double cellSize, double* Ke);
  * (fea.cpp:360-447) apply unchanged.  solid == NULL: the full cuboid.
  * T3D_ERR_ARG if no voxel is solid. */
 int t3d_problem_create_voxels(int nely, int nelx, int nelz, const uint8_t* solid, t3d_problem** out);
+/* The voxel domain with boundary conditions and load given as lists, as the
+ * MATLAB original takes them from a TopVoxel file (matlab/TOP3D_XL.m:458-552,
+ * applied as FEA_ApplyBoundaryCondition, :1319-1360; the C++ port has only the
+ * cantilever of fea.cpp:360-447).  Nodes are (x, y, z) on the original
+ * nelx x nely x nelz box, y counted from its bottom like the solid mask's ey:
+ * fix[4 i] = x y z mask (bit c set: DOF c fixed), loadNode[3 i] = x y z,
+ * loadF[3 i] = the node's force.  A node listed twice keeps its first entry,
+ * a loaded node is not fixed, nodes touched by no element are fixed.
+ * T3D_ERR_ARG for a node outside the box. */
+int t3d_problem_create_bc(int nely, int nelx, int nelz, const uint8_t* solid, int nFix, const int32_t* fix,
+                          int nLoad, const int32_t* loadNode, const double* loadF, t3d_problem** out);
+
+/* A TopVoxel file (matlab/TOP3D_XL.m:458-586, version 1.0; inputs/README.md),
+ * parsed.  The file's voxel and node indices (1-based, MATLAB order: y rows
+ * from the top, then x, then z) become the masks and node lists above; the
+ * forces and fixation flags are taken as written. */
+typedef struct t3d_voxel_model {
+    int nelx, nely, nelz;
+    uint8_t* solid;        /* [nely*nelx*nelz] solid[(ez*nelx + ex)*nely + ey] != 0 */
+    uint8_t* passive;      /* same indexing: passive elements of the file */
+    float* density;        /* same indexing: the file's densities (NULL when it has none) */
+    int nFix;
+    int32_t* fix;          /* [4*nFix] x y z mask */
+    int nLoad;
+    int32_t* loadNode;     /* [3*nLoad] x y z */
+    double* loadF;         /* [3*nLoad] */
+    int extraLoadCases;    /* further load cases in the file; the loop solves the first */
+} t3d_voxel_model;
+/* T3D_ERR_ARG (with the reason in t3d_last_error) for a file that is not TopVoxel 1.0 */
+int t3d_topvoxel_read(const char* path, t3d_voxel_model** out);
+int t3d_voxel_model_destroy(t3d_voxel_model* m);
+/* t3d_problem_create_bc with the model's domain and lists */
+int t3d_problem_create_model(const t3d_voxel_model* m, t3d_problem** out);
 int t3d_problem_destroy(t3d_problem* p);
 /* sizes[0..9] = resX resY resZ numElements numNodes numDOFs numFree nnx nny nnz */
 int t3d_problem_sizes(const t3d_problem* p, int64_t* sizes);
@@ -312,6 +345,9 @@ typedef struct t3d_params {
                                 (t3d_problem_create_voxels); NULL = the cuboid */
     const uint8_t* passive;  /* optional passive voxels, same indexing: held at density 1 by
                                 the OC update (t3d_oc_update_passive); NULL = none */
+    const t3d_voxel_model* model; /* optional: the domain, boundary conditions and load of a
+                                voxel model (t3d_topvoxel_read); its passive elements apply
+                                unless passive is set; nely/nelx/nelz must be the model's */
 } t3d_params;
 typedef struct t3d_loop_record {
     int loop, pcgIters, filterIters, bisectSteps;

@@ -100,6 +100,13 @@ def lib():
         L.o_model_new.argtypes = [C.c_int, C.c_int, C.c_int]
         L.o_model_new_voxels.restype = C.POINTER(OModel)
         L.o_model_new_voxels.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.o_model_new_bc.restype = C.POINTER(OModel)
+        L.o_model_new_bc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                     C.c_void_p, C.c_void_p]
+        L.o_top3d_run_bc.restype = C.c_int
+        L.o_top3d_run_bc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.POINTER(OParams), C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.o_model_free.argtypes = [C.POINTER(OModel)]
         L.o_canon_sum.restype = C.c_double
         L.o_canon_sum.argtypes = [f64p, C.c_size_t]
@@ -168,12 +175,22 @@ def _arr(ptr, n, dtype):
 class Model:
     """Oracle model (fea.cpp:181-447): arrays are copied out as numpy."""
 
-    def __init__(self, nely, nelx, nelz, solid=None):
+    def __init__(self, nely, nelx, nelz, solid=None, fix=None, loads=None):
         """solid: optional uint8 array indexed [ez, ex, ey] (flat (ez*nelx + ex)*nely + ey),
-        nonzero = solid voxel; None = the full cuboid."""
+        nonzero = solid voxel; None = the full cuboid.  fix / loads: boundary
+        conditions as lists (o_model_new_bc): fix int32 [n, 4] = x y z mask, loads a
+        pair (nodes int32 [m, 3], forces [m, 3]); None for both: the cantilever."""
         L = lib()
         self.nely, self.nelx, self.nelz = nely, nelx, nelz
-        if solid is None:
+        if fix is not None or loads is not None:
+            sm = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+            fx, ln, lf = bc_arrays(fix, loads)
+            self._keep = (sm, fx, ln, lf)
+            self.ptr = L.o_model_new_bc(nely, nelx, nelz, None if sm is None else sm.ctypes.data, len(fx),
+                                        fx.ctypes.data, len(ln), ln.ctypes.data, lf.ctypes.data)
+            if not self.ptr:
+                raise ValueError("boundary condition node outside the box")
+        elif solid is None:
             self.ptr = L.o_model_new(nely, nelx, nelz)
         else:
             sm = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
@@ -327,7 +344,13 @@ def default_params(**kw):
     return p
 
 
-def run(nely, nelx, nelz, solid=None, passive=None, **kw):
+def bc_arrays(fix, loads):
+    fx = np.ascontiguousarray(np.zeros((0, 4)) if fix is None else fix, np.int32).reshape(-1, 4)
+    ln, lf = (np.zeros((0, 3)), np.zeros((0, 3))) if loads is None else loads
+    return fx, np.ascontiguousarray(ln, np.int32).reshape(-1, 3), np.ascontiguousarray(lf, np.float64).reshape(-1, 3)
+
+
+def run(nely, nelx, nelz, solid=None, passive=None, fix=None, loads=None, **kw):
     """Solid solve + OC loops (driver.cpp:23-218), on a voxel domain when solid is
     given, with passive voxels held at 1 (matlab/TOP3D_XL.m:154) when passive is
     given (uint8 masks indexed [ez, ex, ey]). Returns (records, x, Csolid, solidIters)."""
@@ -340,8 +363,18 @@ def run(nely, nelx, nelz, solid=None, passive=None, **kw):
     si = C.c_int()
     sm = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
     pm = None if passive is None else np.ascontiguousarray(passive, np.uint8).reshape(-1)
-    n = lib().o_top3d_run_ex(nely, nelx, nelz, None if sm is None else sm.ctypes.data,
-                             None if pm is None else pm.ctypes.data, C.byref(p), rec, x, C.byref(Cs), C.byref(si))
+    if fix is not None or loads is not None:
+        fx, ln, lf = bc_arrays(fix, loads)
+        n = lib().o_top3d_run_bc(nely, nelx, nelz, None if sm is None else sm.ctypes.data,
+                                 None if pm is None else pm.ctypes.data, len(fx), fx.ctypes.data, len(ln),
+                                 ln.ctypes.data, lf.ctypes.data, C.byref(p), C.cast(rec, C.c_void_p),
+                                 x.ctypes.data, C.byref(Cs), C.byref(si))
+        if n < 0:
+            raise ValueError("boundary condition node outside the box")
+    else:
+        n = lib().o_top3d_run_ex(nely, nelx, nelz, None if sm is None else sm.ctypes.data,
+                                 None if pm is None else pm.ctypes.data, C.byref(p), rec, x, C.byref(Cs),
+                                 C.byref(si))
     recs = [{f: getattr(r, f) for f, _ in OLoopRec._fields_} for r in rec[:n]]
     return recs, x, Cs.value, si.value
 

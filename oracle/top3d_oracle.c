@@ -226,7 +226,7 @@ o_model* o_model_new(int nely, int nelx, int nelz)
 /* solid[(ez*nelx + ex)*nely + ey] != 0: solid voxel of the original region
  * (NULL: all); void voxels carry no element, like the padding
  * (fea.cpp:203-227, generalised from the cuboid's all-solid region) */
-o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid)
+static o_model* build_mesh(int nely, int nelx, int nelz, const uint8_t* solid)
 {
     o_model* m = (o_model*)calloc(1, sizeof(o_model));
     m->nelx = nelx; m->nely = nely; m->nelz = nelz;
@@ -285,28 +285,16 @@ o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid)
     for (int c = 0; c < 8; ++c) pos[c] = m->bucketStart[c];
     for (int e = 0; e < ne; ++e) m->bucket[pos[m->color[e]]++] = e;
     o_voxel_ke(0.3, 1.0, m->Ke); /* fea.cpp:345 */
-
-    /* boundary conditions (fea.cpp:360-447) */
-    const int nnz = m->nnz;
     m->isFree = XALLOC(uint8_t, m->nDOF);
-    memset(m->isFree, 1, (size_t)m->nDOF);
     m->F = XALLOC(double, m->nDOF);
-    for (int iz = 0; iz < nnz; ++iz)
-        for (int iy = 0; iy < nny; ++iy) {
-            int n = nnx * nny * iz + nnx * iy + 0;
-            m->isFree[3 * n] = m->isFree[3 * n + 1] = m->isFree[3 * n + 2] = 0;
-        }
-    {
-        int ixLoad = nelx < nnx - 1 ? nelx : nnx - 1;
-        int izMax = (nz / 6) > 1 ? (nz / 6) : 1;
-        int count = (izMax + 1) * nny;
-        double fz = -1.0 / (double)count;
-        for (int iz = 0; iz <= izMax; ++iz)
-            for (int iy = 0; iy < nny; ++iy) {
-                int n = nnx * nny * iz + nnx * iy + ixLoad;
-                m->F[3 * n + 2] += fz;
-            }
-    }
+    return m;
+}
+
+/* free-DOF maps after the boundary conditions: nodes touched by no element
+ * fixed (fea.cpp:405-417), freeDofIndex and eDofFreeMat (fea.cpp:419-447) */
+static void finish_bcs(o_model* m)
+{
+    const int ne = m->ne;
     {
         uint8_t* used = XALLOC(uint8_t, m->nN);
         for (size_t i = 0; i < 8 * (size_t)ne; ++i) used[m->eNodMat[i]] = 1;
@@ -329,6 +317,74 @@ o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid)
             for (int c = 0; c < 3; ++c)
                 m->eDofFree[24 * (size_t)e + 3 * a + c] = g2f[3 * m->eNodMat[8 * (size_t)e + a] + c];
     free(g2f);
+}
+
+o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid)
+{
+    o_model* m = build_mesh(nely, nelx, nelz, solid);
+    const int nnx = m->nnx, nny = m->nny, nnz = m->nnz, nz = m->resZ;
+    /* boundary conditions (fea.cpp:360-447) */
+    memset(m->isFree, 1, (size_t)m->nDOF);
+    for (int iz = 0; iz < nnz; ++iz)
+        for (int iy = 0; iy < nny; ++iy) {
+            int n = nnx * nny * iz + nnx * iy + 0;
+            m->isFree[3 * n] = m->isFree[3 * n + 1] = m->isFree[3 * n + 2] = 0;
+        }
+    {
+        int ixLoad = nelx < nnx - 1 ? nelx : nnx - 1;
+        int izMax = (nz / 6) > 1 ? (nz / 6) : 1;
+        int count = (izMax + 1) * nny;
+        double fz = -1.0 / (double)count;
+        for (int iz = 0; iz <= izMax; ++iz)
+            for (int iy = 0; iy < nny; ++iy) {
+                int n = nnx * nny * iz + nnx * iy + ixLoad;
+                m->F[3 * n + 2] += fz;
+            }
+    }
+    finish_bcs(m);
+    return m;
+}
+
+/* fixations and loads as lists on the original box's nodes (x, y, z), y from
+ * the box's bottom (padded node row resY - y), as a TopVoxel file gives them
+ * (matlab/TOP3D_XL.m:470-552) and FEA_ApplyBoundaryCondition applies them
+ * (:1319-1360): the first entry of a node counts (unique), loaded nodes are not
+ * fixed (setdiff), a load sets the node's force, flagged DOFs are fixed */
+o_model* o_model_new_bc(int nely, int nelx, int nelz, const uint8_t* solid, int nFix, const int32_t* fix, int nLoad,
+                        const int32_t* loadNode, const double* loadF)
+{
+    o_model* m = build_mesh(nely, nelx, nelz, solid);
+    const int nnx = m->nnx, nny = m->nny;
+    memset(m->isFree, 1, (size_t)m->nDOF);
+    memset(m->F, 0, sizeof(double) * (size_t)m->nDOF);
+    uint8_t* loaded = XALLOC(uint8_t, m->nN);
+    uint8_t* seen = XALLOC(uint8_t, m->nN);
+    int bad = 0;
+    for (int i = 0; i < nLoad && !bad; ++i) {
+        const int32_t* c = loadNode + 3 * i;
+        if (c[0] < 0 || c[0] > nelx || c[1] < 0 || c[1] > nely || c[2] < 0 || c[2] > nelz) { bad = 1; break; }
+        const int n = nnx * nny * c[2] + nnx * (m->resY - c[1]) + c[0];
+        if (loaded[n]) continue;
+        loaded[n] = 1;
+        for (int d = 0; d < 3; ++d) m->F[3 * (size_t)n + d] = loadF[3 * i + d];
+    }
+    for (int i = 0; i < nFix && !bad; ++i) {
+        const int32_t* c = fix + 4 * i;
+        if (c[0] < 0 || c[0] > nelx || c[1] < 0 || c[1] > nely || c[2] < 0 || c[2] > nelz) { bad = 1; break; }
+        const int n = nnx * nny * c[2] + nnx * (m->resY - c[1]) + c[0];
+        if (seen[n] || loaded[n]) continue;
+        seen[n] = 1;
+        for (int d = 0; d < 3; ++d)
+            if ((c[3] >> d) & 1) m->isFree[3 * (size_t)n + d] = 0;
+    }
+    free(loaded);
+    free(seen);
+    if (bad) {
+        free(m->isFree); free(m->F); m->isFree = NULL; m->F = NULL;
+        o_model_free(m);
+        return NULL;
+    }
+    finish_bcs(m);
     return m;
 }
 
@@ -1332,10 +1388,29 @@ static uint8_t* passive_compact(const o_model* m, const uint8_t* passive)
     return pc;
 }
 
+static int run_model(o_model* m, const uint8_t* passive, const o_params* P, o_loop_rec* rec, float* x_out,
+                     double* CsolidOut, int* solidIters);
+
 int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, const o_params* P,
                    o_loop_rec* rec, float* x_out, double* CsolidOut, int* solidIters)
 {
     o_model* m = solid ? o_model_new_voxels(nely, nelx, nelz, solid) : o_model_new(nely, nelx, nelz);
+    return run_model(m, passive, P, rec, x_out, CsolidOut, solidIters);
+}
+
+int o_top3d_run_bc(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, int nFix,
+                   const int32_t* fix, int nLoad, const int32_t* loadNode, const double* loadF, const o_params* P,
+                   o_loop_rec* rec, float* x_out, double* CsolidOut, int* solidIters)
+{
+    o_model* m = o_model_new_bc(nely, nelx, nelz, solid, nFix, fix, nLoad, loadNode, loadF);
+    if (!m) return -1;
+    return run_model(m, passive, P, rec, x_out, CsolidOut, solidIters);
+}
+
+/* the loop on a built model (takes ownership of m) */
+static int run_model(o_model* m, const uint8_t* passive, const o_params* P, o_loop_rec* rec, float* x_out,
+                     double* CsolidOut, int* solidIters)
+{
     uint8_t* pc = passive_compact(m, passive);
     o_filter* flt = o_filter_new(m, P->rMin);
     o_mg* g = o_mg_new(m, P->nonDyadic, P->mgMaxLevels, P->mgWeight, P->Emin, P->NlimitDofs);

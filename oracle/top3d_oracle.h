@@ -61,6 +61,11 @@ typedef struct o_model {
 o_model* o_model_new(int nely, int nelx, int nelz);
 /* general voxel domain: solid[(ez*nelx + ex)*nely + ey] != 0 (NULL: cuboid) */
 o_model* o_model_new_voxels(int nely, int nelx, int nelz, const uint8_t* solid);
+/* the voxel domain with fixations and loads as lists on the original box's
+ * nodes (x, y, z), y from the box's bottom: fix[4i] = x y z mask, loadNode[3i],
+ * loadF[3i] (matlab/TOP3D_XL.m:470-552, :1319-1360); NULL for a node outside */
+o_model* o_model_new_bc(int nely, int nelx, int nelz, const uint8_t* solid, int nFix, const int32_t* fix, int nLoad,
+                        const int32_t* loadNode, const double* loadF);
 void o_model_free(o_model* m);
 void o_voxel_ke(double nu, double cellSize, double Ke[576]);
 void o_padding(int nelx, int nely, int nelz, int* rx, int* ry, int* rz, int* L);
@@ -181,6 +186,10 @@ int o_top3d_run(int nely, int nelx, int nelz, const o_params* p,
 /* the loop on a voxel domain (solid mask as o_model_new_voxels, NULL = cuboid)
  * with passive elements (voxel-order mask, NULL = none) */
 int o_top3d_run_ex(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, const o_params* p,
+                   o_loop_rec* rec, float* x_out, double* Csolid, int* solidIters);
+/* the same with the boundary conditions of o_model_new_bc (-1 for a bad node) */
+int o_top3d_run_bc(int nely, int nelx, int nelz, const uint8_t* solid, const uint8_t* passive, int nFix,
+                   const int32_t* fix, int nLoad, const int32_t* loadNode, const double* loadF, const o_params* p,
                    o_loop_rec* rec, float* x_out, double* Csolid, int* solidIters);
 
 /* ---- voxel-surface STL (io.cpp:80-87, geometry_out/voxel_surface.hpp:12-93,
