@@ -131,8 +131,8 @@ struct SeqShared {
 __device__ __forceinline__ seqsum::Tr shfl_up_tr(const seqsum::Tr& t, int d)
 {
     seqsum::Tr o;
-    o.r0 = __shfl_up((long long)t.r0, d, 64);
-    o.r1 = __shfl_up((long long)t.r1, d, 64);
+    o.r0 = uint32_t(__shfl_up(int(t.r0), d, 64));
+    o.r1 = uint32_t(__shfl_up(int(t.r1), d, 64));
     return o;
 }
 
@@ -222,56 +222,64 @@ __device__ float block_seq_sum(int64_t n, const Gen& gen, SeqShared& sh)
     return res;
 }
 
-// passive (nullable): elements held at density 1 (matlab/TOP3D_XL.m:154)
-struct GenXnew {
-    const float* x;
-    const float* dc;
-    const uint8_t* passive;
-    float lmid, move;
-    __device__ float operator()(int64_t i) const
-    {
-        return (passive && passive[i]) ? 1.0f : oc_xnew(x[i], dc[i], lmid, move);
-    }
-};
 struct GenX {
     const float* x;
     __device__ float operator()(int64_t i) const { return x[i]; }
 };
 
 // ---------------------------------------------------------------------------
-// The OC bisection with its exact sums spread over the GPU.  Per bisection
-// step: k_bis_start takes lmid and sums the first elements sequentially; then
-// rounds of k_ss_agg (every workgroup forms the transducer aggregate of one
-// chunk of OC densities in a window after the sum's position, under the
-// current binade) and k_ss_carry (one wavefront composes the window's
-// aggregates in chunk order, finds the first binade crossing, locates its
-// element and restarts after it in the new binade; without a crossing it moves
-// past the window and doubles it) until the sum is complete; k_bis_end takes
-// the bisection decision.  Every kernel returns at once when its flags say
-// there is nothing to do.  OC densities are >= 1e-3 (or 1 for passive
-// elements), so after the first 1024 elements the sum is >= 1 and crosses at
-// most log2(n) + 1 binades; with at most log2(n / 2^15) window doublings,
-// OC_ROUNDS rounds always complete it (k_bis_end flags an error otherwise).
+// The OC bisection and the statistics of xnew, each step an exact
+// std::accumulate<float> over all elements, spread over the GPU.  One sum:
+//   k_ocs_start  adds the first OC_PREFIX elements one by one (more while the
+//                sum is below 1e-30), so the running sum is a normal float s0
+//                before element i0; the bisection step's multiplier is set here;
+//   k_cs_sum     double sum of every chunk of OC_CH elements after i0;
+//   k_cs_bounds  prefix of the chunk sums.  Every float addition of
+//                non-negative terms rounds by a factor in [1-u, 1+u], u = 2^-24,
+//                so before chunk c (n = c OC_CH additions after s0) the running
+//                sum lies in [P (1-u)^n, P (1+u)^n], P = s0 + the exact sum of
+//                the elements before it; the binades of that interval are the
+//                chunk's candidates (at most OC_KMAX, else none).  A bisection
+//                step whose interval for the whole sum already decides
+//                vol > V0 ends here;
+//   k_cs_agg     per chunk and candidate binade, the transducer aggregate of the
+//                chunk (seqsum.hpp);
+//   k_cs_walk    one workgroup walks the chunks in order, composing the
+//                aggregates for the running sum's binade over windows of OC_WIN
+//                chunks; the first chunk that leaves the binade, or that has no
+//                aggregate for it, is summed element by element (a workgroup
+//                scan per binade crossing, each crossing one real float add),
+//                and the walk goes on with the next chunk.
+// The candidates only decide which aggregates exist: a chunk without the one the
+// walk needs is summed element by element, so the sum is exact whatever the
+// bounds.  Every kernel returns at once when the bisection has ended (B->done)
+// or the sum is complete (B->ssDone), so the host launches OC_STEPS steps
+// between checks.
 // ---------------------------------------------------------------------------
-constexpr int OC_CT = 256;                 // threads of an aggregate workgroup
+constexpr int OC_CT = 256;                 // threads of a chunk workgroup
 constexpr int OC_CPER = 8;                 // elements per thread
 constexpr int OC_CH = OC_CT * OC_CPER;     // elements per chunk
-constexpr int OC_ROUNDS = 64;
-constexpr int OC_GROUP = 8;               // rounds launched between host checks
-constexpr int OC_PREFIX = 64;             // elements summed one by one before the rounds (any count is exact)
-constexpr int OC_W0 = 16;                 // first window, chunks
-constexpr int OC_AGG_GRID = 1024;         // workgroups of k_ss_agg (grid-stride over the window)
+constexpr int OC_KMAX = 8;                 // candidate binades per chunk
+constexpr int OC_BT = 1024;                // threads of k_cs_bounds
+constexpr int OC_WT = 1024;                // threads of the walk
+constexpr int OC_WCPT = 1;                 // chunks per walk thread
+constexpr int OC_WIN = OC_WT * OC_WCPT;    // chunks per window of the walk
+constexpr int OC_EPT = OC_CH / OC_WT;      // elements per walk thread in a chunk
+constexpr int OC_PREFIX = 256;             // elements added one by one first
+constexpr int OC_STEPS = 4;                // bisection steps launched between host checks
+constexpr int OC_APPLY_GRID = 2048;        // workgroups of k_oc_apply (grid-stride)
 
 struct OcBis {
     float l1, l2, lmid;
     int steps, done, err;
-    float s;          // running float sum
-    int ssDone;
-    long long i0;     // next element of the sum
-    long long W;      // window of the next round, chunks
+    float s;          // the running float sum
+    int ssDone;       // the sum is complete, or the step decided by its bounds
+    long long i0;     // first element after the prefix
+    int decided;      // 1: vol > V0, 2: vol <= V0, known from the bounds alone
 };
 
-// the summed elements: 0 = OC densities at B->lmid, 1 = x, 2 = x (1 - x)
+// the summed elements: 0 = OC densities at B->lmid (passive elements, when the
+// nullable mask is given, held at 1: matlab/TOP3D_XL.m:154), 1 = x, 2 = x (1 - x)
 __device__ __forceinline__ float oc_gen(int mode, int64_t i, const float* __restrict__ x, const float* __restrict__ dc,
                                         const uint8_t* __restrict__ passive, float lmid, float move)
 {
@@ -280,183 +288,423 @@ __device__ __forceinline__ float oc_gen(int mode, int64_t i, const float* __rest
     return (passive && passive[i]) ? 1.0f : oc_xnew(x[i], dc[i], lmid, move);
 }
 
-__device__ __forceinline__ seqsum::Tr shfl_down_tr(const seqsum::Tr& t, int d)
+// Wave scans of transducers with DPP row shifts (row_shr:n, GFX9 control
+// 0x110 + n) inside each row of 16 lanes and the row totals read with
+// readlane: no LDS round trips, unlike __shfl_up (ds_bpermute).
+template <int CTRL>
+__device__ __forceinline__ seqsum::Tr dpp_tr(const seqsum::Tr& t)
 {
-    seqsum::Tr o;
-    o.r0 = __shfl_down((long long)t.r0, d, 64);
-    o.r1 = __shfl_down((long long)t.r1, d, 64);
-    return o;
+    return seqsum::Tr{uint32_t(__builtin_amdgcn_update_dpp(0, int(t.r0), CTRL, 0xf, 0xf, false)),
+                      uint32_t(__builtin_amdgcn_update_dpp(0, int(t.r1), CTRL, 0xf, 0xf, false))};
 }
+__device__ __forceinline__ seqsum::Tr readlane_tr(const seqsum::Tr& t, int l)
+{
+    return seqsum::Tr{uint32_t(__builtin_amdgcn_readlane(int(t.r0), l)),
+                      uint32_t(__builtin_amdgcn_readlane(int(t.r1), l))};
+}
+// inclusive (inc) and exclusive (ex) scan of T over the 64 lanes in lane order;
+// every lane of the wave takes part
+__device__ __forceinline__ void wave_scan_tr(const seqsum::Tr& T, seqsum::Tr& inc, seqsum::Tr& ex)
+{
+    using namespace seqsum;
+    const int lane = threadIdx.x & 63, r = lane & 15, row = lane >> 4;
+    Tr x = T;
+    {
+        const Tr y = dpp_tr<0x111>(x);
+        if (r >= 1) x = tr_compose(y, x);
+    }
+    {
+        const Tr y = dpp_tr<0x112>(x);
+        if (r >= 2) x = tr_compose(y, x);
+    }
+    {
+        const Tr y = dpp_tr<0x114>(x);
+        if (r >= 4) x = tr_compose(y, x);
+    }
+    {
+        const Tr y = dpp_tr<0x118>(x);
+        if (r >= 8) x = tr_compose(y, x);
+    }
+    const Tr e = dpp_tr<0x111>(x);   // the row's inclusive scan at lane - 1
+    const Tr t0 = readlane_tr(x, 15), t1 = readlane_tr(x, 31), t2 = readlane_tr(x, 47);
+    const Tr p2 = tr_compose(t0, t1), p3 = tr_compose(p2, t2);
+    Tr pre;   // composition of the rows before this lane's row
+    pre.r0 = row == 0 ? 0u : row == 1 ? t0.r0 : row == 2 ? p2.r0 : p3.r0;
+    pre.r1 = row == 0 ? 0u : row == 1 ? t0.r1 : row == 2 ? p2.r1 : p3.r1;
+    inc = tr_compose(pre, x);
+    ex = r == 0 ? pre : tr_compose(pre, e);
+}
+
+__device__ __forceinline__ int64_t oc_chunks(int ne, int64_t i0) { return (ne - i0 + OC_CH - 1) / OC_CH; }
 
 __global__ void k_bis_init(OcBis* __restrict__ B)
 {
     B->l1 = 0.0f; B->l2 = 1e9f; B->lmid = 0.0f;
-    B->steps = 0; B->done = 0; B->err = 0; B->s = 0.0f; B->ssDone = 1; B->i0 = 0; B->W = OC_W0;
+    B->steps = 0; B->done = 0; B->err = 0; B->s = 0.0f; B->ssDone = 1; B->i0 = 0; B->decided = 0;
 }
 
-__global__ void k_bis_start(OcBis* __restrict__ B, int ne, const float* __restrict__ x, const float* __restrict__ dc,
-                            const uint8_t* __restrict__ passive, float move)
+// start of a sum: mode 0 also begins the bisection step (driver.cpp:170-174)
+__global__ void __launch_bounds__(OC_PREFIX) k_ocs_start(OcBis* __restrict__ B, int mode, int ne,
+                                                         const float* __restrict__ x, const float* __restrict__ dc,
+                                                         const uint8_t* __restrict__ passive, float move)
 {
-    if (B->done) return;
-    const float l1 = B->l1, l2 = B->l2;
-    if (!(double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6)) {   // driver.cpp:170
-        B->done = 1;
-        return;
+    __shared__ float v[OC_PREFIX];
+    float lmid = 0.0f;
+    if (mode == 0) {
+        if (B->done) return;
+        const float l1 = B->l1, l2 = B->l2;
+        if (!(double(((__fsub_rn(l2, l1)) / (__fadd_rn(l1, l2)))) > 1e-6)) {   // driver.cpp:170
+            if (threadIdx.x == 0) B->done = 1;
+            return;
+        }
+        lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
     }
-    const float lmid = __fmul_rn(0.5f, __fadd_rn(l1, l2));
-    B->lmid = lmid;
-    GenXnew gx{x, dc, passive, lmid, move};
-    float s = 0.0f;
-    int64_t i = 0;
-    for (; i < ne && (i < OC_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, gx(i));
-    B->s = s;
-    B->i0 = i;
-    B->W = OC_W0;
-    B->ssDone = i >= ne ? 1 : 0;
+    const int t = threadIdx.x;
+    if (t < ne) v[t] = oc_gen(mode, t, x, dc, passive, lmid, move);
+    __syncthreads();
+    if (t == 0) {
+        const int64_t np = ne < OC_PREFIX ? ne : OC_PREFIX;
+        float s = 0.0f;
+        int64_t i = 0;
+        for (; i < np; ++i) s = __fadd_rn(s, v[i]);
+        for (; i < ne && !(s >= 1e-30f); ++i) s = __fadd_rn(s, oc_gen(mode, i, x, dc, passive, lmid, move));
+        if (mode == 0) {
+            B->lmid = lmid;
+        } else {
+            B->done = 0;
+            B->err = 0;
+        }
+        B->s = s;
+        B->i0 = i;
+        B->ssDone = i >= ne ? 1 : 0;
+        B->decided = 0;
+    }
 }
 
-// aggT[c] = transducer of chunk c = [i0 + c OC_CH, ...) under the binade of s
-__global__ void __launch_bounds__(OC_CT) k_ss_agg(const OcBis* __restrict__ B, int mode, int ne,
+// csum[c] = double sum of chunk c (order irrelevant: it only bounds the float sum)
+__global__ void __launch_bounds__(OC_CT) k_cs_sum(const OcBis* __restrict__ B, int mode, int ne,
                                                   const float* __restrict__ x, const float* __restrict__ dc,
                                                   const uint8_t* __restrict__ passive, float move,
-                                                  seqsum::Tr* __restrict__ aggT)
+                                                  double* __restrict__ csum)
+{
+    if (B->done || B->ssDone) return;
+    const int64_t i0 = B->i0;
+    const int64_t c = blockIdx.x;
+    if (c >= oc_chunks(ne, i0)) return;
+    const float lmid = B->lmid;
+    const int64_t base = i0 + c * OC_CH + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < OC_CPER; ++k) {
+        const int64_t idx = base + k * OC_CT;
+        if (idx < ne) s += double(oc_gen(mode, idx, x, dc, passive, lmid, move));
+    }
+    for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w, 64);
+    __shared__ double ws[OC_CT / 64];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < OC_CT / 64; ++w) t += ws[w];
+        csum[c] = t;
+    }
+}
+
+// einfo[c] = {ulp exponent of the chunk's lowest candidate binade, candidates
+// (0: none)}; thread t takes a contiguous run of chunks, one workgroup scan
+// gives each run its prefix
+__global__ void __launch_bounds__(OC_BT) k_cs_bounds(OcBis* __restrict__ B, int mode, int ne, float V0,
+                                                     const double* __restrict__ csum, int2* __restrict__ einfo)
+{
+    if (B->done || B->ssDone) return;
+    const int64_t i0 = B->i0;
+    const int64_t nch = oc_chunks(ne, i0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t run = (nch + OC_BT - 1) / OC_BT;
+    const int64_t c0 = int64_t(tid) * run, c1 = (c0 + run < nch) ? c0 + run : nch;
+    double own = 0.0;
+    for (int64_t c = c0; c < c1; ++c) own += csum[c];
+    double inc = own;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    double ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 0.0;
+    __shared__ double ws[OC_BT / 64];
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    const double s0 = double(B->s);
+    double P = s0 + ex, tot = s0;
+    for (int w = 0; w < OC_BT / 64; ++w) {
+        if (w < wave) P += ws[w];
+        tot += ws[w];
+    }
+    // (1 - u)^n >= exp(-n u (1 + 2u)) and (1 + u)^n <= exp(n u); the factor
+    // 1 -/+ 1e-6 covers the double rounding of P and of exp
+    const double uu = 0x1p-24 * (1.0 + 0x1p-22);
+    for (int64_t c = c0; c < c1; ++c) {
+        const double t = double(c * OC_CH) * uu;
+        const double lo = P * exp(-t) * (1.0 - 1e-6), hi = P * exp(t) * (1.0 + 1e-6);
+        int elo = 0, ehi = 0;
+        frexp(lo, &elo);
+        frexp(hi, &ehi);
+        const int K = ehi - elo + 1;
+        einfo[c] = make_int2(elo - 24, K <= OC_KMAX ? K : 0);
+        P += csum[c];
+    }
+    // vol = s / (float)ne against V0 (driver.cpp:181-182), from the bounds of
+    // the whole sum when they decide it: a float quotient above V0 (1 + 2^-23)
+    // rounds above V0, one below V0 rounds to at most V0
+    if (mode == 0 && tid == 0) {
+        const double t = double(ne - i0) * uu;
+        const double nef = double(float(ne)), v0 = double(V0);
+        const double lo = tot * exp(-t) * (1.0 - 1e-6) / nef, hi = tot * exp(t) * (1.0 + 1e-6) / nef;
+        if (lo > v0 * (1.0 + 0x1p-22)) {
+            B->decided = 1;
+            B->ssDone = 1;
+        } else if (hi < v0 * (1.0 - 0x1p-22)) {
+            B->decided = 2;
+            B->ssDone = 1;
+        }
+    }
+}
+
+// agg[c OC_KMAX + j] = transducer of chunk c under the ulp exponent einfo[c].x + j
+__global__ void __launch_bounds__(OC_CT) k_cs_agg(const OcBis* __restrict__ B, int mode, int ne,
+                                                  const float* __restrict__ x, const float* __restrict__ dc,
+                                                  const uint8_t* __restrict__ passive, float move,
+                                                  const int2* __restrict__ einfo, seqsum::Tr* __restrict__ agg)
 {
     using namespace seqsum;
     if (B->done || B->ssDone) return;
     const int64_t i0 = B->i0;
-    const int64_t nch = (ne - i0 + OC_CH - 1) / OC_CH;
-    const int64_t nw = nch < B->W ? nch : B->W;
+    const int64_t c = blockIdx.x;
+    if (c >= oc_chunks(ne, i0)) return;
+    const int2 ei = einfo[c];
+    const int K = ei.y;
+    if (K == 0) return;
+    const float lmid = B->lmid;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the chunk's values, generated with coalesced loads into LDS (element j
+    // at j + j/8: conflict-free reads of 8 consecutive elements per thread)
+    __shared__ float sv[OC_CH + OC_CH / 8];
+    const int64_t base = i0 + c * OC_CH;
+#pragma unroll
+    for (int k = 0; k < OC_CPER; ++k) {
+        const int j = k * OC_CT + tid;
+        const int64_t idx = base + j;
+        sv[j + (j >> 3)] = idx < ne ? oc_gen(mode, idx, x, dc, passive, lmid, move) : 0.0f;   // a zero adds nothing
+    }
+    __syncthreads();
+    float v[OC_CPER];
+#pragma unroll
+    for (int k = 0; k < OC_CPER; ++k) v[k] = sv[tid * (OC_CPER + 1) + k];
+    __shared__ Tr wt[OC_KMAX][OC_CT / 64];
+#pragma unroll
+    for (int j = 0; j < OC_KMAX; ++j) {
+        if (j < K) {   // uniform over the workgroup
+            Tr F = tr_identity();
+#pragma unroll
+            for (int k = 0; k < OC_CPER; ++k) F = tr_compose(F, tr_elem(v[k], ei.x + j));
+            Tr inc, ex;
+            wave_scan_tr(F, inc, ex);
+            if (lane == 63) wt[j][wave] = inc;
+        }
+    }
+    __syncthreads();
+    if (tid < K) {
+        Tr acc = tr_identity();
+        for (int w = 0; w < OC_CT / 64; ++w) acc = tr_compose(acc, wt[tid][w]);
+        agg[c * OC_KMAX + tid] = acc;
+    }
+}
+
+struct WalkShared {
+    seqsum::Tr wt[OC_WT / 64 + 1];   // exclusive wave prefixes, then the total
+    unsigned long long cross;
+    long long m;
+    int ue;
+};
+
+// scan of one transducer per thread over the walk's workgroup in thread order;
+// entered and left by every thread, resets sh.cross
+__device__ __forceinline__ void walk_scan(const seqsum::Tr& T, seqsum::Tr& excl, seqsum::Tr& inc, seqsum::Tr& tot,
+                                          WalkShared& sh)
+{
+    using namespace seqsum;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    Tr x, ex;
+    wave_scan_tr(T, x, ex);
+    if (lane == 63) sh.wt[wave] = x;
+    __syncthreads();
+    if (wave == 0) {   // the wave totals scanned by the first wave
+        constexpr int NW = OC_WT / 64;
+        const Tr w = lane < NW ? sh.wt[lane] : tr_identity();
+        Tr wi, we;
+        wave_scan_tr(w, wi, we);
+        if (lane < NW) sh.wt[lane] = we;
+        if (lane == NW - 1) sh.wt[NW] = wi;
+        if (lane == 0) sh.cross = ~0ull;
+    }
+    __syncthreads();
+    const Tr pw = sh.wt[wave];
+    excl = tr_compose(pw, ex);
+    inc = tr_compose(pw, x);
+    tot = sh.wt[OC_WT / 64];
+}
+
+// adds the elements [from, end) of one chunk to the running sum (m, ue); thread
+// t holds elements base + t OC_EPT + k in v[k]
+__device__ __forceinline__ void walk_elems(int64_t base, int64_t from, int64_t end, const float (&v)[OC_EPT], int64_t& m, int& ue,
+                           WalkShared& sh)
+{
+    using namespace seqsum;
+    const int64_t own = base + int64_t(threadIdx.x) * OC_EPT;
+    for (;;) {
+        Tr T = tr_identity();
+#pragma unroll
+        for (int k = 0; k < OC_EPT; ++k)
+            if (own + k >= from && own + k < end) T = tr_compose(T, tr_elem(v[k], ue));
+        Tr excl, inc, tot;
+        walk_scan(T, excl, inc, tot, sh);
+        // the running sum only grows, so exactly one thread holds the first
+        // crossing: the one whose prefix stays in the binade and whose own
+        // elements leave it; it alone writes sh.cross
+        int64_t mm = m + tr_apply(excl, m), mmc = 0;
+        float vc = 0.0f;
+        unsigned long long my = ~0ull;
+        if (mm < M_TOP) {
+#pragma unroll
+            for (int k = 0; k < OC_EPT; ++k) {
+                if (own + k >= from && own + k < end && my == ~0ull) {
+                    const int64_t r = tr_apply(tr_elem(v[k], ue), mm);
+                    if (mm + r >= M_TOP) {
+                        my = (unsigned long long)(own + k);
+                        mmc = mm;
+                        vc = v[k];
+                    } else {
+                        mm += r;
+                    }
+                }
+            }
+        }
+        if (my != ~0ull) sh.cross = my;
+        __syncthreads();
+        const unsigned long long cr = sh.cross;
+        if (cr == ~0ull) {
+            m += tr_apply(tot, m);
+            return;
+        }
+        if (my == cr) {   // the crossing element: a real float add, then the new binade
+            const float s = __fadd_rn(ldexpf(float(mmc), ue), vc);
+            int64_t m2;
+            int u2;
+            decompose(s, m2, u2);
+            sh.m = m2;
+            sh.ue = u2;
+        }
+        __syncthreads();
+        m = sh.m;
+        ue = sh.ue;
+        from = int64_t(cr) + 1;
+    }
+}
+
+// the aggregate of a chunk for candidate j (none: capped, which reads as a crossing)
+__device__ __forceinline__ seqsum::Tr pick_tr(const seqsum::Tr (&a)[OC_KMAX], int j)
+{
+    seqsum::Tr T{seqsum::R_CAP, seqsum::R_CAP};
+#pragma unroll
+    for (int jj = 0; jj < OC_KMAX; ++jj)
+        if (jj == j) T = a[jj];
+    return T;
+}
+
+__global__ void __launch_bounds__(OC_WT) k_cs_walk(OcBis* __restrict__ B, int mode, int ne,
+                                                   const float* __restrict__ x, const float* __restrict__ dc,
+                                                   const uint8_t* __restrict__ passive, float move,
+                                                   const int2* __restrict__ einfo, const seqsum::Tr* __restrict__ agg)
+{
+    using namespace seqsum;
+    if (B->done || B->ssDone) return;
+    __shared__ WalkShared sh;
+    const int tid = threadIdx.x;
+    const int64_t i0 = B->i0;
+    const int64_t nch = oc_chunks(ne, i0);
+    const float lmid = B->lmid;
     int64_t m;
     int ue;
     decompose(B->s, m, ue);
-    const float lmid = B->lmid;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ Tr wt[OC_CT / 64];
-    for (int64_t c = blockIdx.x; c < nw; c += gridDim.x) {   // uniform over the workgroup
-        const int64_t c0 = i0 + c * OC_CH;
-        Tr F = tr_identity();
+    for (int64_t cw = 0; cw < nch; cw += OC_WIN) {
+        const int64_t wend = (cw + OC_WIN < nch) ? cw + OC_WIN : nch;
+        const int64_t cb = cw + int64_t(tid) * OC_WCPT;   // this thread's chunks: cb .. cb + OC_WCPT - 1
+        int lo[OC_WCPT];
+        Tr a[OC_WCPT][OC_KMAX];
 #pragma unroll
-        for (int k = 0; k < OC_CPER; ++k) {
-            const int64_t idx = c0 + int64_t(tid) * OC_CPER + k;
-            if (idx < ne) F = tr_compose(F, tr_elem(oc_gen(mode, idx, x, dc, passive, lmid, move), ue));
-        }
-        // ordered wave reduction: lane i takes lane i + d's aggregate after its own
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const Tr o = shfl_down_tr(F, d);
-            if ((lane & (2 * d - 1)) == 0 && lane + d < 64) F = tr_compose(F, o);
-        }
-        if (lane == 0) wt[wave] = F;
-        __syncthreads();
-        if (tid == 0) {
-            Tr acc = tr_identity();
-            for (int w = 0; w < OC_CT / 64; ++w) acc = tr_compose(acc, wt[w]);
-            aggT[c] = acc;
-        }
-        __syncthreads();
-    }
-}
-
-// one wavefront: compose the chunk aggregates in order from i0; at the first
-// binade crossing, find its element, add it with a real float add and restart
-// after it; otherwise the sum is complete
-__global__ void __launch_bounds__(64) k_ss_carry(OcBis* __restrict__ B, int mode, int ne, const float* __restrict__ x,
-                                                 const float* __restrict__ dc, const uint8_t* __restrict__ passive,
-                                                 float move, const seqsum::Tr* __restrict__ aggT)
-{
-    using namespace seqsum;
-    if (B->done || B->ssDone) return;
-    const int lane = threadIdx.x;
-    const int64_t i0 = B->i0;
-    const float s = B->s;
-    int64_t m;
-    int ue;
-    decompose(s, m, ue);
-    const int64_t nch = (ne - i0 + OC_CH - 1) / OC_CH;
-    const int64_t nw = nch < B->W ? nch : B->W;
-    int64_t cross = -1;   // chunk holding the crossing
-    for (int64_t b = 0; b < nw && cross < 0; b += 64) {
-        const int64_t c = b + lane;
-        Tr inc = (c < nw) ? aggT[c] : tr_identity();
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const Tr o = shfl_up_tr(inc, d);
-            if (lane >= d) inc = tr_compose(o, inc);
-        }
-        const bool crosses = c < nw && m + tr_apply(inc, m) >= M_TOP;
-        const unsigned long long ball = __ballot(crosses);
-        if (ball) {
-            const int first = __ffsll((long long)ball) - 1;
-            Tr ex = shfl_up_tr(inc, 1);
-            if (lane == 0) ex = tr_identity();
-            const int64_t r = tr_apply(ex, m);
-            m += __shfl(r, first, 64);          // m before the crossing chunk
-            cross = b + first;
-        } else {
-            const Tr tot{__shfl(inc.r0, 63, 64), __shfl(inc.r1, 63, 64)};
-            m += tr_apply(tot, m);
-        }
-    }
-    if (cross < 0) {   // no crossing in the window: move past it (same binade), double it
-        if (lane == 0) {
-            B->s = ldexpf(float(m), ue);
-            const int64_t next = i0 + nw * OC_CH;
-            B->i0 = next < ne ? next : ne;
-            B->ssDone = next >= ne ? 1 : 0;
-            B->W = 2 * B->W;
-        }
-        return;
-    }
-    // the crossing chunk: 64 lanes x 32 elements, a wave scan, then one lane walks its run
-    const float lmid = B->lmid;
-    const int64_t cs = i0 + cross * OC_CH;
-    constexpr int RUN = OC_CH / 64;
-    Tr T = tr_identity();
-    for (int k = 0; k < RUN; ++k) {
-        const int64_t idx = cs + int64_t(lane) * RUN + k;
-        if (idx < ne) T = tr_compose(T, tr_elem(oc_gen(mode, idx, x, dc, passive, lmid, move), ue));
-    }
-    Tr inc = T;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const Tr o = shfl_up_tr(inc, d);
-        if (lane >= d) inc = tr_compose(o, inc);
-    }
-    const bool crosses = m + tr_apply(inc, m) >= M_TOP;
-    const unsigned long long ball = __ballot(crosses);
-    const int first = __ffsll((long long)ball) - 1;   // the chunk crosses, so some lane does
-    Tr ex = shfl_up_tr(inc, 1);
-    if (lane == 0) ex = tr_identity();
-    if (lane == first) {
-        int64_t mm = m + tr_apply(ex, m);
-        for (int k = 0; k < RUN; ++k) {
-            const int64_t idx = cs + int64_t(lane) * RUN + k;
-            const float v = oc_gen(mode, idx, x, dc, passive, lmid, move);
-            const int64_t r = tr_apply(tr_elem(v, ue), mm);
-            if (mm + r >= M_TOP) {
-                B->s = __fadd_rn(ldexpf(float(mm), ue), v);
-                B->i0 = idx + 1;
-                B->ssDone = (idx + 1 >= ne) ? 1 : 0;
-                break;
+        for (int q = 0; q < OC_WCPT; ++q) {
+            const int64_t c = cb + q;
+            int K = 0;
+            lo[q] = 0;
+            if (c < nch) {
+                const int2 ei = einfo[c];
+                lo[q] = ei.x;
+                K = ei.y;
             }
-            mm += r;
+#pragma unroll
+            for (int j = 0; j < OC_KMAX; ++j) a[q][j] = (j < K) ? agg[c * OC_KMAX + j] : Tr{R_CAP, R_CAP};
+        }
+        for (int64_t cs = cw; cs < wend;) {
+            Tr T = tr_identity();
+#pragma unroll
+            for (int q = 0; q < OC_WCPT; ++q)
+                if (cb + q >= cs && cb + q < wend) T = tr_compose(T, pick_tr(a[q], ue - lo[q]));
+            Tr excl, inc, tot;
+            walk_scan(T, excl, inc, tot, sh);
+            // the running sum only grows, so one thread holds the first chunk
+            // that leaves the binade: its prefix stays inside, its inclusive
+            // prefix does not; it finds the chunk and alone writes sh.cross
+            int64_t mr = m + tr_apply(excl, m);
+            if (mr < M_TOP && m + tr_apply(inc, m) >= M_TOP) {
+                bool found = false;
+#pragma unroll
+                for (int q = 0; q < OC_WCPT; ++q) {
+                    if (!found && cb + q >= cs && cb + q < wend) {
+                        const int64_t r = tr_apply(pick_tr(a[q], ue - lo[q]), mr);
+                        if (mr + r >= M_TOP) {
+                            sh.cross = (unsigned long long)(cb + q);
+                            sh.m = mr;
+                            found = true;
+                        } else {
+                            mr += r;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const unsigned long long cc = sh.cross;
+            if (cc == ~0ull) {   // no crossing up to the window's end
+                m += tr_apply(tot, m);
+                cs = wend;
+                continue;
+            }
+            m = sh.m;
+            const int64_t base = i0 + int64_t(cc) * OC_CH;
+            const int64_t end = (base + OC_CH < ne) ? base + OC_CH : ne;
+            float v[OC_EPT];
+#pragma unroll
+            for (int k = 0; k < OC_EPT; ++k) {
+                const int64_t idx = base + int64_t(tid) * OC_EPT + k;
+                v[k] = idx < end ? oc_gen(mode, idx, x, dc, passive, lmid, move) : 0.0f;
+            }
+            walk_elems(base, base, end, v, m, ue, sh);
+            cs = int64_t(cc) + 1;
         }
     }
-}
-
-// start of an exact sum of x (mode 1) or x (1 - x) (mode 2): the elements are
-// added one by one until the sum is a normal float, then the rounds take over
-__global__ void k_es_start(OcBis* __restrict__ B, int mode, int ne, const float* __restrict__ x)
-{
-    float s = 0.0f;
-    int64_t i = 0;
-    for (; i < ne && (i < OC_PREFIX || !(s >= 1e-30f)); ++i) s = __fadd_rn(s, oc_gen(mode, i, x, nullptr, nullptr, 0.0f, 0.0f));
-    B->done = 0;
-    B->err = 0;
-    B->s = s;
-    B->i0 = i;
-    B->W = OC_W0;
-    B->ssDone = i >= ne ? 1 : 0;
+    if (tid == 0) {
+        B->s = ldexpf(float(m), ue);
+        B->ssDone = 1;
+    }
 }
 
 // sharpness = 4 sum(x(1-x))/ne (mode 2, out[0]) and volume fraction sum(x)/ne
@@ -470,13 +718,19 @@ __global__ void k_es_out(const OcBis* __restrict__ B, int mode, int ne, float* _
 __global__ void k_bis_end(OcBis* __restrict__ B, int ne, float V0)
 {
     if (B->done) return;
-    if (!B->ssDone) {   // more binade crossings than the rounds allow: cannot happen for OC densities
-        B->err = 1;
-        B->done = 1;
-        return;
+    bool above;
+    if (B->decided) {
+        above = B->decided == 1;
+    } else {
+        if (!B->ssDone) {   // the walk always completes its sum
+            B->err = 1;
+            B->done = 1;
+            return;
+        }
+        const float vol = ((B->s) / (float(ne)));
+        above = __fsub_rn(vol, V0) > 0;
     }
-    const float vol = ((B->s) / (float(ne)));
-    if (__fsub_rn(vol, V0) > 0) B->l1 = B->lmid;
+    if (above) B->l1 = B->lmid;
     else B->l2 = B->lmid;
     ++B->steps;
 }
@@ -488,22 +742,25 @@ __global__ void k_bis_out(const OcBis* __restrict__ B, float* __restrict__ out)
     out[1] = float(B->steps);
 }
 
-// xnew for the final multiplier and change = max|xnew - x| (driver.cpp:184)
-__global__ void k_oc_apply(int ne, const float* __restrict__ x, const float* __restrict__ dc,
-                           const uint8_t* __restrict__ passive, const float* __restrict__ lm, float move,
-                           float* __restrict__ xnew, unsigned* __restrict__ changeBits)
+// xnew for the final multiplier and change = max|xnew - x| (driver.cpp:184);
+// the max of non-negative floats is the max of their bit patterns
+__global__ void __launch_bounds__(256) k_oc_apply(int ne, const float* __restrict__ x, const float* __restrict__ dc,
+                                                  const uint8_t* __restrict__ passive, const float* __restrict__ lm,
+                                                  float move, float* __restrict__ xnew,
+                                                  unsigned* __restrict__ changeBits)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    float d = 0.0f;
-    if (e < ne) {
-        const float v = (passive && passive[e]) ? 1.0f : oc_xnew(x[e], dc[e], lm[0], move);
+    const float lmid = lm[0];
+    unsigned b = 0u;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < ne; e += int64_t(gridDim.x) * 256) {
+        const float v = (passive && passive[e]) ? 1.0f : oc_xnew(x[e], dc[e], lmid, move);
         xnew[e] = v;
-        d = fabsf(__fsub_rn(v, x[e]));
+        b = max(b, __float_as_uint(fabsf(__fsub_rn(v, x[e]))));
     }
-    // max of non-negative floats = max of their bit patterns
-    unsigned b = __float_as_uint(d);
     for (int w = 32; w >= 1; w >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, w, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(changeBits, b);
+    __shared__ unsigned wb[4];
+    if ((threadIdx.x & 63) == 0) wb[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(changeBits, max(max(wb[0], wb[1]), max(wb[2], wb[3])));
 }
 
 // generic sequential float sum of an array (for tests)
@@ -703,37 +960,38 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
     const int ne = p->ne;
     const int64_t nchMax = (int64_t(ne) + OC_CH - 1) / OC_CH + 1;
     if (!p->ocBis) T3D_HIP(hipMalloc(&p->ocBis, sizeof(OcBis)));
+    // per chunk: OC_KMAX aggregates, the double sum, the candidate binades
+    const size_t perChunk = OC_KMAX * sizeof(seqsum::Tr) + sizeof(double) + sizeof(int2);
     if (p->ocAggCap < nchMax) {
         if (p->ocAgg) T3D_HIP(hipFree(p->ocAgg));
         p->ocAgg = nullptr;
-        T3D_HIP(hipMalloc(&p->ocAgg, sizeof(seqsum::Tr) * size_t(nchMax)));
+        T3D_HIP(hipMalloc(&p->ocAgg, perChunk * size_t(nchMax)));
         p->ocAggCap = nchMax;
     }
     OcBis* B = static_cast<OcBis*>(p->ocBis);
     seqsum::Tr* agg = static_cast<seqsum::Tr*>(p->ocAgg);
-    const unsigned aggGrid = unsigned(std::min<int64_t>(nchMax, OC_AGG_GRID));
+    double* csum = reinterpret_cast<double*>(agg + nchMax * OC_KMAX);
+    int2* einfo = reinterpret_cast<int2*>(csum + nchMax);
+    const dim3 grid = dim3(unsigned(nchMax));
+    auto exact_sum = [&](int mode, const float* xs, const float* dcs, const uint8_t* pas, float mv) {
+        hipLaunchKernelGGL(k_ocs_start, dim3(1), dim3(OC_PREFIX), 0, st, B, mode, ne, xs, dcs, pas, mv);
+        hipLaunchKernelGGL(k_cs_sum, grid, dim3(OC_CT), 0, st, (const OcBis*)B, mode, ne, xs, dcs, pas, mv, csum);
+        hipLaunchKernelGGL(k_cs_bounds, dim3(1), dim3(OC_BT), 0, st, B, mode, ne, V0, (const double*)csum, einfo);
+        hipLaunchKernelGGL(k_cs_agg, grid, dim3(OC_CT), 0, st, (const OcBis*)B, mode, ne, xs, dcs, pas, mv,
+                           (const int2*)einfo, agg);
+        hipLaunchKernelGGL(k_cs_walk, dim3(1), dim3(OC_WT), 0, st, B, mode, ne, xs, dcs, pas, mv,
+                           (const int2*)einfo, (const seqsum::Tr*)agg);
+    };
     hipLaunchKernelGGL(k_bis_init, dim3(1), dim3(1), 0, st, B);
-    // bisection steps in blocks of 8, checked on the host between blocks; the
-    // float bisection from [0, 1e9] ends within ~180 steps even when l1 stays 0
-    // one step at a time: rounds in groups of OC_GROUP, the sum's completion
-    // (ssDone) read back between groups; a finished bisection ends the loop
-    int hostDone[2] = {0, 0};
-    int hostSS[4] = {0, 0, 0, 0};   // done, err, s, ssDone
-    for (int step = 0; step < 256 && !hostDone[0]; ++step) {
-        hipLaunchKernelGGL(k_bis_start, dim3(1), dim3(1), 0, st, B, ne, x, dc, passive, move);
-        for (int r = 0; r < OC_ROUNDS; r += OC_GROUP) {
-            for (int k = 0; k < OC_GROUP; ++k) {
-                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, 0, ne, x, dc,
-                                   passive, move, agg);
-                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, 0, ne, x, dc, passive, move,
-                                   (const seqsum::Tr*)agg);
-            }
-            T3D_HIP(hipGetLastError());
-            T3D_HIP(hipMemcpyAsync(hostSS, &B->done, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-            T3D_HIP(hipStreamSynchronize(st));
-            if (hostSS[0] || hostSS[3]) break;   // bisection finished, or this step's sum complete
+    // bisection steps in groups of OC_STEPS, the end flag read back between
+    // groups; the float bisection from [0, 1e9] ends within ~180 steps even
+    // when l1 stays 0
+    int hostDone[2] = {0, 0};   // done, err
+    for (int step = 0; step < 256 && !hostDone[0]; step += OC_STEPS) {
+        for (int k = 0; k < OC_STEPS; ++k) {
+            exact_sum(0, x, dc, passive, move);
+            hipLaunchKernelGGL(k_bis_end, dim3(1), dim3(1), 0, st, B, ne, V0);
         }
-        hipLaunchKernelGGL(k_bis_end, dim3(1), dim3(1), 0, st, B, ne, V0);
         T3D_HIP(hipGetLastError());
         T3D_HIP(hipMemcpyAsync(hostDone, &B->done, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
@@ -742,25 +1000,13 @@ int launch_oc(t3d_problem* p, const float* x, const float* dc, float V0, float m
     hipLaunchKernelGGL(k_bis_out, dim3(1), dim3(1), 0, st, (const OcBis*)B, scal);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipMemsetAsync(scal + 2, 0, sizeof(float), st));
-    hipLaunchKernelGGL(k_oc_apply, dim3(unsigned((p->ne + 255) / 256)), dim3(256), 0, st, p->ne, x, dc, passive,
-                       scal, move, xnew, reinterpret_cast<unsigned*>(scal + 2));
+    const unsigned applyGrid = unsigned(std::min<int64_t>((int64_t(ne) + 255) / 256, OC_APPLY_GRID));
+    hipLaunchKernelGGL(k_oc_apply, dim3(applyGrid), dim3(256), 0, st, p->ne, x, dc, passive, scal, move, xnew,
+                       reinterpret_cast<unsigned*>(scal + 2));
     T3D_HIP(hipGetLastError());
     // sharpness and volume fraction of xnew, exact, spread over the GPU like the bisection's sums
     for (int mode = 2; mode >= 1; --mode) {
-        hipLaunchKernelGGL(k_es_start, dim3(1), dim3(1), 0, st, B, mode, ne, (const float*)xnew);
-        int hs[4] = {0, 0, 0, 0};
-        for (int r = 0; r < 32 * OC_ROUNDS && !hs[3]; r += OC_GROUP) {
-            for (int k = 0; k < OC_GROUP; ++k) {
-                hipLaunchKernelGGL(k_ss_agg, dim3(aggGrid), dim3(OC_CT), 0, st, (const OcBis*)B, mode, ne,
-                                   (const float*)xnew, (const float*)nullptr, (const uint8_t*)nullptr, 0.0f, agg);
-                hipLaunchKernelGGL(k_ss_carry, dim3(1), dim3(64), 0, st, B, mode, ne, (const float*)xnew,
-                                   (const float*)nullptr, (const uint8_t*)nullptr, 0.0f, (const seqsum::Tr*)agg);
-            }
-            T3D_HIP(hipGetLastError());
-            T3D_HIP(hipMemcpyAsync(hs, &B->done, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-            T3D_HIP(hipStreamSynchronize(st));
-        }
-        T3D_CHECK(hs[3], T3D_ERR_STATE, "OC statistics: exact sum did not complete");
+        exact_sum(mode, xnew, nullptr, nullptr, 0.0f);
         hipLaunchKernelGGL(k_es_out, dim3(1), dim3(1), 0, st, (const OcBis*)B, mode, ne, scal + 3);
         T3D_HIP(hipGetLastError());
     }
