@@ -807,12 +807,18 @@ __global__ void __launch_bounds__(256) k_compliance(Grid g, int ne, const int32_
                 ue[3 * a + 1] = U[3 * nn + 1];
                 ue[3 * a + 2] = U[3 * nn + 2];
             }
+            // the row loop stays rolled (unrolled, the 576 products need more
+            // registers than a lane has); ue_i is read again from U (cached)
             double c = 0.0;
+#pragma unroll 1
             for (int i = 0; i < 24; ++i) {
                 double rd = 0.0;
 #pragma unroll
                 for (int j = 0; j < 24; ++j) rd = rd + sK[i * 24 + j] * ue[j];
-                c = c + ue[i] * rd;
+                const int a = i / 3, b = a & 3;
+                const int ox = (b == 1 || b == 2) ? 1 : 0, oy = (b <= 1) ? 1 : 0, oz = a >> 2;
+                const int64_t nn = (int64_t(cz + oz) * g.nny + (ty + oy)) * g.nnx + (cx + ox);
+                c = c + U[3 * nn + (i - 3 * a)] * rd;
             }
             ce[e] = c;
             item = Ee * c;
