@@ -107,12 +107,24 @@ struct KxuTile {
     static constexpr int EN = EX * EY * EZ;
 };
 
-template <int NZ>
+// The PCG's direction update p = z + beta p_prev (solver.cpp:203-210) done by
+// the operator that consumes p: every tile forms p for its nodes and halo
+// from z and p_prev while staging them, and stores it for the nodes it owns
+// (p_prev and p are different buffers, so a neighbour's halo reads the old p).
+struct FuseP {
+    const double* z;
+    const double* pPrev;
+    const double* rzNew;   // beta = rzNew / max(1e-30, rzOld)
+    const double* rzOld;
+    double* p;
+};
+
+template <int NZ, bool FUSEP = false>
 __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __restrict__ Ke,
                                                      const double* __restrict__ Ecell,
                                                      const uint8_t* __restrict__ nodeFixed,
                                                      const double* __restrict__ u, double* __restrict__ y,
-                                                     const int* __restrict__ done)
+                                                     const int* __restrict__ done, FuseP fp = FuseP{})
 {
     if (done && *done) return;
     using T = KxuTile<NZ>;
@@ -121,6 +133,11 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
     const TileId tile = xcd_tile();
     const int x0 = tile.x * T::TX, y0 = tile.y * T::TY, z0 = tile.z * T::TZ;
     const int tid = threadIdx.x;
+    double beta = 0.0;
+    if constexpr (FUSEP) {
+        const double o = *fp.rzOld;
+        beta = *fp.rzNew / (1.0e-30 < o ? o : 1.0e-30);
+    }
     // cooperative loads (zero outside the grid)
     for (int i = tid; i < T::UN; i += 512) {
         const int lx = i % T::UX, ly = (i / T::UX) % T::UY, lz = i / (T::UX * T::UY);
@@ -128,7 +145,16 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
         double v0 = 0.0, v1 = 0.0, v2 = 0.0;
         if (X >= 0 && Y >= 0 && Z >= 0 && X < g.nnx && Y < g.nny && Z < g.nnz) {
             const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-            v0 = u[3 * n]; v1 = u[3 * n + 1]; v2 = u[3 * n + 2];
+            if constexpr (FUSEP) {
+                v0 = fp.z[3 * n] + beta * fp.pPrev[3 * n];
+                v1 = fp.z[3 * n + 1] + beta * fp.pPrev[3 * n + 1];
+                v2 = fp.z[3 * n + 2] + beta * fp.pPrev[3 * n + 2];
+                if (lx >= 1 && lx <= T::TX && ly >= 1 && ly <= T::TY && lz >= 1 && lz <= T::TZ) {
+                    fp.p[3 * n] = v0; fp.p[3 * n + 1] = v1; fp.p[3 * n + 2] = v2;
+                }
+            } else {
+                v0 = u[3 * n]; v1 = u[3 * n + 1]; v2 = u[3 * n + 2];
+            }
         }
         sU[i] = v0; sU[T::UN + i] = v1; sU[2 * T::UN + i] = v2;
     }
@@ -231,6 +257,21 @@ __global__ void __launch_bounds__(256) k_stream_copy(int64_t n, const double* __
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done)
 {
     return launch_kxu_lds<1>(p, u, y, done);
+}
+
+// p = z + beta p_prev, then y = K p (see FuseP)
+int launch_kxu_f64_fusep(t3d_problem* p, const double* z, const double* pPrev, const double* rzNew,
+                         const double* rzOld, double* pOut, double* y, const int* done)
+{
+    using T = KxuTile<1>;
+    const Grid& g = p->g;
+    dim3 grid(unsigned((g.nnx + T::TX - 1) / T::TX), unsigned((g.nny + T::TY - 1) / T::TY),
+              unsigned((g.nnz + T::TZ - 1) / T::TZ));
+    const FuseP fp{z, pPrev, rzNew, rzOld, pOut};
+    hipLaunchKernelGGL((k_kxu_f64_lds<1, true>), grid, dim3(512), 0, p->stream, g, p->Ke, p->Ecell, p->nodeFixed,
+                       (const double*)pOut, y, done, fp);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
 }
 
 }  // namespace t3d

@@ -207,16 +207,13 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDev Lf, LevelDev Lc, int 
 // (transfers.cpp:378-445, preconditioner.cpp:230-253):
 //   t = (w*r)/max(1e-30,D);  x = (t + P(xc)) + t;  masked -> 0
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_prolong_smooth(LevelDev Lf, LevelDev Lc, int span,
-                                                        const double* __restrict__ xc,
-                                                        const double* __restrict__ rf,
-                                                        const double* __restrict__ Df,
-                                                        const uint8_t* __restrict__ fmask, double weight,
-                                                        double* __restrict__ xf, const int* __restrict__ done)
+// the value of DOF (n, c): its trilinear prolongation from the coarse nodes
+// and the smoothing of its residual
+__device__ __forceinline__ double prolong_smooth_dof(const LevelDev& Lf, const LevelDev& Lc, int span,
+                                                     const double* __restrict__ xc, const double* __restrict__ rf,
+                                                     const double* __restrict__ Df, unsigned m, double weight,
+                                                     int64_t n, int c)
 {
-    if (done && *done) return;
-    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= Lf.nN) return;
     const int fx = int(n % Lf.nnx), fy = int((n / Lf.nnx) % Lf.nny), fz = int(n / (int64_t(Lf.nnx) * Lf.nny));
     int cx = fx / span, cy = fy / span, cz = fz / span;
     int rx = fx % span, ry = fy % span, rz = fz % span;
@@ -231,21 +228,64 @@ __global__ void __launch_bounds__(256) k_prolong_smooth(LevelDev Lf, LevelDev Lc
     const int64_t b01 = b00 + 1, b10 = b00 + Lc.nnx, b11 = b10 + 1;
     const int64_t s = int64_t(Lc.nnx) * Lc.nny;
     const int64_t n00 = b00 + s, n01 = n00 + 1, n10 = n00 + Lc.nnx, n11 = n10 + 1;
+    const double v = xc[3 * b00 + c] * wx0 * wy0 * wz0 + xc[3 * b01 + c] * wx1 * wy0 * wz0 +
+                     xc[3 * b10 + c] * wx0 * wy1 * wz0 + xc[3 * b11 + c] * wx1 * wy1 * wz0 +
+                     xc[3 * n00 + c] * wx0 * wy0 * wz1 + xc[3 * n01 + c] * wx1 * wy0 * wz1 +
+                     xc[3 * n10 + c] * wx0 * wy1 * wz1 + xc[3 * n11 + c] * wx1 * wy1 * wz1;
+    double out = 0.0;
+    if (!((m >> c) & 1u)) {
+        const double d = Df[3 * n + c];
+        const double t = weight * rf[3 * n + c] / (1.0e-30 < d ? d : 1.0e-30);
+        out = (t + v) + t;
+    }
+    return out;
+}
+
+__global__ void __launch_bounds__(256) k_prolong_smooth(LevelDev Lf, LevelDev Lc, int span,
+                                                        const double* __restrict__ xc,
+                                                        const double* __restrict__ rf,
+                                                        const double* __restrict__ Df,
+                                                        const uint8_t* __restrict__ fmask, double weight,
+                                                        double* __restrict__ xf, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lf.nN) return;
     const unsigned m = fmask[n];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const double v = xc[3 * b00 + c] * wx0 * wy0 * wz0 + xc[3 * b01 + c] * wx1 * wy0 * wz0 +
-                         xc[3 * b10 + c] * wx0 * wy1 * wz0 + xc[3 * b11 + c] * wx1 * wy1 * wz0 +
-                         xc[3 * n00 + c] * wx0 * wy0 * wz1 + xc[3 * n01 + c] * wx1 * wy0 * wz1 +
-                         xc[3 * n10 + c] * wx0 * wy1 * wz1 + xc[3 * n11 + c] * wx1 * wy1 * wz1;
-        double out = 0.0;
-        if (!((m >> c) & 1u)) {
-            const double d = Df[3 * n + c];
-            const double t = weight * rf[3 * n + c] / (1.0e-30 < d ? d : 1.0e-30);
-            out = (t + v) + t;
+    for (int c = 0; c < 3; ++c) xf[3 * n + c] = prolong_smooth_dof(Lf, Lc, span, xc, rf, Df, m, weight, n, c);
+}
+
+// The finest level's k_prolong_smooth evaluated per DOF in the canonical tile
+// geometry (tile t holds DOFs 1024 t + 256 k + lane, k = 0..3), with the
+// tile's partial of r.z summed as k_canon_dot_tiles sums it: the V-cycle's
+// last pass and the PCG's dot product in one read of r.
+__global__ void __launch_bounds__(256) k_prolong_smooth_rz(LevelDev Lf, LevelDev Lc, int span,
+                                                           const double* __restrict__ xc,
+                                                           const double* __restrict__ rf,
+                                                           const double* __restrict__ Df,
+                                                           const uint8_t* __restrict__ fmask, double weight,
+                                                           double* __restrict__ xf, double* __restrict__ partial,
+                                                           const int* __restrict__ done)
+{
+    if (done && *done) return;
+    __shared__ double sh[CANON_LANES];
+    const int64_t nD = 3 * Lf.nN;
+    const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < CANON_PER_LANE; ++k) {
+        const int64_t i = base + int64_t(k) * CANON_LANES;
+        if (i < nD) {
+            const int64_t n = i / 3;
+            const int c = int(i - 3 * n);
+            const double out = prolong_smooth_dof(Lf, Lc, span, xc, rf, Df, fmask[n], weight, n, c);
+            xf[i] = out;
+            s = s + rf[i] * out;
         }
-        xf[3 * n + c] = out;
     }
+    const double r = block_tree_256(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
 // single-level hierarchy: z = mask ? 0 : r / max(1e-30, D)
@@ -830,19 +870,23 @@ static int mg_apply_standard(t3d_mg* g, const double* r0, double* z, const int* 
     return T3D_OK;
 }
 
-int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
+int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done, double* rzDst)
 {
     t3d_problem* p = g->p;
     hipStream_t st = p->stream;
     const int L = g->nlev - 1;
     const int B = 256;
-    if (L == 0) {
-        hipLaunchKernelGGL(k_diag_solve, dim3(unsigned((g->ld[0].nN + B - 1) / B)), dim3(B), 0, st, g->ld[0].nN, r0,
-                           g->D[0], g->mask[0], z, done);
-        T3D_HIP(hipGetLastError());
-        return T3D_OK;
+    if (L == 0 || (g->mode == 1 && g->vtype == 1)) {
+        if (L == 0) {
+            hipLaunchKernelGGL(k_diag_solve, dim3(unsigned((g->ld[0].nN + B - 1) / B)), dim3(B), 0, st, g->ld[0].nN,
+                               r0, g->D[0], g->mask[0], z, done);
+            T3D_HIP(hipGetLastError());
+        } else {
+            int rc = mg_apply_standard(g, r0, z, done);
+            if (rc) return rc;
+        }
+        return rzDst ? canon_dot_dev(p, r0, z, 3 * g->ld[0].nN, rzDst, done) : T3D_OK;
     }
-    if (g->mode == 1 && g->vtype == 1) return mg_apply_standard(g, r0, z, done);
     const bool chol = g->Nc > 0;
     for (int l = 0; l < L; ++l) {
         const double* rf = (l == 0) ? r0 : g->r[l];
@@ -864,6 +908,17 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done)
     for (int l = L - 1; l >= 0; --l) {
         const double* rf = (l == 0) ? r0 : g->r[l];
         double* xo = (l == 0) ? z : g->x[l];
+        if (l == 0 && rzDst) {   // with the r.z partials, in the canonical geometry
+            const int64_t nD = 3 * g->ld[0].nN;
+            int rc = ensure_red(p, nD);
+            if (rc) return rc;
+            const int64_t tiles = (nD + CANON_TILE - 1) / CANON_TILE;
+            hipLaunchKernelGGL(k_prolong_smooth_rz, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, st, g->ld[0],
+                               g->ld[1], g->H.lv[1].span, g->x[1], rf, g->D[0], g->mask[0], g->H.weight, xo, p->red,
+                               done);
+            T3D_HIP(hipGetLastError());
+            return canon_finish(p, tiles, rzDst, done);
+        }
         hipLaunchKernelGGL(k_prolong_smooth, dim3(unsigned((g->ld[l].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
                            g->ld[l + 1], g->H.lv[l + 1].span, g->x[l + 1], rf, g->D[l], g->mask[l], g->H.weight, xo,
                            done);

@@ -14,7 +14,10 @@
 
 namespace t3d {
 
-// per-iteration scalar slots in p->pcgSc
+// per-iteration scalar slots in p->pcgSc.  The direction update
+// p = z + beta p (solver.cpp:203-210) is done by the next iteration's operator
+// (launch_kxu_f64_fusep) and r.z by the V-cycle's last pass (mg_apply_dev), in
+// the same arithmetic and summation order.
 struct PcgSlots {
     double* normb2;   // [1]
     double* stop;     // [1]
@@ -93,17 +96,6 @@ __global__ void k_pcg_check(const double* __restrict__ rn2, const double* __rest
     }
 }
 
-// p = z + beta p, beta = rz_new / max(1e-30, rz_old)
-__global__ void k_pcg_update_p(int64_t n, const double* __restrict__ rz_new, const double* __restrict__ rz_old,
-                               const double* __restrict__ z, double* __restrict__ p, const int* __restrict__ done)
-{
-    if (*done) return;
-    const double o = *rz_old;
-    const double beta = *rz_new / (1.0e-30 < o ? o : 1.0e-30);
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
-}
-
 static int pcg_alloc(t3d_problem* p, int maxIt)
 {
     const size_t nD = size_t(3 * p->g.nN);
@@ -111,6 +103,7 @@ static int pcg_alloc(t3d_problem* p, int maxIt)
     if (!p->wr) T3D_HIP(hipMalloc(&p->wr, sizeof(double) * nD));
     if (!p->wz) T3D_HIP(hipMalloc(&p->wz, sizeof(double) * nD));
     if (!p->wp) T3D_HIP(hipMalloc(&p->wp, sizeof(double) * nD));
+    if (!p->wp2) T3D_HIP(hipMalloc(&p->wp2, sizeof(double) * nD));
     if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
     if (!p->pcgFlags) T3D_HIP(hipMalloc(&p->pcgFlags, sizeof(int) * 4));
     if (!p->hostFlags) T3D_HIP(hipHostMalloc(&p->hostFlags, sizeof(int) * 4, hipHostMallocDefault));
@@ -147,6 +140,9 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
     S.rn2 = S.denom + maxIt;
     int* flags = p->pcgFlags;
     double *r = p->wr, *z = p->wz, *pp = p->wp, *Ap = p->wAp;
+    // p of iteration it lives in pb[it & 1]: the operator of iteration it >= 1
+    // forms it from z (r without a preconditioner) and p of iteration it - 1
+    double* pb[2] = {p->wp, p->wp2};
     // r = b - K x
     if ((rc = launch_kxu_f64(p, x, Ap))) return rc;
     hipLaunchKernelGGL(k_sub, dim3(nb), dim3(B), 0, st, n, b, Ap, r);
@@ -157,7 +153,7 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
         zr = z;
     }
     if ((rc = canon_dot_dev(p, r, zr, n, S.rz))) return rc;
-    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(B), 0, st, n, zr, pp);
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(B), 0, st, n, zr, pb[0]);
     T3D_HIP(hipGetLastError());
     if ((rc = canon_dot_dev(p, b, b, n, S.normb2))) return rc;
     hipLaunchKernelGGL(k_pcg_stop, dim3(1), dim3(1), 0, st, S.normb2, tol, S.stop, flags);
@@ -167,7 +163,10 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
     for (int it0 = 0; it0 < maxIt; it0 += CHUNK) {
         const int it1 = std::min(maxIt, it0 + CHUNK);
         for (int it = it0; it < it1; ++it) {
-            if ((rc = launch_kxu_f64(p, pp, Ap, flags))) return rc;
+            pp = pb[it & 1];
+            if (it == 0) rc = launch_kxu_f64(p, pp, Ap, flags);
+            else rc = launch_kxu_f64_fusep(p, M ? z : r, pb[(it - 1) & 1], S.rz + it, S.rz + it - 1, pp, Ap, flags);
+            if (rc) return rc;
             if ((rc = canon_dot_dev(p, pp, Ap, n, S.denom + it, flags))) return rc;
             hipLaunchKernelGGL(k_pcg_update_xr, dim3(ntile), dim3(CANON_LANES), 0, st, n, S.rz + it, S.denom + it, pp,
                                Ap, x, r, p->red, flags);
@@ -175,16 +174,14 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
             if ((rc = canon_finish(p, ntile, S.rn2 + it, flags))) return rc;
             hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(1), 0, st, S.rn2 + it, S.stop, it, flags);
             T3D_HIP(hipGetLastError());
+            // z = M r with r.z in the V-cycle's last pass; p = z + beta p is
+            // formed by the next iteration's operator
             if (M) {
-                if ((rc = mg_apply_dev(M, r, z, flags))) return rc;
-                if ((rc = canon_dot_dev(p, r, z, n, S.rz + it + 1, flags))) return rc;
-                hipLaunchKernelGGL(k_pcg_update_p, dim3(nb), dim3(B), 0, st, n, S.rz + it + 1, S.rz + it, z, pp,
-                                   flags);
+                if ((rc = mg_apply_dev(M, r, z, flags, S.rz + it + 1))) return rc;
             } else {
                 hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, st, S.rn2 + it, S.rz + it + 1, flags);
-                hipLaunchKernelGGL(k_pcg_update_p, dim3(nb), dim3(B), 0, st, n, S.rn2 + it, S.rz + it, r, pp, flags);
+                T3D_HIP(hipGetLastError());
             }
-            T3D_HIP(hipGetLastError());
         }
         T3D_HIP(hipMemcpyAsync(p->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));

@@ -15,29 +15,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 // ---------------------------------------------------------------------------
 // canonical reduction kernels (definition shared with the oracle)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double block_tree_256(double s, double* sh)
-{
-    const int l = threadIdx.x;
-    sh[l] = s;
-    __syncthreads();
-#pragma unroll
-    for (int w = CANON_LANES / 2; w >= 64; w >>= 1) {
-        if (l < w) sh[l] = sh[l] + sh[l + w];
-        __syncthreads();
-    }
-    // last 6 halvings inside wave 0: lane l adds lane l+w
-    if (l < 64) {
-        double v = sh[l];
-#pragma unroll
-        for (int w = 32; w >= 1; w >>= 1) {
-            double o = __shfl_down(v, w, 64);
-            v = v + o;   // only lanes l < w are consumed; they add exactly s[l+w]
-        }
-        if (l == 0) sh[0] = v;
-    }
-    __syncthreads();
-    return sh[0];
-}
 
 __global__ void __launch_bounds__(256) k_canon_dot_tiles(const double* __restrict__ a, const double* __restrict__ b,
                                                          int64_t n, double* __restrict__ out,
@@ -232,7 +209,7 @@ int t3d_problem_destroy(t3d_problem* p)
 {
     if (!p) return T3D_OK;
     void* bufs[] = {p->Ke, p->nodeFixed, p->compactToCell, p->cellToCompact, p->freeDofIndex,
-                    p->Ecell, p->EcellF, p->bFull, p->red, p->scal, p->wr, p->wz, p->wp, p->wAp, p->pcgSc,
+                    p->Ecell, p->EcellF, p->bFull, p->red, p->scal, p->wr, p->wz, p->wp, p->wp2, p->wAp, p->pcgSc,
                     p->pcgFlags, p->stlIns, p->stlOff, p->ocBis, p->ocAgg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);

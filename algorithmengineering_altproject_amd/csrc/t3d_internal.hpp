@@ -35,6 +35,32 @@ constexpr int CANON_LANES = 256;
 constexpr int CANON_PER_LANE = 4;
 constexpr int CANON_TILE = CANON_LANES * CANON_PER_LANE;
 
+// the canonical tile tree over one workgroup of CANON_LANES lanes: lane
+// partials halved pairwise (lane l adds lane l + w), the result in every thread
+__device__ __forceinline__ double block_tree_256(double s, double* sh)
+{
+    const int l = threadIdx.x;
+    sh[l] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = CANON_LANES / 2; w >= 64; w >>= 1) {
+        if (l < w) sh[l] = sh[l] + sh[l + w];
+        __syncthreads();
+    }
+    // last 6 halvings inside wave 0: lane l adds lane l+w
+    if (l < 64) {
+        double v = sh[l];
+#pragma unroll
+        for (int w = 32; w >= 1; w >>= 1) {
+            double o = __shfl_down(v, w, 64);
+            v = v + o;   // only lanes l < w are consumed; they add exactly s[l+w]
+        }
+        if (l == 0) sh[0] = v;
+    }
+    __syncthreads();
+    return sh[0];
+}
+
 struct Grid {
     int resX, resY, resZ;   // cells
     int nnx, nny, nnz;      // nodes per axis
@@ -90,7 +116,7 @@ struct t3d_problem {
     int64_t redCap = 0;
     double* scal = nullptr;             // device scalars
     // PCG workspace (full layout), allocated on first use
-    double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wAp = nullptr;
+    double *wr = nullptr, *wz = nullptr, *wp = nullptr, *wp2 = nullptr, *wAp = nullptr;
     double* pcgSc = nullptr;            // per-iteration scalars
     int pcgScCap = 0;
     int* pcgFlags = nullptr;            // [0] done, [1] iterations
@@ -115,8 +141,12 @@ int canon_finish_buf(t3d_problem* p, double* buf, int64_t ntiles, double* dst, c
 int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst,
                   const int* done = nullptr);
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
+// p = z + rzNew / max(1e-30, rzOld) * pPrev into pOut (!= pPrev), then y = K p
+int launch_kxu_f64_fusep(t3d_problem* p, const double* z, const double* pPrev, const double* rzNew,
+                         const double* rzOld, double* pOut, double* y, const int* done);
 __global__ void k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ done);
-int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done);
+// z = M r; with rzDst also r.z by the canonical tree into that device scalar
+int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done, double* rzDst = nullptr);
 int compliance_dev(t3d_problem* p, const double* E, const double* U, double* ce, double* Cdst);
 int cobj_dev(t3d_problem* p, const double* E, const double* ce, const double* Csolid, double* dst);
 int launch_simp(t3d_problem* p, const float* x, float E0, float Emin, double* E);
