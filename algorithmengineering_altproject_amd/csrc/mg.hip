@@ -202,6 +202,83 @@ __global__ void __launch_bounds__(256) k_restrict(LevelDev Lf, LevelDev Lc, int 
     }
 }
 
+// The same restriction with one thread per coarse DOF (node n = t / 3,
+// component c = t % 3) for the spans the hierarchy uses: neighbouring lanes
+// read neighbouring doubles, and the loads of a fine row (2 SPAN - 1 nodes)
+// are issued together before its terms are added.  Every coarse value and the
+// weight sum are the same sequence of additions as in k_restrict.
+template <int SPAN>
+__global__ void __launch_bounds__(256) k_restrict_c(LevelDev Lf, LevelDev Lc, const double* __restrict__ rf,
+                                                    const uint8_t* __restrict__ cmask, double* __restrict__ rc,
+                                                    const double* __restrict__ Dc, double* __restrict__ xc,
+                                                    const int* __restrict__ done, int norm)
+{
+    if (done && *done) return;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 3 * Lc.nN) return;
+    const int64_t n = t / 3;
+    const int c = int(t - 3 * n);
+    const int cx = int(n % Lc.nnx), cy = int((n / Lc.nnx) % Lc.nny), cz = int(n / (int64_t(Lc.nnx) * Lc.nny));
+    constexpr int W = 2 * SPAN - 1;
+    const double inv = 1.0 / SPAN;
+    const int fx0 = cx * SPAN, fy0 = cy * SPAN, fz0 = cz * SPAN;
+    const int iz0 = (fz0 - SPAN + 1 < 0) ? (SPAN - 1 - fz0) : 0;
+    const int iz1 = (fz0 + SPAN - 1 >= Lf.nnz) ? (W - (fz0 + SPAN - Lf.nnz)) : W;
+    const int iy0 = (fy0 - SPAN + 1 < 0) ? (SPAN - 1 - fy0) : 0;
+    const int iy1 = (fy0 + SPAN - 1 >= Lf.nny) ? (W - (fy0 + SPAN - Lf.nny)) : W;
+    const int ix0 = (fx0 - SPAN + 1 < 0) ? (SPAN - 1 - fx0) : 0;
+    const int ix1 = (fx0 + SPAN - 1 >= Lf.nnx) ? (W - (fx0 + SPAN - Lf.nnx)) : W;
+    double v = 0.0, wsum = 0.0;
+    for (int iz = iz0; iz < iz1; ++iz) {
+        const double wz = hat_w(iz, SPAN, inv);
+        const int64_t zo = int64_t(fz0 + iz - (SPAN - 1)) * Lf.nny * Lf.nnx;
+        for (int iy = iy0; iy < iy1; ++iy) {
+            const double wzy = wz * hat_w(iy, SPAN, inv);
+            const int64_t yo = zo + int64_t(fy0 + iy - (SPAN - 1)) * Lf.nnx;
+            double f[W];
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix)
+                f[ix] = (ix >= ix0 && ix < ix1) ? rf[3 * (yo + (fx0 + ix - (SPAN - 1))) + c] : 0.0;
+#pragma unroll
+            for (int ix = 0; ix < W; ++ix)
+                if (ix >= ix0 && ix < ix1) {
+                    const double w = wzy * hat_w(ix, SPAN, inv);
+                    v = v + f[ix] * w;
+                    wsum = wsum + w;
+                }
+        }
+    }
+    double r = 0.0;
+    if (!norm) r = v;
+    else if (wsum > 1e-12) r = v * (1.0 / wsum);
+    const unsigned m = cmask[n];
+    if ((m >> c) & 1u) r = 0.0;
+    rc[t] = r;
+    if (Dc) {
+        const double a = Dc[t];
+        xc[t] = ((m >> c) & 1u) ? 0.0 : r / (1.0e-30 < a ? a : 1.0e-30);
+    }
+}
+
+// k_restrict for level l -> l + 1, by coarse DOF for spans 2 and 4
+static int launch_restrict(hipStream_t st, const LevelDev& Lf, const LevelDev& Lc, int span, const double* rf,
+                           const uint8_t* cmask, double* rc, const double* Dc, double* xc, const int* done, int norm)
+{
+    const int B = 256;
+    if (span == 2 || span == 4) {
+        const dim3 grid(unsigned((3 * Lc.nN + B - 1) / B));
+        if (span == 2)
+            hipLaunchKernelGGL(k_restrict_c<2>, grid, dim3(B), 0, st, Lf, Lc, rf, cmask, rc, Dc, xc, done, norm);
+        else
+            hipLaunchKernelGGL(k_restrict_c<4>, grid, dim3(B), 0, st, Lf, Lc, rf, cmask, rc, Dc, xc, done, norm);
+    } else {
+        hipLaunchKernelGGL(k_restrict, dim3(unsigned((Lc.nN + B - 1) / B)), dim3(B), 0, st, Lf, Lc, span, rf, cmask,
+                           rc, Dc, xc, done, norm);
+    }
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
 // ---------------------------------------------------------------------------
 // prolongation + additive correction + post-smoothing + mask
 // (transfers.cpp:378-445, preconditioner.cpp:230-253):
@@ -259,9 +336,11 @@ __global__ void __launch_bounds__(256) k_prolong_smooth(LevelDev Lf, LevelDev Lc
 // The finest level's k_prolong_smooth evaluated per DOF in the canonical tile
 // geometry (tile t holds DOFs 1024 t + 256 k + lane, k = 0..3), with the
 // tile's partial of r.z summed as k_canon_dot_tiles sums it: the V-cycle's
-// last pass and the PCG's dot product in one read of r.
-__global__ void __launch_bounds__(256) k_prolong_smooth_rz(LevelDev Lf, LevelDev Lc, int span,
-                                                           const double* __restrict__ xc,
+// last pass and the PCG's dot product in one read of r.  Per DOF, for the
+// spans the hierarchy uses and grids below 2^32 nodes (32-bit coordinates;
+// the launcher checks): the same operations as prolong_smooth_dof.
+template <int SPAN>
+__global__ void __launch_bounds__(256) k_prolong_smooth_rz(LevelDev Lf, LevelDev Lc, const double* __restrict__ xc,
                                                            const double* __restrict__ rf,
                                                            const double* __restrict__ Df,
                                                            const uint8_t* __restrict__ fmask, double weight,
@@ -272,14 +351,39 @@ __global__ void __launch_bounds__(256) k_prolong_smooth_rz(LevelDev Lf, LevelDev
     __shared__ double sh[CANON_LANES];
     const int64_t nD = 3 * Lf.nN;
     const int64_t base = int64_t(blockIdx.x) * CANON_TILE + threadIdx.x;
+    const uint32_t nnx = uint32_t(Lf.nnx), nny = uint32_t(Lf.nny);
+    const double inv = 1.0 / SPAN;
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < CANON_PER_LANE; ++k) {
         const int64_t i = base + int64_t(k) * CANON_LANES;
         if (i < nD) {
-            const int64_t n = i / 3;
-            const int c = int(i - 3 * n);
-            const double out = prolong_smooth_dof(Lf, Lc, span, xc, rf, Df, fmask[n], weight, n, c);
+            const uint32_t n = uint32_t(i / 3);
+            const int c = int(i - 3 * int64_t(n));
+            const uint32_t q = n / nnx;
+            const int fx = int(n - q * nnx), fy = int(q % nny), fz = int(q / nny);
+            int cx = fx / SPAN, cy = fy / SPAN, cz = fz / SPAN;
+            int rx = fx % SPAN, ry = fy % SPAN, rz = fz % SPAN;
+            if (cx >= Lc.resX) { cx = Lc.resX - 1; rx = SPAN; }
+            if (cy >= Lc.resY) { cy = Lc.resY - 1; ry = SPAN; }
+            if (cz >= Lc.resZ) { cz = Lc.resZ - 1; rz = SPAN; }
+            const double wx1 = double(rx) * inv, wx0 = 1.0 - wx1;
+            const double wy1 = double(ry) * inv, wy0 = 1.0 - wy1;
+            const double wz1 = double(rz) * inv, wz0 = 1.0 - wz1;
+            const int64_t b00 = (int64_t(Lc.nnx) * Lc.nny * cz + int64_t(Lc.nnx) * cy + cx);
+            const int64_t b01 = b00 + 1, b10 = b00 + Lc.nnx, b11 = b10 + 1;
+            const int64_t sz = int64_t(Lc.nnx) * Lc.nny;
+            const int64_t n00 = b00 + sz, n01 = n00 + 1, n10 = n00 + Lc.nnx, n11 = n10 + 1;
+            const double v = xc[3 * b00 + c] * wx0 * wy0 * wz0 + xc[3 * b01 + c] * wx1 * wy0 * wz0 +
+                             xc[3 * b10 + c] * wx0 * wy1 * wz0 + xc[3 * b11 + c] * wx1 * wy1 * wz0 +
+                             xc[3 * n00 + c] * wx0 * wy0 * wz1 + xc[3 * n01 + c] * wx1 * wy0 * wz1 +
+                             xc[3 * n10 + c] * wx0 * wy1 * wz1 + xc[3 * n11 + c] * wx1 * wy1 * wz1;
+            double out = 0.0;
+            if (!((fmask[n] >> c) & 1u)) {
+                const double d = Df[i];
+                const double t = weight * rf[i] / (1.0e-30 < d ? d : 1.0e-30);
+                out = (t + v) + t;
+            }
             xf[i] = out;
             s = s + rf[i] * out;
         }
@@ -849,10 +953,9 @@ static int mg_apply_standard(t3d_mg* g, const double* r0, double* z, const int* 
         if ((rc = apply(l, xl, g->ys[l]))) return rc;
         hipLaunchKernelGGL(k_std_point, grid(l), dim3(B), 0, st, 1, g->ld[l].nN, rl, (const double*)g->ys[l],
                            (const double*)nullptr, (const uint8_t*)g->mask[l], w, g->ys[l], done);
-        hipLaunchKernelGGL(k_restrict, grid(l + 1), dim3(B), 0, st, g->ld[l], g->ld[l + 1], g->H.lv[l + 1].span,
-                           (const double*)g->ys[l], (const uint8_t*)g->mask[l + 1], g->r[l + 1],
-                           (const double*)nullptr, g->x[L], done, 0);
-        T3D_HIP(hipGetLastError());
+        int rc = launch_restrict(st, g->ld[l], g->ld[l + 1], g->H.lv[l + 1].span, (const double*)g->ys[l],
+                                 (const uint8_t*)g->mask[l + 1], g->r[l + 1], nullptr, g->x[L], done, 0);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
                        (const double*)g->Ainv, (const double*)g->r[L], (const uint8_t*)g->mask[L], g->x[L], done);
@@ -891,10 +994,9 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done, double
     for (int l = 0; l < L; ++l) {
         const double* rf = (l == 0) ? r0 : g->r[l];
         const bool last = (l + 1 == L);
-        hipLaunchKernelGGL(k_restrict, dim3(unsigned((g->ld[l + 1].nN + B - 1) / B)), dim3(B), 0, st, g->ld[l],
-                           g->ld[l + 1], g->H.lv[l + 1].span, rf, g->mask[l + 1], g->r[l + 1],
-                           (last && !chol) ? g->D[L] : nullptr, g->x[L], done, g->mode == 0 ? 1 : 0);
-        T3D_HIP(hipGetLastError());
+        int rc = launch_restrict(st, g->ld[l], g->ld[l + 1], g->H.lv[l + 1].span, rf, g->mask[l + 1], g->r[l + 1],
+                                 (last && !chol) ? g->D[L] : nullptr, g->x[L], done, g->mode == 0 ? 1 : 0);
+        if (rc) return rc;
     }
     if (g->mode == 1) {   // x = Ainv r = Li^T Li r, masked
         hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
@@ -908,14 +1010,19 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done, double
     for (int l = L - 1; l >= 0; --l) {
         const double* rf = (l == 0) ? r0 : g->r[l];
         double* xo = (l == 0) ? z : g->x[l];
-        if (l == 0 && rzDst) {   // with the r.z partials, in the canonical geometry
+        const int span0 = g->H.lv[1].span;
+        if (l == 0 && rzDst && (span0 == 2 || span0 == 4) && g->ld[0].nN < (int64_t(1) << 32)) {
+            // with the r.z partials, in the canonical geometry
             const int64_t nD = 3 * g->ld[0].nN;
             int rc = ensure_red(p, nD);
             if (rc) return rc;
             const int64_t tiles = (nD + CANON_TILE - 1) / CANON_TILE;
-            hipLaunchKernelGGL(k_prolong_smooth_rz, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, st, g->ld[0],
-                               g->ld[1], g->H.lv[1].span, g->x[1], rf, g->D[0], g->mask[0], g->H.weight, xo, p->red,
-                               done);
+            if (span0 == 2)
+                hipLaunchKernelGGL(k_prolong_smooth_rz<2>, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, st, g->ld[0],
+                                   g->ld[1], g->x[1], rf, g->D[0], g->mask[0], g->H.weight, xo, p->red, done);
+            else
+                hipLaunchKernelGGL(k_prolong_smooth_rz<4>, dim3(unsigned(tiles)), dim3(CANON_LANES), 0, st, g->ld[0],
+                                   g->ld[1], g->x[1], rf, g->D[0], g->mask[0], g->H.weight, xo, p->red, done);
             T3D_HIP(hipGetLastError());
             return canon_finish(p, tiles, rzDst, done);
         }
@@ -924,7 +1031,7 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done, double
                            done);
         T3D_HIP(hipGetLastError());
     }
-    return T3D_OK;
+    return rzDst ? canon_dot_dev(p, r0, z, 3 * g->ld[0].nN, rzDst, done) : T3D_OK;
 }
 
 }  // namespace t3d
