@@ -79,16 +79,20 @@ int canon_finish_buf(t3d_problem* p, double* buf, int64_t ntiles, double* dst, c
     double* src = buf;
     double* alt = buf + ntiles;
     int64_t m = ntiles;
+    if (m <= 1) {
+        hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, p->stream, src, dst, done);
+        T3D_HIP(hipGetLastError());
+        return T3D_OK;
+    }
     while (m > 1) {
         const int64_t m2 = (m + CANON_TILE - 1) / CANON_TILE;
-        hipLaunchKernelGGL(k_canon_sum_tiles, dim3(unsigned(m2)), dim3(CANON_LANES), 0, p->stream, src, m, alt,
-                           done);
+        // the last level's single tile writes the result itself
+        hipLaunchKernelGGL(k_canon_sum_tiles, dim3(unsigned(m2)), dim3(CANON_LANES), 0, p->stream, src, m,
+                           m2 == 1 ? dst : alt, done);
         T3D_HIP(hipGetLastError());
         std::swap(src, alt);
         m = m2;
     }
-    hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, p->stream, src, dst, done);
-    T3D_HIP(hipGetLastError());
     return T3D_OK;
 }
 
