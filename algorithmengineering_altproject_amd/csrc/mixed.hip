@@ -233,6 +233,9 @@ __global__ void k_dinv(int64_t n, const double* __restrict__ D, float* __restric
 // v_mfma_f32_32x32x16_bf16 with compensated products accumulated in FP32:
 //   PIECES = 2 (bf16 x3): Ke = Kh + Kl, U = Uh + Ul (round to nearest even),
 //     F ~ Kh Uh + Kh Ul + Kl Uh  (16-bit operands, ~4e-6 relative in K u);
+//   U is each voxel's corner values less their per-component mean (a rigid
+//     translation, Ke times it is zero), so the operand rounding is relative
+//     to the variation of u over the voxel, not to u;
 //   PIECES = 3 (bf16 x6): three pieces each, F ~ KhUh + KhUm + KmUh + KhUl +
 //     KmUm + KlUh  (24-bit operands, FP32-level accuracy).
 // DOFs are ordered component-major in the matrix dimensions (k = 8 c + b for
@@ -341,11 +344,22 @@ __global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __r
                 // component of this lane's 8 K entries: ks 0 -> kh, ks 1 -> 2 (lanes 32..63: padding)
                 const int c = ks == 0 ? kh : 2;
                 const bool live = ks == 0 || kh == 0;
+                // the voxel's eight corner values of component c, less their
+                // mean: a rigid translation, which the element matrix maps to
+                // zero, so the split's rounding scales with how u varies over
+                // the voxel instead of with u itself
+                float uv[8];
+                float mean = 0.f;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    uv[b] = live ? sP[(b >> 2) ? qt : qb][c][(ty + off_y(b)) * PX + jx + off_x(b)] : 0.f;
+                    mean += uv[b];
+                }
+                mean *= 0.125f;
                 u16x8 B[PIECES];
 #pragma unroll
                 for (int b = 0; b < 8; ++b) {
-                    float v = 0.f;
-                    if (live) v = sP[(b >> 2) ? qt : qb][c][(ty + off_y(b)) * PX + jx + off_x(b)];
+                    const float v = uv[b] - mean;
                     const unsigned short h = bf16_rne(v);
                     B[0][b] = h;
                     const float r1 = v - bf16_val(h);
@@ -1151,13 +1165,13 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         T3D_HIP(hipMemcpyAsync(&rn2Now, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
         // a non-finite residual, or one grown 1e6-fold from the start, cannot
-        // recover: the FP32 operator is too inexact for this problem
+        // recover (non-finite input, or an operator too inexact for the problem)
         if (!std::isfinite(rn2Now) || rn2Now > 1.0e12 * rn2First) {
             launch_fold(s, x, nullptr);
             T3D_HIP(hipStreamSynchronize(st));
             if (iters) *iters = it1;
             set_error("t3d_pcg_mixed: the residual diverged (non-finite or grown 1e6-fold) by iteration " +
-                      std::to_string(it1) + "; the FP32 operator is too inexact for this problem");
+                      std::to_string(it1));
             return T3D_ERR_DIVERGED;
         }
         if (s->timing) {   // launches of iterations that ran (after convergence the kernel returns at once)

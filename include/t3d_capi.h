@@ -221,11 +221,11 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b_full_dev, double* x_full_dev, do
  * in FP32, form 3: the same with 3-piece operands and 6 products (bf16 x6) */
 int t3d_kxu_f32(t3d_mixed* s, int form, const float* u_full_dev, float* y_full_dev);
 /* operator of t3d_pcg_mixed: 0 Walsh form (default), 2 MFMA bf16 x3, 3 MFMA bf16 x6.
- * Form 2's products round the element matrix and the displacements to 16
- * significant bits, so its error scales with the displacement itself, not
- * with the strains that K u measures: on smooth fields (the low modes) it is
- * O(1) relative at 96x192x96 and above, and t3d_pcg_mixed diverges there
- * (T3D_ERR_DIVERGED); it converges only on small grids. */
+ * The MFMA forms split each voxel's corner displacements less their mean
+ * (a rigid translation, which Ke maps to zero), so the rounding of form 2's
+ * 16-bit operands scales with how the displacement varies across a voxel,
+ * not with the displacement itself; the mixed solve with form 2 converges at
+ * 256x512x256 in the iterations of form 0. */
 int t3d_mixed_set_operator(t3d_mixed* s, int form);
 /* average milliseconds per launch over reps launches, HIP events on the
  * problem's stream: op 0..3 the FP32 operator of form 0..3; op 4 the solver's

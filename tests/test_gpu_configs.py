@@ -111,10 +111,15 @@ def test_config2_mixed_vs_fp64_43M(t3d):
 
 def test_config3_mfma_operators_and_solve_101M(t3d):
     """256x512x256 (33.5M voxels, 101M free DOFs), rho ~ U[1e-3, 1] (seed 3)
-    PDE-filtered with rMin 2: the MFMA operators against the FP64 operator
-    (bf16 x3: 2-piece products, |y - y64| <= 3e-5 max|y64|; bf16 x6: <= 5e-7),
-    and the mixed solve with the bf16 x6 operator and the Galerkin multigrid:
-    true residual below 1e-6, compliance within 1e-6 of the FP64 solve."""
+    PDE-filtered with rMin 2: the MFMA operators against the FP64 operator on a
+    random vector (bf16 x3: 2-piece operands, |y - y64| <= 3e-5 max|y64|;
+    bf16 x6: <= 5e-7) and on the smooth FP64 solution rounded to FP32, where
+    K x is small against |K| |x| (relative to the FP64 operator on the same
+    values: bf16 x3 <= 1e-3, bf16 x6 <= 1e-5, the FP32 Walsh form <= 5e-3;
+    measured 1.4e-4, 1.4e-6, 8.8e-4 -- the MFMA forms split each voxel's
+    displacement less its mean, so their rounding follows the strains); the
+    mixed solves with either MFMA operator and the Galerkin multigrid reach a
+    true residual below 1e-6 with compliance within 1e-6 of the FP64 solve."""
     dims = (256, 512, 256)
     p = t3d.Problem(*dims)
     raw = t3d.DeviceArray.from_host(np.random.default_rng(3).uniform(1e-3, 1.0, p.ne).astype(np.float32))
@@ -140,9 +145,9 @@ def test_config3_mfma_operators_and_solve_101M(t3d):
     for form, bound in ((2, 3e-5), (3, 5e-7)):
         mx.kxu_f32(u32, y32, form)
         err = float(np.abs(y32.download().astype(np.float64) - y64).max() / ymax)
-        print("config 3 operator form %d: rel err %.3e" % (form, err))
+        print("config 3 operator form %d, random u: rel err %.3e" % (form, err))
         assert err <= bound
-    del y64, u32, y32
+    del y64
     db = t3d.DeviceArray(p.nDOF)
     p.load_vector(db)
     ce = t3d.DeviceArray(p.ne)
@@ -150,22 +155,33 @@ def test_config3_mfma_operators_and_solve_101M(t3d):
     x.zero()
     it64 = t3d.pcg(p, M, db, x, 1e-6, 500)
     C64 = t3d.compliance(p, dE, x, ce)
-    mx.set_operator(3)
-    x.zero()
-    itm = mx.pcg(db, x, 1e-6, 500, trueEvery=25)
-    Cm = t3d.compliance(p, dE, x, ce)
-    res = _true_rel_residual(t3d, p, db, x)
-    print("config 3 solve: FP64 %d it, mixed bf16x6 %d it, true residual %.3e, C rel diff %.3e"
-          % (it64, itm, res, abs(Cm - C64) / C64))
-    assert it64 < 500 and itm < 500
-    assert res < 1e-6
-    assert abs(Cm - C64) <= 1e-6 * abs(C64)
-    # bf16 x3: its error scales with the displacement, O(1) on the low modes at
-    # this size, and the solve reports the divergence instead of running on
-    mx.set_operator(2)
-    x.zero()
-    with pytest.raises(t3d.T3DError, match=r"\(-5\).*diverged"):
-        mx.pcg(db, x, 1e-6, 500, trueEvery=25)
+    # the smooth solution (K x = b, |b| small against |K| |x|), rounded to
+    # FP32, against the FP64 operator on the same rounded values
+    xr = x.download().astype(np.float32)
+    x32 = t3d.DeviceArray.from_host(xr)
+    dxr = t3d.DeviceArray.from_host(xr.astype(np.float64))
+    dy = t3d.DeviceArray(p.nDOF)
+    p.kxu_f64(dxr, dy)
+    ys = dy.download()
+    del dy, dxr
+    ysmax = np.abs(ys).max()
+    for form, bound in ((2, 1e-3), (3, 1e-5), (0, 5e-3)):
+        mx.kxu_f32(x32, y32, form)
+        err = float(np.abs(y32.download().astype(np.float64) - ys).max() / ysmax)
+        print("config 3 operator form %d, solution x: rel err %.3e" % (form, err))
+        assert err <= bound
+    del u32, y32, x32
+    for form in (3, 2):
+        mx.set_operator(form)
+        x.zero()
+        itm = mx.pcg(db, x, 1e-6, 500, trueEvery=25)
+        Cm = t3d.compliance(p, dE, x, ce)
+        res = _true_rel_residual(t3d, p, db, x)
+        print("config 3 solve: FP64 %d it, mixed MFMA form %d %d it, true residual %.3e, C rel diff %.3e"
+              % (it64, form, itm, res, abs(Cm - C64) / C64))
+        assert it64 < 500 and itm < 500
+        assert res < 1e-6
+        assert abs(Cm - C64) <= 1e-6 * abs(C64)
 
 
 def test_config4_batch64_matches_single_solves(t3d):

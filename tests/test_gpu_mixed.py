@@ -128,6 +128,25 @@ def test_mixed_residual_replacement_off(t3d):
     assert it < 3000 and _true_rel_residual(p, db, x) < 1e-4
 
 
+def test_mixed_reports_divergence(t3d):
+    """A residual that becomes non-finite ends the solve with T3D_ERR_DIVERGED
+    after the chunk in which it happened, instead of running to the iteration
+    cap; forced from the host with one NaN in the right-hand side (the kernels
+    run as usual)."""
+    p, dE, M, mx, db = _setup(t3d, (8, 16, 8), 0.12)
+    b = db.download()
+    b[np.flatnonzero(b)[0]] = np.nan
+    dbn = t3d.DeviceArray.from_host(b)
+    x = t3d.DeviceArray(p.nDOF)
+    x.zero()
+    with pytest.raises(t3d.T3DError, match=r"\(-5\).*diverged"):
+        mx.pcg(dbn, x, 1e-6, 500, trueEvery=25)
+    # the solver object stays usable
+    x.zero()
+    it = mx.pcg(db, x, 1e-6, 500, trueEvery=25)
+    assert it < 500 and _true_rel_residual(p, db, x) < 1e-6
+
+
 def test_repeated_solves_identical(t3d):
     """Zeroing x through the C ABI and solving again reproduces the first solve
     bit for bit (FP64 plain and mixed): the host helpers complete before they
