@@ -616,7 +616,7 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         const int64_t nD = 3 * p->g.nN;
         if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
         if ((rc = canon_dot_dev(p, b[i], b[i], nD, s->sc + S_NORMB2))) return rc;
-        if ((rc = launch_kxu_f64(p, x[i], p->wAp))) return rc;
+        if ((rc = launch_kxu_f64_fma(p, x[i], p->wAp))) return rc;
         if ((rc = mixed_true_residual(s, b[i]))) return rc;
         T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * nD, st));
         T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * nD, st));

@@ -727,7 +727,7 @@ int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it)
     const int64_t n = 3 * p->g.nN;
     const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     launch_fold(s, x, (const int*)s->iflags);
-    int rc = launch_kxu_f64(p, x, p->wAp, s->iflags);
+    int rc = launch_kxu_f64_fma(p, x, p->wAp, s->iflags);
     if (rc) return rc;
     hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
                        s->part, (const int*)s->iflags);
@@ -743,7 +743,7 @@ int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double*
     t3d_problem* p = s->p;
     int rc = mixed_fold(s, x);
     if (rc) return rc;
-    if ((rc = launch_kxu_f64(p, x, p->wAp))) return rc;
+    if ((rc = launch_kxu_f64_fma(p, x, p->wAp))) return rc;
     if ((rc = mixed_true_residual(s, b))) return rc;
     T3D_HIP(hipMemcpyAsync(stop, s->sc + S_STOP, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     T3D_HIP(hipMemcpyAsync(rn2, s->sc + S_RN2, sizeof(double), hipMemcpyDeviceToHost, p->stream));
@@ -1113,7 +1113,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
     if ((rc = canon_dot_dev(p, b, b, n, S + S_NORMB2))) return rc;
     hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(1), 0, st, tol, S, flags);
-    if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
+    if ((rc = launch_kxu_f64_fma(p, x, Ap64))) return rc;
     hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
                        nullptr);
     hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, -1, 0, 0);
@@ -1151,7 +1151,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
                 // x += dx, then residual replacement with the FP64 operator (the
                 // direction is kept)
                 launch_fold(s, x, (const int*)flags);
-                if ((rc = launch_kxu_f64(p, x, Ap64, flags))) return rc;
+                if ((rc = launch_kxu_f64_fma(p, x, Ap64, flags))) return rc;
                 hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r,
                                    s->part, (const int*)flags);
                 hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
@@ -1191,7 +1191,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         // true residual b - K x (FP64); otherwise continue from there
         const int itDone = s->hostFlags[1] - 1;
         launch_fold(s, x, nullptr);
-        if ((rc = launch_kxu_f64(p, x, Ap64))) return rc;
+        if ((rc = launch_kxu_f64_fma(p, x, Ap64))) return rc;
         hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
                            nullptr);
         hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
