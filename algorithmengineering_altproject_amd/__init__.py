@@ -104,6 +104,9 @@ def lib():
         "t3d_batch_setup": (C.c_int, [vp, vp]),
         "t3d_batch_set_tiles": (C.c_int, [vp, C.c_int, C.c_int]),
         "t3d_batch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "t3d_batch_set_timing": (C.c_int, [vp, C.c_int]),
+        "t3d_batch_get_timing": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_longlong)]),
         "t3d_batch_solve": (C.c_int, [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "t3d_mixed_set_operator": (C.c_int, [vp, C.c_int]),
         "t3d_mixed_time_op": (C.c_int, [vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
@@ -681,6 +684,25 @@ class Batch:
         a, b = C.c_int(), C.c_int()
         check(lib().t3d_batch_info(self.h, C.byref(a), C.byref(b)), "t3d_batch_info")
         return a.value, b.value
+
+    def set_timing(self, on=True):
+        """HIP-event timing of the next solves (totals zeroed): per kernel of the
+        directly launched iterations (useGraph=False), per graph replay (useGraph=True)."""
+        check(lib().t3d_batch_set_timing(self.h, int(on)), "t3d_batch_set_timing")
+
+    def timing(self):
+        """{name: (total ms, launches)} of the kernels timed, plus "graph" for the replays."""
+        buf = C.create_string_buffer(64)
+        ms, n = C.c_double(), C.c_longlong()
+        nk = lib().t3d_batch_get_timing(self.h, -1, buf, 64, C.byref(ms), C.byref(n))
+        if nk < 0:
+            check(nk, "t3d_batch_get_timing")
+        out = {"graph": (ms.value, n.value)}
+        for k in range(nk):
+            check(0 if lib().t3d_batch_get_timing(self.h, k, buf, 64, C.byref(ms), C.byref(n)) >= 0 else -1,
+                  "t3d_batch_get_timing")
+            out[buf.value.decode()] = (ms.value, n.value)
+        return out
 
     def solve(self, dbs, dxs, tol, maxIt, trueEvery=50, flexible=True, useGraph=True):
         bp = (C.c_void_p * self.n)(*[d.ptr.value for d in dbs])

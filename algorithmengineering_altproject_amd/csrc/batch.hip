@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <string>
 #include <vector>
 
 #include "chol.hpp"
@@ -50,7 +52,8 @@ struct BProb {
     const double* LT;
     double* rc64;
     double* xc64;
-    const double* Ainv;   // coarsest inverse Li^T Li
+    const double* Ainv;   // coarsest inverse Li^T Li (packed lower tiles, coarse.hpp)
+    double* cP;           // partial products of the coarsest product
     const float* Ecell;
     const uint8_t* nodeFixed;
     float *pA, *pB, *Ap, *z, *r, *dx, *tmp;
@@ -147,16 +150,25 @@ __global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P
                      bx, cy, cz, 0, 0, q.norm);
 }
 
-// coarsest solve x = Ainv rc64 (coarse.hpp): G blocks per listed problem,
-// masked, FP64 and FP32 copies
-__global__ void __launch_bounds__(64 * CO_WAVES) kb_coarse_mv(const BProb* __restrict__ P,
-                                                              const int* __restrict__ list, unsigned G)
+// coarsest solve x = Ainv rc64 (coarse.hpp): one block per packed tile of
+// every listed problem, then one per row block (masked, FP64 and FP32 copies)
+__global__ void __launch_bounds__(256) kb_coarse_tiles(const BProb* __restrict__ P, const BMap* __restrict__ map,
+                                                       unsigned n)
 {
-    const BProb& q = P[list[blockIdx.x / G]];
+    coarse_tiles_loop(n, [=](unsigned e) {
+        const BMap m = map[e];
+        const BProb& q = P[m.prob];
+        return CoarseTile{q.Nc, q.Ainv, q.rc64, q.cP, unsigned(m.local), *q.flags == 0};
+    });
+}
+
+__global__ void __launch_bounds__(CT) kb_coarse_sum(const BProb* __restrict__ P, const BMap* __restrict__ map)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
     if (*q.flags) return;
-    const int gw = int(blockIdx.x % G) * CO_WAVES + int(threadIdx.x) / 64;
     const int L = q.nlev - 1;
-    coarse_mv_body<2>(q.Nc, q.Ainv, q.rc64, q.lv[L].mask, q.xc64, q.lv[L].x, gw, int(G) * CO_WAVES);
+    coarse_sum_body(q.Nc, q.cP, q.lv[L].mask, q.xc64, q.lv[L].x, m.local);
 }
 
 // coarse-level prolongation + smoothing into level l (l >= 1)
@@ -237,9 +249,10 @@ struct t3d_batch {
     unsigned nYZ[8] = {};
     BMap* mapPro[8] = {};
     unsigned nPro[8] = {};
-    int* cholList = nullptr;
-    int nChol = 0;
-    unsigned cholG = 1;   // workgroups per problem of the coarsest products
+    BMap* mapCT = nullptr;   // coarsest product: (problem, packed tile)
+    unsigned nCT = 0;
+    BMap* mapCS = nullptr;   // coarsest product: (problem, row block)
+    unsigned nCS = 0;
     int maxLev = 0;
     int* itDev = nullptr;
     int* flagsDev = nullptr;    // [2n] every problem's done flag and iteration, gathered for one copy
@@ -249,6 +262,18 @@ struct t3d_batch {
     bool tuned = false, fixedTiles = false;
     // graphs per parity
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // timing (t3d_batch_set_timing): HIP events around every kernel of the
+    // directly launched lockstep iterations, and around every graph replay
+    bool timing = false;
+    std::vector<hipEvent_t> kev;         // kev[k], kev[k + 1] bracket kernel k of an iteration
+    std::vector<std::string> kname;      // name of kernel k of the current iteration
+    std::vector<std::string> tname;      // accumulated names
+    std::vector<double> tms;             // accumulated milliseconds per name
+    std::vector<long long> tn;           // launches per name
+    std::vector<hipEvent_t> gev;         // two per graph replay of a chunk
+    int gUsed = 0;
+    double graphMs = 0.0;
+    long long graphN = 0;
     bool setupDone = false;
     std::vector<char> active;   // problems whose blocks are in the maps
 };
@@ -269,9 +294,10 @@ static void free_maps(t3d_batch* B)
         B->mapRX[l] = B->mapYZ[l] = B->mapPro[l] = nullptr;
         B->nRX[l] = B->nYZ[l] = B->nPro[l] = 0;
     }
-    fr(B->cholList);
-    B->mapKW = B->mapRows = nullptr;
-    B->cholList = nullptr;
+    fr(B->mapCT);
+    fr(B->mapCS);
+    B->mapKW = B->mapRows = B->mapCT = B->mapCS = nullptr;
+    B->nCT = B->nCS = 0;
     for (int k = 0; k < 2; ++k)
         if (B->graph[k]) {
             (void)hipGraphExecDestroy(B->graph[k]);
@@ -297,10 +323,8 @@ static int batch_build(t3d_batch* B)
     free_maps(B);
     if (int(B->active.size()) != B->n) B->active.assign(B->n, 1);
     B->hP.assign(B->n, BProb{});
-    std::vector<BMap> kw, rows, rx[8], yz[8], pro[8];
-    std::vector<int> chol;
+    std::vector<BMap> kw, rows, rx[8], yz[8], pro[8], ct, cs;
     B->maxLev = 0;
-    B->cholG = 1;
     for (int i = 0; i < B->n; ++i) {
         t3d_problem* p = B->P[i];
         t3d_mixed* s = B->X[i];
@@ -328,6 +352,7 @@ static int batch_build(t3d_batch* B)
         q.rc64 = s->rc64;
         q.xc64 = s->xc64;
         q.Ainv = s->Ainv;
+        q.cP = s->cP;
         T3D_CHECK(v.Nc == s->NcInv, T3D_ERR_STATE, "t3d_batch: inverse coarsest factor not set up");
         q.Ecell = p->EcellF;
         q.nodeFixed = p->nodeFixed;
@@ -365,8 +390,8 @@ static int batch_build(t3d_batch* B)
             for (unsigned t = 0; t < nyz; ++t) yz[l].push_back({i, int(t)});
         }
         if (v.Nc > 0) {
-            chol.push_back(i);
-            B->cholG = std::max(B->cholG, coarse_blocks(v.Nc));
+            for (unsigned t = 0; t < coarse_tiles(v.Nc); ++t) ct.push_back({i, int(t)});
+            for (int t = 0; t < coarse_nt(v.Nc); ++t) cs.push_back({i, t});
         }
     }
     T3D_HIP(hipMemcpy(B->dP, B->hP.data(), sizeof(BProb) * B->n, hipMemcpyHostToDevice));
@@ -378,11 +403,8 @@ static int batch_build(t3d_batch* B)
         if ((rc = upload_map(yz[l], &B->mapYZ[l], &B->nYZ[l]))) return rc;
         if ((rc = upload_map(pro[l], &B->mapPro[l], &B->nPro[l]))) return rc;
     }
-    B->nChol = int(chol.size());
-    if (B->nChol) {
-        T3D_HIP(hipMalloc(&B->cholList, sizeof(int) * chol.size()));
-        T3D_HIP(hipMemcpy(B->cholList, chol.data(), sizeof(int) * chol.size(), hipMemcpyHostToDevice));
-    }
+    if ((rc = upload_map(ct, &B->mapCT, &B->nCT))) return rc;
+    if ((rc = upload_map(cs, &B->mapCS, &B->nCS))) return rc;
     T3D_HIP(hipDeviceSynchronize());
     return T3D_OK;
 }
@@ -437,41 +459,93 @@ static void launch_pro(t3d_batch* B)
     }
 }
 
+// kernel timing of a directly launched iteration (no-op unless `on`): the
+// event after kernel k closes its interval, named `name`
+static void tmark(t3d_batch* B, bool on, const char* name)
+{
+    if (!on) return;
+    const size_t k = B->kname.size();
+    if (B->kev.size() < k + 2) {
+        hipEvent_t e;
+        while (B->kev.size() < k + 2 && hipEventCreate(&e) == hipSuccess) B->kev.push_back(e);
+        if (B->kev.size() < k + 2) return;
+    }
+    (void)hipEventRecord(B->kev[k + 1], B->stream);
+    B->kname.push_back(name);
+}
+
 // batched V-cycle: z = M32(r), rz and beta per problem
-static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady)
+static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady, bool tm = false)
 {
     hipStream_t st = B->stream;
     if (!tmpReady) launch_rx(B);
     for (int l = 0; l < B->maxLev; ++l) {
-        if (l >= 1 && B->nRX[l])
+        if (l >= 1 && B->nRX[l]) {
             hipLaunchKernelGGL(kb_restrict_x, dim3(B->nRX[l]), dim3(64), 0, st, (const BProb*)B->dP,
                                (const BMap*)B->mapRX[l], l);
-        if (B->nYZ[l])
+            tmark(B, tm, "kb_restrict_x");
+        }
+        if (B->nYZ[l]) {
             hipLaunchKernelGGL(kb_restrict_yz, dim3(B->nYZ[l]), dim3(64), 0, st, (const BProb*)B->dP,
                                (const BMap*)B->mapYZ[l], l);
+            tmark(B, tm, "kb_restrict_yz");
+        }
     }
-    if (B->nChol) {
-        const dim3 g(unsigned(B->nChol) * B->cholG), blk(64 * CO_WAVES);
-        hipLaunchKernelGGL(kb_coarse_mv, g, blk, 0, st, (const BProb*)B->dP, (const int*)B->cholList, B->cholG);
+    if (B->nCT) {
+        hipLaunchKernelGGL(kb_coarse_tiles, dim3(coarse_tile_blocks(B->nCT)), dim3(256), 0, st, (const BProb*)B->dP,
+                           (const BMap*)B->mapCT, B->nCT);
+        tmark(B, tm, "kb_coarse_tiles");
+        hipLaunchKernelGGL(kb_coarse_sum, dim3(B->nCS), dim3(CT), 0, st, (const BProb*)B->dP, (const BMap*)B->mapCS);
+        tmark(B, tm, "kb_coarse_sum");
     }
     for (int l = B->maxLev - 1; l >= 1; --l)
-        if (B->nPro[l])
+        if (B->nPro[l]) {
             hipLaunchKernelGGL(kb_prolong, dim3(B->nPro[l]), dim3(64), 0, st, (const BProb*)B->dP,
                                (const BMap*)B->mapPro[l], l);
+            tmark(B, tm, "kb_prolong");
+        }
     launch_pro(B);
+    tmark(B, tm, "kb_pro_rows");
     hipLaunchKernelGGL(kb_fin_beta, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP, flexible, first);
+    tmark(B, tm, "kb_fin_beta");
 }
 
-// one lockstep iteration of every active problem (no residual replacement)
-static void batch_iteration(t3d_batch* B, int parity, int flexible)
+// one lockstep iteration of every active problem (no residual replacement);
+// tm: time each kernel (direct launches only, never under capture)
+static void batch_iteration(t3d_batch* B, int parity, int flexible, bool tm = false)
 {
     hipStream_t st = B->stream;
+    if (tm) {
+        B->kname.clear();
+        tmark(B, true, "");   // kev[1]: the start
+        B->kname.clear();
+        std::swap(B->kev[0], B->kev[1]);
+    }
     launch_kw(B, parity);
+    tmark(B, tm, "kb_kxw32");
     hipLaunchKernelGGL(kb_fin_alpha, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP);
+    tmark(B, tm, "kb_fin_alpha");
     launch_xr(B, parity, 1);
+    tmark(B, tm, "kb_xr_rows");
     hipLaunchKernelGGL(kb_fin_rn2, dim3(B->n), dim3(1024), 0, st, (const BProb*)B->dP, (const int*)B->itDev, 1);
-    batch_vcycle(B, flexible, 0, true);
+    tmark(B, tm, "kb_fin_rn2");
+    batch_vcycle(B, flexible, 0, true, tm);
     hipLaunchKernelGGL(kb_tick, dim3(1), dim3(1), 0, st, B->itDev);
+    if (tm && hipEventSynchronize(B->kev[B->kname.size()]) == hipSuccess) {
+        for (size_t k = 0; k < B->kname.size(); ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, B->kev[k], B->kev[k + 1]) != hipSuccess) continue;
+            size_t j = 0;
+            while (j < B->tname.size() && B->tname[j] != B->kname[k]) ++j;
+            if (j == B->tname.size()) {
+                B->tname.push_back(B->kname[k]);
+                B->tms.push_back(0.0);
+                B->tn.push_back(0);
+            }
+            B->tms[j] += ms;
+            B->tn[j] += 1;
+        }
+    }
 }
 
 }  // namespace t3d
@@ -487,6 +561,8 @@ int t3d_batch_destroy(t3d_batch* B)
     if (B->itDev) (void)hipFree(B->itDev);
     if (B->flagsDev) (void)hipFree(B->flagsDev);
     if (B->flagsHost) (void)hipHostFree(B->flagsHost);
+    for (hipEvent_t e : B->kev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : B->gev) (void)hipEventDestroy(e);
     for (int i = 0; i < B->n; ++i) {
         if (B->P[i]) B->P[i]->stream = B->ownStreams[i];
         t3d_mixed_destroy(B->X[i]);
@@ -584,6 +660,42 @@ int t3d_batch_set_tiles(t3d_batch* B, int kwNZ, int rowsPerBlock)
     B->fixedTiles = true;
     B->tuned = true;
     return B->setupDone ? batch_build(B) : T3D_OK;
+}
+
+// timing of the next solves: on != 0 zeroes the totals and creates the events
+int t3d_batch_set_timing(t3d_batch* B, int on)
+{
+    T3D_CHECK(B, T3D_ERR_ARG, "t3d_batch_set_timing: null");
+    B->timing = on != 0;
+    B->tname.clear();
+    B->tms.clear();
+    B->tn.clear();
+    B->graphMs = 0.0;
+    B->graphN = 0;
+    B->gUsed = 0;
+    if (on)
+        while (B->gev.size() < 2 * 32) {   // two per replay of a chunk of iterations
+            hipEvent_t e;
+            T3D_HIP(hipEventCreate(&e));
+            B->gev.push_back(e);
+        }
+    return T3D_OK;
+}
+
+int t3d_batch_get_timing(const t3d_batch* B, int k, char* name, int nameLen, double* ms_total, long long* launches)
+{
+    T3D_CHECK(B && ms_total && launches, T3D_ERR_ARG, "t3d_batch_get_timing: null");
+    if (k < 0) {   // the graph replays
+        *ms_total = B->graphMs;
+        *launches = B->graphN;
+        if (name && nameLen > 0) std::snprintf(name, size_t(nameLen), "%s", "graph");
+        return int(B->tname.size());
+    }
+    T3D_CHECK(k < int(B->tname.size()), T3D_ERR_ARG, "t3d_batch_get_timing: no such kernel");
+    *ms_total = B->tms[size_t(k)];
+    *launches = B->tn[size_t(k)];
+    if (name && nameLen > 0) std::snprintf(name, size_t(nameLen), "%s", B->tname[size_t(k)].c_str());
+    return int(B->tname.size());
 }
 
 int t3d_batch_info(const t3d_batch* B, int* kwNZ, int* rowsPerBlock)
@@ -701,10 +813,17 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
             const int par = it & 1;
             const bool replace = trueEvery > 0 && ((it + 1) % trueEvery) == 0;
             if (!replace) {
-                if (useGraph)
+                if (useGraph) {
+                    const bool tg = B->timing && B->gUsed + 2 <= int(B->gev.size());
+                    if (tg) T3D_HIP(hipEventRecord(B->gev[B->gUsed], st));
                     T3D_HIP(hipGraphLaunch(B->graph[par], st));
-                else
-                    batch_iteration(B, par, flexible);
+                    if (tg) {
+                        T3D_HIP(hipEventRecord(B->gev[B->gUsed + 1], st));
+                        B->gUsed += 2;
+                    }
+                } else {
+                    batch_iteration(B, par, flexible, B->timing);
+                }
             } else {
                 launch_kw(B, par);
                 hipLaunchKernelGGL(kb_fin_alpha, dim3(n), dim3(1024), 0, st, (const BProb*)B->dP);
@@ -726,6 +845,13 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         T3D_HIP(hipGetLastError());
         T3D_HIP(hipMemcpyAsync(B->flagsHost, B->flagsDev, sizeof(int) * 2 * size_t(n), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k + 1 < B->gUsed; k += 2) {   // graph replays of this chunk
+            float ms = 0.f;
+            T3D_HIP(hipEventElapsedTime(&ms, B->gev[k], B->gev[k + 1]));
+            B->graphMs += ms;
+            B->graphN += 1;
+        }
+        B->gUsed = 0;
         bool any = false, allDone = true;
         for (int i = 0; i < n; ++i) {
             if (finished[i]) continue;

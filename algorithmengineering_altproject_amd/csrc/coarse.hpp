@@ -6,11 +6,16 @@
 // must walk row by row.  The mixed solver instead keeps Ainv = Li^T Li with
 // Li = L^-1 (both computed on the device once per multigrid set-up) and applies
 //     x = Ainv b                    (x masked on output, as the substitution)
-// as one dense matrix-vector product spread over many workgroups: one
-// wavefront per row, lanes striding over the row, a fixed-order butterfly
-// sum.  Same factor, same FP64 arithmetic class; the rounding differs from the
-// substitution at the level of the matrix's condition number times 2^-53.
-// The FP64 plain path keeps the reference's substitution (chol.hpp).
+// as one dense matrix-vector product spread over many workgroups.  Ainv is
+// symmetric, so only its lower triangle is kept, in 64 x 64 tiles (tile
+// (I, J), J <= I, at index I (I + 1) / 2 + J, row-major inside, zero beyond
+// N): each tile is read once and gives two partial products, A_IJ x_J for row
+// block I and (I > J) A_IJ^T x_I for row block J, written to a partial array;
+// a second pass sums each row block's partials in a fixed order.  Half the
+// bytes of the full matrix, in two launches.  Same factor, same FP64
+// arithmetic class; the rounding differs from the substitution at the level
+// of the matrix's condition number times 2^-53.  The FP64 plain path keeps
+// the reference's substitution (chol.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,12 +25,28 @@
 
 namespace t3d {
 
-constexpr int CO_WAVES = 4;   // wavefronts per workgroup of the products
-constexpr int CO_ROWS = 1;    // rows per wavefront (grid size = ceil(N / (CO_WAVES * CO_ROWS)))
+constexpr int CT = 64;   // tile edge of the packed coarsest inverse
 
-inline unsigned coarse_blocks(int N)
+// row blocks of an N x N matrix in tiles of CT
+__host__ __device__ inline int coarse_nt(int N) { return (N + CT - 1) / CT; }
+// tiles of the packed lower triangle
+__host__ __device__ inline unsigned coarse_tiles(int N)
 {
-    return unsigned((N + CO_WAVES * CO_ROWS - 1) / (CO_WAVES * CO_ROWS));
+    const unsigned nT = unsigned(coarse_nt(N));
+    return nT * (nT + 1) / 2;
+}
+// doubles of the packed lower triangle, and of the partial products (nT x nT x CT)
+inline size_t coarse_packed_doubles(int N) { return size_t(coarse_tiles(N)) * CT * CT; }
+inline size_t coarse_partial_doubles(int N) { return size_t(coarse_nt(N)) * size_t(coarse_nt(N)) * CT; }
+
+// (I, J), J <= I, of packed tile index t = I (I + 1) / 2 + J
+__device__ __forceinline__ void coarse_tile_ij(unsigned t, int& I, int& J)
+{
+    int i = int((sqrt(8.0 * double(t) + 1.0) - 1.0) * 0.5);
+    while (unsigned(i) * unsigned(i + 1) / 2 > t) --i;
+    while (unsigned(i + 1) * unsigned(i + 2) / 2 <= t) ++i;
+    I = i;
+    J = int(t - unsigned(i) * unsigned(i + 1) / 2);
 }
 
 // column j of Li = L^-1 (row-major N x N, zero above the diagonal on entry):
@@ -190,14 +211,14 @@ static inline hipError_t launch_tri_inv(hipStream_t st, int N, const double* L, 
 }
 
 // Ainv = Li^T Li for the lower-triangular Li (row-major N x N, zero above the
-// diagonal): the coarsest solve becomes one dense product.  Output tile
-// (ti, tj) of 64 x 64 per 256-thread workgroup, 4 x 4 entries per thread,
-// k from the tile's first nonzero row in chunks of 32 staged in LDS.
-__device__ __forceinline__ void gram_body(int N, const double* __restrict__ Li, double* __restrict__ Ainv, int ti,
+// diagonal), tile (ti, tj), tj <= ti, of the packed lower triangle: 64 x 64
+// entries per 256-thread workgroup, 4 x 4 per thread, k from the tile's first
+// nonzero row in chunks of 32 staged in LDS; entries beyond N are written as 0.
+__device__ __forceinline__ void gram_body(int N, const double* __restrict__ Li, double* __restrict__ Ap, int ti,
                                           int tj)
 {
     __shared__ double Sa[32][65], Sb[32][65];
-    const int i0 = 64 * ti, j0 = 64 * tj;
+    const int i0 = CT * ti, j0 = CT * tj;
     const int t = int(threadIdx.x);
     const int ri = (t / 16) * 4, rj = (t % 16) * 4;
     double acc[4][4] = {};
@@ -216,45 +237,112 @@ __device__ __forceinline__ void gram_body(int N, const double* __restrict__ Li, 
                 for (int b = 0; b < 4; ++b) acc[a][b] = fma(Sa[k][ri + a], Sb[k][rj + b], acc[a][b]);
         __syncthreads();
     }
+    double* T = Ap + (size_t(ti) * size_t(ti + 1) / 2 + size_t(tj)) * (CT * CT);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if (i0 + ri + a < N && j0 + rj + b < N) Ainv[size_t(i0 + ri + a) * N + j0 + rj + b] = acc[a][b];
+        for (int b = 0; b < 4; ++b) T[(ri + a) * CT + rj + b] = acc[a][b];
 }
 
-// rows gw, gw + GW, ... of out = M v over the lower (TRI = 0: k <= i) or upper
-// (TRI = 1: k >= i) triangle, or all (TRI = 2), of the row-major N x N matrix
-// M, one wavefront per row.  With mask, masked rows are zeroed; with out32, an
-// FP32 copy is written too.
-template <int TRI>
-__device__ __forceinline__ void coarse_mv_body(int N, const double* __restrict__ M, const double* __restrict__ v,
-                                               const uint8_t* __restrict__ mask, double* __restrict__ out,
-                                               float* __restrict__ out32, int gw, int GW)
+// First pass of out = Ainv v over the packed tiles: tile (I, J) writes
+// P[I][J] = A_IJ v_J and, for I > J, P[J][I] = A_IJ^T v_I, each a CT-vector
+// at P + (row block * nT + col) * CT.  Each sum has a fixed order: the row
+// products as four 16-term chains added pairwise across lanes, the column
+// products as four 16-term chains added pairwise through LDS.
+struct CoarseTile {
+    int N;
+    const double* Ap;   // packed lower tiles
+    const double* v;
+    double* P;
+    unsigned tile;      // packed tile index
+    bool on;            // false: the tile's problem is finished (nothing read or written)
+};
+
+// 256-thread workgroups, each walking tiles e = blockIdx.x, + gridDim.x, ...
+// of a list of n (entry(e) describes tile e): a tile is staged in LDS from
+// registers while the next tile's loads are in flight.
+template <class F>
+__device__ __forceinline__ void coarse_tiles_loop(unsigned n, F entry)
 {
-    const int lane = int(threadIdx.x) & 63;
-    for (int i = gw; i < N; i += GW) {
-        const double* Mi = M + size_t(i) * N;
-        const int k0 = TRI == 1 ? i : 0;
-        const int k1 = TRI == 0 ? i + 1 : N;
-        // eight independent partial sums, so a wavefront has eight row loads in flight
-        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int u = 0; k0 + u * 64 < k1; u += 8) {
+    __shared__ double sA[CT][CT + 1];
+    __shared__ double sv[2][CT];
+    __shared__ double sq[4][CT];
+    const int t = int(threadIdx.x);
+    unsigned e = blockIdx.x;
+    if (e >= n) return;
+    CoarseTile c = entry(e);
+    double R[16];
+    auto load = [&](const CoarseTile& x) {
+        const double* T = x.Ap + size_t(x.tile) * (CT * CT);
 #pragma unroll
-            for (int w = 0; w < 8; ++w) {
-                const int k = k0 + (u + w) * 64 + lane;
-                if (k < k1) acc[w] = fma(Mi[k], v[k], acc[w]);
-            }
-        }
-        double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        for (int k = 0; k < 16; ++k) R[k] = x.on ? T[t + 256 * k] : 0.0;
+    };
+    load(c);
+    for (;;) {
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
-        if (lane == 0) {
-            if (mask && ((mask[i / 3] >> (i % 3)) & 1u)) a = 0.0;
-            out[i] = a;
-            if (out32) out32[i] = float(a);
+        for (int k = 0; k < 16; ++k) sA[t / CT + 4 * k][t % CT] = R[k];
+        int I, J;
+        coarse_tile_ij(c.tile, I, J);
+        if (t < CT) {
+            const int j = CT * J + t;
+            sv[0][t] = (c.on && j < c.N) ? c.v[j] : 0.0;
+        } else if (t < 2 * CT) {
+            const int i = CT * I + t - CT;
+            sv[1][t - CT] = (c.on && i < c.N) ? c.v[i] : 0.0;
         }
+        const unsigned en = e + gridDim.x;
+        const bool more = en < n;
+        CoarseTile cn = c;
+        if (more) {   // the next tile's loads, in flight during this tile's arithmetic
+            cn = entry(en);
+            load(cn);
+        }
+        __syncthreads();
+        const int nT = coarse_nt(c.N);
+        {   // row r = t / 4, columns 16 q .. 16 q + 15 (q = t % 4)
+            const int r = t / 4, q = t % 4;
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a = fma(sA[r][16 * q + k], sv[0][16 * q + k], a);
+            a += __shfl_xor(a, 1, 64);
+            a += __shfl_xor(a, 2, 64);
+            if (c.on && q == 0) c.P[(size_t(I) * nT + J) * CT + r] = a;
+        }
+        {   // column cc = t % 64, rows 16 h .. 16 h + 15 (h = t / 64)
+            const int cc = t % CT, h = t / CT;
+            double b = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) b = fma(sA[16 * h + k][cc], sv[1][16 * h + k], b);
+            sq[h][cc] = b;
+        }
+        __syncthreads();
+        if (c.on && I != J && t < CT) c.P[(size_t(J) * nT + I) * CT + t] = (sq[0][t] + sq[1][t]) + (sq[2][t] + sq[3][t]);
+        __syncthreads();   // sA, sv, sq are restaged for the next tile
+        if (!more) break;
+        e = en;
+        c = cn;
     }
+}
+
+// workgroups of the first pass (each loops over tiles; 4 per CU fit the LDS)
+inline unsigned coarse_tile_blocks(unsigned n) { return n < 1024u ? n : 1024u; }
+
+// Second pass: out[i] = sum over the row block's nT partials in order (J = 0,
+// 1, ...), zero for masked DOFs, for the CT rows of block I (CT threads).
+// With out32, an FP32 copy is written too.
+__device__ __forceinline__ void coarse_sum_body(int N, const double* __restrict__ P, const uint8_t* __restrict__ mask,
+                                                double* __restrict__ out, float* __restrict__ out32, int I)
+{
+    const int nT = coarse_nt(N);
+    const int r = int(threadIdx.x);
+    const int i = CT * I + r;
+    if (r >= CT || i >= N) return;
+    const double* Pi = P + size_t(I) * nT * CT + r;
+    double a = 0.0;
+    for (int J = 0; J < nT; ++J) a += Pi[size_t(J) * CT];
+    if (mask && ((mask[i / 3] >> (i % 3)) & 1u)) a = 0.0;
+    out[i] = a;
+    if (out32) out32[i] = float(a);
 }
 
 }  // namespace t3d

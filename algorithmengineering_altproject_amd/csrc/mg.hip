@@ -878,19 +878,27 @@ __global__ void __launch_bounds__(256) k_chol_update(int N, double* __restrict__
 }
 
 // coarsest solve of the Galerkin form through the inverse factor (coarse.hpp)
-__global__ void __launch_bounds__(256) k_mg_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
+__global__ void __launch_bounds__(256) k_mg_gram(int N, const double* __restrict__ Li, double* __restrict__ Ap)
 {
-    gram_body(N, Li, Ainv, int(blockIdx.y), int(blockIdx.x));
+    int I, J;
+    coarse_tile_ij(blockIdx.x, I, J);
+    gram_body(N, Li, Ap, I, J);
 }
 
-__global__ void __launch_bounds__(64 * CO_WAVES) k_mg_coarse_mv(int N, const double* __restrict__ M,
-                                                                const double* __restrict__ v,
-                                                                const uint8_t* __restrict__ mask,
-                                                                double* __restrict__ out, const int* __restrict__ done)
+__global__ void __launch_bounds__(256) k_mg_coarse_tiles(int N, const double* __restrict__ Ap,
+                                                         const double* __restrict__ v, double* __restrict__ P,
+                                                         const int* __restrict__ done)
 {
     if (done && *done) return;
-    coarse_mv_body<2>(N, M, v, mask, out, nullptr, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
-                      int(gridDim.x) * CO_WAVES);
+    coarse_tiles_loop(coarse_tiles(N), [=](unsigned e) { return CoarseTile{N, Ap, v, P, e, true}; });
+}
+
+__global__ void __launch_bounds__(CT) k_mg_coarse_sum(int N, const double* __restrict__ P,
+                                                      const uint8_t* __restrict__ mask, double* __restrict__ out,
+                                                      const int* __restrict__ done)
+{
+    if (done && *done) return;
+    coarse_sum_body(N, P, mask, out, nullptr, int(blockIdx.x));
 }
 
 }  // namespace t3d
@@ -924,11 +932,24 @@ struct t3d_mg {
     int vtype = 0;            // Galerkin V-cycle: 0 'Adapted', 1 'Standard' (FP64 path)
     double* ys[8] = {};       // 'Standard': A_l x and residuals per level l < L
     double* Li = nullptr;     // inverse coarsest factor L^-1
-    double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles
+    double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles (packed lower tiles)
+    double* cP = nullptr;     // partial products of the packed coarsest product
     double* yc = nullptr;     // device flag of the factorisation (after N doubles)
 };
 
 namespace t3d {
+
+// x = Ainv r over the packed tiles of the coarsest inverse, masked (Galerkin form)
+static void launch_mg_coarse(t3d_mg* g, const double* r, double* x, const int* done)
+{
+    hipStream_t st = g->p->stream;
+    const int L = g->nlev - 1;
+    hipLaunchKernelGGL(k_mg_coarse_tiles, dim3(coarse_tile_blocks(coarse_tiles(g->Nc))), dim3(256), 0, st, g->Nc,
+                       (const double*)g->Ainv,
+                       r, g->cP, done);
+    hipLaunchKernelGGL(k_mg_coarse_sum, dim3(unsigned(coarse_nt(g->Nc))), dim3(CT), 0, st, g->Nc,
+                       (const double*)g->cP, (const uint8_t*)g->mask[L], x, done);
+}
 
 // the 'Standard' V-cycle of the Galerkin form (TOP3D_XL.m:783-803); z holds x_0
 static int mg_apply_standard(t3d_mg* g, const double* r0, double* z, const int* done)
@@ -957,8 +978,7 @@ static int mg_apply_standard(t3d_mg* g, const double* r0, double* z, const int* 
                                  (const uint8_t*)g->mask[l + 1], g->r[l + 1], nullptr, g->x[L], done, 0);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
-                       (const double*)g->Ainv, (const double*)g->r[L], (const uint8_t*)g->mask[L], g->x[L], done);
+    launch_mg_coarse(g, g->r[L], g->x[L], done);
     T3D_HIP(hipGetLastError());
     for (int l = L - 1; l >= 0; --l) {
         const double* rl = (l == 0) ? r0 : g->r[l];
@@ -999,8 +1019,7 @@ int mg_apply_dev(t3d_mg* g, const double* r0, double* z, const int* done, double
         if (rc) return rc;
     }
     if (g->mode == 1) {   // x = Ainv r = Li^T Li r, masked
-        hipLaunchKernelGGL(k_mg_coarse_mv, dim3(coarse_blocks(g->Nc)), dim3(64 * CO_WAVES), 0, st, g->Nc,
-                           (const double*)g->Ainv, (const double*)g->r[L], (const uint8_t*)g->mask[L], g->x[L], done);
+        launch_mg_coarse(g, g->r[L], g->x[L], done);
         T3D_HIP(hipGetLastError());
     } else if (chol) {
         hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(CHOL_MAX), 0, st, g->Nc, g->L, g->LT, g->r[L], g->mask[L],
@@ -1241,7 +1260,8 @@ static int mg_galerkin_setup(t3d_mg* g)
     if (!g->L) T3D_HIP(hipMalloc(&g->L, sizeof(double) * n2));
     if (!g->Li) {
         T3D_HIP(hipMalloc(&g->Li, sizeof(double) * n2));
-        T3D_HIP(hipMalloc(&g->Ainv, sizeof(double) * n2));
+        T3D_HIP(hipMalloc(&g->Ainv, sizeof(double) * coarse_packed_doubles(N)));
+        T3D_HIP(hipMalloc(&g->cP, sizeof(double) * coarse_partial_doubles(N)));
         T3D_HIP(hipMalloc(&g->yc, sizeof(double) * (size_t(N) + 1)));
     }
     int* fail = reinterpret_cast<int*>(g->yc + N);
@@ -1267,8 +1287,7 @@ static int mg_galerkin_setup(t3d_mg* g)
         return T3D_ERR_NOT_SPD;
     }
     T3D_HIP(launch_tri_inv(st, N, g->L, g->Li));
-    const unsigned nt = unsigned((N + 63) / 64);
-    hipLaunchKernelGGL(k_mg_gram, dim3(nt, nt), dim3(256), 0, st, N, (const double*)g->Li, g->Ainv);
+    hipLaunchKernelGGL(k_mg_gram, dim3(coarse_tiles(N)), dim3(256), 0, st, N, (const double*)g->Li, g->Ainv);
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(st));
     g->Nc = N;
@@ -1300,6 +1319,7 @@ int t3d_mg_destroy(t3d_mg* g)
     if (g->fixCells) (void)hipFree(g->fixCells);
     if (g->Li) (void)hipFree(g->Li);
     if (g->Ainv) (void)hipFree(g->Ainv);
+    if (g->cP) (void)hipFree(g->cP);
     if (g->yc) (void)hipFree(g->yc);
     if (g->L) (void)hipFree(g->L);
     if (g->LT) (void)hipFree(g->LT);

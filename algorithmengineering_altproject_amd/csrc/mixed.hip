@@ -520,22 +520,29 @@ __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict
     d2f_body(n, a, b, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-// Ainv = Li^T Li (coarse.hpp)
-__global__ void __launch_bounds__(256) k_gram(int N, const double* __restrict__ Li, double* __restrict__ Ainv)
+// Ainv = Li^T Li into the packed lower tiles (coarse.hpp), one tile per workgroup
+__global__ void __launch_bounds__(256) k_gram(int N, const double* __restrict__ Li, double* __restrict__ Ap)
 {
-    gram_body(N, Li, Ainv, int(blockIdx.y), int(blockIdx.x));
+    int I, J;
+    coarse_tile_ij(blockIdx.x, I, J);
+    gram_body(N, Li, Ap, I, J);
 }
 
-// the coarsest solve's product (coarse.hpp)
-template <int TRI>
-__global__ void __launch_bounds__(64 * CO_WAVES) k_coarse_mv(int N, const double* __restrict__ M,
-                                                             const double* __restrict__ v,
-                                                             const uint8_t* __restrict__ mask, double* __restrict__ out,
-                                                             float* __restrict__ out32, const int* __restrict__ done)
+// the coarsest solve's product over the packed tiles (coarse.hpp): partials, then their sums
+__global__ void __launch_bounds__(256) k_coarse_tiles(int N, const double* __restrict__ Ap,
+                                                      const double* __restrict__ v, double* __restrict__ P,
+                                                      const int* __restrict__ done)
 {
     if (done && *done) return;
-    coarse_mv_body<TRI>(N, M, v, mask, out, out32, int(blockIdx.x) * CO_WAVES + int(threadIdx.x) / 64,
-                        int(gridDim.x) * CO_WAVES);
+    coarse_tiles_loop(coarse_tiles(N), [=](unsigned e) { return CoarseTile{N, Ap, v, P, e, true}; });
+}
+
+__global__ void __launch_bounds__(CT) k_coarse_sum(int N, const double* __restrict__ P,
+                                                   const uint8_t* __restrict__ mask, double* __restrict__ out,
+                                                   float* __restrict__ out32, const int* __restrict__ done)
+{
+    if (done && *done) return;
+    coarse_sum_body(N, P, mask, out, out32, int(blockIdx.x));
 }
 
 }  // namespace t3d
@@ -550,8 +557,9 @@ static void launch_coarse(t3d_mixed* s, const uint8_t* mask, float* x32, const i
 {
     hipStream_t st = s->p->stream;
     const int N = s->NcInv;
-    const dim3 g(coarse_blocks(N)), blk(64 * CO_WAVES);
-    hipLaunchKernelGGL(k_coarse_mv<2>, g, blk, 0, st, N, (const double*)s->Ainv, (const double*)s->rc64, mask,
+    hipLaunchKernelGGL(k_coarse_tiles, dim3(coarse_tile_blocks(coarse_tiles(N))), dim3(256), 0, st, N, (const double*)s->Ainv,
+                       (const double*)s->rc64, s->cP, done);
+    hipLaunchKernelGGL(k_coarse_sum, dim3(unsigned(coarse_nt(N))), dim3(CT), 0, st, N, (const double*)s->cP, mask,
                        s->xc64, x32, done);
 }
 
@@ -700,6 +708,32 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
     return T3D_OK;
 }
 
+// The fused operator kernel's memory streams as a plain copy (the bandwidth
+// this GPU gives that access mix without a stencil): reads u, z, dx and the
+// cell moduli, writes pcur = u, y = z and dx in place, one float4 of each
+// node vector per thread (3 nN / 4 of them; the moduli by the first ncell / 4
+// threads).
+__global__ void __launch_bounds__(256) k_copy_fused_streams(int64_t n4, int64_t e4, const float4* __restrict__ u,
+                                                            const float4* __restrict__ z, float4* __restrict__ dx,
+                                                            const float4* __restrict__ E, float4* __restrict__ pcur,
+                                                            float4* __restrict__ y)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    float4 a = u[i];
+    const float4 b = z[i], c = dx[i];
+    if (i < e4) {   // + 0 * E keeps the modulus read; the copy's values are not used
+        const float4 e = E[i];
+        a.x += 0.f * e.x;
+        a.y += 0.f * e.y;
+        a.z += 0.f * e.z;
+        a.w += 0.f * e.w;
+    }
+    pcur[i] = a;
+    y[i] = b;
+    dx[i] = c;
+}
+
 }  // namespace t3d
 
 namespace t3d {
@@ -789,7 +823,7 @@ int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
     void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64,
-                 s->Li, s->ownAinv ? s->Ainv : nullptr, s->yc64,
+                 s->Li, s->ownAinv ? s->Ainv : nullptr, s->yc64, s->cP,
                  s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
         if (q) (void)hipFree(q);
@@ -976,12 +1010,18 @@ int t3d_mixed_setup(t3d_mixed* s)
             s->ownAinv = true;
         }
         if (!s->Li) T3D_HIP(hipMalloc(&s->Li, sizeof(double) * n2));
-        if (!s->Ainv) T3D_HIP(hipMalloc(&s->Ainv, sizeof(double) * n2));
+        if (!s->Ainv) T3D_HIP(hipMalloc(&s->Ainv, sizeof(double) * coarse_packed_doubles(v.Nc)));
         T3D_HIP(launch_tri_inv(st, v.Nc, v.L, s->Li));
-        const unsigned nt = unsigned((v.Nc + 63) / 64);
-        hipLaunchKernelGGL(k_gram, dim3(nt, nt), dim3(256), 0, st, v.Nc, (const double*)s->Li, s->Ainv);
+        hipLaunchKernelGGL(k_gram, dim3(coarse_tiles(v.Nc)), dim3(256), 0, st, v.Nc, (const double*)s->Li, s->Ainv);
         T3D_HIP(hipGetLastError());
         s->NcInv = v.Nc;
+    }
+    if (s->NcInv > 0 && s->cPn < coarse_partial_doubles(s->NcInv)) {
+        if (s->cP) (void)hipFree(s->cP);
+        s->cP = nullptr;
+        s->cPn = 0;
+        T3D_HIP(hipMalloc(&s->cP, sizeof(double) * coarse_partial_doubles(s->NcInv)));
+        s->cPn = coarse_partial_doubles(s->NcInv);
     }
     T3D_HIP(hipStreamSynchronize(st));
     return T3D_OK;
@@ -1034,9 +1074,9 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form)
 // buffer, y = K p, block partials of p.y), as launched once per PCG iteration
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, float* ms)
 {
-    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 6, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
+    T3D_CHECK(s && u && y && reps > 0 && ms && op >= 0 && op <= 7, T3D_ERR_ARG, "t3d_mixed_time_op: bad argument");
     const int order = s->kwXcd;
-    if (op >= 5) {   // 5: fused, 6: Walsh-form operator, in the other tile order
+    if (op == 5 || op == 6) {   // 5: fused, 6: Walsh-form operator, in the other tile order
         s->kwXcd = !order;
         op = op == 5 ? 4 : 0;
     }
@@ -1045,11 +1085,16 @@ int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, 
     T3D_HIP(hipEventCreate(&e0));
     T3D_HIP(hipEventCreate(&e1));
     T3D_HIP(hipEventRecord(e0, p->stream));
+    const int64_t n4 = 3 * p->g.nN / 4;
     for (int i = 0; i < reps; ++i) {
         if (op < 4)
             launch_kx32(s, op, u, y);
-        else
+        else if (op == 4)
             launch_fused(s, u, s->pB, y, 0);
+        else
+            hipLaunchKernelGGL(t3d::k_copy_fused_streams, dim3(unsigned((n4 + 255) / 256)), dim3(256), 0, p->stream, n4,
+                               p->g.ncell / 4, (const float4*)u, (const float4*)s->z, (float4*)s->dx,
+                               (const float4*)p->EcellF, (float4*)s->pB, (float4*)y);
     }
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipEventRecord(e1, p->stream));

@@ -21,8 +21,10 @@ struct t3d_mixed {
     double* rc64 = nullptr;    // coarsest residual / solution in FP64 for Cholesky
     double* xc64 = nullptr;
     double* Li = nullptr;      // inverse Cholesky factor L^-1 (row-major, lower; set-up only)
-    double* Ainv = nullptr;    // Li^T Li, the coarsest inverse applied per V-cycle
+    double* Ainv = nullptr;    // Li^T Li, the coarsest inverse applied per V-cycle (packed lower tiles)
     bool ownAinv = true;       // false: Ainv is the multigrid object's (Galerkin form)
+    double* cP = nullptr;      // partial products of the packed coarsest product
+    size_t cPn = 0;            // doubles allocated at cP
     double* yc64 = nullptr;    // Li * rc64
     int NcInv = 0;             // size of Li (0: diagonal coarsest level)
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;

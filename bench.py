@@ -394,6 +394,34 @@ def main_batch(args, t3d):
     res = [true_residual(t3d, p, db, x) for p, db, x in zip(B.problems, dbs, xs)]
     log("batched: %.1f ms for all %d solves (%.1f solves/s), tiles %s, iterations %d..%d, max true residual %.2e"
         % (ms_batch, len(probs), len(probs) * 1e3 / ms_batch, tiles, min(its), max(its), max(res)))
+    # kernel-level evidence outside the timed steps: HIP events around every graph replay of one solve,
+    # then around every kernel of the same lockstep iterations launched directly (one sync per iteration)
+    ktiming = None
+    if not args.no_graph:
+        B.set_timing(True)
+        for x in xs:
+            x.zero()
+        B.solve(dbs, xs, btol, args.maxit, trueEvery=bte, useGraph=True)
+        g_ms, g_n = B.timing()["graph"]
+        B.set_timing(True)
+        for x in xs:
+            x.zero()
+        B.solve(dbs, xs, btol, args.maxit, trueEvery=bte, useGraph=False)
+        kt = B.timing()
+        B.set_timing(False)
+        per = {k: {"us_per_launch": 1e3 * v[0] / v[1], "launches": v[1]} for k, v in kt.items() if k != "graph" and v[1]}
+        n_it = kt["kb_kxw32"][1] if "kb_kxw32" in kt else 0
+        direct = sum(v[0] for k, v in kt.items() if k != "graph") / max(1, n_it)
+        ktiming = {"graph_replay_ms_per_iteration": g_ms / max(1, g_n), "graph_replays": g_n,
+                   "direct_launch_ms_per_iteration": direct, "direct_iterations": n_it,
+                   "per_kernel_direct": per,
+                   "timing": "HIP events on the batch stream: around each graph replay (one lockstep iteration of "
+                             "every active problem); around each kernel of the directly launched iterations "
+                             "(launch gap included)"}
+        log("graph replay: %.3f ms per lockstep iteration (%d replays); direct launches: %.3f ms per iteration"
+            % (ktiming["graph_replay_ms_per_iteration"], g_n, direct))
+        for k, v in sorted(per.items(), key=lambda kv: -kv[1]["us_per_launch"] * kv[1]["launches"]):
+            log("  %-18s %9.1f us per launch, %6d launches" % (k, v["us_per_launch"], v["launches"]))
     # the same problems one after another with the single-problem solver
     ms_seq, its_seq = None, []
     if not args.no_seq:
@@ -449,6 +477,7 @@ def main_batch(args, t3d):
             "sequential_ms": ms_seq,
             "sequential_solves_per_s": len(probs) * 1e3 / ms_seq if ms_seq else None,
             "speedup_vs_sequential": ms_seq / ms_batch if ms_seq else None,
+            "kernel_timing": ktiming,
         },
         "roofline": None,
         "cpu_baseline": None,
@@ -567,6 +596,11 @@ def main():
     copy_ms = p.time_op(1, du, dy, args.kxu_reps)
     copy_gbps = 2 * 8 * p.nDOF / copy_ms / 1e6
     log("stream copy on this GPU: %.0f GB/s" % copy_gbps)
+    # the fused kernel's own streams (read u, z, dx, moduli; write two vectors and dx) as a plain float4 copy
+    mx.time_kxu(u32, y32, 3, 7)
+    copyf_ms = mx.time_kxu(u32, y32, args.kxu_reps, 7)
+    copyf_gbps = (16 * 6 * (p.nDOF // 4) + 16 * (p.ne // 4)) / copyf_ms / 1e6
+    log("copy with the fused kernel's streams (3 in + 3 out + moduli): %.0f GB/s" % copyf_gbps)
     b64 = 8 * (2 * p.nF + p.ne)       # read u, write y (free DOFs), read E
     b32 = 4 * (2 * p.nF + p.ne)
     bfused = 4 * (6 * p.nF + p.ne)    # read p_prev, z, dx; write p, y, dx; read E
@@ -774,6 +808,8 @@ def main():
             "ms_per_launch_back_to_back": fused_ms,
             "stream_copy_gbps_this_gpu": copy_gbps,
             "frac_of_stream_copy": bfused / fused_solve_ms / 1e6 / copy_gbps,
+            "copy_of_fused_streams_gbps": copyf_gbps,
+            "frac_of_copy_of_fused_streams": bfused / fused_solve_ms / 1e6 / copyf_gbps,
             "algorithmic_bytes": bfused,
         },
         "cpu_baseline": cpu,

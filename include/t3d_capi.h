@@ -231,7 +231,10 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form);
  * problem's stream: op 0..3 the FP32 operator of form 0..3; op 4 the solver's
  * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials);
  * op 5 / op 6: the Walsh-form fused kernel / operator in the other workgroup
- * tile order (hardware order vs XCD-grouped) */
+ * tile order (hardware order vs XCD-grouped); op 7: the fused kernel's memory
+ * streams as a plain float4 copy (reads u, z, the solver's dx and the cell
+ * moduli, writes two node vectors and dx in place; 4 (18 nN + ncell) bytes),
+ * the bandwidth the same access mix reaches without the stencil */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 /* HIP-event timing of the solver's fused operator kernel inside t3d_pcg_mixed
  * (no counterpart in the reference; for the benchmark's roofline): on != 0
@@ -267,6 +270,15 @@ int t3d_batch_setup(t3d_batch* b, const double* const* E_dev);
  * {2,4,8}; (0, 0) = autotune on the next solve (the default) */
 int t3d_batch_set_tiles(t3d_batch* b, int kwNZ, int rowsPerBlock);
 int t3d_batch_info(const t3d_batch* b, int* kwNZ, int* rowsPerBlock);
+/* kernel timing of the next t3d_batch_solve calls (no reference counterpart):
+ * on != 0 zeroes the totals.  Solves with useGraph = 0 record HIP events
+ * around every kernel of each lockstep iteration (one stream synchronisation
+ * per iteration); solves with useGraph = 1 record events around every graph
+ * replay.  t3d_batch_get_timing: k >= 0 the k-th kernel name's total
+ * milliseconds and launches, k < 0 the graph replays ("graph"); returns the
+ * number of kernel names, or a negative status. */
+int t3d_batch_set_timing(t3d_batch* b, int on);
+int t3d_batch_get_timing(const t3d_batch* b, int k, char* name, int nameLen, double* ms_total, long long* launches);
 /* b_dev[i], x_dev[i]: FP64 full layout of problem i (x = initial guess and
  * result); iters[i] as t3d_pcg_mixed; useGraph != 0 replays hipGraphs */
 int t3d_batch_solve(t3d_batch* b, const double* const* b_dev, double* const* x_dev, double tol, int maxIt,

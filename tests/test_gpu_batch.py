@@ -140,3 +140,46 @@ def test_batch_galerkin_matches_sequential(t3d):
         assert its[i] == it_ref and its[i] < 150
         assert np.array_equal(x.download(), x_ref)
         assert true_rel_residual(t3d, p, db, x) < tol
+
+
+def test_batch_timing_leaves_results_unchanged(t3d):
+    """t3d_batch_set_timing: the graph replays and, without graphs, every kernel
+    of the lockstep iterations are timed with HIP events; timing changes no
+    result (same iterations, bit-identical solutions as an untimed solve), the
+    per-kernel launch counts agree with the iterations run, and the times are
+    positive."""
+    tol, maxIt, te = 1e-6, 500, 25
+    B, dEs, dbs = build(t3d, DIMS, V0S, VOIDS, galerkin=True)
+    B.set_tiles(16, 4)
+    xs = [t3d.DeviceArray(p.nDOF) for p in B.problems]
+    for x in xs:
+        x.zero()
+    its = B.solve(dbs, xs, tol, maxIt, trueEvery=te, useGraph=True)
+    ref = [x.download() for x in xs]
+    for graph in (True, False):
+        B.set_timing(True)
+        for x in xs:
+            x.zero()
+        its_t = B.solve(dbs, xs, tol, maxIt, trueEvery=te, useGraph=graph)
+        t = B.timing()
+        B.set_timing(False)
+        assert list(its_t) == list(its)
+        for x, r in zip(xs, ref):
+            assert np.array_equal(x.download(), r)
+        # lockstep iterations run: the slowest problem's, less the residual
+        # replacements, up to the end of its chunk of 32 iterations
+        n_plain = max(its) - max(its) // te
+        n_max = -(-(max(its) + 1) // 32) * 32
+        if graph:
+            ms, n = t["graph"]
+            assert n_plain - 2 <= n <= n_max and ms > 0.0
+            assert len(t) == 1
+        else:
+            assert t["graph"][1] == 0
+            for name in ("kb_kxw32", "kb_fin_alpha", "kb_xr_rows", "kb_fin_rn2", "kb_restrict_yz",
+                         "kb_coarse_tiles", "kb_coarse_sum", "kb_pro_rows", "kb_fin_beta"):
+                ms, n = t[name]
+                assert ms > 0.0, name
+            n_kw = t["kb_kxw32"][1]
+            assert n_plain - 2 <= n_kw <= n_max
+            assert t["kb_pro_rows"][1] == n_kw and t["kb_coarse_tiles"][1] == n_kw
