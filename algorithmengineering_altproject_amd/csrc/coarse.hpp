@@ -339,7 +339,15 @@ __device__ __forceinline__ void coarse_sum_body(int N, const double* __restrict_
     if (r >= CT || i >= N) return;
     const double* Pi = P + size_t(I) * nT * CT + r;
     double a = 0.0;
-    for (int J = 0; J < nT; ++J) a += Pi[size_t(J) * CT];
+    int J = 0;
+    for (; J + 8 <= nT; J += 8) {   // eight loads in flight, added in order
+        double w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = Pi[size_t(J + u) * CT];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += w[u];
+    }
+    for (; J < nT; ++J) a += Pi[size_t(J) * CT];
     if (mask && ((mask[i / 3] >> (i % 3)) & 1u)) a = 0.0;
     out[i] = a;
     if (out32) out32[i] = float(a);

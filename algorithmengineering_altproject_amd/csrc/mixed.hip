@@ -227,211 +227,18 @@ __global__ void k_dinv(int64_t n, const double* __restrict__ D, float* __restric
     if (i < n) Dinv[i] = float(1.0 / (1.0e-30 < D[i] ? D[i] : 1.0e-30));
 }
 
-// ---------------------------------------------------------------------------
-// FP32 operator, MFMA form (split bf16).  The voxel contraction
-// F (24 x voxels) = Ke (24x24) U (24 x voxels) runs on the matrix cores as
-// v_mfma_f32_32x32x16_bf16 with compensated products accumulated in FP32:
-//   PIECES = 2 (bf16 x3): Ke = Kh + Kl, U = Uh + Ul (round to nearest even),
-//     F ~ Kh Uh + Kh Ul + Kl Uh  (16-bit operands, ~4e-6 relative in K u);
-//   U is each voxel's corner values less their per-component mean (a rigid
-//     translation, Ke times it is zero), so the operand rounding is relative
-//     to the variation of u over the voxel, not to u;
-//   PIECES = 3 (bf16 x6): three pieces each, F ~ KhUh + KhUm + KmUh + KhUl +
-//     KmUm + KlUh  (24-bit operands, FP32-level accuracy).
-// DOFs are ordered component-major in the matrix dimensions (k = 8 c + b for
-// component c of local node b), so in the instruction's operand layouts
-// (gfx950: A[i][k]: lane l holds i = l%32, k = 8 (l/32) + e; B[k][j]: k =
-// 8 (l/32) + e, j = l%32; D[i][j]: lane l register r holds i = 8 (r/4) +
-// 4 (l/32) + r%4, j = l%32) lane l feeds the 8 nodes of one component of voxel
-// column l % 32 and receives component r/4 of the bottom-face nodes (l < 32) or
-// the top-face nodes (l >= 32).
-// Block and z-sweep as the Walsh form: 256 threads own 31 x 7 node columns; a
-// wave takes two rows of 32 voxel columns (two 32-column MFMA tiles).
-// ---------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ unsigned short bf16_rne(float x)
-{
-    const unsigned u = __float_as_uint(x);
-    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
-
-// node offset o(b) of local node b (fea.cpp node order)
-__device__ __forceinline__ constexpr int off_x(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
-__device__ __forceinline__ constexpr int off_y(int b) { return ((b & 3) <= 1) ? 1 : 0; }
-
-template <int NZ, bool FUSED, int PIECES>
-__global__ void __launch_bounds__(256) k_kxm32(Grid g, const unsigned short* __restrict__ KA,
-                                               const float* __restrict__ Ecell, const uint8_t* __restrict__ nodeFixed,
-                                               const float* __restrict__ u, const float* __restrict__ z,
-                                               float* __restrict__ pcur, float* __restrict__ y,
-                                               double* __restrict__ partial, const double* __restrict__ S,
-                                               const int* __restrict__ done, float* __restrict__ dxv, int pid)
-{
-    using namespace wk;
-    if (FUSED && *done) return;
-    __shared__ float sP[3][3][PN];          // three staged node planes, [buffer][component][node]
-    __shared__ float sBot[12][CX * CY];     // bottom-face forces of the current voxel layer
-    __shared__ float sTop[2][12][CX * CY];  // top-face forces, by layer parity
-    __shared__ double sh[4];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const int jx = lane & 31, kh = lane >> 5;
-    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
-    const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
-    const float alphaDx = FUSED ? float(S[S_ALPHA_DX]) : 0.0f;
-    if (FUSED && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0)
-        const_cast<double*>(S)[S_PLAST] = double(pid);
-    // A operands (element matrix pieces), [K step][piece][lane][8]
-    bf16x8 A[2][PIECES];
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-        for (int pc = 0; pc < PIECES; ++pc) {
-            u16x8 h;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = KA[((ks * PIECES + pc) * 64 + lane) * 8 + e];
-            A[ks][pc] = __builtin_bit_cast(bf16x8, h);
-        }
-    auto stage = [&](int Z, int q) {
-        const bool zin = Z >= 0 && Z < g.nnz;
-        const bool own = Z >= z0 && Z < z0 + NZ;
-        for (int f = tid; f < 3 * PN; f += CX * CY) {
-            const int row = f / (3 * PX);
-            const int k = f - row * (3 * PX);
-            const int lx = k / 3, c = k - 3 * lx;
-            const int X = x0 - 1 + lx, Y = y0 - 1 + row;
-            float v = 0.f;
-            if (zin && X >= 0 && X < g.nnx && Y >= 0 && Y < g.nny) {
-                const int64_t gi = 3 * ((int64_t(Z) * g.nny + Y) * g.nnx + X) + c;
-                v = u[gi];
-                if (FUSED) {
-                    const float pp = v;
-                    v = fmaf(beta, v, z[gi]);
-                    if (own && lx >= 1 && lx <= NX && row >= 1 && row <= NY) {
-                        pcur[gi] = v;
-                        dxv[gi] = fmaf(alphaDx, pp, dxv[gi]);
-                    }
-                }
-            }
-            sP[q][c][row * PX + lx] = v;
-        }
-    };
-    stage(z0 - 1, 0);
-    stage(z0, 1);
-    __syncthreads();
-    double dot = 0.0;
-    const int nsteps = min(NZ, g.nnz - z0) + 1;
-    for (int s = 0; s < nsteps; ++s) {
-        const int qb = s % 3, qt = (s + 1) % 3;   // planes cz (bottom faces) and cz + 1 (top faces)
-        const int cz = z0 - 1 + s;
-        const int par = cz & 1;
-#pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-            const int ty = 2 * wv + nb;               // voxel row of this MFMA tile
-            const int cx = x0 - 1 + jx, cy = y0 - 1 + ty;
-            float E = 0.f;
-            if (cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY && cz >= 0 && cz < g.resZ) {
-                E = Ecell[(int64_t(cz) * g.resY + cy) * g.resX + cx];
-                if (E <= 1.0e-16f) E = 0.f;   // fea.cpp:743-744
-            }
-            f32x16 acc = {};
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                // component of this lane's 8 K entries: ks 0 -> kh, ks 1 -> 2 (lanes 32..63: padding)
-                const int c = ks == 0 ? kh : 2;
-                const bool live = ks == 0 || kh == 0;
-                // the voxel's eight corner values of component c, less their
-                // mean: a rigid translation, which the element matrix maps to
-                // zero, so the split's rounding scales with how u varies over
-                // the voxel instead of with u itself
-                float uv[8];
-                float mean = 0.f;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    uv[b] = live ? sP[(b >> 2) ? qt : qb][c][(ty + off_y(b)) * PX + jx + off_x(b)] : 0.f;
-                    mean += uv[b];
-                }
-                mean *= 0.125f;
-                u16x8 B[PIECES];
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const float v = uv[b] - mean;
-                    const unsigned short h = bf16_rne(v);
-                    B[0][b] = h;
-                    const float r1 = v - bf16_val(h);
-                    const unsigned short m = bf16_rne(r1);
-                    B[1][b] = m;
-                    if (PIECES == 3) B[PIECES - 1][b] = bf16_rne(r1 - bf16_val(m));
-                }
-                const bf16x8 B0 = __builtin_bit_cast(bf16x8, B[0]), B1 = __builtin_bit_cast(bf16x8, B[1]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B0, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B1, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B0, acc, 0, 0, 0);
-                if (PIECES == 3) {
-                    const bf16x8 B2 = __builtin_bit_cast(bf16x8, B[PIECES - 1]);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B2, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B1, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][PIECES - 1], B0, acc, 0, 0, 0);
-                }
-            }
-            // registers r < 12: component r/4 of node 4 kh + r%4 of voxel column (jx, ty)
-            const int col = ty * CX + jx;
-            float* dst = kh ? &sTop[par][0][0] : &sBot[0][0];
-#pragma unroll
-            for (int r = 0; r < 12; ++r) {
-                const int c = r / 4, bq = r % 4;   // face node bq: corner (off_x, off_y)
-                const int slot = (2 * off_y(bq) + off_x(bq)) * 3 + c;
-                dst[slot * (CX * CY) + col] = E * acc[r];
-            }
-        }
-        __syncthreads();
-        if (s > 0) {   // plane cz: bottom faces of layer cz + top faces of layer cz - 1
-            const int tx = tid % CX, tyn = tid / CX;
-            const int X = x0 + tx, Y = y0 + tyn, Z = cz;
-            if (tx < NX && tyn < NY && X < g.nnx && Y < g.nny) {
-                const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
-                const unsigned fx = nodeFixed[n];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    // node (X, Y) is corner (ox, oy) of voxel column (X - ox, Y - oy)
-                    float acc2 = 0.f;
-#pragma unroll
-                    for (int oy = 0; oy < 2; ++oy)
-#pragma unroll
-                        for (int ox = 0; ox < 2; ++ox) {
-                            const int colv = (tyn + 1 - oy) * CX + (tx + 1 - ox);
-                            const int slot = (2 * oy + ox) * 3 + c;
-                            acc2 += sBot[slot][colv] + sTop[par ^ 1][slot][colv];
-                        }
-                    const float v = ((fx >> c) & 1u) ? 0.f : acc2;
-                    y[3 * n + c] = v;
-                    if (FUSED) dot = fma(double(sP[qb][c][(tyn + 1) * PX + tx + 1]), double(v), dot);
-                }
-            }
-        }
-        if (s + 1 < nsteps) stage(z0 + s + 1, (s + 2) % 3);
-        __syncthreads();
-    }
-    if (FUSED) {
-        const double t = block_sum_256(dot, sh);
-        if (tid == 0) partial[linear_block()] = t;
-    }
-}
-
 // xcd != 0: XCD-grouped tile order (each XCD works through a contiguous range
 // of tiles, x fastest, so x- and y-neighbours that share halo nodes run on
 // one L2); the partial slot is the tile index in both orders
-template <int NZ, bool FUSED, int CYT>
+// PIECES = 0: the Walsh form (KW); 2, 3: the MFMA form with bf16 x3 / x6 products (KA)
+template <int NZ, bool FUSED, int CYT, int PIECES = 0>
 __global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
                                                   const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
                                                   const float* __restrict__ z, float* __restrict__ pcur,
                                                   float* __restrict__ y, double* __restrict__ partial,
                                                   const double* __restrict__ S, const int* __restrict__ done, int xcd,
-                                                  float* __restrict__ dxv, int pid)
+                                                  float* __restrict__ dxv, int pid,
+                                                  const unsigned short* __restrict__ KA = nullptr)
 {
     unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (xcd) {
@@ -441,8 +248,8 @@ __global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, c
         bz = t.z;
     }
     const unsigned slot = (bz * gridDim.y + by) * gridDim.x + bx;
-    kxw32_body<NZ, FUSED, CYT>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, dxv, pid, bx, by, bz, slot,
-                               gridDim.x);
+    kxw32_body<NZ, FUSED, CYT, PIECES>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, dxv, pid, bx, by,
+                                       bz, slot, gridDim.x, KA);
 }
 
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
@@ -581,11 +388,6 @@ dim3 kww_grid(const Grid& g)
                 unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
 }
 
-dim3 kw_grid(const Grid& g)
-{
-    return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + wk::NY - 1) / wk::NY),
-                unsigned((g.nnz + KW_NZ - 1) / KW_NZ));
-}
 
 // y = K u in FP32; form 0: Walsh form, 1: node-gather form, 2: MFMA bf16 x3,
 // 3: MFMA bf16 x6
@@ -593,13 +395,13 @@ static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
 {
     t3d_problem* p = s->p;
     if (form == 2)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
-                           nullptr, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0);
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY, 2>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
+                           (const float*)nullptr, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
+                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA2);
     else if (form == 3)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, false, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u,
-                           nullptr, nullptr, y, nullptr, nullptr, nullptr, nullptr, 0);
+        hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY, 3>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
+                           (const float*)nullptr, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
+                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA3);
     else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
@@ -619,16 +421,12 @@ unsigned row_blocks(const LevelF& L) { return unsigned((int64_t(L.nny) * L.nnz +
 static void launch_fused(t3d_mixed* s, const float* pprev, float* pcur, float* Ap, int pid)
 {
     t3d_problem* p = s->p;
-    if (s->form == 2)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 2>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA2, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
-                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
-                           s->dx, pid);
-    else if (s->form == 3)
-        hipLaunchKernelGGL((k_kxm32<KW_NZ, true, 3>), kw_grid(p->g), dim3(256), 0, p->stream, p->g,
-                           (const unsigned short*)s->KA3, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed,
-                           pprev, (const float*)s->z, pcur, Ap, s->part, (const double*)s->sc, (const int*)s->iflags,
-                           s->dx, pid);
+    if (s->form == 2 || s->form == 3)
+        hipLaunchKernelGGL((s->form == 2 ? k_kxw32<KW_NZ, true, KW_CY, 2> : k_kxw32<KW_NZ, true, KW_CY, 3>),
+                           kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g, (const float*)nullptr,
+                           (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev, (const float*)s->z, pcur, Ap,
+                           s->part, (const double*)s->sc, (const int*)s->iflags, s->kwXcd, s->dx, pid,
+                           (const unsigned short*)(s->form == 2 ? s->KA2 : s->KA3));
     else
         hipLaunchKernelGGL((k_kxw32<KW_NZ, true, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, pprev,
@@ -710,26 +508,36 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
 
 // The fused operator kernel's memory streams as a plain copy (the bandwidth
 // this GPU gives that access mix without a stencil): reads u, z, dx and the
-// cell moduli, writes pcur = u, y = z and dx in place, one float4 of each
+// cell moduli, writes pcur = z + w u, y = z and dx += w u, one float4 of each
 // node vector per thread (3 nN / 4 of them; the moduli by the first ncell / 4
-// threads).
+// threads, added as w E).  w is a kernel argument (0 when timed), so no
+// access can be elided at compile time.
 __global__ void __launch_bounds__(256) k_copy_fused_streams(int64_t n4, int64_t e4, const float4* __restrict__ u,
                                                             const float4* __restrict__ z, float4* __restrict__ dx,
                                                             const float4* __restrict__ E, float4* __restrict__ pcur,
-                                                            float4* __restrict__ y)
+                                                            float4* __restrict__ y, float w)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n4) return;
-    float4 a = u[i];
-    const float4 b = z[i], c = dx[i];
-    if (i < e4) {   // + 0 * E keeps the modulus read; the copy's values are not used
+    const float4 a = u[i], b = z[i];
+    float4 c = dx[i];
+    float4 pv = b;
+    if (i < e4) {
         const float4 e = E[i];
-        a.x += 0.f * e.x;
-        a.y += 0.f * e.y;
-        a.z += 0.f * e.z;
-        a.w += 0.f * e.w;
+        pv.x = fmaf(w, e.x, pv.x);
+        pv.y = fmaf(w, e.y, pv.y);
+        pv.z = fmaf(w, e.z, pv.z);
+        pv.w = fmaf(w, e.w, pv.w);
     }
-    pcur[i] = a;
+    pv.x = fmaf(w, a.x, pv.x);
+    pv.y = fmaf(w, a.y, pv.y);
+    pv.z = fmaf(w, a.z, pv.z);
+    pv.w = fmaf(w, a.w, pv.w);
+    c.x = fmaf(w, a.x, c.x);
+    c.y = fmaf(w, a.y, c.y);
+    c.z = fmaf(w, a.z, c.z);
+    c.w = fmaf(w, a.w, c.w);
+    pcur[i] = pv;
     y[i] = b;
     dx[i] = c;
 }
@@ -896,7 +704,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
                           "t3d_mixed_create: element matrix does not have the Walsh-form structure");
     }
     const int64_t nD = 3 * p->g.nN;
-    const dim3 kg = kw_grid(p->g);
+    const dim3 kg = kww_grid(p->g);
     const LevelF L0 = levf(v, 0);
     // partials: the largest layout any caller uses (Walsh tiles down to 8 planes
     // deep, row kernels down to 2 rows per block, the vector kernels)
@@ -908,11 +716,12 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
         ntmp = std::max<int64_t>(ntmp, 3 * int64_t(Lc.nnx) * Lf.nny * Lf.nnz);
     }
-    // MFMA A operands: A[i][k] = Ke[3 b_i + c_i][3 b_k + c_k] for the
-    // component-major indices i = 8 c_i + b_i, k = 8 c_k + b_k (24x24 padded to
-    // 32x32), split into 2 or 3 bf16 pieces (round to nearest even), in the
-    // instruction's lane layout [K step][piece][lane][8]: lane l holds row l%32,
-    // columns 16 ks + 8 (l/32) + e
+    // MFMA A operands: A[i][k] = Ke[3 b_i + c_i][3 b_k + c_k] for the output
+    // row order of the MFMA form (mfma_row_c / mfma_row_b: each lane receives
+    // all eight nodes of its components) and the component-major inputs
+    // k = 8 c_k + b_k (24x24 padded to 32x32), split into 2 or 3 bf16 pieces
+    // (round to nearest even), in the instruction's lane layout
+    // [K step][piece][lane][8]: lane l holds row l%32, columns 16 ks + 8 (l/32) + e
     std::vector<unsigned short> KAh2(2 * 2 * 64 * 8), KAh3(2 * 3 * 64 * 8);
     {
         auto rne = [](float x) {
@@ -931,8 +740,9 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
                 for (int e = 0; e < 8; ++e) {
                     const int i = l % 32, k = 16 * ks + 8 * (l / 32) + e;
                     float a = 0.0f;
-                    if (i < 24 && k < 24) {
-                        const int ci = i / 8, bi = i % 8, ck = k / 8, bk = k % 8;
+                    const int ci = mfma_row_c(i), bi = mfma_row_b(i);
+                    if (ci >= 0 && k < 24) {
+                        const int ck = k / 8, bk = k % 8;
                         a = float(p->host.Ke[(3 * bi + ci) * 24 + 3 * bk + ck]);
                     }
                     const unsigned short h = rne(a);
@@ -1094,7 +904,7 @@ int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u, float* y, int reps, 
         else
             hipLaunchKernelGGL(t3d::k_copy_fused_streams, dim3(unsigned((n4 + 255) / 256)), dim3(256), 0, p->stream, n4,
                                p->g.ncell / 4, (const float4*)u, (const float4*)s->z, (float4*)s->dx,
-                               (const float4*)p->EcellF, (float4*)s->pB, (float4*)y);
+                               (const float4*)p->EcellF, (float4*)s->pB, (float4*)y, 0.0f);
     }
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipEventRecord(e1, p->stream));
@@ -1151,7 +961,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     double* S = s->sc;
     int* flags = s->iflags;
-    const dim3 kg = s->form == 0 ? kww_grid(g) : kw_grid(g);   // tiles of the fused operator kernel
+    const dim3 kg = kww_grid(g);   // tiles of the fused operator kernel
     const unsigned nfused = kg.x * kg.y * kg.z;
     const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
     const unsigned xrb = row_blocks(L0);

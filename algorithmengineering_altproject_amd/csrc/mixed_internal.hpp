@@ -77,7 +77,6 @@ inline t3d::LevelF levf(const t3d::MGView& v, int l)
 namespace t3d {
 constexpr int KW_NZ = 16;   // node planes per Walsh-form tile of the single-problem solver
 constexpr int KW_CY = 16;   // voxel rows per Walsh-form tile (workgroup of 32 x KW_CY threads)
-dim3 kw_grid(const Grid& g);    // tiles of the MFMA operators (32 x 8 voxel columns)
 dim3 kww_grid(const Grid& g);   // tiles of the Walsh-form operator (32 x KW_CY voxel columns)
 unsigned row_blocks(const LevelF& L);
 // z32 = M32(r) with rz and beta (mixed.hip); tmpReady: the finest restriction's

@@ -162,15 +162,64 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
 }
 
-// CYT: voxel rows of a tile (the workgroup is 32 x CYT threads)
-template <int NZ, bool FUSED, int CYT = wk::CY>
+// ---------------------------------------------------------------------------
+// MFMA form of the same operator (split bf16): a wavefront contracts two rows
+// of 32 voxel columns, F (32 x 32) = Ke (24 x 24, padded) U (24 x 32), on the
+// matrix cores as v_mfma_f32_32x32x16_bf16 with compensated products
+// accumulated in FP32:
+//   PIECES = 2 (bf16 x3): Ke = Kh + Kl, U = Uh + Ul (round to nearest even),
+//     F ~ Kh Uh + Kh Ul + Kl Uh;
+//   PIECES = 3 (bf16 x6): three pieces each, F ~ KhUh + KhUm + KmUh + KhUl +
+//     KmUm + KlUh (24-bit operands, FP32-level accuracy).
+// U is each voxel's corner values less their per-component mean (a rigid
+// translation, which Ke maps to zero), so the operand rounding follows the
+// variation of u over the voxel, not u itself.
+// Operand layouts of the instruction on gfx950: A[i][k]: lane l holds row
+// i = l % 32, columns k = 8 (l / 32) + e; B[k][j]: lane l holds k = 8 (l / 32) + e
+// of voxel column j = l % 32; D[i][j]: lane l register r holds row
+// i = 8 (r / 4) + 4 (l / 32) + r % 4 of column j = l % 32.  Input DOFs are
+// component-major (k = 8 c + b for local node b); output rows are ordered so
+// that each lane receives all eight nodes of its components: lanes l < 32
+// components 0 and 1 (registers 8 j + b, j = c), lanes l >= 32 component 2
+// (registers 0..7); rows beyond the 24 DOFs are zero.  Each lane therefore
+// keeps the top-face forces of its voxel in registers for the layer above,
+// as the Walsh form does.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+// node offset o(b) of local node b (fea.cpp node order); node 4 + m lies above node m
+__host__ __device__ constexpr int node_off_x(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__host__ __device__ constexpr int node_off_y(int b) { return ((b & 3) <= 1) ? 1 : 0; }
+
+// output DOF (component c, local node b) of MFMA row i, or c = -1 for a padding row
+__host__ __device__ constexpr int mfma_row_c(int i)
+{
+    return ((i / 4) % 2 == 0) ? (i / 8) / 2 : ((i / 8) < 2 ? 2 : -1);
+}
+__host__ __device__ constexpr int mfma_row_b(int i) { return 4 * ((i / 8) % 2) + i % 4; }
+
+// CYT: voxel rows of a tile (the workgroup is 32 x CYT threads).  PIECES = 0:
+// the Walsh form (KW); PIECES = 2 or 3: the MFMA form (KA: the element
+// matrix's bf16 pieces in the A layout, [K step][piece][lane][8]), which needs
+// CYT = 16 (eight wavefronts of two voxel rows each).
+template <int NZ, bool FUSED, int CYT = wk::CY, int PIECES = 0>
 __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW, const float* __restrict__ Ecell,
                                                const uint8_t* __restrict__ nodeFixed, const float* __restrict__ u,
                                                const float* __restrict__ z, float* __restrict__ pcur,
                                                float* __restrict__ y, double* __restrict__ partial,
                                                const double* __restrict__ S, const int* __restrict__ done,
                                                float* __restrict__ dxv, int pid,
-    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_)
+    unsigned bx_, unsigned by_, unsigned bz_, unsigned slot_, unsigned gdx_,
+    const unsigned short* __restrict__ KA = nullptr)
 {
     using wk::IN;
     using wk::NNZ;
@@ -181,9 +230,10 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     constexpr int NT = CX * CY;
     constexpr int SPT = (3 * PN + NT - 1) / NT;   // staging slots per thread
     constexpr int SPN = 3 * PN + 64;              // staged plane + 64 write-only dummy slots
+    static_assert(PIECES == 0 || CY == 2 * (NT / 64), "MFMA form: two voxel rows per wavefront");
     if (FUSED && *done) return;
     __shared__ float sP[3][SPN];       // three staged node planes, [buffer][component * PN + node]
-    __shared__ float sC[12][NT];       // face-corner forces [(2 oy + ox) * 3 + c][thread]
+    __shared__ float sC[12][NT];       // face-corner forces [(2 oy + ox) * 3 + c][voxel column]
     __shared__ double sh[NT / 64];
     const int tid = threadIdx.x;
     const int tx = tid % CX, ty = tid / CX;
@@ -194,10 +244,12 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     const float beta = FUSED ? float(S[S_BETA]) : 0.0f;
     const float alphaDx = FUSED ? float(S[S_ALPHA_DX]) : 0.0f;
     if (FUSED && slot_ == 0 && tid == 0) const_cast<double*>(S)[S_PLAST] = double(pid);
-    const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
+    const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column (Walsh form)
     const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
     const int X = x0 + tx, Y = y0 + ty;             // the node this thread gathers
     const bool ownNode = tx < NX && ty < NY && X < g.nnx && Y < g.nny;
+    // MFMA form: lane (jx, kh) of wavefront wv takes voxel columns (jx, 2 wv + nb)
+    const int lane = tid & 63, wv = tid >> 6, jx = lane & 31, kh = lane >> 5;
     // Node planes are staged in LDS; rows of PX nodes are 3*PX contiguous
     // floats in memory.  Each thread owns SPT slots of a staged plane, whose
     // byte offsets within a node plane do not depend on z: they are computed
@@ -228,7 +280,19 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     const int64_t plane = int64_t(3) * g.nnx * g.nny;
     const uint32_t planeB = uint32_t(4 * plane);
     const int64_t layer = int64_t(g.resX) * g.resY;
-    const uint32_t eoff = colIn ? uint32_t(4 * (cy * g.resX + cx)) : BUF_OOB;
+    // byte offsets of the moduli of the voxel columns this thread computes
+    // (Walsh: its own; MFMA: (jx, 2 wv) and (jx, 2 wv + 1))
+    uint32_t eoff[PIECES ? 2 : 1];
+    if constexpr (PIECES == 0) {
+        eoff[0] = colIn ? uint32_t(4 * (cy * g.resX + cx)) : BUF_OOB;
+    } else {
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int mx = x0 - 1 + jx, my = y0 - 1 + 2 * wv + nb;
+            eoff[nb] = (mx >= 0 && mx < g.resX && my >= 0 && my < g.resY) ? uint32_t(4 * (my * g.resX + mx)) : BUF_OOB;
+        }
+    }
+    constexpr int NE = PIECES ? 2 : 1;
     const uint32_t noff = ownNode ? uint32_t(Y * g.nnx + X) : BUF_OOB;
     auto plane_rsrc = [&](const float* base, int Z) {
         const bool zin = Z >= 0 && Z < g.nnz;
@@ -266,10 +330,12 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
             sq[slds[j]] = v;
         }
     };
-    // voxel modulus of layer cz (0 outside the grid), unclamped
-    auto modulus = [&](int cz) {
+    // voxel moduli of layer cz (0 outside the grid), unclamped
+    auto modulus = [&](int cz, float (&Ev)[NE]) {
         const bool in = cz >= 0 && cz < g.resZ;
-        return buf_ld(buf_rsrc(Ecell + (in ? cz * layer : 0), in ? uint32_t(4 * layer) : 0u), eoff);
+        const __amdgpu_buffer_rsrc_t re = buf_rsrc(Ecell + (in ? cz * layer : 0), in ? uint32_t(4 * layer) : 0u);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) Ev[k] = buf_ld(re, eoff[k]);
     };
     // fixed-DOF bits of this thread's node in plane Z
     auto fixed_bits = [&](int Z) {
@@ -286,6 +352,28 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
             walsh2(sP[q][c * PN + b], sP[q][c * PN + b + 1], sP[q][c * PN + b + PX], sP[q][c * PN + b + PX + 1],
                    F[c]);
     };
+    // MFMA form: the element matrix pieces (A operands) and, per voxel row
+    // nb and component slot j (j = c for lanes < 32, j = 0 <-> c = 2 above),
+    // the E-scaled top-face forces of the layer below
+    bf16x8 A[2][PIECES ? PIECES : 1];
+    float mcarry[2][2][4];
+    if constexpr (PIECES > 0) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int pc = 0; pc < PIECES; ++pc) {
+                u16x8 h;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = KA[((ks * PIECES + pc) * 64 + lane) * 8 + e];
+                A[ks][pc] = __builtin_bit_cast(bf16x8, h);
+            }
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) mcarry[nb][j][m] = 0.f;
+    }
     float Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -295,68 +383,129 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     commit(z0 - 1, 0);
     fetch(z0);
     commit(z0, 1);
-    float Enext = modulus(z0 - 1);
+    float Enext[NE];
+    modulus(z0 - 1, Enext);
     unsigned fxNext = 0u;
     __syncthreads();
-    face(0, Fp);
+    if constexpr (PIECES == 0) face(0, Fp);
     double dot = 0.0;
     const int nsteps = min(NZ, g.nnz - z0) + 1;
     for (int s = 0; s < nsteps; ++s) {
         const int qn = (s + 1) % 3, qo = s % 3;
         const bool more = s + 1 < nsteps;
         const int cz = z0 - 1 + s;
-        // voxel (cx, cy, cz) and the fixed bits of node (X, Y, cz), loaded last step
-        const float E = Enext <= 1.0e-16f ? 0.f : Enext;   // fea.cpp:743-744
+        // voxel layer cz and the fixed bits of node (X, Y, cz), loaded last step
+        float E[NE];
+#pragma unroll
+        for (int k = 0; k < NE; ++k) E[k] = Enext[k] <= 1.0e-16f ? 0.f : Enext[k];   // fea.cpp:743-744
         const unsigned fx = fxNext;
         if (more) {   // loads of the next step, in flight during this one
             fetch(z0 + s + 1);
-            Enext = modulus(cz + 1);
+            modulus(cz + 1, Enext);
             fxNext = fixed_bits(cz + 1);
         }
-        face(qn, Fn);
-        float uh[24], vh[24];
+        if constexpr (PIECES == 0) {
+            face(qn, Fn);
+            float uh[24], vh[24];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int my = 0; my < 2; ++my)
+                for (int my = 0; my < 2; ++my)
 #pragma unroll
-                for (int mx = 0; mx < 2; ++mx) {
-                    uh[(2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] + Fn[c][my][mx]);
-                    uh[(4 + 2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] - Fn[c][my][mx]);
+                    for (int mx = 0; mx < 2; ++mx) {
+                        uh[(2 * my + mx) * 3 + c] = E[0] * (Fp[c][my][mx] + Fn[c][my][mx]);
+                        uh[(4 + 2 * my + mx) * 3 + c] = E[0] * (Fp[c][my][mx] - Fn[c][my][mx]);
+                    }
+#pragma unroll
+            for (int i = 0; i < 24; ++i) vh[i] = 0.f;
+#pragma unroll
+            for (int k = 0; k < NNZ; ++k) vh[OUT[k]] = fmaf(KW[k], uh[IN[k]], vh[OUT[k]]);
+            // faces: bottom (oz = 0) = vh[mz=0] + vh[mz=1] closes plane cz; top carries to cz + 1
+            float A4[3][2][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int my = 0; my < 2; ++my)
+#pragma unroll
+                    for (int mx = 0; mx < 2; ++mx) {
+                        const float v0 = vh[(2 * my + mx) * 3 + c], v1 = vh[(4 + 2 * my + mx) * 3 + c];
+                        A4[c][my][mx] = carry[c][my][mx] + (v0 + v1);
+                        carry[c][my][mx] = v0 - v1;
+                    }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
+            if (s > 0) {   // corner forces of plane cz (plane z0 - 1 belongs to the block below)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float C4[2][2];
+                    walsh2(A4[c][0][0], A4[c][0][1], A4[c][1][0], A4[c][1][1], C4);
+                    // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
+                    sC[0 * 3 + c][tid] = C4[0][0];
+                    sC[1 * 3 + c][tid] = C4[0][1];
+                    sC[2 * 3 + c][tid] = C4[1][0];
+                    sC[3 * 3 + c][tid] = C4[1][1];
                 }
-#pragma unroll
-        for (int i = 0; i < 24; ++i) vh[i] = 0.f;
-#pragma unroll
-        for (int k = 0; k < NNZ; ++k) vh[OUT[k]] = fmaf(KW[k], uh[IN[k]], vh[OUT[k]]);
-        // faces: bottom (oz = 0) = vh[mz=0] + vh[mz=1] closes plane cz; top carries to cz + 1
-        float A[3][2][2];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int my = 0; my < 2; ++my)
-#pragma unroll
-                for (int mx = 0; mx < 2; ++mx) {
-                    const float v0 = vh[(2 * my + mx) * 3 + c], v1 = vh[(4 + 2 * my + mx) * 3 + c];
-                    A[c][my][mx] = carry[c][my][mx] + (v0 + v1);
-                    carry[c][my][mx] = v0 - v1;
-                }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
-        if (s > 0) {   // plane z0 - 1 belongs to the block below
-            // plane Z = cz: corner forces of this voxel column, then the node gather
-            const int Z = cz;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float C4[2][2];
-                walsh2(A[c][0][0], A[c][0][1], A[c][1][0], A[c][1][1], C4);
-                // walsh2 returns [my][mx] slots; as the inverse they are corners [oy][ox]
-                sC[0 * 3 + c][tid] = C4[0][0];
-                sC[1 * 3 + c][tid] = C4[0][1];
-                sC[2 * 3 + c][tid] = C4[1][0];
-                sC[3 * 3 + c][tid] = C4[1][1];
             }
+        } else {
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                const int tyv = 2 * wv + nb;   // voxel row of this MFMA tile
+                f32x16 acc = {};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    // component of this lane's 8 input DOFs: ks 0 -> kh, ks 1 -> 2 (lanes 32..63: zero padding)
+                    const int c = ks == 0 ? kh : 2;
+                    const bool live = ks == 0 || kh == 0;
+                    float uv[8];
+                    float mean = 0.f;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        uv[b] = live ? sP[(b >> 2) ? qn : qo][c * PN + (tyv + node_off_y(b)) * PX + jx + node_off_x(b)]
+                                     : 0.f;
+                        mean += uv[b];
+                    }
+                    mean *= 0.125f;
+                    u16x8 Bp[PIECES];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const float v = uv[b] - mean;
+                        const unsigned short h = bf16_rne(v);
+                        Bp[0][b] = h;
+                        const float r1 = v - bf16_val(h);
+                        const unsigned short m = bf16_rne(r1);
+                        Bp[1][b] = m;
+                        if (PIECES == 3) Bp[PIECES - 1][b] = bf16_rne(r1 - bf16_val(m));
+                    }
+                    const bf16x8 B0 = __builtin_bit_cast(bf16x8, Bp[0]), B1 = __builtin_bit_cast(bf16x8, Bp[1]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B0, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B1, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B0, acc, 0, 0, 0);
+                    if (PIECES == 3) {
+                        const bf16x8 B2 = __builtin_bit_cast(bf16x8, Bp[PIECES - 1]);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][0], B2, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][1], B1, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[ks][PIECES - 1], B0, acc, 0, 0, 0);
+                    }
+                }
+                // registers 8 j + m: bottom node m, 8 j + 4 + m: top node 4 + m, of
+                // component j (lanes < 32) or 2 (lanes >= 32, j = 0)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (kh == 1 && j == 1) continue;
+                    const int c = kh == 0 ? j : 2;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const int slot = (2 * node_off_y(m) + node_off_x(m)) * 3 + c;
+                        sC[slot][tyv * CX + jx] = fmaf(E[nb], acc[8 * j + m], mcarry[nb][j][m]);
+                        mcarry[nb][j][m] = E[nb] * acc[8 * j + 4 + m];
+                    }
+                }
+            }
+        }
+        if (s > 0) {   // plane Z = cz: the node gather (plane z0 - 1 belongs to the block below)
+            const int Z = cz;
             __syncthreads();
             if (ownNode) {
                 // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)

@@ -232,8 +232,8 @@ int t3d_mixed_set_operator(t3d_mixed* s, int form);
  * fused kernel of the selected operator (p = z + beta*u, y = K p, p.y partials);
  * op 5 / op 6: the Walsh-form fused kernel / operator in the other workgroup
  * tile order (hardware order vs XCD-grouped); op 7: the fused kernel's memory
- * streams as a plain float4 copy (reads u, z, the solver's dx and the cell
- * moduli, writes two node vectors and dx in place; 4 (18 nN + ncell) bytes),
+ * streams as a plain float4 streaming pass (reads u, z, the solver's dx and
+ * the cell moduli, writes two node vectors and dx; 4 (18 nN + ncell) bytes),
  * the bandwidth the same access mix reaches without the stencil */
 int t3d_mixed_time_op(t3d_mixed* s, int op, const float* u_full_dev, float* y_full_dev, int reps, float* ms);
 /* HIP-event timing of the solver's fused operator kernel inside t3d_pcg_mixed
