@@ -794,7 +794,7 @@ def main():
         },
         "roofline": {
             "kernel": ("fused FP32 Walsh-form K*u of the mixed solve (k_kxw32<16,true>)" if form == 0
-                       else "fused FP32 MFMA bf16 x%d K*u of the mixed solve (k_kxm32<16,true,%d>)"
+                       else "fused FP32 MFMA bf16 x%d K*u of the mixed solve (k_kxw32<16,true,16,%d>)"
                        % (3 if form == 2 else 6, form)),
             "bound": "hbm",
             "achieved": bfused / fused_solve_ms / 1e6,
