@@ -95,6 +95,7 @@ def lib():
         "t3d_mixed_destroy": (C.c_int, [vp]),
         "t3d_mixed_setup": (C.c_int, [vp]),
         "t3d_mixed_coarse_solve": (C.c_int, [vp, vp, vp]),
+        "t3d_mixed_vcycle": (C.c_int, [vp, vp, vp]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
         "t3d_batch_create": (C.c_int, [C.c_int, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
         "t3d_batch_create_voxels": (C.c_int, [C.c_int, vp, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
@@ -594,6 +595,10 @@ class Mixed:
 
     def setup(self):
         check(lib().t3d_mixed_setup(self.h), "t3d_mixed_setup")
+
+    def vcycle(self, dr, dz):
+        """z = M32(r): one FP32 V-cycle of the multigrid object's cycle type (float32 device arrays)."""
+        check(lib().t3d_mixed_vcycle(self.h, dr.ptr, dz.ptr), "t3d_mixed_vcycle")
 
     def coarse_solve(self, db, dx):
         """Coarsest-level solve of the mixed V-cycle (inverse Cholesky factor), FP64 device arrays."""

@@ -929,7 +929,7 @@ struct t3d_mg {
     double* Eeff = nullptr;   // level-1 effective moduli, [cells][span^3]
     int32_t* fixCells = nullptr;   // level-1 cells covering finest cells with fixed DOFs
     int nFix = 0;
-    int vtype = 0;            // Galerkin V-cycle: 0 'Adapted', 1 'Standard' (FP64 path)
+    int vtype = 0;            // Galerkin V-cycle: 0 'Adapted', 1 'Standard'
     double* ys[8] = {};       // 'Standard': A_l x and residuals per level l < L
     double* Li = nullptr;     // inverse coarsest factor L^-1
     double* Ainv = nullptr;   // Li^T Li: the coarsest inverse of the FP64 and mixed V-cycles (packed lower tiles)
@@ -1069,7 +1069,8 @@ struct MGView {
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
     const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
-    int vtype;           // 0 'Adapted', 1 'Standard' (FP64 path only)
+    int vtype;           // 0 'Adapted', 1 'Standard' (Galerkin form)
+    const double* Kel[8];   // Galerkin element matrices of levels >= 1 ([cells][576]), else null
 };
 int mg_view(const t3d_mg* g, MGView& v)
 {
@@ -1089,6 +1090,7 @@ int mg_view(const t3d_mg* g, MGView& v)
     v.norm = g->mode == 0 ? 1 : 0;
     v.Ainv = g->mode == 1 ? g->Ainv : nullptr;
     v.vtype = g->vtype;
+    for (int l = 0; l < 8; ++l) v.Kel[l] = (g->mode == 1 && l >= 1 && l < g->nlev) ? g->Kel[l] : nullptr;
     return T3D_OK;
 }
 }  // namespace t3d

@@ -311,15 +311,143 @@ __global__ void __launch_bounds__(256) k_rx_rows(int nnx, int NR, const float* _
     rx_rows_body<RPB>(nnx, NR, r, cnx, span, t, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
-template <int U>
+template <int U, int TW = 2, bool PARTIALS = true>
 __global__ void __launch_bounds__(256) k_pro_rows(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                   const float* __restrict__ rf, const float* __restrict__ Df,
                                                   const uint8_t* __restrict__ fmask, float weight,
                                                   float* __restrict__ xf, const float* __restrict__ Ap,
                                                   double* __restrict__ partial, const int* __restrict__ done)
 {
-    pro_rows_body<RPB, U>(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, Ap, partial, done, blockIdx.x, blockIdx.y,
-                          blockIdx.z, linear_block(), gridDim.x);
+    pro_rows_body<RPB, U, TW, PARTIALS>(Lf, Lc, span, xc, rf, Df, fmask, weight, xf, Ap, partial, done, blockIdx.x,
+                                        blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// MATLAB's 'Standard' V-cycle of the Galerkin form (TOP3D_XL.m:783-803) in
+// FP32, as the FP64 path's (mg.hip) with the mixed solver's arithmetic:
+//   down:  x_l = w r_l Dinv_l;  r_{l+1} = P^T (r_l - A_l x_l)
+//   up:    x_l += P x_{l+1};    x_l += w (r_l - A_l x_l) Dinv_l
+// every result masked; A_0 is the FP32 operator of the solver (fixed DOFs 0),
+// A_l (l >= 1) the level's Galerkin element matrices in FP32.
+// ---------------------------------------------------------------------------
+// op 0: x = w r Dinv;  op 1: x = r - y;  op 2: x += w (r - y) Dinv  (masked -> 0); one node per thread
+__global__ void __launch_bounds__(256) k_std32_point(int op, int64_t nN, const float* __restrict__ r,
+                                                     const float* y, const float* __restrict__ Dinv,
+                                                     const uint8_t* __restrict__ mask, float w, float* x,
+                                                     const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= nN) return;
+    const unsigned m = mask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int64_t i = 3 * n + c;
+        float out = 0.f;
+        if (!((m >> c) & 1u)) {
+            if (op == 0)
+                out = w * r[i] * Dinv[i];
+            else if (op == 1)
+                out = r[i] - y[i];
+            else
+                out = fmaf(w * (r[i] - y[i]), Dinv[i], x[i]);
+        }
+        x[i] = out;
+    }
+}
+
+// FP32 copy of a level's element matrices, entry-major: out[e * ncell + cell] =
+// in[cell * 576 + e] (row-major 24 x 24 per cell, local DOF 3a + c), so that
+// neighbouring nodes read neighbouring cells' entries
+__global__ void __launch_bounds__(256) k_kel32(int64_t ncell, const double* __restrict__ in, float* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 576 * ncell) return;
+    const int64_t e = i / ncell, c = i - e * ncell;
+    out[i] = float(in[c * 576 + e]);
+}
+
+// y = A_l x from the level's FP32 element matrices (entry-major, k_kel32): one
+// thread per node gathers the rows of its corner in each of the (up to) eight
+// cells around it (k_gal_apply's order)
+__global__ void __launch_bounds__(256) k_gal_apply32(LevelF L, const float* __restrict__ Kel,
+                                                     const float* __restrict__ x, float* __restrict__ y,
+                                                     const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= L.nN) return;
+    const int ix = int(n % L.nnx), iy = int((n / L.nnx) % L.nny), iz = int(n / (int64_t(L.nnx) * L.nny));
+    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int cx = ix - dx, cy = iy - dy, cz = iz - dz;
+                if (cx < 0 || cy < 0 || cz < 0 || cx >= L.resX || cy >= L.resY || cz >= L.resZ) continue;
+                const int a = 4 * dz + (dy ? (dx ? 1 : 0) : (dx ? 2 : 3));   // the node's corner in the cell
+                const int64_t nc = int64_t(L.resX) * L.resY * L.resZ;
+                const float* K = Kel + int64_t(3 * a * 24) * nc + (int64_t(cz) * L.resY + cy) * L.resX + cx;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int ox = node_off_x(b), oy = node_off_y(b), oz = b >> 2;
+                    const int64_t nb = (int64_t(cz + oz) * L.nny + (cy + oy)) * L.nnx + (cx + ox);
+                    const float u0 = x[3 * nb], u1 = x[3 * nb + 1], u2 = x[3 * nb + 2];
+                    const float* Kb = K + int64_t(3 * b) * nc;
+                    acc0 = fmaf(Kb[0], u0, fmaf(Kb[nc], u1, fmaf(Kb[2 * nc], u2, acc0)));
+                    acc1 = fmaf(Kb[24 * nc], u0, fmaf(Kb[25 * nc], u1, fmaf(Kb[26 * nc], u2, acc1)));
+                    acc2 = fmaf(Kb[48 * nc], u0, fmaf(Kb[49 * nc], u1, fmaf(Kb[50 * nc], u2, acc2)));
+                }
+            }
+    y[3 * n] = acc0;
+    y[3 * n + 1] = acc1;
+    y[3 * n + 2] = acc2;
+}
+
+// xf += P xc (trilinear, masked -> 0), one fine node per thread
+__global__ void __launch_bounds__(256) k_prolong_add32(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
+                                                       const uint8_t* __restrict__ fmask, float* __restrict__ xf,
+                                                       const int* __restrict__ done)
+{
+    if (done && *done) return;
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= Lf.nN) return;
+    const int fx = int(n % Lf.nnx), fy = int((n / Lf.nnx) % Lf.nny), fz = int(n / (int64_t(Lf.nnx) * Lf.nny));
+    const Prolong P(Lc, span, fx, fy, fz);
+    float v[3];
+    P.eval(Lc, xc, v);
+    const unsigned m = fmask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xf[3 * n + c] = ((m >> c) & 1u) ? 0.f : xf[3 * n + c] + v[c];
+}
+
+// the finest post-smoothing z += w (r - y) Dinv (masked -> 0) with the
+// partials of r.z and z.Ap that give beta (fixed grid, block b writes
+// partial[b] and partial[gridDim.x + b])
+__global__ void __launch_bounds__(256) k_std32_final(int64_t nN, const float* __restrict__ r,
+                                                     const float* __restrict__ y, const float* __restrict__ Dinv,
+                                                     const uint8_t* __restrict__ mask, float w, float* __restrict__ z,
+                                                     const float* __restrict__ Ap, double* __restrict__ partial,
+                                                     const int* __restrict__ done)
+{
+    if (done && *done) return;
+    __shared__ double sh[4];
+    double srz = 0.0, szap = 0.0;
+    for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < nN; n += int64_t(gridDim.x) * blockDim.x) {
+        const unsigned m = mask[n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = 3 * n + c;
+            const float ri = r[i];
+            const float out = ((m >> c) & 1u) ? 0.f : fmaf(w * (ri - y[i]), Dinv[i], z[i]);
+            z[i] = out;
+            srz = fma(double(ri), double(out), srz);
+            szap = fma(double(out), double(Ap[i]), szap);
+        }
+    }
+    const double a = block_sum_256(srz, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = a;
+    const double b = block_sum_256(szap, sh);
+    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = b;
 }
 
 __global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b)
@@ -391,21 +519,21 @@ dim3 kww_grid(const Grid& g)
 
 // y = K u in FP32; form 0: Walsh form, 1: node-gather form, 2: MFMA bf16 x3,
 // 3: MFMA bf16 x6
-static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y)
+static void launch_kx32(t3d_mixed* s, int form, const float* u, float* y, const int* done = nullptr)
 {
     t3d_problem* p = s->p;
     if (form == 2)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY, 2>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)nullptr, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
-                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA2);
+                           nullptr, y, nullptr, nullptr, done, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA2);
     else if (form == 3)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY, 3>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)nullptr, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
-                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA3);
+                           nullptr, y, nullptr, nullptr, done, s->kwXcd, nullptr, 0, (const unsigned short*)s->KA3);
     else if (form == 0)
         hipLaunchKernelGGL((k_kxw32<KW_NZ, false, KW_CY>), kww_grid(p->g), dim3(32 * KW_CY), 0, p->stream, p->g,
                            (const float*)s->KW, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
-                           nullptr, y, nullptr, nullptr, nullptr, s->kwXcd, nullptr, 0);
+                           nullptr, y, nullptr, nullptr, done, s->kwXcd, nullptr, 0);
     else
         hipLaunchKernelGGL((k_kx32<KX_NZ, false>), kx_grid(p->g, KX_NZ), dim3(256), 0, p->stream, p->g,
                            (const float*)s->Ke32, (const float*)p->EcellF, (const uint8_t*)p->nodeFixed, u, nullptr,
@@ -448,9 +576,88 @@ static void launch_fold(t3d_mixed* s, double* x, const int* done)
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
 // x-pass of the finest restriction is already in s->tmp (fused into the x, r
 // update)
+// z32 = M32(r) with MATLAB's 'Standard' V-cycle (the kernels above); rz and beta
+static int mixed_vcycle_std(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, const int* done)
+{
+    t3d_problem* p = s->p;
+    hipStream_t st = p->stream;
+    const int L = v.nlev - 1;
+    const bool chol = v.Nc > 0;
+    const float w = float(v.weight);
+    T3D_CHECK(L >= 1 && s->ys32[0], T3D_ERR_STATE, "mixed 'Standard' V-cycle: t3d_mixed_setup not called");
+    auto nodes = [](int64_t nN) { return dim3(unsigned((nN + 255) / 256)); };
+    // y = A_l x
+    auto apply = [&](int l, const float* x, float* y) {
+        if (l == 0)
+            launch_kx32(s, s->form, x, y, done);
+        else
+            hipLaunchKernelGGL(k_gal_apply32, nodes(v.nN[l]), dim3(256), 0, st, levf(v, l), (const float*)s->Kel32[l],
+                               x, y, done);
+    };
+    for (int l = 0; l < L; ++l) {
+        const bool last = (l + 1 == L);
+        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
+        const int span = v.span[l + 1];
+        const float* rl = l == 0 ? r : s->r32[l];
+        float* xl = l == 0 ? s->z : s->x32[l];
+        float* yl = s->ys32[l];
+        hipLaunchKernelGGL(k_std32_point, nodes(Lf.nN), dim3(256), 0, st, 0, Lf.nN, rl, (const float*)nullptr,
+                           (const float*)s->D32[l], v.mask[l], w, xl, done);
+        apply(l, xl, yl);
+        hipLaunchKernelGGL(k_std32_point, nodes(Lf.nN), dim3(256), 0, st, 1, Lf.nN, rl, (const float*)yl,
+                           (const float*)nullptr, v.mask[l], w, yl, done);
+        // r_{l+1} = P^T (r_l - A_l x_l)
+        if (l == 0) {
+            hipLaunchKernelGGL(k_rx_rows, dim3(row_blocks(Lf)), dim3(256), sizeof(float) * RPB * 3 * Lf.nnx, st,
+                               Lf.nnx, Lf.nny * Lf.nnz, (const float*)yl, Lc.nnx, span, s->tmp, done);
+        } else {
+            const dim3 g1(unsigned((Lc.nnx + 63) / 64), unsigned(Lf.nny), unsigned(Lf.nnz));
+            hipLaunchKernelGGL(k_restrict_x<float>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, (const float*)yl, s->tmp,
+                               done);
+        }
+        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
+        hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
+                           s->r32[l + 1], (last && !chol) ? (const float*)s->D32[L] : nullptr, s->x32[L],
+                           (last && chol) ? s->rc64 : nullptr, done, v.norm);
+        T3D_HIP(hipGetLastError());
+    }
+    if (chol) {
+        T3D_CHECK(s->NcInv == v.Nc, T3D_ERR_STATE, "mixed V-cycle: t3d_mixed_setup not called after t3d_mg_setup");
+        launch_coarse(s, v.mask[L], s->x32[L], done);
+    }
+    for (int l = L - 1; l >= 1; --l) {
+        const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
+        hipLaunchKernelGGL(k_prolong_add32, nodes(Lf.nN), dim3(256), 0, st, Lf, Lc, v.span[l + 1],
+                           (const float*)s->x32[l + 1], v.mask[l], s->x32[l], done);
+        apply(l, s->x32[l], s->ys32[l]);
+        hipLaunchKernelGGL(k_std32_point, nodes(Lf.nN), dim3(256), 0, st, 2, Lf.nN, (const float*)s->r32[l],
+                           (const float*)s->ys32[l], (const float*)s->D32[l], v.mask[l], w, s->x32[l], done);
+    }
+    // finest: z = w r Dinv + P x_1 (row kernel, the pre-smoothed value recomputed), then the post-smoothing
+    const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
+    const unsigned nb = row_blocks(L0);
+    if (3 * L0.nnx <= 256 * 5)
+        hipLaunchKernelGGL((k_pro_rows<5, 1, false>), dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0,
+                           L1, v.span[1], (const float*)s->x32[1], r, (const float*)s->D32[0], v.mask[0], w, s->z,
+                           (const float*)nullptr, (double*)nullptr, done);
+    else
+        hipLaunchKernelGGL((k_pro_rows<8, 1, false>), dim3(nb), dim3(256), sizeof(float) * RPB * 3 * L1.nnx, st, L0,
+                           L1, v.span[1], (const float*)s->x32[1], r, (const float*)s->D32[0], v.mask[0], w, s->z,
+                           (const float*)nullptr, (double*)nullptr, done);
+    apply(0, s->z, s->ys32[0]);
+    const unsigned fb = unsigned(std::min<int64_t>(VB, (L0.nN + 255) / 256));
+    hipLaunchKernelGGL(k_std32_final, dim3(fb), dim3(256), 0, st, L0.nN, r, (const float*)s->ys32[0],
+                       (const float*)s->D32[0], v.mask[0], w, s->z, (const float*)s->Ap, s->part, done);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, st, (const double*)s->part, fb, s->sc, flexible, first,
+                       done);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
 int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
                         const int* done)
 {
+    if (v.vtype == 1) return mixed_vcycle_std(s, v, r, flexible, first, done);
     t3d_problem* p = s->p;
     hipStream_t st = p->stream;
     const int L = v.nlev - 1;
@@ -641,6 +848,8 @@ int t3d_mixed_destroy(t3d_mixed* s)
         if (s->D32[l]) (void)hipFree(s->D32[l]);
         if (s->r32[l]) (void)hipFree(s->r32[l]);
         if (s->x32[l]) (void)hipFree(s->x32[l]);
+        if (s->Kel32[l]) (void)hipFree(s->Kel32[l]);
+        if (s->ys32[l]) (void)hipFree(s->ys32[l]);
     }
     delete s;
     return T3D_OK;
@@ -710,7 +919,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     // deep, row kernels down to 2 rows per block, the vector kernels)
     const int64_t kwMax = int64_t(kg.x) * kg.y * ((p->g.nnz + 7) / 8);
     const int64_t rowsMax = (int64_t(L0.nny) * L0.nnz + 1) / 2;
-    const int64_t npart = std::max<int64_t>({kwMax, 2 * rowsMax, VB}) + 16;
+    const int64_t npart = std::max<int64_t>({kwMax, 2 * rowsMax, 2 * int64_t(VB)}) + 16;
     int64_t ntmp = 0;
     for (int l = 0; l + 1 < v.nlev; ++l) {
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
@@ -795,9 +1004,33 @@ int t3d_mixed_setup(t3d_mixed* s)
     MGView v;
     int rc = mg_view(s->M, v);
     if (rc) return rc;
-    T3D_CHECK(v.vtype == 0, T3D_ERR_STATE,
-              "t3d_mixed_setup: the 'Standard' V-cycle runs in the FP64 path only (t3d_pcg, t3d_mg_apply)");
     hipStream_t st = s->p->stream;
+    if (v.vtype == 1) {   // 'Standard' V-cycle: FP32 element matrices and per-level work vectors
+        const int L = v.nlev - 1;
+        for (int l = 0; l < L; ++l) {
+            const size_t ny = size_t(3) * size_t(v.nN[l]);
+            if (s->ys32n[l] < ny) {
+                if (s->ys32[l]) (void)hipFree(s->ys32[l]);
+                s->ys32[l] = nullptr;
+                s->ys32n[l] = 0;
+                T3D_HIP(hipMalloc(&s->ys32[l], sizeof(float) * ny));
+                s->ys32n[l] = ny;
+            }
+            if (l == 0) continue;
+            T3D_CHECK(v.Kel[l], T3D_ERR_STATE, "t3d_mixed_setup: the 'Standard' V-cycle needs the Galerkin form");
+            const size_t nk = size_t(576) * size_t(v.resX[l]) * size_t(v.resY[l]) * size_t(v.resZ[l]);
+            if (s->Kel32n[l] < nk) {
+                if (s->Kel32[l]) (void)hipFree(s->Kel32[l]);
+                s->Kel32[l] = nullptr;
+                s->Kel32n[l] = 0;
+                T3D_HIP(hipMalloc(&s->Kel32[l], sizeof(float) * nk));
+                s->Kel32n[l] = nk;
+            }
+            hipLaunchKernelGGL(k_kel32, dim3(unsigned((nk + 255) / 256)), dim3(256), 0, st, int64_t(nk / 576),
+                               v.Kel[l], s->Kel32[l]);
+            T3D_HIP(hipGetLastError());
+        }
+    }
     for (int l = 0; l < v.nlev; ++l) {
         const int64_t n = 3 * v.nN[l];
         hipLaunchKernelGGL(k_dinv, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, n, v.D[l], s->D32[l]);
@@ -833,6 +1066,23 @@ int t3d_mixed_setup(t3d_mixed* s)
         T3D_HIP(hipMalloc(&s->cP, sizeof(double) * coarse_partial_doubles(s->NcInv)));
         s->cPn = coarse_partial_doubles(s->NcInv);
     }
+    T3D_HIP(hipStreamSynchronize(st));
+    return T3D_OK;
+}
+
+// z = M32(r): one application of the mixed solver's FP32 V-cycle (the
+// multigrid object's cycle type) to an FP32 full-layout residual
+int t3d_mixed_vcycle(t3d_mixed* s, const float* r, float* z)
+{
+    T3D_CHECK(s && r && z, T3D_ERR_ARG, "t3d_mixed_vcycle: null argument");
+    MGView v;
+    int rc = mg_view(s->M, v);
+    if (rc) return rc;
+    hipStream_t st = s->p->stream;
+    hipLaunchKernelGGL(k_mx_clear_done, dim3(1), dim3(1), 0, st, s->iflags);
+    T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * 3 * size_t(s->p->g.nN), st));   // the beta partials read it
+    if ((rc = mixed_vcycle(s, v, r, 0, 1, false, s->iflags))) return rc;
+    T3D_HIP(hipMemcpyAsync(z, s->z, sizeof(float) * 3 * size_t(s->p->g.nN), hipMemcpyDeviceToDevice, st));
     T3D_HIP(hipStreamSynchronize(st));
     return T3D_OK;
 }
@@ -965,6 +1215,8 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     const unsigned nfused = kg.x * kg.y * kg.z;
     const LevelF L0 = levf(v, 0), L1 = levf(v, 1);
     const unsigned xrb = row_blocks(L0);
+    // the 'Standard' V-cycle restricts r - A_0 x_0, not r: the x/r update then skips the restriction's x-pass
+    const bool stdCycle = v.vtype == 1;
     // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
     if ((rc = canon_dot_dev(p, b, b, n, S + S_NORMB2))) return rc;
     hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(1), 0, st, tol, S, flags);
@@ -999,7 +1251,8 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             // dx += alpha p ; r -= alpha Ap ; rn2 ; convergence test
             hipLaunchKernelGGL(k_xr_rows, dim3(xrb), dim3(256), sizeof(float) * RPB * 3 * L0.nnx, st, L0.nnx,
                                L0.nny * L0.nnz, (const float*)s->Ap, r, s->part,
-                               (const double*)S, L1.nnx, v.span[1], replace ? nullptr : s->tmp, (const int*)flags);
+                               (const double*)S, L1.nnx, v.span[1], (replace || stdCycle) ? nullptr : s->tmp,
+                               (const int*)flags);
             hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, xrb, S, flags, it,
                                replace ? 0 : 1, 1);
             if (replace) {
@@ -1013,7 +1266,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
                                    it, 1, 1);
             }
             // z = M r ; beta ; rz
-            if ((rc = mixed_vcycle(s, v, r, flexible, 0, !replace, flags))) return rc;
+            if ((rc = mixed_vcycle(s, v, r, flexible, 0, !replace && !stdCycle, flags))) return rc;
         }
         double rn2Now = 0.0;
         T3D_HIP(hipMemcpyAsync(s->hostFlags, flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));

@@ -25,6 +25,13 @@ struct t3d_mixed {
     bool ownAinv = true;       // false: Ainv is the multigrid object's (Galerkin form)
     double* cP = nullptr;      // partial products of the packed coarsest product
     size_t cPn = 0;            // doubles allocated at cP
+    // 'Standard' V-cycle: FP32 Galerkin element matrices of levels 1 .. L-1
+    // (entry-major: entry e of cell c at e * ncell + c) and per-level work
+    // vectors A_l x_l / residuals (levels 0 .. L-1)
+    float* Kel32[8] = {};
+    size_t Kel32n[8] = {};
+    float* ys32[8] = {};
+    size_t ys32n[8] = {};
     double* yc64 = nullptr;    // Li * rc64
     int NcInv = 0;             // size of Li (0: diagonal coarsest level)
     float *pA = nullptr, *pB = nullptr, *Ap = nullptr, *z = nullptr;
@@ -57,7 +64,8 @@ struct MGView {
     double weight;
     int norm;   // 1: the reference's normalised restriction; 0: the Galerkin form's P^T
     const double* Ainv;  // coarsest inverse Li^T Li when the multigrid keeps it
-    int vtype;           // 0 'Adapted', 1 'Standard' (FP64 path only)
+    int vtype;           // 0 'Adapted', 1 'Standard' (Galerkin form)
+    const double* Kel[8];   // Galerkin element matrices of levels >= 1 ([cells][576]), else null
 };
 int mg_view(const t3d_mg* g, MGView& v);
 __global__ void k_chol_solve(int N, const double* __restrict__ L, const double* __restrict__ LT,

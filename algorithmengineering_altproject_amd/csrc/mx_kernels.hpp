@@ -231,7 +231,7 @@ __device__ __forceinline__ void kxw32_body(Grid g, const float* __restrict__ KW,
     constexpr int SPT = (3 * PN + NT - 1) / NT;   // staging slots per thread
     constexpr int SPN = 3 * PN + 64;              // staged plane + 64 write-only dummy slots
     static_assert(PIECES == 0 || CY == 2 * (NT / 64), "MFMA form: two voxel rows per wavefront");
-    if (FUSED && *done) return;
+    if (done && *done) return;   // the fused form always has a flag; the operator has one inside a solve
     __shared__ float sP[3][SPN];       // three staged node planes, [buffer][component * PN + node]
     __shared__ float sC[12][NT];       // face-corner forces [(2 oy + ox) * 3 + c][voxel column]
     __shared__ double sh[NT / 64];
@@ -900,7 +900,11 @@ __device__ __forceinline__ void rx_rows_body(int nnx, int NR, const float* __res
 // finest level: z = mask ? 0 : 2t + P(x_c), t = w r/D,
 // with the partials of r.z and z.Ap (two per block).  The y/z part of the
 // trilinear prolongation is evaluated once per row and coarse x in LDS.
-template <int R, int U = 8>
+// TW: weight of the smoothing term t (2: the 'Adapted' cycle's pre- and
+// post-smoothing from the same residual; 1: the pre-smoothed value of the
+// 'Standard' cycle, whose post-smoothing follows with a fresh residual);
+// PARTIALS: also the r.z and z.Ap partials
+template <int R, int U = 8, int TW = 2, bool PARTIALS = true>
 __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, const float* __restrict__ xc,
                                                   const float* __restrict__ rf, const float* __restrict__ Df,
                                                   const uint8_t* __restrict__ fmask, float weight,
@@ -973,7 +977,7 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
                 const uint32_t off = 4u * uint32_t(k0 + int(threadIdx.x) + u * 256);
                 rv[u] = buf_ld(rR, off);
                 dv[u] = buf_ld(rD, off);
-                av[u] = buf_ld(rA, off);
+                av[u] = PARTIALS ? buf_ld(rA, off) : 0.f;
                 mv[u] = __builtin_amdgcn_raw_buffer_load_b8(rM, xs[u], 0, 0);
             }
 #pragma unroll
@@ -981,18 +985,21 @@ __device__ __forceinline__ void pro_rows_body(LevelF Lf, LevelF Lc, int span, co
                 const uint32_t off = 4u * uint32_t(k0 + int(threadIdx.x) + u * 256);
                 const float v = (1.f - w1[u]) * q[qo[u]] + w1[u] * q[qo[u] + 3];
                 float out = 0.f;
-                if (!((mv[u] >> cc[u]) & 1u)) out = 2.f * (weight * rv[u] * dv[u]) + v;
+                if (!((mv[u] >> cc[u]) & 1u)) out = float(TW) * (weight * rv[u] * dv[u]) + v;
                 buf_st(out, rX, off);
-                // beyond the row rv = av = 0: the partials gain exact zeros
-                srz = fma(double(rv[u]), double(out), srz);
-                szap = fma(double(out), double(av[u]), szap);
+                if (PARTIALS) {   // beyond the row rv = av = 0: the partials gain exact zeros
+                    srz = fma(double(rv[u]), double(out), srz);
+                    szap = fma(double(out), double(av[u]), szap);
+                }
             }
         }
     }
-    const double a = block_sum_256(srz, sh);
-    if (threadIdx.x == 0) partial[bx_] = a;
-    const double b = block_sum_256(szap, sh);
-    if (threadIdx.x == 0) partial[gdx_ + bx_] = b;
+    if (PARTIALS) {
+        const double a = block_sum_256(srz, sh);
+        if (threadIdx.x == 0) partial[bx_] = a;
+        const double b = block_sum_256(szap, sh);
+        if (threadIdx.x == 0) partial[gdx_ + bx_] = b;
+    }
 }
 
 // rz_new = r.z, zAp = z.Ap -> beta (0 on the first V-cycle of a solve); rz = rz_new

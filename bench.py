@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--mg", choices=["galerkin", "reference"], default="galerkin",
                     help="multigrid of the timed mixed solves: the Galerkin hierarchy of the MATLAB original "
                          "(default) or the C++ port's additive V-cycle")
+    ap.add_argument("--vcycle", choices=["adapted", "standard"], default="adapted",
+                    help="V-cycle of the Galerkin multigrid: the 'Adapted' cycle (default) or MATLAB's 'Standard' "
+                         "cycle (residuals recomputed with the level operators; FP32 levels in the mixed solver)")
     ap.add_argument("--tol", type=float, default=None,
                     help="tolerance of the timed solves (default 1e-6 with --mg galerkin, else --ref-tol)")
     ap.add_argument("--ref-tol", type=float, default=float(np.float32(1e-3)),
@@ -562,7 +565,7 @@ def main():
     gal_setup_s = gal_resetup_s = None
     if galerkin:
         t0 = time.time()
-        Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True)
+        Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True, standard=args.vcycle == "standard")
         Mg.setup(dE)
         mxg = t3d.Mixed(p, Mg)
         mxg.setup()
@@ -663,16 +666,17 @@ def main():
                      "compliance_rel_diff_mixed": abs(Cm - C64g) / abs(C64g)}
         log("FP64 PCG (Galerkin MG): %d iterations, %.1f ms/solve, compliance %.9e, true residual %.3e; "
             "mixed speed-up %.2fx" % (it64g, ms64g, C64g, res64g, ms64g / ms_step))
-        # the same hierarchy with MATLAB's 'Standard' V-cycle (FP64 path only)
-        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 1), "t3d_mg_set_vcycle")
+        # the same hierarchy with the other V-cycle (FP64)
+        other = "adapted" if args.vcycle == "standard" else "standard"
+        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 1 if other == "standard" else 0), "t3d_mg_set_vcycle")
         it64s, ms64s = fp64_solve(t3d, p, Mg, db, dsol, tol, args.maxit)
         C64s = comp()
         res64s = true_residual(t3d, p, db, dsol)
-        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 0), "t3d_mg_set_vcycle")
-        fp64_same["standard_vcycle"] = {"ms_per_solve": ms64s, "pcg_iters": it64s, "ms_per_iteration": ms64s / it64s,
+        t3d.check(t3d.lib().t3d_mg_set_vcycle(Mg.h, 1 if other == "adapted" else 0), "t3d_mg_set_vcycle")
+        fp64_same[other + "_vcycle"] = {"ms_per_solve": ms64s, "pcg_iters": it64s, "ms_per_iteration": ms64s / it64s,
                                         "compliance_rel_err": rel(C64s), "true_residual": res64s}
-        log("FP64 PCG (Galerkin MG, 'Standard' V-cycle): %d iterations, %.1f ms/solve, %.3f ms/iteration, "
-            "true residual %.3e" % (it64s, ms64s, ms64s / it64s, res64s))
+        log("FP64 PCG (Galerkin MG, '%s' V-cycle): %d iterations, %.1f ms/solve, %.3f ms/iteration, "
+            "true residual %.3e" % (other, it64s, ms64s, ms64s / it64s, res64s))
 
     fp64 = None
     if not args.no_fp64:   # the FP64 plain path: the reference's algorithm, bit-exact, at the reference's tol
@@ -743,8 +747,8 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     opname = {0: "Walsh-form", 2: "MFMA bf16 x3", 3: "MFMA bf16 x6"}[form]
-    mgname = ("Galerkin multigrid of the MATLAB original ('Adapted' V-cycle, P^T restriction, coarsest Cholesky) "
-              "on the C++ port's levels" if galerkin else "the C++ port's additive V-cycle")
+    mgname = ("Galerkin multigrid of the MATLAB original ('%s' V-cycle, P^T restriction, coarsest Cholesky) "
+              "on the C++ port's levels" % args.vcycle.capitalize() if galerkin else "the C++ port's additive V-cycle")
     line = {
         "metric": METRIC,
         "value": 1e3 / ms_step,

@@ -174,8 +174,9 @@ int t3d_mg_set_mode(t3d_mg* g, int mode);
  * restriction of the residual itself, one Jacobi correction), 1 'Standard'
  * (TOP3D_XL.m:783-803: the residual recomputed with the level operator before
  * restriction and after prolongation, A_l applied from the level's element
- * matrices).  'Standard' runs in the FP64 path (t3d_pcg, t3d_mg_apply); the
- * mixed and batched solvers refuse it (T3D_ERR_STATE).  Needs mode 1. */
+ * matrices).  'Standard' runs in the FP64 path (t3d_pcg, t3d_mg_apply) and in
+ * the mixed solver (FP32 levels; set it before t3d_mixed_setup); the batched
+ * solver refuses it (T3D_ERR_STATE).  Needs mode 1. */
 int t3d_mg_set_vcycle(t3d_mg* g, int type);
 
 /* ---- PCG ----
@@ -205,6 +206,12 @@ int t3d_mixed_setup(t3d_mixed* s);
  * chol_solve_lower (cpp/src/multigrid/coarsest.cpp:92-122) through Li.
  * T3D_ERR_STATE when the coarsest level has no Cholesky factor. */
 int t3d_mixed_coarse_solve(t3d_mixed* s, const double* b_dev, double* x_dev);
+/* z = M32(r): one application of the mixed solver's FP32 V-cycle (the
+ * multigrid object's cycle: 'Adapted' or 'Standard') to an FP32 full-layout
+ * residual (device arrays of 3 nN floats); the counterpart of the
+ * preconditioner closure (cpp/src/multigrid/preconditioner.cpp:88-265) in the
+ * mixed solver's arithmetic */
+int t3d_mixed_vcycle(t3d_mixed* s, const float* r_dev, float* z_dev);
 /* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
  * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old.
  * Convergence is reported only when the true residual ||b - K x|| (FP64
