@@ -56,6 +56,12 @@ def parse():
     ap.add_argument("--vcycle", choices=["adapted", "standard"], default="adapted",
                     help="V-cycle of the Galerkin multigrid: the 'Adapted' cycle (default) or MATLAB's 'Standard' "
                          "cycle (residuals recomputed with the level operators; FP32 levels in the mixed solver)")
+    ap.add_argument("--non-dyadic", type=int, choices=[0, 1], default=1,
+                    help="hierarchy of the Galerkin multigrid: first coarsening by 4 (1, the reference driver's and "
+                         "the MATLAB original's default) or by 2 at every level (0)")
+    ap.add_argument("--levels", type=int, default=5, help="multigrid levels of the Galerkin hierarchy (maxLevels)")
+    ap.add_argument("--no-dyadic-check", action="store_true",
+                    help="skip the comparison solve on the hierarchy coarsened by 2 at every level (6 levels)")
     ap.add_argument("--tol", type=float, default=None,
                     help="tolerance of the timed solves (default 1e-6 with --mg galerkin, else --ref-tol)")
     ap.add_argument("--ref-tol", type=float, default=float(np.float32(1e-3)),
@@ -565,7 +571,8 @@ def main():
     gal_setup_s = gal_resetup_s = None
     if galerkin:
         t0 = time.time()
-        Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True, standard=args.vcycle == "standard")
+        Mg = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True, standard=args.vcycle == "standard",
+                    nonDyadic=args.non_dyadic, maxLevels=args.levels)
         Mg.setup(dE)
         mxg = t3d.Mixed(p, Mg)
         mxg.setup()
@@ -678,6 +685,32 @@ def main():
         log("FP64 PCG (Galerkin MG, '%s' V-cycle): %d iterations, %.1f ms/solve, %.3f ms/iteration, "
             "true residual %.3e" % (other, it64s, ms64s, ms64s / it64s, res64s))
 
+    dyadic = None
+    if galerkin and not args.no_fp64 and not args.no_dyadic_check and args.non_dyadic == 1:
+        # the same solve on the hierarchy coarsened by 2 at every level (the MATLAB original's nonDyadic_ = 0),
+        # 6 levels down to the same coarsest grid: the mixed solver and the FP64 solver with that multigrid
+        t0 = time.time()
+        Md = t3d.MG(p, Emin=float(np.float32(1e-9)), galerkin=True, standard=args.vcycle == "standard",
+                    nonDyadic=0, maxLevels=args.levels + 1)
+        Md.setup(dE)
+        mxd = t3d.Mixed(p, Md)
+        mxd.setup()
+        mxd.set_operator(form)
+        setup_d = time.time() - t0
+        itd, msd = timed_solves(t3d, mxd, db, dsol, tol, args.maxit, te, 1, args.steps)
+        Cd = comp()
+        resd = true_residual(t3d, p, db, dsol)
+        it64d, ms64d = fp64_solve(t3d, p, Md, db, dsol, tol, args.maxit)
+        C64d = comp()
+        dyadic = {"levels": [list(d) for d in Md.dims], "coarsest_dofs": Md.info()[0], "setup_s": setup_d,
+                  "mixed": {"solves_per_s": 1e3 / msd, "ms_per_solve": msd, "pcg_iters": itd,
+                            "ms_per_iteration": msd / itd, "compliance_rel_err": rel(Cd), "true_residual": resd},
+                  "fp64": {"ms_per_solve": ms64d, "pcg_iters": it64d, "compliance_rel_err": rel(C64d)},
+                  "speedup_mixed_vs_fp64": ms64d / msd}
+        log("Galerkin MG coarsened by 2 at every level (%d levels): mixed %d iterations %.1f ms/solve "
+            "(%.3f ms/iteration), FP64 %d iterations %.1f ms/solve" % (Md.nlev, itd, msd, msd / itd, it64d, ms64d))
+        del mxd, Md
+
     fp64 = None
     if not args.no_fp64:   # the FP64 plain path: the reference's algorithm, bit-exact, at the reference's tol
         it64, ms64 = fp64_solve(t3d, p, M, db, dsol, args.ref_tol, args.maxit)
@@ -747,8 +780,10 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
     opname = {0: "Walsh-form", 2: "MFMA bf16 x3", 3: "MFMA bf16 x6"}[form]
+    levels = ("the C++ port's levels (%d)" % Mg.nlev if args.non_dyadic == 1
+              else "%d levels coarsened by 2" % Mg.nlev) if galerkin else ""
     mgname = ("Galerkin multigrid of the MATLAB original ('%s' V-cycle, P^T restriction, coarsest Cholesky) "
-              "on the C++ port's levels" % args.vcycle.capitalize() if galerkin else "the C++ port's additive V-cycle")
+              "on %s" % (args.vcycle.capitalize(), levels) if galerkin else "the C++ port's additive V-cycle")
     line = {
         "metric": METRIC,
         "value": 1e3 / ms_step,
@@ -783,6 +818,8 @@ def main():
             "galerkin_setup_s": gal_setup_s,
             "galerkin_resetup_s": gal_resetup_s,
             "fp64_same_multigrid": fp64_same,
+            "hierarchy": {"nonDyadic": args.non_dyadic, "levels": [list(d) for d in Mg.dims]} if galerkin else None,
+            "galerkin_dyadic_hierarchy": dyadic,
             "fp64_plain": fp64,
             "fp64_plain_96x192x96": small,
             "mixed_reference_vcycle": mixed_ref,

@@ -415,6 +415,7 @@ def ref(fast=False):
         return _ref[key]
     R = C.CDLL(ref_path(fast))
     ii = [C.c_int, C.c_int, C.c_int]
+    R.ref_set_hierarchy.argtypes = [C.c_int, C.c_int]
     R.ref_sizes.argtypes = ii + [i32p]
     R.ref_model.argtypes = ii + [i32p, i32p, i32p, f64p, f64p, u8p, i32p, i32p]
     R.ref_bfree.argtypes = ii + [f64p]
@@ -453,12 +454,16 @@ def ref(fast=False):
 
 
 class RefModel:
-    """The reference's own model arrays (via oracle/_ref)."""
+    """The reference's own model arrays (via oracle/_ref); the multigrid
+    functions use the hierarchy (nonDyadic, maxLevels) of MGPrecondConfig
+    (multigrid.hpp:9-13), by default the reference driver's (1, 5)."""
 
-    def __init__(self, nely, nelx, nelz, fast=False):
+    def __init__(self, nely, nelx, nelz, fast=False, nonDyadic=1, maxLevels=5):
         R = ref(fast)
         self.R = R
         self.dims = (nely, nelx, nelz)
+        self.hier = (int(nonDyadic), int(maxLevels))
+        self._sel()
         s = np.zeros(7, np.int32)
         R.ref_sizes(nely, nelx, nelz, s)
         self.resX, self.resY, self.resZ, self.ne, self.nN, self.nF, self.nlev = [int(v) for v in s]
@@ -476,6 +481,9 @@ class RefModel:
         self.bFree = np.zeros(nF)
         R.ref_bfree(nely, nelx, nelz, self.bFree)
 
+    def _sel(self):
+        self.R.ref_set_hierarchy(*self.hier)
+
     def kxu(self, E, u):
         y = np.zeros(self.nF)
         self.R.ref_kxu(*self.dims, np.ascontiguousarray(E, np.float64), np.ascontiguousarray(u, np.float64), y)
@@ -488,21 +496,25 @@ class RefModel:
         return C_, ce
 
     def levels(self):
+        self._sel()
         out = np.zeros(4 * 8, np.int32)
         n = self.R.ref_mg_levels(*self.dims, out)
         return [tuple(int(v) for v in out[4 * l:4 * l + 4]) for l in range(n)]
 
     def mask(self, level, n_nodes):
+        self._sel()
         out = np.zeros(3 * n_nodes, np.uint8)
         self.R.ref_mg_mask(*self.dims, level, out)
         return out
 
     def diag(self, E, level, n_nodes):
+        self._sel()
         out = np.zeros(3 * n_nodes)
         self.R.ref_mg_diag(*self.dims, np.ascontiguousarray(E, np.float64), level, out)
         return out
 
     def coarsest(self, E, Emin, N):
+        self._sel()
         Kc = np.zeros(N * N)
         Lc = np.zeros(N * N)
         ok = C.c_int()
@@ -510,22 +522,26 @@ class RefModel:
         return Kc.reshape(N, N), Lc.reshape(N, N), ok.value
 
     def restrict_vec3(self, lc, rf, nc):
+        self._sel()
         rc = np.zeros(3 * nc)
         self.R.ref_restrict_vec3(*self.dims, lc, np.ascontiguousarray(rf, np.float64), rc)
         return rc
 
     def prolong_vec3(self, lc, xc, nf):
+        self._sel()
         xf = np.zeros(3 * nf)
         self.R.ref_prolong_vec3(*self.dims, lc, np.ascontiguousarray(xc, np.float64), xf)
         return xf
 
     def mg_apply(self, E, Emin, rFree):
+        self._sel()
         z = np.zeros(self.nF)
         self.R.ref_mg_apply(*self.dims, np.ascontiguousarray(E, np.float64), Emin,
                             np.ascontiguousarray(rFree, np.float64), z)
         return z
 
     def pcg(self, E, Emin, bFree, x0, tol, maxIt, useMG=True):
+        self._sel()
         x = np.ascontiguousarray(x0, np.float64).copy()
         it = self.R.ref_pcg(*self.dims, np.ascontiguousarray(E, np.float64), Emin,
                             np.ascontiguousarray(bFree, np.float64), x, tol, maxIt, 1 if useMG else 0)
@@ -545,6 +561,7 @@ class RefModel:
 
 def ref_run(nely, nelx, nelz, V0=0.12, nLoop=3, rMin=3 ** 0.5, Emin=1e-6, cgTol=1e-3, cgMaxIt=800):
     R = ref(False)
+    R.ref_set_hierarchy(1, 5)
     rec = np.zeros(6 * max(1, nLoop))
     nel = nely * nelx * nelz
     x = np.zeros(nel, np.float32)

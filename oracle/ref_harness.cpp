@@ -39,6 +39,10 @@ struct RefState {
     bool haveFilter = false;
 };
 std::unique_ptr<RefState> g;
+// hierarchy of the multigrid functions below (MGPrecondConfig, multigrid.hpp:9-13):
+// the reference driver's nonDyadic = true, maxLevels = 5 unless ref_set_hierarchy changes it
+bool hNonDyadic = true;
+int hMaxLevels = 5;
 
 RefState& state(int nely, int nelx, int nelz)
 {
@@ -48,7 +52,7 @@ RefState& state(int nely, int nelx, int nelz)
         InitialSettings(g->pb.params);
         CreateVoxelFEAmodel_Cuboid(g->pb, nely, nelx, nelz);
         ApplyBoundaryConditions(g->pb);
-        g->cfg.nonDyadic = true; g->cfg.maxLevels = 5; g->cfg.weight = 0.6f;
+        g->cfg.nonDyadic = hNonDyadic; g->cfg.maxLevels = hMaxLevels; g->cfg.weight = 0.6f;
         mg::build_static_once(g->pb, g->cfg, g->H, g->masks);
     }
     return *g;
@@ -56,6 +60,14 @@ RefState& state(int nely, int nelx, int nelz)
 }  // namespace
 
 extern "C" {
+
+// the hierarchy used from here on (the cached problem is rebuilt when it changes)
+void ref_set_hierarchy(int nonDyadic, int maxLevels)
+{
+    if ((nonDyadic != 0) != hNonDyadic || maxLevels != hMaxLevels) g.reset();
+    hNonDyadic = nonDyadic != 0;
+    hMaxLevels = maxLevels;
+}
 
 // sizes: resX resY resZ ne nN nF nlev
 void ref_sizes(int nely, int nelx, int nelz, int* out)

@@ -32,12 +32,12 @@ def _E(m, seed):
     return E
 
 
-def _setup(t3d, dims, spheres, seed):
+def _setup(t3d, dims, spheres, seed, hier=(1, 5)):
     p, m = _both(t3d, dims, spheres)
     E = _E(m, seed)
-    g = O.MG(m, Emin=1e-9)
+    g = O.MG(m, Emin=1e-9, nonDyadic=hier[0], maxLevels=hier[1])
     Go = GalerkinMG(m, g, E)
-    G = t3d.MG(p, Emin=1e-9, galerkin=True)
+    G = t3d.MG(p, Emin=1e-9, galerkin=True, nonDyadic=hier[0], maxLevels=hier[1])
     G.setup(t3d.DeviceArray.from_host(E))
     return p, m, E, Go, G
 
@@ -231,3 +231,45 @@ def test_standard_pcg_fp64(t3d):
     # still far fewer than the 'Adapted' cycle needs
     assert res < tol and it_m < it_a
     assert abs(C_m - C_g) <= 1e-9 * abs(C_g)
+
+
+@pytest.mark.parametrize("dims,spheres,hier", [((16, 32, 16), None, (0, 6)), ((24, 48, 24), None, (0, 6)),
+                                               ((16, 32, 16), [(10.0, 8.0, 8.0, 4.5)], (0, 6))])
+def test_dyadic_hierarchy(t3d, dims, spheres, hier):
+    """The Galerkin form on a hierarchy coarsened by 2 at every level
+    (nonDyadic = 0, the MATLAB original's nonDyadic_ = 0,
+    matlab/TOP3D_XL.m:1110-1121): operators, V-cycle and both solvers as on
+    the default hierarchy."""
+    p, m, E, Go, G = _setup(t3d, dims, spheres, 9, hier)
+    assert G.nlev == len(Go.A) and all(d[3] == 2 for d in G.dims[1:])
+    for l in range(G.nlev):
+        d = G.diag(l)
+        assert np.abs(d - Go.D[l]).max() <= 1e-12 * np.abs(Go.D[l]).max(), l
+        assert np.array_equal(G.mask(l).astype(bool), Go.mask[l]), l
+    Kc, L = G.coarsest()
+    assert np.abs(Kc - Go.Ac).max() <= 1e-12 * np.abs(Go.Ac).max()
+    r = m.free_to_full(np.random.default_rng(3).uniform(-1, 1, m.nF))
+    z_o = Go.apply(r)
+    dz = t3d.DeviceArray(m.nDOF)
+    G.apply(t3d.DeviceArray.from_host(r), dz)
+    assert np.abs(dz.download() - z_o).max() <= 1e-11 * np.abs(z_o).max()
+    tol = 1e-8
+    it_o, x_o = Go.pcg(m.b, tol, 500)
+    db = t3d.DeviceArray(m.nDOF)
+    p.load_vector(db)
+    dx = t3d.DeviceArray(m.nDOF)
+    dx.zero()
+    it_g = t3d.pcg(p, G, db, dx, tol, 500)
+    assert abs(it_g - it_o) <= 1 and it_g < 500, (it_g, it_o)
+    dE = t3d.DeviceArray.from_host(E)
+    dce = t3d.DeviceArray(p.ne)
+    C64 = t3d.compliance(p, dE, dx, dce)
+    mx = t3d.Mixed(p, G)
+    mx.setup()
+    dx.zero()
+    it_m = mx.pcg(db, dx, tol, 500)
+    assert it_m < 500 and it_m <= 1.5 * it_g + 5, (it_m, it_g)
+    dy = t3d.DeviceArray(m.nDOF)
+    p.kxu_f64(dx, dy)
+    assert np.linalg.norm(m.b - dy.download()) / np.linalg.norm(m.b) < tol
+    assert abs(t3d.compliance(p, dE, dx, dce) - C64) <= 1e-6 * abs(C64)

@@ -95,3 +95,38 @@ def test_pcg_unpreconditioned(t3d):
     it_g = t3d.pcg(p, None, db, dx, 1e-6, 300)
     assert it_g == it_o
     assert np.array_equal(dx.download(), x_o)
+
+
+# the other hierarchies of MGPrecondConfig (multigrid.hpp:9-13): every
+# coarsening by 2 (nonDyadic = false; the finest transfers then run with span 2)
+# and more levels; the oracle's levels are pinned against the reference build
+# for the same hierarchies (test_oracle_vs_reference.py)
+@pytest.mark.parametrize("hier", [(0, 5), (0, 6), (1, 6)])
+@pytest.mark.parametrize("dims", [(8, 16, 8), (16, 32, 16), (12, 20, 8)])
+def test_hierarchies_bitexact(t3d, dims, hier):
+    p = t3d.Problem(*dims)
+    m = O.Model(*dims)
+    g = O.MG(m, nonDyadic=hier[0], maxLevels=hier[1])
+    G = t3d.MG(p, nonDyadic=hier[0], maxLevels=hier[1])
+    assert [d for d in G.dims] == [tuple(d) for d in g.dims]
+    for l in range(g.nlev):
+        assert np.array_equal(G.mask(l), g.masks[l])
+    E = _E(m, 6)
+    g.setup(E)
+    G.setup(t3d.DeviceArray.from_host(E))
+    for l in range(g.nlev):
+        assert np.array_equal(G.diag(l), g.diag[l]), l
+    nc, ok = G.info()
+    assert nc == g.Nc and ok == g.cholOk
+    r = m.free_to_full(np.random.default_rng(8).uniform(-1, 1, m.nF))
+    dz = t3d.DeviceArray(m.nDOF)
+    G.apply(t3d.DeviceArray.from_host(r), dz)
+    assert np.array_equal(dz.download(), g.apply(r))
+    it_o, x_o = g.pcg(E, m.b, np.zeros(m.nDOF), 1e-8, 300)
+    db = t3d.DeviceArray(m.nDOF)
+    p.load_vector(db)
+    dx = t3d.DeviceArray(m.nDOF)
+    dx.zero()
+    it_g = t3d.pcg(p, G, db, dx, 1e-8, 300)
+    assert it_g == it_o
+    assert np.array_equal(dx.download(), x_o)

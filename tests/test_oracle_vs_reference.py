@@ -37,7 +37,11 @@ def test_model_kxu_mg_bitexact(dims):
     C2, ce2 = r.compliance(E, u)
     assert abs(C1 - C2) <= 1e-13 * abs(C2)
     assert np.abs(ce1 - ce2).max() <= 1e-13 * np.abs(ce2).max()
-    g = O.MG(m)
+    _check_mg(m, r, O.MG(m), E, rng)
+
+
+def _check_mg(m, r, g, E, rng):
+    """levels, masks, diagonals, transfers, coarsest matrix and factor, one V-cycle"""
     assert [tuple(d) for d in g.dims] == r.levels()
     for l in range(g.nlev):
         assert np.array_equal(g.masks[l], r.mask(l, g.nN[l]))
@@ -62,14 +66,32 @@ def test_model_kxu_mg_bitexact(dims):
     assert np.abs(z1 - z2).max() <= 1e-12 * np.abs(z2).max()
 
 
-@pytest.mark.parametrize("dims,tol,maxIt", [((8, 16, 8), 1e-3, 800), ((8, 16, 8), 1e-8, 300),
-                                            ((30, 60, 30), 1e-3, 60)])
-def test_pcg_iterations_and_solution(dims, tol, maxIt):
+# the other hierarchies of MGPrecondConfig (multigrid.hpp:9-13; hierarchy.cpp:84-138):
+# every coarsening by 2 (nonDyadic = false), and more levels
+HIERARCHIES = [(0, 5), (0, 6), (1, 6)]
+
+
+@pytest.mark.parametrize("hier", HIERARCHIES)
+@pytest.mark.parametrize("dims", [(8, 16, 8), (16, 32, 16), (12, 20, 8)])
+def test_mg_hierarchies_bitexact(dims, hier):
     m = O.Model(*dims)
-    r = O.RefModel(*dims)
+    r = O.RefModel(*dims, nonDyadic=hier[0], maxLevels=hier[1])
+    rng = np.random.default_rng(7)
+    E = rng.uniform(1e-3, 1, m.ne)
+    E[::11] = 1e-17
+    g = O.MG(m, nonDyadic=hier[0], maxLevels=hier[1])
+    _check_mg(m, r, g, E, rng)
+
+
+@pytest.mark.parametrize("dims,tol,maxIt,hier", [((8, 16, 8), 1e-3, 800, (1, 5)), ((8, 16, 8), 1e-8, 300, (1, 5)),
+                                                 ((30, 60, 30), 1e-3, 60, (1, 5)),
+                                                 ((16, 32, 16), 1e-8, 300, (0, 6))])
+def test_pcg_iterations_and_solution(dims, tol, maxIt, hier):
+    m = O.Model(*dims)
+    r = O.RefModel(*dims, nonDyadic=hier[0], maxLevels=hier[1])
     rng = np.random.default_rng(1)
     E = rng.uniform(1e-3, 1, m.ne)
-    g = O.MG(m)
+    g = O.MG(m, nonDyadic=hier[0], maxLevels=hier[1])
     g.setup(E)
     it1, x1 = g.pcg(E, m.b, np.zeros(m.nDOF), tol, maxIt)
     it2, x2 = r.pcg(E, 1e-6, r.bFree, np.zeros(m.nF), tol, maxIt)
