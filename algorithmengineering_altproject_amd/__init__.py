@@ -96,6 +96,7 @@ def lib():
         "t3d_mixed_setup": (C.c_int, [vp]),
         "t3d_mixed_coarse_solve": (C.c_int, [vp, vp, vp]),
         "t3d_mixed_vcycle": (C.c_int, [vp, vp, vp]),
+        "t3d_mixed_true_residual": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "t3d_kxu_f32": (C.c_int, [vp, C.c_int, vp, vp]),
         "t3d_batch_create": (C.c_int, [C.c_int, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
         "t3d_batch_create_voxels": (C.c_int, [C.c_int, vp, vp, vp, vp, C.c_float, C.POINTER(C.c_void_p)]),
@@ -599,6 +600,13 @@ class Mixed:
     def vcycle(self, dr, dz):
         """z = M32(r): one FP32 V-cycle of the multigrid object's cycle type (float32 device arrays)."""
         check(lib().t3d_mixed_vcycle(self.h, dr.ptr, dz.ptr), "t3d_mixed_vcycle")
+
+    def true_residual(self, db, dx, dr):
+        """r = b - K x with the FP64 operator of the residual replacement, into the
+        float32 device array dr; returns the FP64 r.r."""
+        rn2 = C.c_double()
+        check(lib().t3d_mixed_true_residual(self.h, db.ptr, dx.ptr, dr.ptr, C.byref(rn2)), "t3d_mixed_true_residual")
+        return rn2.value
 
     def coarse_solve(self, db, dx):
         """Coarsest-level solve of the mixed V-cycle (inverse Cholesky factor), FP64 device arrays."""

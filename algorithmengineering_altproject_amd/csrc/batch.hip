@@ -726,10 +726,8 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         t3d_problem* p = B->P[i];
         t3d_mixed* s = B->X[i];
         const int64_t nD = 3 * p->g.nN;
-        if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * nD));
         if ((rc = canon_dot_dev(p, b[i], b[i], nD, s->sc + S_NORMB2))) return rc;
-        if ((rc = launch_kxu_f64_fma(p, x[i], p->wAp))) return rc;
-        if ((rc = mixed_true_residual(s, b[i]))) return rc;
+        if ((rc = mixed_true_residual(s, b[i], x[i]))) return rc;
         T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * nD, st));
         T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * nD, st));
         T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * nD, st));
@@ -890,8 +888,11 @@ int t3d_batch_solve(t3d_batch* B, const double* const* b, double* const* x, doub
         it0 = it1;
     }
     for (int i = 0; i < n; ++i) {
-        if (!finished[i]) result[i] = maxIt - skipped[i];
-        if ((rc = mixed_fold(B->X[i], x[i]))) return rc;
+        // a finished problem has folded its last increment at its confirmation
+        if (!finished[i]) {
+            result[i] = maxIt - skipped[i];
+            if ((rc = mixed_fold(B->X[i], x[i]))) return rc;
+        }
         iters[i] = result[i];
     }
     T3D_HIP(hipStreamSynchronize(st));

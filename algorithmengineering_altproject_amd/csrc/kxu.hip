@@ -119,10 +119,7 @@ struct FuseP {
     double* p;
 };
 
-// FMA: fused multiply-adds instead of the reference's separate products and
-// sums (for the mixed solvers' FP64 true residual, which needs FP64 accuracy
-// but not the reference's rounding; half the VALU instructions)
-template <int NZ, bool FUSEP = false, bool FMA = false>
+template <int NZ, bool FUSEP = false>
 __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __restrict__ Ke,
                                                      const double* __restrict__ Ecell,
                                                      const uint8_t* __restrict__ nodeFixed,
@@ -206,15 +203,9 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
 #pragma unroll
                 for (int k = 0; k < NZ; ++k) {
                     const double ue = sU[c * T::UN + ub[k] + off];
-                    if constexpr (FMA) {
-                        f[k][0] = fma(k0, ue, f[k][0]);
-                        f[k][1] = fma(k1, ue, f[k][1]);
-                        f[k][2] = fma(k2, ue, f[k][2]);
-                    } else {
-                        f[k][0] = f[k][0] + k0 * ue;
-                        f[k][1] = f[k][1] + k1 * ue;
-                        f[k][2] = f[k][2] + k2 * ue;
-                    }
+                    f[k][0] = f[k][0] + k0 * ue;
+                    f[k][1] = f[k][1] + k1 * ue;
+                    f[k][2] = f[k][2] + k2 * ue;
                 }
             }
         }
@@ -222,15 +213,9 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
 #pragma unroll
         for (int k = 0; k < NZ; ++k)
             if (!(E[k] <= 1.0e-16)) {
-                if constexpr (FMA) {
-                    acc[k][0] = fma(f[k][0], E[k], acc[k][0]);
-                    acc[k][1] = fma(f[k][1], E[k], acc[k][1]);
-                    acc[k][2] = fma(f[k][2], E[k], acc[k][2]);
-                } else {
-                    acc[k][0] = acc[k][0] + f[k][0] * E[k];
-                    acc[k][1] = acc[k][1] + f[k][1] * E[k];
-                    acc[k][2] = acc[k][2] + f[k][2] * E[k];
-                }
+                acc[k][0] = acc[k][0] + f[k][0] * E[k];
+                acc[k][1] = acc[k][1] + f[k][1] * E[k];
+                acc[k][2] = acc[k][2] + f[k][2] * E[k];
             }
     }
 #pragma unroll
@@ -272,19 +257,6 @@ __global__ void __launch_bounds__(256) k_stream_copy(int64_t n, const double* __
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done)
 {
     return launch_kxu_lds<1>(p, u, y, done);
-}
-
-// y = K u in FP64 with fused multiply-adds (not the reference's rounding)
-int launch_kxu_f64_fma(t3d_problem* p, const double* u, double* y, const int* done)
-{
-    using T = KxuTile<1>;
-    const Grid& g = p->g;
-    dim3 grid(unsigned((g.nnx + T::TX - 1) / T::TX), unsigned((g.nny + T::TY - 1) / T::TY),
-              unsigned((g.nnz + T::TZ - 1) / T::TZ));
-    hipLaunchKernelGGL((k_kxu_f64_lds<1, false, true>), grid, dim3(512), 0, p->stream, g, p->Ke, p->Ecell,
-                       p->nodeFixed, u, y, done, FuseP{});
-    T3D_HIP(hipGetLastError());
-    return T3D_OK;
 }
 
 // p = z + beta p_prev, then y = K p (see FuseP)

@@ -168,22 +168,6 @@ __global__ void __launch_bounds__(256) k_kx32(Grid g, const float* __restrict__ 
         if (tid == 0) partial[linear_block()] = t;
     }
 }
-// r = b - Ap (FP64 arithmetic, stored in FP32) ; partial of the FP64 r.r
-__global__ void __launch_bounds__(256) k_mx_true_residual(int64_t n, const double* __restrict__ b,
-                                                          const double* __restrict__ Ap, float* __restrict__ r,
-                                                          double* __restrict__ partial, const int* __restrict__ done)
-{
-    if (done && *done) return;
-    __shared__ double sh[4];
-    double s = 0.0;
-    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
-        const double v = b[i] - Ap[i];
-        r[i] = float(v);
-        s = fma(v, v, s);
-    }
-    const double t = block_sum_256(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
 // x += dx + alpha p (the increment of the last iteration, not yet applied by
 // an operator launch) ; dx = 0 ; followed by k_mx_applied
 __global__ void __launch_bounds__(256) k_mx_fold(int64_t n, double* __restrict__ x, float* __restrict__ dx,
@@ -250,6 +234,201 @@ __global__ void __launch_bounds__(32 * CYT, CYT >= 16 ? 4 : 5) k_kxw32(Grid g, c
     const unsigned slot = (bz * gridDim.y + by) * gridDim.x + bx;
     kxw32_body<NZ, FUSED, CYT, PIECES>(g, KW, Ecell, nodeFixed, u, z, pcur, y, partial, S, done, dxv, pid, bx, by,
                                        bz, slot, gridDim.x, KA);
+}
+
+// ---------------------------------------------------------------------------
+// The true residual of the residual replacement, r = b - K x, in FP64 with
+// the Walsh form: the scheme of kxw32_body (a thread per voxel column
+// sweeping z, node planes staged in LDS one step ahead, face transforms
+// shared by the voxels below and above, the four-column node gather in a
+// fixed order) on FP64 planes, FP64 moduli (p->Ecell, zero at or below 1e-16
+// as fea.cpp:743-744) and the 45 FP64 coefficients of K^/64.  The node gather
+// ends in r = b - (K x, 0 at fixed DOFs), stored in FP32, and the block's
+// partial of the FP64 r.r.  256-thread tiles of 32 x 8 voxel columns (31 x 7
+// owned node columns) and R64_NZ node planes.
+// ---------------------------------------------------------------------------
+constexpr int R64_NZ = 16;
+
+__device__ __forceinline__ double buf_ld64(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void walsh2d(double a00, double a10, double a01, double a11, double F[2][2])
+{
+    const double s0 = a00 + a10, d0 = a00 - a10, s1 = a01 + a11, d1 = a01 - a11;
+    F[0][0] = s0 + s1;
+    F[0][1] = d0 + d1;
+    F[1][0] = s0 - s1;
+    F[1][1] = d0 - d1;
+}
+
+static dim3 resid64_grid(const Grid& g)
+{
+    return dim3(unsigned((g.nnx + wk::NX - 1) / wk::NX), unsigned((g.nny + wk::NY - 1) / wk::NY),
+                unsigned((g.nnz + R64_NZ - 1) / R64_NZ));
+}
+
+__global__ void __launch_bounds__(256) k_resid64(Grid g, const double* __restrict__ KW, const double* __restrict__ Ecell,
+                                                 const uint8_t* __restrict__ nodeFixed, const double* __restrict__ x,
+                                                 const double* __restrict__ b, float* __restrict__ r,
+                                                 double* __restrict__ partial, const int* __restrict__ done)
+{
+    using wk::IN;
+    using wk::NNZ;
+    using wk::OUT;
+    constexpr int CX = wk::CX, CY = wk::CY, NX = wk::NX, NY = wk::NY, PX = wk::PX, PN = wk::PN;
+    constexpr int NT = CX * CY;
+    constexpr int SPT = (3 * PN + NT - 1) / NT;   // staging slots per thread
+    constexpr int SPN = 3 * PN + 64;              // staged plane + 64 write-only dummy slots
+    constexpr int NZ = R64_NZ;
+    if (done && *done) return;
+    __shared__ double sP[3][SPN];
+    __shared__ double sC[12][NT];
+    __shared__ double sh[NT / 64];
+    const int tid = threadIdx.x;
+    const int tx = tid % CX, ty = tid / CX;
+    const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY, z0 = blockIdx.z * NZ;
+    const int cx = x0 - 1 + tx, cy = y0 - 1 + ty;   // this thread's voxel column
+    const bool colIn = cx >= 0 && cx < g.resX && cy >= 0 && cy < g.resY;
+    const int X = x0 + tx, Y = y0 + ty;             // the node this thread gathers
+    const bool ownNode = tx < NX && ty < NY && X < g.nnx && Y < g.nny;
+    uint32_t voff[SPT];   // byte offset of staging slot j within a node plane (BUF_OOB: loads 0)
+    int slds[SPT];        // its LDS index c * PN + row * PX + lx (a dummy slot if none)
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int f = tid + j * NT;
+        const int row = f / (3 * PX);
+        const int k = f - row * (3 * PX);
+        const int lx = k / 3, c = k - 3 * lx;
+        const int Xs = x0 - 1 + lx, Ys = y0 - 1 + row;
+        const bool slot = f < 3 * PN;
+        const bool in = slot && Xs >= 0 && Xs < g.nnx && Ys >= 0 && Ys < g.nny;
+        slds[j] = slot ? c * PN + row * PX + lx : 3 * PN + (tid & 63);
+        voff[j] = in ? uint32_t(8 * (3 * (Ys * g.nnx + Xs) + c)) : BUF_OOB;
+    }
+    const int64_t plane = int64_t(3) * g.nnx * g.nny;
+    const int64_t layer = int64_t(g.resX) * g.resY;
+    const uint32_t eoff = colIn ? uint32_t(8 * (cy * g.resX + cx)) : BUF_OOB;
+    const uint32_t noff = ownNode ? uint32_t(Y * g.nnx + X) : BUF_OOB;
+    double fu[SPT];
+    auto fetch = [&](int Z) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        const __amdgpu_buffer_rsrc_t ru = buf_rsrc(x + (zin ? Z * plane : 0), zin ? uint32_t(8 * plane) : 0u);
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) fu[j] = buf_ld64(ru, voff[j]);
+    };
+    auto commit = [&](int q) {
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) sP[q][slds[j]] = fu[j];
+    };
+    auto modulus = [&](int cz) {
+        const bool in = cz >= 0 && cz < g.resZ;
+        return buf_ld64(buf_rsrc(Ecell + (in ? cz * layer : 0), in ? uint32_t(8 * layer) : 0u), eoff);
+    };
+    auto fixed_bits = [&](int Z) {
+        const bool zin = Z >= 0 && Z < g.nnz;
+        const int64_t pn = int64_t(g.nnx) * g.nny;
+        return unsigned(__builtin_amdgcn_raw_buffer_load_b8(
+            buf_rsrc(nodeFixed + (zin ? Z * pn : 0), zin ? uint32_t(pn) : 0u), noff, 0, 0));
+    };
+    auto face = [&](int q, double F[3][2][2]) {
+        const int bb = ty * PX + tx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            walsh2d(sP[q][c * PN + bb], sP[q][c * PN + bb + 1], sP[q][c * PN + bb + PX], sP[q][c * PN + bb + PX + 1],
+                    F[c]);
+    };
+    double Fp[3][2][2], Fn[3][2][2], carry[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) (&carry[c][0][0])[k] = 0.0;
+    fetch(z0 - 1);
+    commit(0);
+    fetch(z0);
+    commit(1);
+    double Enext = modulus(z0 - 1);
+    unsigned fxNext = 0u;
+    __syncthreads();
+    face(0, Fp);
+    double dot = 0.0;
+    const int nsteps = min(NZ, g.nnz - z0) + 1;
+    for (int s = 0; s < nsteps; ++s) {
+        const int qn = (s + 1) % 3, qo = s % 3;
+        const bool more = s + 1 < nsteps;
+        const int cz = z0 - 1 + s;
+        const double E = Enext <= 1.0e-16 ? 0.0 : Enext;
+        const unsigned fx = fxNext;
+        if (more) {   // loads of the next step, in flight during this one
+            fetch(z0 + s + 1);
+            Enext = modulus(cz + 1);
+            fxNext = fixed_bits(cz + 1);
+        }
+        face(qn, Fn);
+        double uh[24], vh[24];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int my = 0; my < 2; ++my)
+#pragma unroll
+                for (int mx = 0; mx < 2; ++mx) {
+                    uh[(2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] + Fn[c][my][mx]);
+                    uh[(4 + 2 * my + mx) * 3 + c] = E * (Fp[c][my][mx] - Fn[c][my][mx]);
+                }
+#pragma unroll
+        for (int i = 0; i < 24; ++i) vh[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NNZ; ++k) vh[OUT[k]] = fma(KW[k], uh[IN[k]], vh[OUT[k]]);
+        double A4[3][2][2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int my = 0; my < 2; ++my)
+#pragma unroll
+                for (int mx = 0; mx < 2; ++mx) {
+                    const double v0 = vh[(2 * my + mx) * 3 + c], v1 = vh[(4 + 2 * my + mx) * 3 + c];
+                    A4[c][my][mx] = carry[c][my][mx] + (v0 + v1);
+                    carry[c][my][mx] = v0 - v1;
+                }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) (&Fp[c][0][0])[k] = (&Fn[c][0][0])[k];
+        if (s > 0) {   // corner forces of plane cz (plane z0 - 1 belongs to the block below)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double C4[2][2];
+                walsh2d(A4[c][0][0], A4[c][0][1], A4[c][1][0], A4[c][1][1], C4);
+                sC[0 * 3 + c][tid] = C4[0][0];
+                sC[1 * 3 + c][tid] = C4[0][1];
+                sC[2 * 3 + c][tid] = C4[1][0];
+                sC[3 * 3 + c][tid] = C4[1][1];
+            }
+            __syncthreads();
+            if (ownNode) {   // node (X, Y) collects corner (ox, oy) of voxel column (X - ox, Y - oy)
+                const bool zin = cz < g.nnz;
+                const __amdgpu_buffer_rsrc_t rb =
+                    buf_rsrc(b + (zin ? cz * plane : 0), zin ? uint32_t(8 * plane) : 0u);
+                const __amdgpu_buffer_rsrc_t rr =
+                    buf_rsrc(r + (zin ? cz * plane : 0), zin ? uint32_t(4 * plane) : 0u);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double c00 = sC[3 * 3 + c][ty * CX + tx];
+                    const double c10 = sC[2 * 3 + c][ty * CX + tx + 1];
+                    const double c01 = sC[1 * 3 + c][(ty + 1) * CX + tx];
+                    const double c11 = sC[0 * 3 + c][(ty + 1) * CX + tx + 1];
+                    const double yv = ((fx >> c) & 1u) ? 0.0 : (c00 + c10) + (c01 + c11);
+                    const double rv = buf_ld64(rb, 8 * (3 * noff + c)) - yv;
+                    buf_st(float(rv), rr, 4 * (3 * noff + c));
+                    dot = fma(rv, rv, dot);
+                }
+            }
+        }
+        if (more) commit((s + 2) % 3);
+        __syncthreads();
+    }
+    const double t = block_sum_256(dot, sh);
+    if (tid == 0) partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
 }
 
 __global__ void __launch_bounds__(1024) k_fin_alpha(const double* __restrict__ partial, unsigned m,
@@ -573,6 +752,22 @@ static void launch_fold(t3d_mixed* s, double* x, const int* done)
     hipLaunchKernelGGL(k_mx_applied, dim3(1), dim3(1), 0, p->stream, s->sc, done);
 }
 
+// r = b - K64 x (FP64 Walsh form, r in FP32) with the FP64 r.r partials in
+// s->part, then rn2 = their sum and, with check, the convergence test of
+// iteration it (k_fin_rn2); skipped for a finished problem when done != nullptr
+static int launch_true_residual(t3d_mixed* s, const double* b, const double* x, const int* done, int it, int check,
+                                int guard)
+{
+    t3d_problem* p = s->p;
+    const dim3 gr = resid64_grid(p->g);
+    hipLaunchKernelGGL(k_resid64, gr, dim3(256), 0, p->stream, p->g, (const double*)s->KW64,
+                       (const double*)p->Ecell, (const uint8_t*)p->nodeFixed, x, b, s->r, s->part, done);
+    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, p->stream, (const double*)s->part, gr.x * gr.y * gr.z,
+                       s->sc, s->iflags, it, check, guard);
+    T3D_HIP(hipGetLastError());
+    return T3D_OK;
+}
+
 // z32 = M32(r); rz and beta from the finest prolongation.  tmpReady: the
 // x-pass of the finest restriction is already in s->tmp (fused into the x, r
 // update)
@@ -755,35 +950,18 @@ namespace t3d {
 
 // helpers of the batched solver (batch.hip), one problem each, on its stream
 
-// r = b - (K64 x, already in p->wAp) in FP32 ; rn2 (no convergence test)
-int mixed_true_residual(t3d_mixed* s, const double* b)
+// r = b - K64 x in FP32 ; rn2 (no convergence test)
+int mixed_true_residual(t3d_mixed* s, const double* b, const double* x)
 {
-    t3d_problem* p = s->p;
-    const int64_t n = 3 * p->g.nN;
-    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
-                       s->part, nullptr);
-    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, p->stream, (const double*)s->part, unsigned(VB), s->sc,
-                       s->iflags, -1, 0, 0);
-    T3D_HIP(hipGetLastError());
-    return T3D_OK;
+    return launch_true_residual(s, b, x, nullptr, -1, 0, 0);
 }
 
 // the residual replacement of iteration it, skipped when the problem is done:
 // x += dx, r = b - K64 x, rn2 and the convergence test
 int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it)
 {
-    t3d_problem* p = s->p;
-    const int64_t n = 3 * p->g.nN;
-    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     launch_fold(s, x, (const int*)s->iflags);
-    int rc = launch_kxu_f64_fma(p, x, p->wAp, s->iflags);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, p->stream, n, b, (const double*)p->wAp, s->r,
-                       s->part, (const int*)s->iflags);
-    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, p->stream, (const double*)s->part, unsigned(VB), s->sc,
-                       s->iflags, it, 1, 1);
-    T3D_HIP(hipGetLastError());
-    return T3D_OK;
+    return launch_true_residual(s, b, x, (const int*)s->iflags, it, 1, 1);
 }
 
 // x += dx ; r = b - K64 x ; returns the FP64 r.r and the stopping threshold
@@ -792,8 +970,7 @@ int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double*
     t3d_problem* p = s->p;
     int rc = mixed_fold(s, x);
     if (rc) return rc;
-    if ((rc = launch_kxu_f64_fma(p, x, p->wAp))) return rc;
-    if ((rc = mixed_true_residual(s, b))) return rc;
+    if ((rc = mixed_true_residual(s, b, x))) return rc;
     T3D_HIP(hipMemcpyAsync(stop, s->sc + S_STOP, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     T3D_HIP(hipMemcpyAsync(rn2, s->sc + S_RN2, sizeof(double), hipMemcpyDeviceToHost, p->stream));
     T3D_HIP(hipStreamSynchronize(p->stream));
@@ -837,7 +1014,7 @@ extern "C" {
 int t3d_mixed_destroy(t3d_mixed* s)
 {
     if (!s) return T3D_OK;
-    void* b[] = {s->Ke32, s->KW, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64,
+    void* b[] = {s->Ke32, s->KW, s->KW64, s->KA2, s->KA3, s->r, s->dx, s->tmp, s->rc64, s->xc64,
                  s->Li, s->ownAinv ? s->Ainv : nullptr, s->yc64, s->cP,
                  s->pA, s->pB, s->Ap, s->z, s->part, s->sc, s->iflags};
     for (void* q : b)
@@ -876,6 +1053,7 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     for (int i = 0; i < 576; ++i) K32[i] = float(p->host.Ke[i]);
     // Walsh form: K^ = (H x I3) Ke (H x I3)^T; check its sparsity against the kernel's pattern
     float KW[wk::NNZ];
+    double KW64[wk::NNZ];
     {
         static const int off[8][3] = {{0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 0},
                                       {0, 1, 1}, {1, 1, 1}, {1, 0, 1}, {0, 0, 1}};   // o(b), fea.cpp node order
@@ -905,7 +1083,8 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         bool inPat[24][24] = {};
         for (int k = 0; k < wk::NNZ; ++k) {
             inPat[outIdx[k]][inIdx[k]] = true;
-            KW[k] = float(Kh[outIdx[k]][inIdx[k]] / 64.0);
+            KW64[k] = Kh[outIdx[k]][inIdx[k]] / 64.0;
+            KW[k] = float(KW64[k]);
         }
         for (int i = 0; i < 24; ++i)
             for (int j = 0; j < 24; ++j)
@@ -919,7 +1098,8 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
     // deep, row kernels down to 2 rows per block, the vector kernels)
     const int64_t kwMax = int64_t(kg.x) * kg.y * ((p->g.nnz + 7) / 8);
     const int64_t rowsMax = (int64_t(L0.nny) * L0.nnz + 1) / 2;
-    const int64_t npart = std::max<int64_t>({kwMax, 2 * rowsMax, 2 * int64_t(VB)}) + 16;
+    const dim3 rg = resid64_grid(p->g);
+    const int64_t npart = std::max<int64_t>({kwMax, 2 * rowsMax, 2 * int64_t(VB), int64_t(rg.x) * rg.y * rg.z}) + 16;
     int64_t ntmp = 0;
     for (int l = 0; l + 1 < v.nlev; ++l) {
         const LevelF Lf = levf(v, l), Lc = levf(v, l + 1);
@@ -972,6 +1152,8 @@ int t3d_mixed_create(t3d_problem* p, t3d_mg* M, t3d_mixed** out)
         hipMemcpy(s->Ke32, K32, sizeof(float) * 576, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->KW, sizeof(float) * wk::NNZ) != hipSuccess ||
         hipMemcpy(s->KW, KW, sizeof(float) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->KW64, sizeof(double) * wk::NNZ) != hipSuccess ||
+        hipMemcpy(s->KW64, KW64, sizeof(double) * wk::NNZ, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&s->pA, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->pB, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->Ap, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->z, sizeof(float) * nD) != hipSuccess ||
         hipMalloc(&s->r, sizeof(float) * nD) != hipSuccess || hipMalloc(&s->dx, sizeof(float) * nD) != hipSuccess ||
@@ -1083,6 +1265,19 @@ int t3d_mixed_vcycle(t3d_mixed* s, const float* r, float* z)
     T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * 3 * size_t(s->p->g.nN), st));   // the beta partials read it
     if ((rc = mixed_vcycle(s, v, r, 0, 1, false, s->iflags))) return rc;
     T3D_HIP(hipMemcpyAsync(z, s->z, sizeof(float) * 3 * size_t(s->p->g.nN), hipMemcpyDeviceToDevice, st));
+    T3D_HIP(hipStreamSynchronize(st));
+    return T3D_OK;
+}
+
+// r = b - K64 x (the replacement's FP64 Walsh-form residual) into r (FP32) and its FP64 r.r
+int t3d_mixed_true_residual(t3d_mixed* s, const double* b, const double* x, float* r, double* rn2)
+{
+    T3D_CHECK(s && b && x && r && rn2, T3D_ERR_ARG, "t3d_mixed_true_residual: null argument");
+    hipStream_t st = s->p->stream;
+    int rc = launch_true_residual(s, b, x, nullptr, -1, 0, 0);
+    if (rc) return rc;
+    T3D_HIP(hipMemcpyAsync(r, s->r, sizeof(float) * 3 * size_t(s->p->g.nN), hipMemcpyDeviceToDevice, st));
+    T3D_HIP(hipMemcpyAsync(rn2, s->sc + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
     T3D_HIP(hipStreamSynchronize(st));
     return T3D_OK;
 }
@@ -1203,12 +1398,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     hipStream_t st = p->stream;
     const Grid& g = p->g;
     const int64_t n = 3 * g.nN;
-    // FP64 K x of the residual replacement (the problem's work vector, shared
-    // with the plain PCG)
-    if (!p->wAp) T3D_HIP(hipMalloc(&p->wAp, sizeof(double) * n));
     float* r = s->r;
-    double* Ap64 = p->wAp;
-    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     double* S = s->sc;
     int* flags = s->iflags;
     const dim3 kg = kww_grid(g);   // tiles of the fused operator kernel
@@ -1220,10 +1410,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     // r = b - K64 x ; stop = tol max(1e-30, |b|) ; z = M r ; rz ; beta = 0
     if ((rc = canon_dot_dev(p, b, b, n, S + S_NORMB2))) return rc;
     hipLaunchKernelGGL(k_mx_init, dim3(1), dim3(1), 0, st, tol, S, flags);
-    if ((rc = launch_kxu_f64_fma(p, x, Ap64))) return rc;
-    hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
-                       nullptr);
-    hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags, -1, 0, 0);
+    if ((rc = launch_true_residual(s, b, x, nullptr, -1, 0, 0))) return rc;
     T3D_HIP(hipMemsetAsync(s->Ap, 0, sizeof(float) * n, st));
     T3D_HIP(hipMemsetAsync(s->pA, 0, sizeof(float) * n, st));
     T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * n, st));
@@ -1237,6 +1424,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     auto pCurOf = [&](int it) { return (it & 1) ? s->pA : s->pB; };
     const int CHUNK = 32;
     int result = maxIt;
+    bool confirmed = false;
     int it0 = 0;
     while (it0 < maxIt) {
         const int it1 = std::min(maxIt, it0 + CHUNK);
@@ -1259,11 +1447,7 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
                 // x += dx, then residual replacement with the FP64 operator (the
                 // direction is kept)
                 launch_fold(s, x, (const int*)flags);
-                if ((rc = launch_kxu_f64_fma(p, x, Ap64, flags))) return rc;
-                hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r,
-                                   s->part, (const int*)flags);
-                hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
-                                   it, 1, 1);
+                if ((rc = launch_true_residual(s, b, x, (const int*)flags, it, 1, 1))) return rc;
             }
             // z = M r ; beta ; rz
             if ((rc = mixed_vcycle(s, v, r, flexible, 0, !replace && !stdCycle, flags))) return rc;
@@ -1299,24 +1483,21 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
         // true residual b - K x (FP64); otherwise continue from there
         const int itDone = s->hostFlags[1] - 1;
         launch_fold(s, x, nullptr);
-        if ((rc = launch_kxu_f64_fma(p, x, Ap64))) return rc;
-        hipLaunchKernelGGL(k_mx_true_residual, dim3(VB), dim3(256), 0, st, n, b, (const double*)Ap64, r, s->part,
-                           nullptr);
-        hipLaunchKernelGGL(k_fin_rn2, dim3(1), dim3(1024), 0, st, (const double*)s->part, unsigned(VB), S, flags,
-                           itDone, 0, 0);
+        if ((rc = launch_true_residual(s, b, x, nullptr, itDone, 0, 0))) return rc;
         double hs[2];
         T3D_HIP(hipMemcpyAsync(hs, S + S_STOP, sizeof(double), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipMemcpyAsync(hs + 1, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
         T3D_HIP(hipStreamSynchronize(st));
         if (std::sqrt(hs[1]) < hs[0]) {
             result = itDone + 1;
+            confirmed = true;
             break;
         }
         hipLaunchKernelGGL(k_mx_clear_done, dim3(1), dim3(1), 0, st, flags);
         if ((rc = mixed_vcycle(s, v, r, flexible, 0, false, flags))) return rc;
         it0 = itDone + 1;
     }
-    launch_fold(s, x, nullptr);
+    if (!confirmed) launch_fold(s, x, nullptr);   // a confirmed solve has folded its last increment
     T3D_HIP(hipGetLastError());
     T3D_HIP(hipStreamSynchronize(st));
     if (iters) *iters = result;

@@ -10,6 +10,7 @@ struct t3d_mixed {
     t3d_mg* M = nullptr;
     float* Ke32 = nullptr;     // element matrix (node-gather form)
     float* KW = nullptr;       // the 45 nonzeros of the Walsh-form element matrix / 64
+    double* KW64 = nullptr;    // the same in FP64 (the true residual)
     unsigned short* KA2 = nullptr;  // element matrix as 2-piece bf16 MFMA A operands
     unsigned short* KA3 = nullptr;  // ... and as 3-piece
     int form = 0;              // operator of the solver: 0 Walsh form, 2 MFMA bf16 x3, 3 MFMA bf16 x6
@@ -92,7 +93,7 @@ unsigned row_blocks(const LevelF& L);
 int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, int first, bool tmpReady,
                  const int* done);
 // single-problem steps of the batched solver (mixed.hip)
-int mixed_true_residual(t3d_mixed* s, const double* b);
+int mixed_true_residual(t3d_mixed* s, const double* b, const double* x);
 int mixed_replace_residual(t3d_mixed* s, const double* b, double* x, int it);
 int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double* stop);
 int mixed_fold(t3d_mixed* s, double* x);

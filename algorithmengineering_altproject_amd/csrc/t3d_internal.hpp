@@ -141,9 +141,6 @@ int canon_finish_buf(t3d_problem* p, double* buf, int64_t ntiles, double* dst, c
 int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, double* dst,
                   const int* done = nullptr);
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
-// y = K u in FP64 with fused multiply-adds: FP64 accuracy, not the reference's
-// rounding (the mixed solvers' true residual)
-int launch_kxu_f64_fma(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
 // p = z + rzNew / max(1e-30, rzOld) * pPrev into pOut (!= pPrev), then y = K p
 int launch_kxu_f64_fusep(t3d_problem* p, const double* z, const double* pPrev, const double* rzNew,
                          const double* rzOld, double* pOut, double* y, const int* done);

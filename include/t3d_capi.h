@@ -212,6 +212,14 @@ int t3d_mixed_coarse_solve(t3d_mixed* s, const double* b_dev, double* x_dev);
  * preconditioner closure (cpp/src/multigrid/preconditioner.cpp:88-265) in the
  * mixed solver's arithmetic */
 int t3d_mixed_vcycle(t3d_mixed* s, const float* r_dev, float* z_dev);
+/* r = b - K x with the FP64 operator of the mixed solver's residual
+ * replacement (Walsh form in FP64; fixed DOFs of K x are 0, so r = b there),
+ * stored in FP32 into r_dev, and *rn2 = the FP64 r.r of the unrounded
+ * residual (device arrays of 3 nN; b and x FP64).  The counterpart of the
+ * reference's residual r = b - K x (cpp/src/solver.cpp:110-122) in the
+ * mixed solver's arithmetic; overwrites the solver's residual, so it is
+ * called between solves. */
+int t3d_mixed_true_residual(t3d_mixed* s, const double* b_dev, const double* x_dev, float* r_dev, double* rn2);
 /* b, x: FP64 full layout (device); trueEvery > 0: residual replacement period;
  * flexible != 0: Polak-Ribiere beta, else the reference's beta = rz_new/rz_old.
  * Convergence is reported only when the true residual ||b - K x|| (FP64

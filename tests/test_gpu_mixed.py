@@ -259,3 +259,43 @@ def test_coarse_solve_inverse_factor(t3d, dims, uniform):
     xrs = np.where(fixed, 0.0, xr)
     res = np.abs(A @ xs - A @ xrs)[free].max()
     assert res <= 64 * np.sqrt(N) * np.linalg.norm(A, 2) * np.max(np.abs(xs)) * eps, res
+
+
+@pytest.mark.parametrize("dims,spheres", [((2, 4, 2), None), ((8, 16, 8), None), ((12, 20, 8), None),
+                                          ((30, 60, 30), None), ((40, 72, 24), None),
+                                          ((16, 32, 16), [(10.0, 8.0, 8.0, 4.5), (22.0, 4.0, 10.0, 3.0)])])
+def test_true_residual_fp64(t3d, dims, spheres):
+    """The residual replacement's r = b - K x (FP64 Walsh form, r stored in
+    FP32) against b - K x with the FP64 plain operator (the reference's
+    summation order): the FP32 values within one FP32 rounding of the FP64
+    residual plus the FP64 reordering error of K x, and the FP64 r.r within
+    1e-10; moduli with skipped elements (<= 1e-16), ragged tiles, a voxel
+    domain with voids."""
+    from test_voxels import spheres_mask
+    solid = spheres_mask(*dims, spheres) if spheres else None
+    p = t3d.Problem(*dims, solid=solid)
+    rng = np.random.default_rng(11)
+    E = rng.uniform(1e-3, 1.0, p.ne)
+    E[::9] = 1e-17
+    dE = t3d.DeviceArray.from_host(E)
+    p.set_modulus(dE)
+    M = t3d.MG(p, Emin=1e-9)
+    M.setup(dE)
+    mx = t3d.Mixed(p, M)
+    mx.setup()
+    db = t3d.DeviceArray(p.nDOF)
+    p.load_vector(db)
+    b = db.download()
+    x = t3d.DeviceArray(p.nDOF)
+    p.free_to_full(t3d.DeviceArray.from_host(rng.uniform(-1, 1, p.nF) * 1e3), x)
+    dy = t3d.DeviceArray(p.nDOF)
+    p.kxu_f64(x, dy)
+    y = dy.download()
+    r_ref = b - y
+    dr = t3d.DeviceArray(p.nDOF, np.float32)
+    rn2 = mx.true_residual(db, x, dr)
+    r = dr.download().astype(np.float64)
+    tol = 2.0 ** -24 * np.abs(r_ref) + 1e-13 * np.abs(y).max()
+    assert np.all(np.abs(r - r_ref) <= tol), np.abs(r - r_ref).max()
+    ref2 = float(np.dot(r_ref, r_ref))
+    assert abs(rn2 - ref2) <= 1e-10 * ref2
