@@ -451,6 +451,19 @@ __global__ void __launch_bounds__(1024) k_fin_beta(const double* __restrict__ pa
     fin_beta_body(partial, m, S, flexible, first, done, blockIdx.x, blockIdx.y, blockIdx.z, linear_block(), gridDim.x);
 }
 
+// restriction of an FP32 level in one pass (span 2 or 4; restrict_c32_t)
+__global__ void __launch_bounds__(64) k_restrict_c32(LevelF Lf, LevelF Lc, int span, const float* __restrict__ rf,
+                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
+                                                     const float* __restrict__ Dc, float* __restrict__ xc,
+                                                     double* __restrict__ rcd, const int* __restrict__ done, int norm)
+{
+    if (done && *done) return;
+    if (span == 4)
+        restrict_c32_t<4>(Lf, Lc, rf, cmask, rc, Dc, xc, rcd, norm, blockIdx.x, blockIdx.y, blockIdx.z);
+    else
+        restrict_c32_t<2>(Lf, Lc, rf, cmask, rc, Dc, xc, rcd, norm, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 template <class TIn>
 __global__ void __launch_bounds__(64) k_restrict_x(LevelF Lf, int cnx, int span, const TIn* __restrict__ rf,
                                                    float* __restrict__ t, const int* __restrict__ done)
@@ -802,6 +815,15 @@ static int mixed_vcycle_std(t3d_mixed* s, const MGView& v, const float* r, int f
         hipLaunchKernelGGL(k_std32_point, nodes(Lf.nN), dim3(256), 0, st, 1, Lf.nN, rl, (const float*)yl,
                            (const float*)nullptr, v.mask[l], w, yl, done);
         // r_{l+1} = P^T (r_l - A_l x_l)
+        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
+        const float* Dc = (last && !chol) ? (const float*)s->D32[L] : nullptr;
+        double* rcd = (last && chol) ? s->rc64 : nullptr;
+        if (l > 0 && (span == 2 || span == 4)) {   // coarse levels: one pass
+            hipLaunchKernelGGL(k_restrict_c32, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)yl, v.mask[l + 1],
+                               s->r32[l + 1], Dc, s->x32[L], rcd, done, v.norm);
+            T3D_HIP(hipGetLastError());
+            continue;
+        }
         if (l == 0) {
             hipLaunchKernelGGL(k_rx_rows, dim3(row_blocks(Lf)), dim3(256), sizeof(float) * RPB * 3 * Lf.nnx, st,
                                Lf.nnx, Lf.nny * Lf.nnz, (const float*)yl, Lc.nnx, span, s->tmp, done);
@@ -810,10 +832,8 @@ static int mixed_vcycle_std(t3d_mixed* s, const MGView& v, const float* r, int f
             hipLaunchKernelGGL(k_restrict_x<float>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, (const float*)yl, s->tmp,
                                done);
         }
-        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
         hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
-                           s->r32[l + 1], (last && !chol) ? (const float*)s->D32[L] : nullptr, s->x32[L],
-                           (last && chol) ? s->rc64 : nullptr, done, v.norm);
+                           s->r32[l + 1], Dc, s->x32[L], rcd, done, v.norm);
         T3D_HIP(hipGetLastError());
     }
     if (chol) {
@@ -864,6 +884,13 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
         const int span = v.span[l + 1];
         const float* Dc = (last && !chol) ? s->D32[L] : nullptr;
         double* rcd = (last && chol) ? s->rc64 : nullptr;
+        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
+        if (l > 0 && (span == 2 || span == 4)) {   // coarse levels: one pass
+            hipLaunchKernelGGL(k_restrict_c32, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->r32[l],
+                               v.mask[l + 1], s->r32[l + 1], Dc, s->x32[L], rcd, done, v.norm);
+            T3D_HIP(hipGetLastError());
+            continue;
+        }
         if (l == 0) {
             if (!tmpReady)
                 hipLaunchKernelGGL(k_rx_rows, dim3(row_blocks(Lf)), dim3(256), sizeof(float) * RPB * 3 * Lf.nnx, st,
@@ -873,7 +900,6 @@ int mixed_vcycle(t3d_mixed* s, const MGView& v, const float* r, int flexible, in
             hipLaunchKernelGGL(k_restrict_x<float>, g1, dim3(64), 0, st, Lf, Lc.nnx, span, (const float*)s->r32[l],
                                s->tmp, done);
         }
-        const dim3 g2(unsigned((Lc.nnx + 63) / 64), unsigned(Lc.nny), unsigned(Lc.nnz));
         hipLaunchKernelGGL(k_restrict_yz, g2, dim3(64), 0, st, Lf, Lc, span, (const float*)s->tmp, v.mask[l + 1],
                            s->r32[l + 1], Dc, s->x32[L], rcd, done, v.norm);
         T3D_HIP(hipGetLastError());

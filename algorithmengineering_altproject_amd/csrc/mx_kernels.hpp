@@ -619,6 +619,45 @@ __device__ __forceinline__ void restrict_x_body(LevelF Lf, int cnx, int span, co
     t[o + 2] = v2;
 }
 
+// the end of the restriction of coarse node (cx, cy, cz): normalisation by
+// the in-range weight sum (norm = 1), coarse mask, stores
+__device__ __forceinline__ void restrict_finish(LevelF Lc, int cx, int cy, int cz, float v0, float v1, float v2,
+                                                float wxs, float wyz, const uint8_t* __restrict__ cmask,
+                                                float* __restrict__ rc, const float* __restrict__ Dc,
+                                                float* __restrict__ xc, double* __restrict__ rcd, int norm)
+{
+    const float ws = wxs * wyz;
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    if (!norm) {   // Galerkin form: the plain transpose of the prolongation
+        r0 = v0;
+        r1 = v1;
+        r2 = v2;
+    } else if (ws > 1e-12f) {
+        const float iw = 1.0f / ws;
+        r0 = v0 * iw;
+        r1 = v1 * iw;
+        r2 = v2 * iw;
+    }
+    const int64_t n = (int64_t(cz) * Lc.nny + cy) * Lc.nnx + cx;
+    const unsigned m = cmask[n];
+    if (m & 1u) r0 = 0.f;
+    if (m & 2u) r1 = 0.f;
+    if (m & 4u) r2 = 0.f;
+    rc[3 * n] = r0;
+    rc[3 * n + 1] = r1;
+    rc[3 * n + 2] = r2;
+    if (rcd) {
+        rcd[3 * n] = r0;
+        rcd[3 * n + 1] = r1;
+        rcd[3 * n + 2] = r2;
+    }
+    if (Dc) {
+        xc[3 * n] = (m & 1u) ? 0.f : r0 * Dc[3 * n];
+        xc[3 * n + 1] = (m & 2u) ? 0.f : r1 * Dc[3 * n + 1];
+        xc[3 * n + 2] = (m & 4u) ? 0.f : r2 * Dc[3 * n + 2];
+    }
+}
+
 // restriction pass 2 (y, z), normalisation by the in-range weight sum, coarse
 // mask; at the coarsest level optionally the diagonal solve xc = rc / D or an
 // FP64 copy of rc for the Cholesky solve (transfers.cpp:449-533).  SP > 0: the
@@ -690,37 +729,9 @@ __device__ __forceinline__ void restrict_yz_t(LevelF Lf, LevelF Lc, int span, co
             }
         }
     }
-    const float ws = wxs * wyz;
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-    if (!norm) {   // Galerkin form: the plain transpose of the prolongation
-        r0 = v0;
-        r1 = v1;
-        r2 = v2;
-    } else if (ws > 1e-12f) {
-        const float iw = 1.0f / ws;
-        r0 = v0 * iw;
-        r1 = v1 * iw;
-        r2 = v2 * iw;
-    }
-    const int64_t n = (int64_t(cz) * Lc.nny + cy) * Lc.nnx + cx;
-    const unsigned m = cmask[n];
-    if (m & 1u) r0 = 0.f;
-    if (m & 2u) r1 = 0.f;
-    if (m & 4u) r2 = 0.f;
-    rc[3 * n] = r0;
-    rc[3 * n + 1] = r1;
-    rc[3 * n + 2] = r2;
-    if (rcd) {
-        rcd[3 * n] = r0;
-        rcd[3 * n + 1] = r1;
-        rcd[3 * n + 2] = r2;
-    }
-    if (Dc) {
-        xc[3 * n] = (m & 1u) ? 0.f : r0 * Dc[3 * n];
-        xc[3 * n + 1] = (m & 2u) ? 0.f : r1 * Dc[3 * n + 1];
-        xc[3 * n + 2] = (m & 4u) ? 0.f : r2 * Dc[3 * n + 2];
-    }
+    restrict_finish(Lc, cx, cy, cz, v0, v1, v2, wxs, wyz, cmask, rc, Dc, xc, rcd, norm);
 }
+
 
 __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span, const float* __restrict__ t,
                                                     const uint8_t* __restrict__ cmask, float* __restrict__ rc,
@@ -735,6 +746,62 @@ __device__ __forceinline__ void restrict_yz_body(LevelF Lf, LevelF Lc, int span,
         restrict_yz_t<2>(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, norm, bx_, by_, bz_);
     else
         restrict_yz_t<0>(Lf, Lc, span, t, cmask, rc, Dc, xc, rcd, norm, bx_, by_, bz_);
+}
+
+// Restriction of an FP32 level with span SP (2 or 4) in one pass: for every
+// (y, z) tap, the x-pass value that restrict_x_body computes at (cx, fy, fz),
+// then the (y, z) sum of restrict_yz_t<SP> -- the same operations in the same
+// order, so the result is bit-identical to the two passes, without their
+// intermediate array and second launch.  For the small coarse levels, whose
+// passes take a launch's latency each.
+template <int SP>
+__device__ __forceinline__ void restrict_c32_t(LevelF Lf, LevelF Lc, const float* __restrict__ rf,
+                                               const uint8_t* __restrict__ cmask, float* __restrict__ rc,
+                                               const float* __restrict__ Dc, float* __restrict__ xc,
+                                               double* __restrict__ rcd, int norm, unsigned bx_, unsigned by_,
+                                               unsigned bz_)
+{
+    const int cx = bx_ * 64 + threadIdx.x;
+    if (cx >= Lc.nnx) return;
+    const int cy = by_, cz = bz_;
+    const float inv = 1.0f / float(SP);
+    float wxs = 0.f;
+    for (int d = -(SP - 1); d <= SP - 1; ++d) {
+        const int fx = cx * SP + d;
+        if (fx >= 0 && fx < Lf.nnx) wxs += 1.0f - float(d < 0 ? -d : d) * inv;
+    }
+    float v0 = 0.f, v1 = 0.f, v2 = 0.f, wyz = 0.f;
+#pragma unroll
+    for (int dz = -(SP - 1); dz <= SP - 1; ++dz) {
+        const int fz = cz * SP + dz;
+        const bool zin = fz >= 0 && fz < Lf.nnz;
+        const float wz = 1.0f - float(dz < 0 ? -dz : dz) / float(SP);
+#pragma unroll
+        for (int dy = -(SP - 1); dy <= SP - 1; ++dy) {
+            const int fy = cy * SP + dy;
+            const bool in = zin && fy >= 0 && fy < Lf.nny;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+            if (in) {   // restrict_x_body at (cx, fy, fz)
+                const int64_t row = (int64_t(fz) * Lf.nny + fy) * Lf.nnx;
+#pragma unroll
+                for (int d = -(SP - 1); d <= SP - 1; ++d) {
+                    const int fx = cx * SP + d;
+                    if (fx < 0 || fx >= Lf.nnx) continue;
+                    const float w = 1.0f - float(d < 0 ? -d : d) * inv;
+                    const int64_t i = 3 * (row + fx);
+                    a0 = fmaf(w, rf[i], a0);
+                    a1 = fmaf(w, rf[i + 1], a1);
+                    a2 = fmaf(w, rf[i + 2], a2);
+                }
+            }
+            const float ww = in ? wz * (1.0f - float(dy < 0 ? -dy : dy) / float(SP)) : 0.f;
+            v0 = fmaf(ww, a0, v0);
+            v1 = fmaf(ww, a1, v1);
+            v2 = fmaf(ww, a2, v2);
+            wyz += ww;
+        }
+    }
+    restrict_finish(Lc, cx, cy, cz, v0, v1, v2, wxs, wyz, cmask, rc, Dc, xc, rcd, norm);
 }
 
 // trilinear prolongation of xc at fine node (fx, fy, fz), clamped at the
