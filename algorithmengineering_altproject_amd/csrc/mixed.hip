@@ -1442,18 +1442,20 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
     T3D_HIP(hipMemsetAsync(s->dx, 0, sizeof(float) * n, st));
     if ((rc = mixed_vcycle(s, v, r, flexible, 1, false, flags))) return rc;
     // the initial residual norm, against which the solve checks for divergence
-    double rn2First = 0.0;
+    double rn2First = 0.0, stopHost = 0.0;
     T3D_HIP(hipMemcpyAsync(&rn2First, S + S_RN2, sizeof(double), hipMemcpyDeviceToHost, st));
+    T3D_HIP(hipMemcpyAsync(&stopHost, S + S_STOP, sizeof(double), hipMemcpyDeviceToHost, st));
     T3D_HIP(hipStreamSynchronize(st));
     // iteration it reads p_prev and writes p_cur; the two buffers alternate
     auto pPrevOf = [&](int it) { return (it & 1) ? s->pB : s->pA; };
     auto pCurOf = [&](int it) { return (it & 1) ? s->pA : s->pB; };
     const int CHUNK = 32;
+    int chunk = CHUNK;
     int result = maxIt;
     bool confirmed = false;
     int it0 = 0;
     while (it0 < maxIt) {
-        const int it1 = std::min(maxIt, it0 + CHUNK);
+        const int it1 = std::min(maxIt, it0 + chunk);
         for (int it = it0; it < it1; ++it) {
             // p = z + beta p_prev ; Ap = K32 p ; alpha = rz / p.Ap
             if (s->timing) T3D_HIP(hipEventRecord(s->tev[2 * (it - it0)], st));
@@ -1502,6 +1504,16 @@ int t3d_pcg_mixed(t3d_mixed* s, const double* b, double* x, double tol, int maxI
             }
         }
         if (!s->hostFlags[0]) {
+            // the next chunk ends two iterations after the convergence that the
+            // mean decay so far predicts (iterations launched beyond it only
+            // return at once), at least 4 and at most CHUNK iterations
+            chunk = CHUNK;
+            const double stop2 = stopHost * stopHost;
+            if (rn2Now > 0.0 && rn2Now < rn2First && stop2 > 0.0 && stop2 < rn2Now) {
+                const double perIt = std::log(rn2Now / rn2First) / double(it1);
+                const double need = std::log(stop2 / rn2Now) / perIt;
+                if (need < double(CHUNK)) chunk = std::max(4, int(std::ceil(need)) + 2);
+            }
             it0 = it1;
             continue;
         }
