@@ -150,6 +150,30 @@ __global__ void __launch_bounds__(64) kb_restrict_yz(const BProb* __restrict__ P
                      bx, cy, cz, 0, 0, q.norm);
 }
 
+// restriction of coarse level l >= 1 into l + 1 in one pass (restrict_c32_t,
+// bit-identical to kb_restrict_x + kb_restrict_yz), on the y/z pass's map
+__global__ void __launch_bounds__(64) kb_restrict_c32(const BProb* __restrict__ P, const BMap* __restrict__ map, int l)
+{
+    const BMap m = map[blockIdx.x];
+    const BProb& q = P[m.prob];
+    if (q.flags && *q.flags) return;
+    const LevelF& Lc = q.lv[l + 1].L;
+    const unsigned gx = unsigned((Lc.nnx + 63) / 64);
+    const unsigned loc = unsigned(m.local);
+    const unsigned bx = loc % gx, cy = (loc / gx) % unsigned(Lc.nny), cz = loc / (gx * unsigned(Lc.nny));
+    const int L = q.nlev - 1;
+    const bool last = (l + 1 == L);
+    const bool chol = q.Nc > 0;
+    const float* Dc = (last && !chol) ? q.lv[L].Dinv : nullptr;
+    double* rcd = (last && chol) ? q.rc64 : nullptr;
+    if (q.lv[l + 1].span == 4)
+        restrict_c32_t<4>(q.lv[l].L, Lc, (const float*)q.lv[l].r, q.lv[l + 1].mask, q.lv[l + 1].r, Dc, q.lv[L].x, rcd,
+                          q.norm, bx, cy, cz);
+    else
+        restrict_c32_t<2>(q.lv[l].L, Lc, (const float*)q.lv[l].r, q.lv[l + 1].mask, q.lv[l + 1].r, Dc, q.lv[L].x, rcd,
+                          q.norm, bx, cy, cz);
+}
+
 // coarsest solve x = Ainv rc64 (coarse.hpp): one block per packed tile of
 // every listed problem, then one per row block (masked, FP64 and FP32 copies)
 __global__ void __launch_bounds__(256) kb_coarse_tiles(const BProb* __restrict__ P, const BMap* __restrict__ map,
@@ -254,6 +278,7 @@ struct t3d_batch {
     BMap* mapCS = nullptr;   // coarsest product: (problem, row block)
     unsigned nCS = 0;
     int maxLev = 0;
+    bool oneRestrict = true;   // every coarse level of every problem restricts in one pass (spans 2, 4)
     int* itDev = nullptr;
     int* flagsDev = nullptr;    // [2n] every problem's done flag and iteration, gathered for one copy
     int* flagsHost = nullptr;   // pinned mirror of flagsDev
@@ -325,6 +350,7 @@ static int batch_build(t3d_batch* B)
     B->hP.assign(B->n, BProb{});
     std::vector<BMap> kw, rows, rx[8], yz[8], pro[8], ct, cs;
     B->maxLev = 0;
+    B->oneRestrict = true;
     for (int i = 0; i < B->n; ++i) {
         t3d_problem* p = B->P[i];
         t3d_mixed* s = B->X[i];
@@ -374,6 +400,8 @@ static int batch_build(t3d_batch* B)
         q.rowBlocks = unsigned((int64_t(g.nny) * g.nnz + B->rpb - 1) / B->rpb);
         const int L = v.nlev - 1;
         B->maxLev = std::max(B->maxLev, L);
+        for (int l = 2; l <= L; ++l)
+            if (v.span[l] != 2 && v.span[l] != 4) B->oneRestrict = false;
         if (!B->active[i]) continue;
         for (unsigned t = 0; t < q.kwTiles; ++t) kw.push_back({i, int(t)});
         for (unsigned t = 0; t < q.rowBlocks; ++t) rows.push_back({i, int(t)});
@@ -480,6 +508,14 @@ static void batch_vcycle(t3d_batch* B, int flexible, int first, bool tmpReady, b
     hipStream_t st = B->stream;
     if (!tmpReady) launch_rx(B);
     for (int l = 0; l < B->maxLev; ++l) {
+        if (l >= 1 && B->oneRestrict) {
+            if (B->nYZ[l]) {
+                hipLaunchKernelGGL(kb_restrict_c32, dim3(B->nYZ[l]), dim3(64), 0, st, (const BProb*)B->dP,
+                                   (const BMap*)B->mapYZ[l], l);
+                tmark(B, tm, "kb_restrict_c32");
+            }
+            continue;
+        }
         if (l >= 1 && B->nRX[l]) {
             hipLaunchKernelGGL(kb_restrict_x, dim3(B->nRX[l]), dim3(64), 0, st, (const BProb*)B->dP,
                                (const BMap*)B->mapRX[l], l);
