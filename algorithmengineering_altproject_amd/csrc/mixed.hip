@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(256) k_resid64(Grid g, const double* __restric
     double dot = 0.0;
     const int nsteps = min(NZ, g.nnz - z0) + 1;
     for (int s = 0; s < nsteps; ++s) {
-        const int qn = (s + 1) % 3, qo = s % 3;
+        const int qn = (s + 1) % 3;
         const bool more = s + 1 < nsteps;
         const int cz = z0 - 1 + s;
         const double E = Enext <= 1.0e-16 ? 0.0 : Enext;
@@ -1006,9 +1006,6 @@ int mixed_confirm(t3d_mixed* s, const double* b, double* x, double* rn2, double*
 // x += dx ; dx = 0 (unconditional)
 int mixed_fold(t3d_mixed* s, double* x)
 {
-    t3d_problem* p = s->p;
-    const int64_t n = 3 * p->g.nN;
-    const unsigned fb = unsigned(std::min<int64_t>(VB, (n + 255) / 256));
     launch_fold(s, x, nullptr);
     T3D_HIP(hipGetLastError());
     return T3D_OK;

@@ -914,10 +914,50 @@ __device__ __forceinline__ void xr_rows_body(int nnx, int NR, const float* __res
     const double alpha = S[S_ALPHA];
     double s = 0.0;
     // the block's rows are one contiguous range starting at an even index
-    // (R is even), so float2 accesses are aligned
+    // (R is even), so float2 accesses are aligned; where both arrays' range
+    // starts on 16 bytes (R L a multiple of 4, or an even block), float4
+    // accesses, two per thread and array issued before their arithmetic
     const int64_t base = row0 * L;
     const int ne = nr * L;
-    for (int e = 2 * threadIdx.x; e < ne; e += 512) {
+    int e0 = 0;
+    if (((reinterpret_cast<uintptr_t>(Ap + base) | reinterpret_cast<uintptr_t>(r + base)) & 15) == 0) {
+        const int n4 = ne / 4;
+        const float4* A4 = reinterpret_cast<const float4*>(Ap + base);
+        float4* R4 = reinterpret_cast<float4*>(r + base);
+        for (int q0 = threadIdx.x; q0 < n4; q0 += 512) {
+            float4 aa[2], rv[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int q = q0 + 256 * u;
+                if (q < n4) {
+                    aa[u] = A4[q];
+                    rv[u] = R4[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int q = q0 + 256 * u;
+                if (q < n4) {
+                    const double v0 = fma(-alpha, double(aa[u].x), double(rv[u].x));
+                    const double v1 = fma(-alpha, double(aa[u].y), double(rv[u].y));
+                    const double v2 = fma(-alpha, double(aa[u].z), double(rv[u].z));
+                    const double v3 = fma(-alpha, double(aa[u].w), double(rv[u].w));
+                    const float4 o = make_float4(float(v0), float(v1), float(v2), float(v3));
+                    R4[q] = o;
+                    s = fma(v0, v0, s);
+                    s = fma(v1, v1, s);
+                    s = fma(v2, v2, s);
+                    s = fma(v3, v3, s);
+                    sR[4 * q] = o.x;
+                    sR[4 * q + 1] = o.y;
+                    sR[4 * q + 2] = o.z;
+                    sR[4 * q + 3] = o.w;
+                }
+            }
+        }
+        e0 = 4 * n4;
+    }
+    for (int e = e0 + 2 * threadIdx.x; e < ne; e += 512) {
         const int64_t i = base + e;
         if (e + 1 < ne) {
             const float2 aa = *reinterpret_cast<const float2*>(Ap + i);
