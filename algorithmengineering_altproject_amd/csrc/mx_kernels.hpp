@@ -55,11 +55,27 @@ __device__ __forceinline__ double block_sum_n(double s, double* sh)
     return t;
 }
 
+// thread t's sequential sum of partial[t], partial[t + 1024], ... (eight
+// loads in flight, added in index order)
+__device__ __forceinline__ double thread_partials_1024(const double* __restrict__ partial, unsigned m)
+{
+    double s = 0.0;
+    unsigned i = threadIdx.x;
+    for (; i + 7u * 1024u < m; i += 8u * 1024u) {
+        double w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = partial[i + 1024u * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += w[u];
+    }
+    for (; i < m; i += 1024) s += partial[i];
+    return s;
+}
+
 // sum of partial[0..m) in a fixed order by a 1024-thread block (valid in thread 0)
 __device__ __forceinline__ double sum_partials_1024(const double* __restrict__ partial, unsigned m, double* sh)
 {
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < m; i += 1024) s += partial[i];
+    double s = thread_partials_1024(partial, m);
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
