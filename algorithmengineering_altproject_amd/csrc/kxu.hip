@@ -111,12 +111,18 @@ struct KxuTile {
 // the operator that consumes p: every tile forms p for its nodes and halo
 // from z and p_prev while staging them, and stores it for the nodes it owns
 // (p_prev and p are different buffers, so a neighbour's halo reads the old p).
+// With x set, the same tile also applies the previous iteration's solution
+// update x += alpha p_prev to the nodes it owns (solver.cpp:184-188), alpha =
+// rzOld / max(1e-30, denomOld) as the iteration computed it: the update the
+// x/r kernel no longer makes, with the same operations on the same values.
 struct FuseP {
     const double* z;
     const double* pPrev;
     const double* rzNew;   // beta = rzNew / max(1e-30, rzOld)
     const double* rzOld;
     double* p;
+    double* x = nullptr;
+    const double* denomOld = nullptr;
 };
 
 template <int NZ, bool FUSEP = false>
@@ -133,10 +139,14 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
     const TileId tile = xcd_tile();
     const int x0 = tile.x * T::TX, y0 = tile.y * T::TY, z0 = tile.z * T::TZ;
     const int tid = threadIdx.x;
-    double beta = 0.0;
+    double beta = 0.0, alphaPrev = 0.0;
     if constexpr (FUSEP) {
         const double o = *fp.rzOld;
         beta = *fp.rzNew / (1.0e-30 < o ? o : 1.0e-30);
+        if (fp.x) {
+            const double d = *fp.denomOld;
+            alphaPrev = o / (1.0e-30 < d ? d : 1.0e-30);
+        }
     }
     // cooperative loads (zero outside the grid)
     for (int i = tid; i < T::UN; i += 512) {
@@ -146,11 +156,19 @@ __global__ void __launch_bounds__(512) k_kxu_f64_lds(Grid g, const double* __res
         if (X >= 0 && Y >= 0 && Z >= 0 && X < g.nnx && Y < g.nny && Z < g.nnz) {
             const int64_t n = (int64_t(Z) * g.nny + Y) * g.nnx + X;
             if constexpr (FUSEP) {
-                v0 = fp.z[3 * n] + beta * fp.pPrev[3 * n];
-                v1 = fp.z[3 * n + 1] + beta * fp.pPrev[3 * n + 1];
-                v2 = fp.z[3 * n + 2] + beta * fp.pPrev[3 * n + 2];
+                const double q0 = fp.pPrev[3 * n], q1 = fp.pPrev[3 * n + 1], q2 = fp.pPrev[3 * n + 2];
+                v0 = fp.z[3 * n] + beta * q0;
+                v1 = fp.z[3 * n + 1] + beta * q1;
+                v2 = fp.z[3 * n + 2] + beta * q2;
                 if (lx >= 1 && lx <= T::TX && ly >= 1 && ly <= T::TY && lz >= 1 && lz <= T::TZ) {
                     fp.p[3 * n] = v0; fp.p[3 * n + 1] = v1; fp.p[3 * n + 2] = v2;
+                    if (fp.x) {
+                        double* xn = fp.x + 3 * n;
+                        const double x0v = xn[0], x1v = xn[1], x2v = xn[2];
+                        xn[0] = x0v + alphaPrev * q0;
+                        xn[1] = x1v + alphaPrev * q1;
+                        xn[2] = x2v + alphaPrev * q2;
+                    }
                 }
             } else {
                 v0 = u[3 * n]; v1 = u[3 * n + 1]; v2 = u[3 * n + 2];
@@ -261,13 +279,14 @@ int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done)
 
 // p = z + beta p_prev, then y = K p (see FuseP)
 int launch_kxu_f64_fusep(t3d_problem* p, const double* z, const double* pPrev, const double* rzNew,
-                         const double* rzOld, double* pOut, double* y, const int* done)
+                         const double* rzOld, double* pOut, double* y, const int* done, double* x,
+                         const double* denomOld)
 {
     using T = KxuTile<1>;
     const Grid& g = p->g;
     dim3 grid(unsigned((g.nnx + T::TX - 1) / T::TX), unsigned((g.nny + T::TY - 1) / T::TY),
               unsigned((g.nnz + T::TZ - 1) / T::TZ));
-    const FuseP fp{z, pPrev, rzNew, rzOld, pOut};
+    const FuseP fp{z, pPrev, rzNew, rzOld, pOut, x, denomOld};
     hipLaunchKernelGGL((k_kxu_f64_lds<1, true>), grid, dim3(512), 0, p->stream, g, p->Ke, p->Ecell, p->nodeFixed,
                        (const double*)pOut, y, done, fp);
     T3D_HIP(hipGetLastError());

@@ -15,9 +15,11 @@
 namespace t3d {
 
 // per-iteration scalar slots in p->pcgSc.  The direction update
-// p = z + beta p (solver.cpp:203-210) is done by the next iteration's operator
-// (launch_kxu_f64_fusep) and r.z by the V-cycle's last pass (mg_apply_dev), in
-// the same arithmetic and summation order.
+// p = z + beta p (solver.cpp:203-210) and the solution update x += alpha p
+// (solver.cpp:184-188) are done by the next iteration's operator
+// (launch_kxu_f64_fusep; after the last iteration by k_pcg_fold_x) and r.z by
+// the V-cycle's last pass (mg_apply_dev), in the same arithmetic and
+// summation order.
 struct PcgSlots {
     double* normb2;   // [1]
     double* stop;     // [1]
@@ -48,9 +50,11 @@ __global__ void k_pcg_stop(const double* __restrict__ normb2, double tol, double
 }
 
 // x += alpha p ; r = r - alpha Ap ; tile partials of r.r (canonical layout)
+// r -= alpha Ap with the canonical tile partials of r.r (solver.cpp:189-199);
+// x += alpha p of this iteration is applied by the next operator
 __global__ void __launch_bounds__(256) k_pcg_update_xr(int64_t n, const double* __restrict__ rz,
-                                                       const double* __restrict__ denom, const double* __restrict__ p,
-                                                       const double* __restrict__ Ap, double* __restrict__ x,
+                                                       const double* __restrict__ denom,
+                                                       const double* __restrict__ Ap,
                                                        double* __restrict__ r, double* __restrict__ partial,
                                                        const int* __restrict__ done)
 {
@@ -64,7 +68,6 @@ __global__ void __launch_bounds__(256) k_pcg_update_xr(int64_t n, const double* 
     for (int k = 0; k < CANON_PER_LANE; ++k) {
         const int64_t i = base + int64_t(k) * CANON_LANES;
         if (i < n) {
-            x[i] = x[i] + alpha * p[i];
             const double v = r[i] - alpha * Ap[i];
             r[i] = v;
             s = s + v * v;
@@ -84,6 +87,16 @@ __global__ void __launch_bounds__(256) k_pcg_update_xr(int64_t n, const double* 
         for (int w = 32; w >= 1; w >>= 1) v = v + __shfl_down(v, w, 64);
         if (l == 0) partial[blockIdx.x] = v;
     }
+}
+
+// x += alpha p of the last iteration run (the one no later operator applies)
+__global__ void k_pcg_fold_x(int64_t n, const double* __restrict__ rz, const double* __restrict__ denom,
+                             const double* __restrict__ p, double* __restrict__ x)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double d = *denom;
+    const double alpha = *rz / (1.0e-30 < d ? d : 1.0e-30);
+    if (i < n) x[i] = x[i] + alpha * p[i];
 }
 
 __global__ void k_pcg_check(const double* __restrict__ rn2, const double* __restrict__ stop, int it,
@@ -165,11 +178,12 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
         for (int it = it0; it < it1; ++it) {
             pp = pb[it & 1];
             if (it == 0) rc = launch_kxu_f64(p, pp, Ap, flags);
-            else rc = launch_kxu_f64_fusep(p, M ? z : r, pb[(it - 1) & 1], S.rz + it, S.rz + it - 1, pp, Ap, flags);
+            else rc = launch_kxu_f64_fusep(p, M ? z : r, pb[(it - 1) & 1], S.rz + it, S.rz + it - 1, pp, Ap, flags,
+                                           x, S.denom + it - 1);
             if (rc) return rc;
             if ((rc = canon_dot_dev(p, pp, Ap, n, S.denom + it, flags))) return rc;
-            hipLaunchKernelGGL(k_pcg_update_xr, dim3(ntile), dim3(CANON_LANES), 0, st, n, S.rz + it, S.denom + it, pp,
-                               Ap, x, r, p->red, flags);
+            hipLaunchKernelGGL(k_pcg_update_xr, dim3(ntile), dim3(CANON_LANES), 0, st, n, S.rz + it, S.denom + it,
+                               Ap, r, p->red, flags);
             T3D_HIP(hipGetLastError());
             if ((rc = canon_finish(p, ntile, S.rn2 + it, flags))) return rc;
             hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(1), 0, st, S.rn2 + it, S.stop, it, flags);
@@ -189,6 +203,11 @@ int t3d_pcg(t3d_problem* p, t3d_mg* M, const double* b, double* x, double tol, i
             result = p->hostFlags[1];
             break;
         }
+    }
+    if (result > 0) {
+        const int k = result - 1;
+        hipLaunchKernelGGL(k_pcg_fold_x, dim3(nb), dim3(B), 0, st, n, S.rz + k, S.denom + k, pb[k & 1], x);
+        T3D_HIP(hipGetLastError());
     }
     T3D_HIP(hipStreamSynchronize(st));
     if (iters) *iters = result;

@@ -143,7 +143,8 @@ int canon_dot_dev(t3d_problem* p, const double* a, const double* b, int64_t n, d
 int launch_kxu_f64(t3d_problem* p, const double* u, double* y, const int* done = nullptr);
 // p = z + rzNew / max(1e-30, rzOld) * pPrev into pOut (!= pPrev), then y = K p
 int launch_kxu_f64_fusep(t3d_problem* p, const double* z, const double* pPrev, const double* rzNew,
-                         const double* rzOld, double* pOut, double* y, const int* done);
+                         const double* rzOld, double* pOut, double* y, const int* done, double* x = nullptr,
+                         const double* denomOld = nullptr);
 __global__ void k_copy_scalar(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ done);
 // z = M r; with rzDst also r.z by the canonical tree into that device scalar
 int mg_apply_dev(t3d_mg* g, const double* r, double* z, const int* done, double* rzDst = nullptr);
